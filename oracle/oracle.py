@@ -1,0 +1,470 @@
+"""CPU FP64 oracle for the BsaLib BFM hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg may
+import this module, and only as the checker / the timed CPU baseline.  The
+product package ``bsalib_amd`` never imports it.
+
+numpy restatements here (small, closed-form pieces) and ctypes bindings to
+``bsa_oracle.c`` (the loop-heavy restatements).  Every function cites the
+reference routine it follows.  How each function is pinned independently of
+its own formula is listed in oracle/README.md.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "_build" / "libbsa_oracle.so"
+TWO_PI = 2.0 * np.pi
+FOUR_PI = 4.0 * np.pi
+
+_lib = None
+
+
+def build(force: bool = False) -> Path:
+    """Compile bsa_oracle.c with gcc (OpenMP) into oracle/_build/."""
+    src = HERE / "bsa_oracle.c"
+    if force or not LIB_PATH.exists() or LIB_PATH.stat().st_mtime < src.stat().st_mtime:
+        LIB_PATH.parent.mkdir(exist_ok=True)
+        subprocess.check_call(["gcc", "-O2", "-fopenmp", "-fPIC", "-shared", "-std=c11",
+                               "-D_DEFAULT_SOURCE", str(src), "-o", str(LIB_PATH), "-lm"])
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        _lib = C.CDLL(str(LIB_PATH))
+    return _lib
+
+
+def _p(a, ct=C.c_double):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _i32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+def _f64F(a):
+    return np.asfortranarray(np.asarray(a, dtype=np.float64))
+
+
+# ---------------------------------------------------------------------------
+# wind PSDs -- src/wind/WindPSDImpl.F90
+# ---------------------------------------------------------------------------
+def eval_psd(prob, f, tc: int, idir: int = 1, pulsation: bool = True) -> np.ndarray:
+    """evalPSD_ (WindPSDImpl.F90:123-144): PSD(nf, NNL) for component ``tc``
+    (1-based), divided by 4*pi under the pulsation convention (:143)."""
+    f = np.asarray(f, dtype=np.float64).reshape(-1)
+    nl = prob.n_load - 1
+    wz = prob.wz_node[nl] - 1
+    U = prob.u_node[nl]
+    sig = prob.turb_std[tc - 1, wz]
+    pid = prob.psd_id
+    if pid == 1:      # vonKarmanPSD_ (:149-226)
+        if tc == 1:
+            Lr = prob.turb_scales[0, idir - 1, wz]
+            LU = Lr / U
+            x = np.abs(f)[:, None] * LU[None, :]
+            den = (1.0 + 70.7 * x * x) ** (5.0 / 6.0)
+            psd = 4.0 * (LU * sig ** 2)[None, :] / den
+        else:
+            Lr = prob.turb_scales[tc - 1, idir - 1, wz]
+            d = 2.0 * f[:, None] * (Lr / U)[None, :]
+            d = d * d
+            r2 = (1.0 + 70.7 * d) ** (11.0 / 6.0) * U[None, :]
+            r3 = Lr[None, :] * (1.0 + 188.4 * d) / r2 * (sig ** 2)[None, :]
+            psd = 4.0 * r3
+    elif pid == 2:    # kaimalPSD_ returns zero (:231-243)
+        psd = np.zeros((f.size, nl.size))
+    elif pid == 3:    # davenportPSD_Greisch_ (:249-277)
+        cst = 0.65 * 1200.0
+        um = prob.u_mean_ref[wz]
+        psd = (cst * sig ** 2 / um)[None, :] / \
+            (1.0 + (cst * f[:, None] / um[None, :]) ** 2.0) ** (5.0 / 6.0)
+    elif pid == 5:    # davenportPSD_Uliege_ (:283-316)
+        Lr = prob.turb_scales[tc - 1, idir - 1, wz]
+        cLU = Lr / U
+        cF = np.abs(f)[:, None] * cLU[None, :]
+        psd = (2.0 / 3.0) * cF * cLU[None, :] * (sig ** 2)[None, :] / (1.0 + cF ** 2) ** (4.0 / 3.0)
+    else:
+        raise ValueError(f"unsupported psd id {pid}")
+    if pulsation:
+        psd = psd / FOUR_PI
+    return np.asfortranarray(psd)
+
+
+def suvw_table(prob, f) -> np.ndarray:
+    """S_uvw(NF, NNL*NTC) as mainClassic_ builds it (BsaClassicImpl.F90:103-119)."""
+    return np.asfortranarray(np.concatenate([eval_psd(prob, f, int(t)) for t in prob.tc], axis=1))
+
+
+# ---------------------------------------------------------------------------
+# modal data -- src/bsa/BsaLibImpl.F90
+# ---------------------------------------------------------------------------
+def kept_modes(prob) -> np.ndarray:
+    """validateModalInfo_ (BsaLibImpl.F90:545-599): keep modes with max|phi| == 1."""
+    mx = np.abs(prob.phi).max(axis=0)
+    return (np.nonzero(mx == 1.0)[0] + 1).astype(np.int32)
+
+
+def psi(prob, modes=None) -> np.ndarray:
+    """setPhitimesCLocalInstance_ (BsaLibImpl.F90:611-671):
+    Psi(im, in, id) = sum_lib wfc(lib, id, in) * phi((n-1)*NLIBS + libs_load(lib), modes(im))."""
+    if modes is None:
+        modes = kept_modes(prob)
+    nm = len(modes)
+    nnl = prob.nnl
+    ndegw = prob.wfc.shape[1]
+    out = np.zeros((nm, nnl, ndegw), order="F")
+    for a in range(nnl):
+        n = int(prob.n_load[a])
+        rows = (n - 1) * prob.nlibs + prob.libs_load - 1
+        ph = prob.phi[np.ix_(rows, np.asarray(modes) - 1)]          # (NLIBSL, nm)
+        for d in range(ndegw):
+            out[:, a, d] = (prob.wfc[:, d, a][:, None] * ph).sum(axis=0)
+    return out
+
+
+def frf(prob, f, modes=None):
+    """H_m(f) = 1/(K - w^2 M + i w C_mm), w = 2 pi f (functionsImpl.F90:1641-1651)."""
+    if modes is None:
+        modes = kept_modes(prob)
+    mo = np.asarray(modes) - 1
+    w = TWO_PI * np.asarray(f, dtype=np.float64)[..., None]
+    r = -(w * w * prob.Mm[mo]) + prob.Km[mo]
+    i = w * np.diag(prob.Cm)[mo]
+    d = r * r + i * i
+    return r / d - 1j * i / d
+
+
+def skewness(m2: np.ndarray, m3: np.ndarray, nm: int) -> np.ndarray:
+    """computeSkewness_ (BsaLibImpl.F90:1630-1707): sk(m,n,p) = m3/(s_m s_n s_p)
+    with s_m = sqrt(m2(m,m)); index m + (n-1)nm + (p-1)nm^2."""
+    sig = np.sqrt(np.asarray(m2).reshape(nm, nm, order="F").diagonal())
+    den = sig[:, None, None] * sig[None, :, None] * sig[None, None, :]
+    return np.asarray(m3).reshape(nm, nm, nm, order="F") / den
+
+
+# ---------------------------------------------------------------------------
+# Classic vectorised path -- literal restatement (tiny sizes only)
+# ---------------------------------------------------------------------------
+def classic_vect(prob, with_bisp: bool = True, nthreads: int = 1) -> dict:
+    """mainClassic_ vectorised branch (BsaClassicImpl.F90:137-212):
+    getFM_full_tnm_vect_cls_ -> intgSpectraVect_ -> getRM_full_vect_cls_ ->
+    intgSpectraVect_ -> psd*w^2 -> intgSpectraVect_."""
+    L = lib()
+    f = prob.freqs()
+    nf = f.size
+    modes = kept_modes(prob)
+    nm = len(modes)
+    P = psi(prob, modes)
+    S = suvw_table(prob, f)
+    psd = np.zeros((nf, nm * nm), order="F")
+    bisp = np.zeros((nf, nf, nm ** 3), order="F") if with_bisp else None
+    L.orc_bfm_vect_cls(C.c_int(nf), _p(f), C.c_int(prob.nnl), C.c_int(prob.nn), C.c_int(nm),
+                       C.c_int(prob.ntc), _p(_i32(prob.tc), C.c_int),
+                       _p(_i32(prob.n_load), C.c_int), _p(P), C.c_int(P.shape[2]),
+                       _p(_f64F(prob.nod_corr)), C.c_long(prob.nod_corr.shape[0]), _p(S),
+                       _p(psd), _p(bisp), C.c_int(nthreads))
+    out = {"f": f, "psd_f": psd.copy(), "bisp_f": None if bisp is None else bisp.copy()}
+    m2 = np.zeros(nm * nm)
+    m3 = np.zeros(nm ** 3)
+    _intg(f, psd, m2, bisp, m3)
+    out["m2mf"], out["m3mf"] = m2.copy(), m3.copy()
+    L.orc_brm_vect_cls(C.c_int(nf), _p(f), C.c_int(nm), _p(modes, C.c_int),
+                       C.c_int(prob.phi.shape[1]), _p(_f64F(prob.Mm)), _p(_f64F(prob.Km)),
+                       _p(_f64F(prob.Cm)), _p(psd), _p(bisp))
+    out["psd_r"], out["bisp_r"] = psd.copy(), None if bisp is None else bisp.copy()
+    _intg(f, psd, m2, bisp, m3)
+    out["m2mr"], out["m3mr"] = m2.copy(), m3.copy()
+    om = TWO_PI * f
+    psd2 = np.asfortranarray(psd * (om * om)[:, None])
+    m2o2 = np.zeros(nm * nm)
+    _intg(f, psd2, m2o2, None, None)
+    out["m2o2mr"] = m2o2
+    out["modes"] = modes
+    return out
+
+
+def _intg(f, psd, m2, bisp, m3, pulsation=True):
+    """intgSpectraVect_ (BsaClassicImpl.F90:567-633)."""
+    nf = f.size
+    lib().orc_intg_vect(C.c_int(nf), _p(f), C.c_int(1 if pulsation else 0), _p(psd),
+                        C.c_int(0 if m2 is None else m2.size), _p(m2), _p(bisp),
+                        C.c_int(0 if m3 is None else m3.size), _p(m3))
+
+
+def intg_weights_2d(nf: int, d: float, rule: str) -> np.ndarray:
+    """Per-point weights equivalent to the reference quadratures:
+    'vect' intgSpectraVect_ (corner 1-3/2 = -1/2, edge 1/2), 'riemann'
+    (Classic scalar, BsaClassicImpl.F90:331,341), 'trapezoid' (zone rule,
+    _shared_poly2d.fi:827-832)."""
+    w = np.ones((nf, nf))
+    if rule == "vect":
+        w[0, :] = w[-1, :] = w[:, 0] = w[:, -1] = 0.5
+        for a in (0, -1):
+            for b in (0, -1):
+                w[a, b] = 1.0 - 1.5
+    elif rule == "trapezoid":
+        e = np.ones(nf)
+        e[0] = e[-1] = 0.5
+        w = e[:, None] * e[None, :]
+    elif rule != "riemann":
+        raise ValueError(rule)
+    return w * d * d
+
+
+# ---------------------------------------------------------------------------
+# factorised FP64 oracle over point sets
+# ---------------------------------------------------------------------------
+def a_table(prob, freqs, modes=None) -> np.ndarray:
+    """A(n,i;f) = sum_j Psi_a(n,j) S_ji(f), S_ji(f) = rho_ji^|f| sqrt(S_j S_i),
+    the projection inside getFM_full_tnm_vect_cls_ (functionsImpl.F90:1450-1476).
+    Returns (nm, nnl, ntc, nft)."""
+    from bsalib_amd.synth import corr_index
+    if modes is None:
+        modes = kept_modes(prob)
+    P = psi(prob, modes)
+    nm, nnl = P.shape[0], P.shape[1]
+    freqs = np.asarray(freqs, dtype=np.float64)
+    out = np.zeros((nm, nnl, prob.ntc, freqs.size), order="F")
+    nl = prob.n_load
+    idx = corr_index(nl[:, None], nl[None, :], prob.nn) - 1
+    for c, t in enumerate(prob.tc):
+        rho = prob.nod_corr[idx, t - 1]
+        S = eval_psd(prob, freqs, int(t))                      # (nft, nnl)
+        s = np.sqrt(S)
+        with np.errstate(divide="ignore"):
+            lr = np.log(rho)
+        for q, fq in enumerate(freqs):
+            af = abs(fq)
+            if af == 0.0:
+                coh = np.ones_like(rho)        # 0**0 == 1 (Fortran semantics)
+            else:
+                coh = np.exp(af * lr)
+            Sm = coh * (s[q][:, None] * s[q][None, :])
+            out[:, :, c, q] = P[:, :, t - 1] @ Sm
+    return out
+
+
+def classic_factorised(prob, rule: str = "vect", truncate: bool = True, nthreads: int = 0,
+                       want_bisp: bool = False, point_sample=None) -> dict:
+    """Factorised FP64 Classic BFM/BRM + m3 on the uniform grid.  ``truncate``
+    reproduces the Classic zero S(f1+f2) beyond +-fmax (functionsImpl.F90:
+    1409-1413); otherwise S(f1+f2) is evaluated exactly (Mesher / BsaCL)."""
+    L = lib()
+    f = prob.freqs()
+    nf = f.size
+    c = (nf - 1) // 2
+    modes = kept_modes(prob)
+    nm = len(modes)
+    P = psi(prob, modes)
+    tcs = prob.tc - 1
+    pa = np.asfortranarray(P[:, :, tcs])
+    pb = np.asfortranarray(P[:, :, tcs + 3])
+    fext = np.arange(-2 * c, 2 * c + 1) * prob.df
+    need_ext = not truncate
+    ft = fext if need_ext else f
+    off = 2 * c if need_ext else c
+    A = a_table(prob, ft, modes)
+    H = frf(prob, ft, modes)                                   # (nft, nm)
+    Hc = np.ascontiguousarray(np.stack([H.real, H.imag], axis=-1))   # (nft, nm, 2)
+    d = prob.df * TWO_PI
+    W = intg_weights_2d(nf, d, rule)
+    if point_sample is None:
+        ii, jj = np.meshgrid(np.arange(nf), np.arange(nf), indexing="ij")
+        ii, jj = ii.ravel(order="F"), jj.ravel(order="F")
+    else:
+        ii, jj = point_sample
+    k3 = ii + jj - c                                            # grid index of f1+f2
+    i1 = (ii - c + off).astype(np.int32)
+    i2 = (jj - c + off).astype(np.int32)
+    if need_ext:
+        i3 = (k3 - c + off).astype(np.int32)
+    else:
+        i3 = np.where((k3 >= 0) & (k3 < nf), k3, -1).astype(np.int32)
+    w = np.ascontiguousarray(W[ii, jj])
+    f3 = f[ii] + f[jj]
+    H3 = frf(prob, f3, modes)                                  # (npts, nm)
+    H3c = np.ascontiguousarray(np.stack([H3.real, H3.imag], axis=-1))
+    m3f = np.zeros(nm ** 3)
+    m3r = np.zeros(nm ** 3)
+    npts = ii.size
+    bfm = np.zeros((nm ** 3, npts), order="F") if want_bisp else None
+    brm = np.zeros((nm ** 3, npts), order="F") if want_bisp else None
+    L.orc_bfm_factorised(C.c_int(npts), _p(i1, C.c_int), _p(i2, C.c_int), _p(i3, C.c_int),
+                         _p(w), C.c_int(nm), C.c_int(prob.nnl), C.c_int(prob.ntc), _p(pa),
+                         _p(pb), _p(np.asfortranarray(A)), C.c_int(ft.size),
+                         _p(Hc), _p(H3c),
+                         _p(m3f), _p(m3r), _p(bfm), _p(brm), C.c_int(nthreads))
+    return {"m3mf": m3f, "m3mr": m3r, "bfm": bfm, "brm": brm, "modes": modes,
+            "points": (ii, jj)}
+
+
+def modal_psd(prob, rule_pulsation: bool = True) -> dict:
+    """Modal force/response PSD and m2 on the Classic grid:
+    psd(n,p;f) = sum_jk Psi_a(n,j) Psi_a(p,k) S_jk(f) (functionsImpl.F90:1546-1558),
+    psd_r = psd * Re[H_p conj(H_n)] (:1686-1689); m2 by intgSpectraVect_."""
+    f = prob.freqs()
+    modes = kept_modes(prob)
+    nm = len(modes)
+    P = psi(prob, modes)
+    A = a_table(prob, f, modes)                                 # (nm, nnl, ntc, nf)
+    psd = np.zeros((f.size, nm * nm), order="F")
+    for c, t in enumerate(prob.tc):
+        # psd(q, n + p*nm) = sum_k A(n,k) Psi_a(p,k)
+        S = np.einsum("nkq,pk->qnp", A[:, :, c, :], P[:, :, t - 1])
+        psd += S.reshape(f.size, nm * nm, order="F")
+    H = frf(prob, f, modes)
+    fr = (H[:, None, :] * np.conj(H[:, :, None])).real        # [q, n, p] = Re[H_p conj(H_n)]
+    psd_r = psd * fr.reshape(f.size, nm * nm, order="F")
+    om = TWO_PI * f
+    d = prob.df * TWO_PI
+    wv = np.full(f.size, d)
+    wv[0] = wv[-1] = d / 2.0
+    return {"psd_f": psd, "psd_r": psd_r, "m2mf": wv @ psd, "m2mr": wv @ psd_r,
+            "m2o2mr": wv @ (psd_r * (om * om)[:, None])}
+
+
+# ---------------------------------------------------------------------------
+# Mesher per-point BFM (non-POD) -- literal restatement
+# ---------------------------------------------------------------------------
+def msh_points(prob, fi, fj) -> np.ndarray:
+    """getFM_full_tnm_scalar_msh_ (functionsImpl.F90:247-465) at each point;
+    returns bfm(nm^3, npts).  PSDs at fi, fj, fi+fj via evalPSD (:302-306)."""
+    L = lib()
+    modes = kept_modes(prob)
+    nm = len(modes)
+    fi = np.atleast_1d(np.asarray(fi, dtype=np.float64))
+    fj = np.atleast_1d(np.asarray(fj, dtype=np.float64))
+    out = np.zeros((nm ** 3, fi.size), order="F")
+    buf = np.zeros(nm ** 3)
+    phi = _f64F(prob.phi)
+    wfc = _f64F(prob.wfc)
+    corr = _f64F(prob.nod_corr)
+    for q in range(fi.size):
+        S1 = np.asfortranarray(np.stack([eval_psd(prob, [fi[q]], int(t))[0] for t in prob.tc], axis=1))
+        S2 = np.asfortranarray(np.stack([eval_psd(prob, [fj[q]], int(t))[0] for t in prob.tc], axis=1))
+        S3 = np.asfortranarray(np.stack([eval_psd(prob, [fi[q] + fj[q]], int(t))[0] for t in prob.tc], axis=1))
+        L.orc_bfm_point_msh(C.c_double(fi[q]), C.c_double(fj[q]), C.c_int(prob.nnl), C.c_int(prob.nn),
+                            C.c_int(prob.nlibs), C.c_int(prob.libs_load.size),
+                            _p(_i32(prob.libs_load), C.c_int), C.c_int(nm), _p(modes, C.c_int),
+                            C.c_int(prob.phi.shape[1]), C.c_int(prob.ntc), _p(_i32(prob.tc), C.c_int),
+                            _p(_i32(prob.n_load), C.c_int), _p(wfc), C.c_int(wfc.shape[1]), _p(phi),
+                            C.c_int(phi.shape[0]), _p(corr), C.c_long(corr.shape[0]), _p(S1),
+                            _p(S2), _p(S3), _p(buf))
+        out[:, q] = buf
+    return out
+
+
+def bsacl_run(prob, fi, fj, psd_id=None, nthreads: int = 0) -> np.ndarray:
+    """bsaclRun with bfm_kernel id 2 (BsaCL/src/bsacl.c:1307-1592,
+    BsaCL/src/bsacl.cl:405-930) in FP64: response third moment over all
+    (fi, fj) pairs, plain Riemann sum times dInfl."""
+    L = lib()
+    modes = kept_modes(prob)
+    nm = len(modes)
+    P = psi(prob, modes)
+    mo = modes - 1
+    Mg = np.ascontiguousarray(prob.Mm[mo])
+    Kg = np.ascontiguousarray(prob.Km[mo])
+    Cg = np.ascontiguousarray(prob.Cm[np.ix_(mo, mo)].T)        # Cg[mi*nm+mi] diagonal
+    fi = np.ascontiguousarray(fi, dtype=np.float64)
+    fj = np.ascontiguousarray(fj, dtype=np.float64)
+    res = np.zeros(nm ** 3)
+    L.orc_bsacl_run(C.c_int(psd_id or prob.psd_id), C.c_int(prob.ntc), _p(_i32(prob.tc), C.c_int),
+                    C.c_int(prob.nnl), C.c_int(nm), _p(_i32(prob.n_load), C.c_int), _p(P),
+                    C.c_int(prob.nn), _p(_f64F(prob.nod_corr)), _p(_f64F(prob.u_node)),
+                    _p(_f64F(prob.turb_scales)), _p(_f64F(prob.turb_std)), _p(Mg), _p(Cg), _p(Kg),
+                    C.c_int(fi.size), _p(fi), C.c_int(fj.size), _p(fj), C.c_int(1), _p(res),
+                    C.c_int(nthreads))
+    return res
+
+
+# ---------------------------------------------------------------------------
+# the reference itself (oracle/_ref/ref_harness, built by oracle/ref/build_ref.sh)
+# ---------------------------------------------------------------------------
+REF_HARNESS = HERE / "_ref" / "ref_harness"
+
+
+def reference_available() -> bool:
+    return REF_HARNESS.exists()
+
+
+def _write_ref_input(prob, path, mode: int, fi=None, fj=None):
+    modes = kept_modes(prob)
+    P = psi(prob, modes)
+    npts = 0 if fi is None else int(np.size(fi))
+    hdr = np.array([prob.nn, prob.nlibs, prob.nnl, prob.libs_load.size, prob.phi.shape[1],
+                    modes.size, prob.ntc, prob.wfc.shape[1], prob.nwz, prob.psd_id,
+                    prob.nfreqs, mode, npts, prob.nod_corr.shape[0]], dtype=np.int32)
+    with open(path, "wb") as fh:
+        fh.write(hdr.tobytes())
+        for a in (prob.n_load, prob.libs_load, modes, prob.tc, prob.wz_node):
+            fh.write(np.asarray(a, dtype=np.int32).tobytes())
+        fh.write(np.float64(prob.df).tobytes())
+        for a in (prob.phi, prob.natf, prob.Mm, prob.Km, prob.Cm, prob.u_node, prob.turb_scales,
+                  prob.turb_std, prob.u_mean_ref, prob.nod_corr, prob.wfc, P):
+            fh.write(np.asarray(a, dtype=np.float64).tobytes(order="F"))
+        if npts:
+            fh.write(np.asarray(fi, dtype=np.float64).tobytes())
+            fh.write(np.asarray(fj, dtype=np.float64).tobytes())
+
+
+def reference_run(prob, mode: int = 1, fi=None, fj=None, workdir=None) -> dict:
+    """Run the reference's own operators through oracle/_ref/ref_harness.
+    mode bit 1: Classic vect BFM + BRM on the full grid; bit 2: Mesher BFM at (fi, fj)."""
+    import tempfile
+    if not reference_available():
+        raise FileNotFoundError(str(REF_HARNESS))
+    modes = kept_modes(prob)
+    nm = modes.size
+    nf = prob.nf
+    with tempfile.TemporaryDirectory(dir=workdir) as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        _write_ref_input(prob, fin, mode, fi, fj)
+        subprocess.run([str(REF_HARNESS), fin, fout], check=True, stdout=subprocess.DEVNULL,
+                       env=dict(os.environ, OMP_NUM_THREADS=os.environ.get("OMP_NUM_THREADS", "8")))
+        raw = np.fromfile(fout, dtype=np.float64)
+    out = {}
+    o = 0
+
+    def take(n, shape):
+        nonlocal o
+        a = raw[o:o + n].reshape(shape, order="F")
+        o += n
+        return a
+    out["suvw"] = take(nf * prob.nnl * prob.ntc, (nf, prob.nnl * prob.ntc))
+    if mode & 1:
+        out["psd_f"] = take(nf * nm * nm, (nf, nm * nm))
+        out["bisp_f"] = take(nf * nf * nm ** 3, (nf, nf, nm ** 3))
+        out["psd_r"] = take(nf * nm * nm, (nf, nm * nm))
+        out["bisp_r"] = take(nf * nf * nm ** 3, (nf, nf, nm ** 3))
+    if mode & 2 and fi is not None:
+        out["bfm_msh"] = take(nm ** 3 * np.size(fi), (nm ** 3, np.size(fi)))
+    return out
+
+
+def reference_time(prob, threads: int) -> float:
+    """Wall time of the reference's getFM_full_tnm_vect_cls_ on ``prob``."""
+    import tempfile
+    with tempfile.TemporaryDirectory() as td:
+        fin = os.path.join(td, "in.bin")
+        _write_ref_input(prob, fin, 1)
+        r = subprocess.run([str(REF_HARNESS), fin, os.path.join(td, "o.bin"), "time"],
+                           check=True, capture_output=True, text=True,
+                           env=dict(os.environ, OMP_NUM_THREADS=str(threads)))
+    for line in r.stdout.splitlines():
+        if line.startswith("REFTIME"):
+            return float(line.split()[1])
+    raise RuntimeError("reference timing line missing:\n" + r.stdout[-2000:])

@@ -1,0 +1,36 @@
+#!/usr/bin/env bash
+# Compile the reference's own BFM operator library (Fortran, unmodified, read in
+# place under /root/reference/src) plus the test-only harness ref_harness.F90
+# into oracle/_ref/ref_harness.  Test infrastructure only.
+#
+# Units compiled (dependency order): CONSTANTS, Timer, BsaIO, Settings,
+# StructureData, WindType, utility, BsaLibData(+Impl), IO/Settings/Structure/
+# Wind implementations, BsaLib_Functions(+Impl).  The reference's own build
+# system is not used.  functionsImpl.F90 references LAPACK dsyev_ only on the
+# Mesher-POD path, which the harness never calls; the image has no LAPACK, so
+# that one symbol is left unresolved at link time (no stand-in is provided).
+set -euo pipefail
+REF=${BSA_REFERENCE:-/root/reference}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT=$(cd "$HERE/.." && pwd)/_ref
+SRC=$REF/src
+FC=${FC:-amdflang}
+FFLAGS="-cpp -O2 -fopenmp -fPIC -I$SRC/CONSTANTS"
+[ -d "$SRC" ] || { echo "reference sources not found at $SRC" >&2; exit 3; }
+rm -rf "$OUT/obj"
+mkdir -p "$OUT/obj"
+cd "$OUT/obj"
+UNITS="CONSTANTS/CONSTANTS.F90 timing/Timer.F90 IO/BsaIO.F90 settings/Settings.F90
+structure/StructureData.F90 wind/WindType.F90 utils/utility.F90 bsa/data/BsaLibData.F90
+bsa/data/BsaLibDataImpl.F90 IO/BsaIOImpl.F90 settings/SettingsImpl.F90
+structure/StructureImpl.F90 wind/WindSetImpl.F90 wind/WindPSDImpl.F90
+bsa/functions/functions.F90 bsa/functions/functionsImpl.F90"
+OBJS=""
+for u in $UNITS; do
+  o=$(basename "$u" .F90).o
+  $FC $FFLAGS -c "$SRC/$u" -o "$o"
+  OBJS="$OBJS $o"
+done
+$FC $FFLAGS -c "$HERE/ref_harness.F90" -o ref_harness.o
+$FC -fopenmp -o "$OUT/ref_harness" ref_harness.o $OBJS -Wl,--unresolved-symbols=ignore-all
+echo "built $OUT/ref_harness"
