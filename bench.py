@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Benchmark of the BFM hot path on one MI355X.
+
+Workload: bridge deck NN=1000, NM=30, u+w turbulence, Classic grid 1025 x 1025
+over [-4, 4] Hz (the 1024^2 configuration, realised as the reference's odd
+1025-point grid).  One step = one full Classic pass: nodal PSDs, projection
+A(f), node contraction over the whole (f1,f2) plane with FRF weighting and
+quadrature -> m3mf, m3mr (NM^3 each), plus the modal PSD path -> m2mf, m2mr,
+m2o2mr.  An "eval" is one (f1,f2) point of the plane with its full NM^3 modal
+bispectrum accumulated into m3 (1025^2 evals per step).
+
+Prints ONE JSON line (see keys below).  --gpus N with N > 1 exits: this build
+runs on a single GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "modal-bispectrum evals/s (NN=1000,NM=30) at skew err≤1e-4: mixed vs FP64 GPU"
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def gsym_flops(prob, nm):
+    """Algorithmic FP64 flops of one k_gsym_f64 launch: every distinct value
+    G[x](y,u; z,v) (u, v on the non-negative half grid, (y,u) <= (z,v)) is a
+    dot product of length NTC*NNL (2 flops per term)."""
+    nh = prob.nfreqs
+    n = nm * nh
+    return 2.0 * prob.ntc * prob.nnl * nm * n * (n + 1) / 2.0
+
+
+def cpu_reference_baseline(threads: int):
+    """Reference's own getFM_full_tnm_vect_cls_ (oracle/_ref, amdflang -O2,
+    OpenMP) timed on a bounded sample of the deck: NNL=10 nodes spread over
+    the span, NM=30, NTC=2, 33-point grid.  Its element-update rate divided by
+    the updates one 1025^2-grid eval needs at NNL=1000 (NTC*NNL^3*NM^3) gives
+    the reference's evals/s at the benchmark size (extrapolated in NNL^3, the
+    reference's exact loop count)."""
+    from bsalib_amd import synth
+    from oracle import oracle as O
+    if not O.reference_available():
+        return None
+    p = synth.deck(nn=10, nm=30, nfreqs=17, df=0.25, span=1000.0)
+    t = O.reference_time(p, threads)
+    updates = p.ntc * p.nnl ** 3 * 30 ** 3 * p.nf ** 2
+    rate = updates / t
+    per_eval = 2 * 1000 ** 3 * 30 ** 3
+    return {"value": rate / per_eval, "unit": "evals/s", "cores": threads, "kind": "reference",
+            "sample": f"getFM_full_tnm_vect_cls_ NNL=10 NM=30 NTC=2 NF=33 ({t:.2f} s, "
+                      f"{rate:.3e} element updates/s), extrapolated by NTC*NNL^3*NM^3 to NNL=1000",
+            "seconds": t}
+
+
+def cpu_factorised_baseline(prob, threads: int, npts: int = 64):
+    """Factorised FP64 oracle (A(f) projection + three node contractions per
+    point, OpenMP over points) on a sample of npts points of the benchmark grid."""
+    from oracle import oracle as O
+    nf = prob.nf
+    rng = np.random.default_rng(7)
+    ii = rng.integers(0, nf, npts)
+    jj = rng.integers(0, nf, npts)
+    t0 = time.perf_counter()
+    O.classic_factorised(prob, rule="vect", truncate=True, nthreads=threads, point_sample=(ii, jj))
+    t = time.perf_counter() - t0
+    return {"value": npts / t, "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": f"{npts} random points of the 1025^2 grid incl. the projection table "
+                      f"({t:.2f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--config", default="config3", choices=["config1", "config2", "config3"])
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build targets one GPU; multi-GPU runs are not part of it "
+              "(the (f1,f2) plane would be sharded by tile pairs with one all-reduce of m3).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from bsalib_amd import abi, synth
+    prob = getattr(synth, args.config)()
+    nm = prob.nm
+    log(f"workload {prob.name} NN={prob.nn} NM={nm} NTC={prob.ntc} NF={prob.nf}")
+    abi.init(0)
+    bound = abi.Bound(prob)
+    for _ in range(args.warmup):
+        abi.classic_run(prob, quad="vect", bound=bound)
+    log("warmup done")
+    kms = []
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        r = abi.classic_run(prob, quad="vect", bound=bound)
+        kms.append(abi.last_kernel_ms()[0])
+    t = time.perf_counter() - t0
+    ms_step = 1e3 * t / args.steps
+    evals = prob.nf ** 2
+    value = evals * args.steps / t
+    k_ms = float(np.mean(kms))
+    flops = gsym_flops(prob, nm)
+    achieved = flops / (k_ms * 1e-3) / 1e12
+    sk = None
+    try:
+        from bsalib_amd.host import skewness
+        sk = float(np.nanmax(np.abs(skewness(r["m2mr"], r["m3mr"], nm))))
+    except Exception:  # noqa: BLE001
+        pass
+    log(f"step {ms_step:.2f} ms, k_gsym {k_ms:.2f} ms, {achieved:.2f} TFLOP/s")
+
+    cpu = None
+    cpu2 = None
+    if not args.no_cpu:
+        threads = int(os.environ.get("BENCH_CPU_THREADS", str(min(16, os.cpu_count() or 1))))
+        try:
+            cpu = cpu_reference_baseline(threads)
+            log(f"cpu reference baseline {cpu}")
+        except Exception as e:  # noqa: BLE001
+            log(f"cpu reference baseline failed: {e}")
+        try:
+            cpu2 = cpu_factorised_baseline(prob, threads)
+            log(f"cpu factorised baseline {cpu2}")
+        except Exception as e:  # noqa: BLE001
+            log(f"cpu factorised baseline failed: {e}")
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded bridge-deck generator)",
+        "config": {"workload": "bridge deck NN=1000 NM=30 NTC=2 (u,w), Classic grid 1025x1025 "
+                               "over [-4,4] Hz, FP64 plain form",
+                   "nn": prob.nn, "nm": nm, "ntc": prob.ntc, "grid": prob.nf,
+                   "precision": "fp64"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_gsym_f64", "kernel_ms": k_ms, "flops_per_launch": flops},
+        "cpu_baseline": cpu,
+        "cpu_baseline_factorised": cpu2,
+        "max_abs_skewness": sk,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

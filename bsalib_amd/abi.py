@@ -1,0 +1,170 @@
+"""ctypes binding of libbsahip.so (C ABI declared in include/bsahip.h).
+
+The library is loaded from bsalib_amd/_lib/ (built in-tree by
+``bsalib_amd.build``).  A missing library or a missing GPU raises; nothing
+here computes on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+from . import host
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libbsahip.so"
+
+PREC_FP64, PREC_FP32, PREC_BF16X3 = 0, 1, 2
+QUAD_VECT, QUAD_RIEMANN, QUAD_TRAPZ = 0 << 4, 1 << 4, 2 << 4
+SUM_TRUNCATED, SUM_EXACT = 0, 1 << 8
+NO_PSD, NO_BISP = 1 << 9, 1 << 10
+
+QUAD = {"vect": QUAD_VECT, "riemann": QUAD_RIEMANN, "trapezoid": QUAD_TRAPZ}
+PREC = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
+
+
+class BsaHipError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"bsahip error {code}: {msg}")
+        self.code = code
+
+
+class Problem_t(C.Structure):
+    """Mirror of ``bsahip_problem`` (include/bsahip.h)."""
+    _fields_ = [
+        ("nn", C.c_int32), ("nnl", C.c_int32), ("nm", C.c_int32), ("ntc", C.c_int32),
+        ("ndegw", C.c_int32), ("nwz", C.c_int32), ("psd_id", C.c_int32),
+        ("pulsation", C.c_int32), ("ncorr_rows", C.c_int32), ("corr_col_mode", C.c_int32),
+        ("n_load", C.POINTER(C.c_int32)), ("tc", C.POINTER(C.c_int32)),
+        ("psi", C.POINTER(C.c_double)), ("nod_corr", C.POINTER(C.c_double)),
+        ("u_node", C.POINTER(C.c_double)), ("wz_node", C.POINTER(C.c_int32)),
+        ("turb_scales", C.POINTER(C.c_double)), ("turb_std", C.POINTER(C.c_double)),
+        ("u_mean_ref", C.POINTER(C.c_double)), ("Mm", C.POINTER(C.c_double)),
+        ("Km", C.POINTER(C.c_double)), ("Cm", C.POINTER(C.c_double)),
+    ]
+
+
+_lib = None
+
+EXPORTS = ["bsahip_init", "bsahip_finalise", "bsahip_last_error", "bsahip_version",
+           "bsahip_classic_run", "bsahip_classic_spectra", "bsahip_bfm_points",
+           "bsahip_last_kernel_time"]
+
+
+def lib():
+    """Load libbsahip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise FileNotFoundError(f"{LIB_PATH} is missing: run `python -m bsalib_amd.build` "
+                                    "(the HIP library is required, there is no CPU path)")
+        L = C.CDLL(str(LIB_PATH))
+        L.bsahip_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise BsaHipError(rc, lib().bsahip_last_error().decode(errors="replace"))
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct)) if a is not None else None
+
+
+class Bound:
+    """A problem packed into the C struct, keeping the numpy buffers alive."""
+
+    def __init__(self, prob, modes=None, corr_col_mode: int = 0, psd_id=None):
+        self.modes = host.kept_modes(prob) if modes is None else np.asarray(modes, np.int32)
+        psi = host.build_psi(prob, self.modes)
+        mo = self.modes - 1
+        self.keep = dict(
+            n_load=np.ascontiguousarray(prob.n_load, dtype=np.int32),
+            tc=np.ascontiguousarray(prob.tc, dtype=np.int32),
+            psi=np.asfortranarray(psi),
+            nod_corr=np.asfortranarray(prob.nod_corr, dtype=np.float64),
+            u_node=np.ascontiguousarray(prob.u_node, dtype=np.float64),
+            wz_node=np.ascontiguousarray(prob.wz_node, dtype=np.int32),
+            turb_scales=np.asfortranarray(prob.turb_scales, dtype=np.float64),
+            turb_std=np.asfortranarray(prob.turb_std, dtype=np.float64),
+            u_mean_ref=np.ascontiguousarray(prob.u_mean_ref, dtype=np.float64),
+            Mm=np.ascontiguousarray(prob.Mm[mo], dtype=np.float64),
+            Km=np.ascontiguousarray(prob.Km[mo], dtype=np.float64),
+            Cm=np.ascontiguousarray(np.diag(prob.Cm)[mo], dtype=np.float64),
+        )
+        k = self.keep
+        s = Problem_t()
+        s.nn, s.nnl, s.nm, s.ntc = prob.nn, prob.nnl, len(self.modes), prob.ntc
+        s.ndegw, s.nwz = psi.shape[2], prob.nwz
+        s.psd_id = prob.psd_id if psd_id is None else psd_id
+        s.pulsation = 1
+        s.ncorr_rows = prob.nod_corr.shape[0]
+        s.corr_col_mode = corr_col_mode
+        for name in ("n_load", "tc", "wz_node"):
+            setattr(s, name, _ptr(k[name], C.c_int32))
+        for name in ("psi", "nod_corr", "u_node", "turb_scales", "turb_std", "u_mean_ref",
+                     "Mm", "Km", "Cm"):
+            setattr(s, name, _ptr(k[name], C.c_double))
+        self.struct = s
+        self.nm = len(self.modes)
+
+
+def init(device: int = 0):
+    _check(lib().bsahip_init(C.c_int(device)))
+
+
+def classic_run(prob, quad: str = "vect", prec: str = "fp64", bound: Bound = None,
+                psd: bool = True, bisp: bool = True) -> dict:
+    """Classic full-grid moments (bsahip_classic_run)."""
+    b = bound or Bound(prob)
+    nm = b.nm
+    out = {k: np.zeros(nm * nm) for k in ("m2mf", "m2mr", "m2o2mr")}
+    out.update({k: np.zeros(nm ** 3) for k in ("m3mf", "m3mr")})
+    flags = PREC[prec] | QUAD[quad] | (0 if psd else NO_PSD) | (0 if bisp else NO_BISP)
+    D = C.POINTER(C.c_double)
+    _check(lib().bsahip_classic_run(C.byref(b.struct), C.c_int32(prob.nfreqs), C.c_double(prob.df),
+                                    C.c_int32(flags), _ptr(out["m2mf"], C.c_double),
+                                    _ptr(out["m2mr"], C.c_double), _ptr(out["m2o2mr"], C.c_double),
+                                    _ptr(out["m3mf"], C.c_double), _ptr(out["m3mr"], C.c_double)))
+    out["modes"] = b.modes
+    return out
+
+
+def classic_spectra(prob, exact: bool = False, bound: Bound = None) -> dict:
+    """Modal PSDs (NF, NM^2) and bispectra (NF, NF, NM^3) on the Classic grid."""
+    b = bound or Bound(prob)
+    nm, nf = b.nm, prob.nf
+    psd_f = np.zeros((nf, nm * nm), order="F")
+    psd_r = np.zeros((nf, nm * nm), order="F")
+    bfm = np.zeros((nf, nf, nm ** 3), order="F")
+    brm = np.zeros((nf, nf, nm ** 3), order="F")
+    flags = SUM_EXACT if exact else SUM_TRUNCATED
+    _check(lib().bsahip_classic_spectra(C.byref(b.struct), C.c_int32(prob.nfreqs),
+                                        C.c_double(prob.df), C.c_int32(flags),
+                                        _ptr(psd_f, C.c_double), _ptr(psd_r, C.c_double),
+                                        _ptr(bfm, C.c_double), _ptr(brm, C.c_double)))
+    return {"psd_f": psd_f, "psd_r": psd_r, "bisp_f": bfm, "bisp_r": brm}
+
+
+def bfm_points(prob, fi, fj, with_brm: bool = True, bound: Bound = None):
+    """Modal bispectra at arbitrary points, S(f1+f2) exact (bsahip_bfm_points)."""
+    b = bound or Bound(prob)
+    fi = np.ascontiguousarray(fi, dtype=np.float64)
+    fj = np.ascontiguousarray(fj, dtype=np.float64)
+    n = fi.size
+    bfm = np.zeros((b.nm ** 3, n), order="F")
+    brm = np.zeros((b.nm ** 3, n), order="F") if with_brm else None
+    _check(lib().bsahip_bfm_points(C.byref(b.struct), C.c_int32(n), _ptr(fi, C.c_double),
+                                   _ptr(fj, C.c_double), C.c_int32(0), _ptr(bfm, C.c_double),
+                                   _ptr(brm, C.c_double)))
+    return bfm, brm
+
+
+def last_kernel_ms():
+    ms = C.c_double(0)
+    n = C.c_int32(0)
+    _check(lib().bsahip_last_kernel_time(C.byref(ms), C.byref(n)))
+    return ms.value, n.value

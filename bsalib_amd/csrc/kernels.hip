@@ -1,0 +1,522 @@
+// Device kernels of the BFM hot path for gfx950 (CDNA4).
+//
+// Notation (the reference's, src/bsa/functions/functionsImpl.F90:1312-1587):
+//   nodes i,j,k (NNL loaded), modes m,n,p (NM kept), components tc (NTC),
+//   Psi_a / Psi_b = phi_times_A_ndegw_(:,:,tc) / (:,:,tc+3),
+//   S_ij(f) = rho_ij^|f| sqrt(S_i(f) S_j(f)),   A(n,i;f) = sum_j Psi_a(n,j) S_ji(f).
+// The reference's modal force bispectrum is
+//   B(m,n,p)(f1,f2) = 2 sum_tc [ G[m](n,f1; p,f2) + G[n](m,f3; p,f2) + G[p](m,f3; n,f1) ],
+//   G[x](y,f; z,g)  = sum_i Psi_b(x,i) A(y,i;f) A(z,i;g),          f3 = f1 + f2,
+// i.e. its three node sums (functionsImpl.F90:1523-1528) are one tensor G read
+// at three argument permutations.  G is symmetric in (y,f)<->(z,g) and depends
+// on |f|, |g| only (PSD and coherence are even in f).  k_gsym_f64 computes each
+// distinct G value once on FP64 MFMA and its epilogue adds it, weighted by the
+// quadrature weight and the modal FRF product, to the up to 12 (m,n,p,f1,f2)
+// places it occupies in the plane.
+//
+// K index of every contraction: k = tc*NNLp + i (NNLp = NNL padded, zeros).
+// U table (A values): U[k*ldU + y*nhp + u], u = |f|/df on the half grid.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+#define KC 32          // K chunk staged in LDS by k_gsym_f64
+#define TILE 64        // G tile edge (rows (y,u) x cols (z,v))
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+__global__ void k_fill(double *__restrict__ p, size_t n, double v)
+{
+   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   const size_t st = (size_t)gridDim.x * blockDim.x;
+   for (; i < n; i += st) p[i] = v;
+}
+
+// Nodal one-sided PSD, then its square root, on a list of |f| values.
+// formula: 1 von Karman u (WindPSDImpl.F90:166-188), 2 von Karman v/w (:190-222),
+// 3 Davenport-Greisch (:249-277), 5 Davenport-Uliege (:283-316), 0 zero (Kaimal,
+// :231-243).  Pulsation convention divides by 4 pi (:143).
+// prm[(tc*nnl + i)*4 + {0,1,2,3}] = {L, sigma, U_node, U_mean_ref(zone)}.
+__global__ void k_psd_sqrt(const double *__restrict__ fabs_list, int nfl,
+                           const double *__restrict__ prm, const int *__restrict__ formula,
+                           int nnl, int ntc, int pulsation, double *__restrict__ sq)
+{
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   const int q = blockIdx.y;
+   const int tc = blockIdx.z;
+   if (i >= nnl || q >= nfl) return;
+   const double *pp = prm + ((size_t)tc * nnl + i) * 4;
+   const double L = pp[0], sg = pp[1], U = pp[2], Um = pp[3];
+   const double f = fabs_list[q];
+   double S = 0.0;
+   switch (formula[tc]) {
+   case 1: {
+      const double LU = L / U;
+      const double x = f * LU;
+      S = 4.0 * (LU * sg * sg) / pow(1.0 + 70.7 * x * x, 5.0 / 6.0);
+   } break;
+   case 2: {
+      double d = 2.0 * f * (L / U);
+      d = d * d;
+      const double r2 = pow(1.0 + 70.7 * d, 11.0 / 6.0) * U;
+      S = 4.0 * (L * (1.0 + 188.4 * d) / r2 * (sg * sg));
+   } break;
+   case 3: {
+      const double c = 0.65 * 1200.0;
+      const double x = c * f / Um;
+      S = (c * sg * sg / Um) / pow(1.0 + x * x, 5.0 / 6.0);
+   } break;
+   case 5: {
+      const double cLU = L / U;
+      const double cF = f * cLU;
+      S = ((2.0 / 3.0) * cF * cLU * (sg * sg)) / pow(1.0 + cF * cF, 4.0 / 3.0);
+   } break;
+   default: S = 0.0;
+   }
+   if (pulsation) S = S / (4.0 * M_PI);
+   sq[((size_t)tc * nfl + q) * nnl + i] = sqrt(S);
+}
+
+// Modal FRF table H[m*nft + q] = 1/(K - w^2 M + i w C), w = 2 pi q df, q >= 0
+// (functionsImpl.F90:1641-1651).  Negative frequencies use conj.
+__global__ void k_frf_table(const double *__restrict__ Mm, const double *__restrict__ Km,
+                            const double *__restrict__ Cm, int nm, int nft, double df,
+                            double2 *__restrict__ H)
+{
+   const int q = blockIdx.x * blockDim.x + threadIdx.x;
+   const int m = blockIdx.y;
+   if (q >= nft || m >= nm) return;
+   const double w = 2.0 * M_PI * ((double)q * df);
+   const double r = -(w * w * Mm[m]) + Km[m];
+   const double im = w * Cm[m];
+   const double d = r * r + im * im;
+   H[(size_t)m * nft + q] = make_double2(r / d, -im / d);
+}
+
+// ---------------------------------------------------------------------------
+// projection A(y,i;f) = s_i(f) sum_j Psi_a(y,j) s_j(f) exp(|f| ln rho_ji)
+// grid (ceil(nnl/64), nfl, ntc), block 256 = 64 nodes i x 4 mode groups
+// ---------------------------------------------------------------------------
+template <int NQ>
+__global__ __launch_bounds__(256) void k_project(
+   const double *__restrict__ pa,      // [tc][nm][nnl]
+   const double *__restrict__ lnr,     // [tc][nnl][nnl]  ln rho_ji (row j, col i)
+   const double *__restrict__ sq,      // [tc][nfl][nnl]  sqrt(S)
+   const double *__restrict__ fabs_list, int nfl, int nnl, int nm, int nnlp,
+   int row_stride,                     // rows per mode in U (nhp)
+   double *__restrict__ U, long ldU)
+{
+   __shared__ double ps[64][NQ * 4 + 1];
+   __shared__ double ee[64][65];
+   const int tc = blockIdx.z;
+   const int q = blockIdx.y;
+   const int i0 = blockIdx.x * 64;
+   const int li = threadIdx.x & 63;
+   const int yg = threadIdx.x >> 6;
+   const double af = fabs_list[q];
+   const double *sqq = sq + ((size_t)tc * nfl + q) * nnl;
+   const double *lnt = lnr + (size_t)tc * nnl * nnl;
+   const double *pat = pa + (size_t)tc * nm * nnl;
+   double acc[NQ];
+#pragma unroll
+   for (int t = 0; t < NQ; ++t) acc[t] = 0.0;
+
+   for (int j0 = 0; j0 < nnl; j0 += 64) {
+      for (int t = threadIdx.x; t < 64 * NQ * 4; t += 256) {
+         const int jj = t / (NQ * 4), y = t % (NQ * 4);
+         const int j = j0 + jj;
+         ps[jj][y] = (j < nnl && y < nm) ? pat[(size_t)y * nnl + j] * sqq[j] : 0.0;
+      }
+      for (int t = threadIdx.x; t < 64 * 64; t += 256) {
+         const int jj = t >> 6, ii = t & 63;
+         const int j = j0 + jj, i = i0 + ii;
+         double e = 0.0;
+         if (j < nnl && i < nnl) {
+            const double l = lnt[(size_t)j * nnl + i];
+            e = (af == 0.0) ? 1.0 : exp(af * l);     // rho^0 == 1 even for rho == 0
+         }
+         ee[jj][ii] = e;
+      }
+      __syncthreads();
+      const int jn = min(64, nnl - j0);
+      for (int jj = 0; jj < jn; ++jj) {
+         const double e = ee[jj][li];
+#pragma unroll
+         for (int t = 0; t < NQ; ++t) acc[t] = fma(ps[jj][yg + 4 * t], e, acc[t]);
+      }
+      __syncthreads();
+   }
+   const int i = i0 + li;
+   if (i < nnl) {
+      const double si = sqq[i];
+#pragma unroll
+      for (int t = 0; t < NQ; ++t) {
+         const int y = yg + 4 * t;
+         if (y < nm) U[((size_t)tc * nnlp + i) * ldU + (size_t)y * row_stride + q] = acc[t] * si;
+      }
+   }
+}
+
+// ---------------------------------------------------------------------------
+// quadrature weight and FRF helpers (FP64)
+// ---------------------------------------------------------------------------
+// rule 0: intgSpectraVect_ (BsaClassicImpl.F90:604-631): interior 1, edge 1/2,
+// corner 1 - 3/2;  rule 1: plain Riemann (:331,341);  rule 2: tensor trapezoid.
+__device__ __forceinline__ double qweight(int F1, int F2, int nh1, int rule, double d2)
+{
+   const int a1 = abs(F1), a2 = abs(F2);
+   if (a1 > nh1 || a2 > nh1) return 0.0;
+   const bool e1 = (a1 == nh1), e2 = (a2 == nh1);
+   if (rule == 1) return d2;
+   if (rule == 0) return (e1 && e2) ? (1.0 - 1.5) * d2 : ((e1 || e2) ? 0.5 * d2 : d2);
+   return d2 * (e1 ? 0.5 : 1.0) * (e2 ? 0.5 : 1.0);
+}
+
+__device__ __forceinline__ double2 hget(const double2 *__restrict__ H, int nft, int m, int F)
+{
+   double2 h = H[(size_t)m * nft + abs(F)];
+   if (F < 0) h.y = -h.y;
+   return h;
+}
+
+// Re[a b conj(c)]  (functionsImpl.F90:1706-1710 / bsacl.cl:892-897)
+__device__ __forceinline__ double re3(double2 a, double2 b, double2 c)
+{
+   const double pr = a.x * b.x - a.y * b.y;
+   const double pi = a.x * b.y + a.y * b.x;
+   return pr * c.x + pi * c.y;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+   return v;
+}
+
+// ---------------------------------------------------------------------------
+// G contraction on FP64 MFMA + plane epilogue
+// grid: (n_tile_pairs * nm) workgroups, 256 threads (4 waves, 2x2 of 32x32)
+// ---------------------------------------------------------------------------
+struct GsymArgs {
+   const double *U;       // [K][ldU]
+   long ldU;
+   const double *pb;      // [nm][K]  Psi_b per K index
+   int K;                 // multiple of KC
+   int nm, nhp, nh;       // modes, rows per mode, valid half-grid points
+   int ntile;             // tiles per side
+   long npairs;           // ntile*(ntile+1)/2
+   const double2 *H;      // [nm][nft]
+   int nft;               // >= 2*nh - 1
+   double d2;             // (2 pi df)^2 or df^2
+   int rule;
+   double *m3f, *m3r;     // [nm^3] accumulated (atomic)
+};
+
+__device__ __forceinline__ void pair_from_index(long t, int &R, int &Q)
+{
+   long q = (long)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+   while ((q + 1) * (q + 2) / 2 <= t) ++q;
+   while (q * (q + 1) / 2 > t) --q;
+   Q = (int)q;
+   R = (int)(t - q * (q + 1) / 2);
+}
+
+__global__ __launch_bounds__(256) void k_gsym_f64(GsymArgs a)
+{
+   __shared__ double As[KC][TILE];
+   __shared__ double Bs[KC][TILE];
+
+   // XCD-aware order: workgroups dispatched to one XCD take consecutive
+   // logical ids, so the nm modes x that share one (R,Q) tile pair run on the
+   // same L2.
+   const long nwg = (long)gridDim.x;
+   const long wid = blockIdx.x;
+   long lid = wid;
+   const long per = nwg / 8;
+   if (wid < per * 8) lid = (wid % 8) * per + wid / 8;
+   const int x = (int)(lid % a.nm);
+   const long tp = lid / a.nm;
+   int R, Q;
+   pair_from_index(tp, R, Q);
+   const bool diag = (R == Q);
+
+   const int tid = threadIdx.x;
+   const int lane = tid & 63;
+   const int w = tid >> 6;
+   const int wr = w >> 1, wc = w & 1;
+
+   // staging: each thread moves 8 consecutive doubles of one k row per tile
+   const int sk = tid >> 3;            // 0..31
+   const int so = (tid & 7) * 8;       // 0..56
+   const double *gA = a.U + (size_t)R * TILE + so;
+   const double *gB = a.U + (size_t)Q * TILE + so;
+   const double *pbx = a.pb + (size_t)x * a.K;
+
+   d4 acc[2][2];
+#pragma unroll
+   for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[i][j] = (d4){0.0, 0.0, 0.0, 0.0};
+
+   double ra[8], rb[8];
+   double sc;
+   {
+      const double *pA = gA + (size_t)sk * a.ldU;
+      const double *pB = gB + (size_t)sk * a.ldU;
+#pragma unroll
+      for (int t = 0; t < 8; ++t) { ra[t] = pA[t]; rb[t] = pB[t]; }
+      sc = pbx[sk];
+   }
+   for (int k0 = 0; k0 < a.K; k0 += KC) {
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < 8; ++t) { As[sk][so + t] = ra[t] * sc; Bs[sk][so + t] = rb[t]; }
+      __syncthreads();
+      if (k0 + KC < a.K) {
+         const double *pA = gA + (size_t)(k0 + KC + sk) * a.ldU;
+         const double *pB = gB + (size_t)(k0 + KC + sk) * a.ldU;
+#pragma unroll
+         for (int t = 0; t < 8; ++t) { ra[t] = pA[t]; rb[t] = pB[t]; }
+         sc = pbx[k0 + KC + sk];
+      }
+#pragma unroll
+      for (int kk = 0; kk < KC; kk += 4) {
+         const int kr = kk + (lane >> 4);
+         const double a0 = As[kr][wr * 32 + (lane & 15)];
+         const double a1 = As[kr][wr * 32 + 16 + (lane & 15)];
+         const double b0 = Bs[kr][wc * 32 + (lane & 15)];
+         const double b1 = Bs[kr][wc * 32 + 16 + (lane & 15)];
+         acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+         acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+         acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+         acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+      }
+   }
+
+   // ---- epilogue: for v_mfma_f64_16x16x4f64, register t of lane l holds
+   // D[(l/16) + 4t][l%16] of its 16x16 block (rows interleaved by 4).
+   const int nh1 = a.nh - 1;
+   const int nm = a.nm;
+   const size_t nm2 = (size_t)nm * nm;
+#pragma unroll
+   for (int br = 0; br < 2; ++br) {
+#pragma unroll
+      for (int bc = 0; bc < 2; ++bc) {
+         const int r0 = R * TILE + wr * 32 + br * 16;
+         const int q0 = Q * TILE + wc * 32 + bc * 16;
+         const int y = r0 / a.nhp, z = q0 / a.nhp;
+         if (y >= nm || z >= nm) continue;            // uniform per block
+         const int v = q0 - z * a.nhp + (lane & 15);
+         double s[12];
+#pragma unroll
+         for (int t = 0; t < 12; ++t) s[t] = 0.0;
+         if (v < a.nh) {
+            const double2 hxv = hget(a.H, a.nft, x, v);
+            const double2 hyv = hget(a.H, a.nft, y, v);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+               const int u = r0 - y * a.nhp + (lane >> 4) + 4 * t;
+               const double g = 2.0 * acc[br][bc][t];   // B = 2 * sum of the three G terms
+               if (u >= a.nh) continue;
+               const double ms = (u == 0 && v == 0) ? 1.0 : 2.0;
+               const double mo = (u > 0 && v > 0) ? 2.0 : 0.0;
+               const double2 hxu = hget(a.H, a.nft, x, u);
+               const double2 hzu = hget(a.H, a.nft, z, u);
+               const double2 hyp = hget(a.H, a.nft, y, u + v);
+               const double2 hym = hget(a.H, a.nft, y, u - v);
+               const double2 hzp = hget(a.H, a.nft, z, u + v);
+               const double2 hzm = hget(a.H, a.nft, z, u - v);
+               const double2 cyv = make_double2(hyv.x, -hyv.y);
+               const double2 cxv = make_double2(hxv.x, -hxv.y);
+               const double2 cxu = make_double2(hxu.x, -hxu.y);
+               const double2 czu = make_double2(hzu.x, -hzu.y);
+               const int dm = u - v;
+               const double Wa = qweight(u, v, nh1, a.rule, a.d2);
+               const double Wb = qweight(dm, v, nh1, a.rule, a.d2);
+               const double Wd = qweight(u + v, v, nh1, a.rule, a.d2);
+               // (m,n,p) = (x,y,z): T1 at (u,v) and (u,-v)
+               s[0] += g * Wa * (ms + mo);
+               s[1] += g * Wa * (ms * re3(hxu, hyv, hzp) + mo * re3(hxu, cyv, hzm));
+               // (y,x,z): T2 at (u-v, v) and (u+v, -v)
+               s[2] += g * (ms * Wb + mo * Wd);
+               s[3] += g * (ms * Wb * re3(hym, hxv, hzu) + mo * Wd * re3(hyp, cxv, hzu));
+               // (y,z,x): T3 at (v, u-v) and (-v, u+v)
+               s[4] += g * (ms * Wb + mo * Wd);
+               s[5] += g * (ms * Wb * re3(hyv, hzm, hxu) + mo * Wd * re3(cyv, hzp, hxu));
+               if (!diag) {
+                  const double We = qweight(dm, u, nh1, a.rule, a.d2);
+                  const double Wf = qweight(u + v, u, nh1, a.rule, a.d2);
+                  const double2 hzmn = make_double2(hzm.x, -hzm.y);   // H_z(v-u)
+                  const double2 hymn = make_double2(hym.x, -hym.y);   // H_y(v-u)
+                  // (x,z,y): T1 at (v,u) and (v,-u)
+                  s[6] += g * Wa * (ms + mo);
+                  s[7] += g * Wa * (ms * re3(hxv, hzu, hyp) + mo * re3(hxv, czu, hymn));
+                  // (z,x,y): T2 at (v-u, u) and (v+u, -u)
+                  s[8] += g * (ms * We + mo * Wf);
+                  s[9] += g * (ms * We * re3(hzmn, hxu, hyv) + mo * Wf * re3(hzp, cxu, hyv));
+                  // (z,y,x): T3 at (u, v-u) and (-u, u+v)
+                  s[10] += g * (ms * We + mo * Wf);
+                  s[11] += g * (ms * We * re3(hzu, hymn, hxv) + mo * Wf * re3(czu, hyp, hxv));
+               }
+            }
+         }
+#pragma unroll
+         for (int t = 0; t < 12; ++t) s[t] = wave_sum(s[t]);
+         if (lane == 0) {
+            const size_t i_xyz = x + (size_t)nm * y + nm2 * z;
+            const size_t i_yxz = y + (size_t)nm * x + nm2 * z;
+            const size_t i_yzx = y + (size_t)nm * z + nm2 * x;
+            atomicAdd(a.m3f + i_xyz, s[0]);
+            atomicAdd(a.m3r + i_xyz, s[1]);
+            atomicAdd(a.m3f + i_yxz, s[2]);
+            atomicAdd(a.m3r + i_yxz, s[3]);
+            atomicAdd(a.m3f + i_yzx, s[4]);
+            atomicAdd(a.m3r + i_yzx, s[5]);
+            if (!diag) {
+               const size_t i_xzy = x + (size_t)nm * z + nm2 * y;
+               const size_t i_zxy = z + (size_t)nm * x + nm2 * y;
+               const size_t i_zyx = z + (size_t)nm * y + nm2 * x;
+               atomicAdd(a.m3f + i_xzy, s[6]);
+               atomicAdd(a.m3r + i_xzy, s[7]);
+               atomicAdd(a.m3f + i_zxy, s[8]);
+               atomicAdd(a.m3r + i_zxy, s[9]);
+               atomicAdd(a.m3f + i_zyx, s[10]);
+               atomicAdd(a.m3r + i_zyx, s[11]);
+            }
+         }
+      }
+   }
+}
+
+// ---------------------------------------------------------------------------
+// modal PSD and second moments (functionsImpl.F90:1546-1558, 1686-1689;
+// BsaClassicImpl.F90:197-210, 567-589).  One block per (n,p); deterministic.
+// psd(n,p;u) = sum_k U[k][n*nhp+u] Psi_a(p,k); the grid holds +-u.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_m2(const double *__restrict__ U, long ldU,
+                                            const double *__restrict__ pa, int K, int nm,
+                                            int nhp, int nh, const double2 *__restrict__ H,
+                                            int nft, double d, double om_step,
+                                            double *__restrict__ m2f, double *__restrict__ m2r,
+                                            double *__restrict__ m2o)
+{
+   __shared__ double red[3][256];
+   const int n = blockIdx.x % nm, p = blockIdx.x / nm;
+   double sf = 0.0, sr = 0.0, so = 0.0;
+   for (int u = threadIdx.x; u < nh; u += blockDim.x) {
+      double s = 0.0;
+      const double *col = U + (size_t)n * nhp + u;
+      const double *pp = pa + (size_t)p * K;
+      for (int k = 0; k < K; ++k) s = fma(col[(size_t)k * ldU], pp[k], s);
+      double w = (u == nh - 1) ? 0.5 * d : d;
+      if (u > 0) w *= 2.0;                                 // +u and -u
+      const double2 hp = H[(size_t)p * nft + u], hn = H[(size_t)n * nft + u];
+      const double fr = hp.x * hn.x + hp.y * hn.y;          // Re[H_p conj(H_n)]
+      const double om = om_step * u;
+      sf += w * s;
+      sr += w * s * fr;
+      so += w * s * fr * om * om;
+   }
+   red[0][threadIdx.x] = sf;
+   red[1][threadIdx.x] = sr;
+   red[2][threadIdx.x] = so;
+   __syncthreads();
+   for (int o = 128; o > 0; o >>= 1) {
+      if (threadIdx.x < o)
+         for (int c = 0; c < 3; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + o];
+      __syncthreads();
+   }
+   if (threadIdx.x == 0) {
+      const int id = n + nm * p;
+      if (m2f) m2f[id] = red[0][0];
+      if (m2r) m2r[id] = red[1][0];
+      if (m2o) m2o[id] = red[2][0];
+   }
+}
+
+// Modal PSD on grid indices (for the spectra output): psd(q, n+p*nm).
+__global__ void k_psd_grid(const double *__restrict__ U, long ldU, const double *__restrict__ pa,
+                           int K, int nm, int nhp, int nf, int c, const double2 *__restrict__ H,
+                           int nft, double *__restrict__ psd_f, double *__restrict__ psd_r)
+{
+   const int q = blockIdx.x * blockDim.x + threadIdx.x;
+   const int np = blockIdx.y;
+   if (q >= nf) return;
+   const int n = np % nm, p = np / nm;
+   const int u = abs(q - c);
+   double s = 0.0;
+   for (int k = 0; k < K; ++k) s = fma(U[(size_t)k * ldU + (size_t)n * nhp + u], pa[(size_t)p * K + k], s);
+   const double2 hp = H[(size_t)p * nft + u], hn = H[(size_t)n * nft + u];
+   if (psd_f) psd_f[(size_t)np * nf + q] = s;
+   if (psd_r) psd_r[(size_t)np * nf + q] = s * (hp.x * hn.x + hp.y * hn.y);
+}
+
+// ---------------------------------------------------------------------------
+// Per-point bispectrum: B(m,n,p) at points given by row indices into U
+// (i1 -> f1, i2 -> f2, i3 -> f1+f2, i3 < 0 means S(f1+f2) = 0) and the
+// signed frequencies for the FRF.  out index = pt*spt + mnp*smn.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bfm_points(
+   const double *__restrict__ U, long ldU, int rows, const double *__restrict__ pb, int K,
+   int nm, int npts, const int *__restrict__ i1, const int *__restrict__ i2,
+   const int *__restrict__ i3, const double *__restrict__ f1s, const double *__restrict__ f2s,
+   const double *__restrict__ Mm, const double *__restrict__ Km, const double *__restrict__ Cm,
+   double *__restrict__ bfm, double *__restrict__ brm, long spt, long smn)
+{
+   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+   const long nm3 = (long)nm * nm * nm;
+   if (gid >= nm3 * npts) return;
+   const long pt = gid / nm3;
+   const int mnp = (int)(gid % nm3);
+   const int m = mnp % nm, n = (mnp / nm) % nm, p = mnp / (nm * nm);
+   const int a1 = i1[pt], a2 = i2[pt], a3 = i3[pt];
+   double s = 0.0;
+   for (int k = 0; k < K; ++k) {
+      const double *row = U + (size_t)k * ldU;
+      const double An1 = row[(size_t)n * rows + a1], Ap2 = row[(size_t)p * rows + a2];
+      double v = pb[(size_t)m * K + k] * An1 * Ap2;
+      if (a3 >= 0) {
+         const double Am3 = row[(size_t)m * rows + a3];
+         v += pb[(size_t)n * K + k] * Am3 * Ap2 + pb[(size_t)p * K + k] * Am3 * An1;
+      }
+      s += v;
+   }
+   s *= 2.0;
+   if (bfm) bfm[pt * spt + mnp * smn] = s;
+   if (brm) {
+      const double f1 = f1s[pt], f2 = f2s[pt];
+      auto H = [&](int mo, double f) {
+         const double w = 2.0 * M_PI * f;
+         const double r = -(w * w * Mm[mo]) + Km[mo];
+         const double im = w * Cm[mo];
+         const double d = r * r + im * im;
+         return make_double2(r / d, -im / d);
+      };
+      const double w1 = 2.0 * M_PI * f1, w2 = 2.0 * M_PI * f2;
+      // H at f1+f2 uses w1 + w2 as the reference does (functionsImpl.F90:1659-1663)
+      const double w3 = w1 + w2;
+      const double r3 = -(w3 * w3 * Mm[p]) + Km[p], i3v = w3 * Cm[p], d3 = r3 * r3 + i3v * i3v;
+      const double2 h3 = make_double2(r3 / d3, -i3v / d3);
+      brm[pt * spt + mnp * smn] = s * re3(H(m, f1), H(n, f2), h3);
+   }
+}
+
+// Lane-layout self test of v_mfma_f64_16x16x4f64 (one wave, 512 doubles out):
+// out[t*64 + l] = row index seen by register t of lane l, out[256 + t*64 + l]
+// = its column index (A = row index in column 0, B = column index in row 0).
+__global__ void k_mfma_f64_layout(double *__restrict__ out)
+{
+   const int l = threadIdx.x;
+   const bool k0 = (l >> 4) == 0;
+   d4 z = (d4){0.0, 0.0, 0.0, 0.0};
+   d4 rows = __builtin_amdgcn_mfma_f64_16x16x4f64(k0 ? (double)(l & 15) : 0.0, k0 ? 1.0 : 0.0, z, 0, 0, 0);
+   d4 cols = __builtin_amdgcn_mfma_f64_16x16x4f64(k0 ? 1.0 : 0.0, k0 ? (double)(l & 15) : 0.0, z, 0, 0, 0);
+#pragma unroll
+   for (int t = 0; t < 4; ++t) {
+      out[t * 64 + l] = rows[t];
+      out[256 + t * 64 + l] = cols[t];
+   }
+}

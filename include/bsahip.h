@@ -1,0 +1,140 @@
+/*
+ * bsahip.h -- C ABI of the MI355X (gfx950) BFM hot path of BsaLib.
+ *
+ * One shared library, libbsahip.so, exports these entry points (plain
+ * pointers, sizes and integer error codes; callable from C, C++, Fortran
+ * iso_c_binding and Python ctypes) next to the BsaCL symbol set of bsacl.h.
+ *
+ * Array layouts are the reference's own (Fortran column-major, 1-based ids):
+ *   psi          phi_times_A_ndegw_(NM, NNL, NDEGW)   src/bsa/BsaLibImpl.F90:611-671
+ *   nod_corr     ((NN*NN+NN)/2, ncorr_cols)           src/utils/utility.F90:60-79
+ *   turb_scales  L(3, 3, NWZ), turb_std sigma(3, NWZ) src/wind/WindType.F90:69-209
+ *   m2*          (NM*NM)  index n + (p-1)*NM
+ *   m3*          (NM^3)   index m + (n-1)*NM + (p-1)*NM^2
+ *   bisp         (NF, NF, NM^3)                        functionsImpl.F90:1312-1587
+ *
+ * Every function returns 0 on success and a negative BSAHIP_E* code on
+ * failure; bsahip_last_error() returns the message of the last failure on the
+ * calling thread.  A missing or unusable GPU is an error, never a fallback.
+ */
+#ifndef BSAHIP_H_
+#define BSAHIP_H_
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define BSAHIP_OK              0
+#define BSAHIP_EINVAL        -1   /* bad size / pointer / flag combination  */
+#define BSAHIP_EDEVICE       -2   /* HIP runtime failure                     */
+#define BSAHIP_ENOMEM        -3   /* device allocation failed                */
+#define BSAHIP_EUNSUPPORTED  -4   /* valid request this build does not do    */
+#define BSAHIP_ESTATE        -5   /* call order violated                     */
+
+/* precision of the node contraction (flags bits 0-3) */
+#define BSAHIP_PREC_FP64       0  /* FP64 MFMA, the plain form                */
+#define BSAHIP_PREC_FP32       1  /* FP32 operands, FP32 MFMA accumulation    */
+#define BSAHIP_PREC_BF16X3     2  /* split-bf16 x3 products, FP32 accumulate  */
+#define BSAHIP_PREC_MASK     0xF
+
+/* quadrature over the (f1,f2) plane (flags bits 4-7) */
+#define BSAHIP_QUAD_VECT     (0 << 4) /* intgSpectraVect_: edges 1/2, corners -1/2
+                                         (src/bsa/classic/BsaClassicImpl.F90:567-633) */
+#define BSAHIP_QUAD_RIEMANN  (1 << 4) /* every point dw^2 (BsaClassicImpl.F90:331,341) */
+#define BSAHIP_QUAD_TRAPZ    (2 << 4) /* tensor trapezoid (_shared_poly2d.fi:827-832)   */
+#define BSAHIP_QUAD_MASK     (0xF << 4)
+
+/* S(f1+f2) beyond the grid (flag bit 8) */
+#define BSAHIP_SUM_TRUNCATED (0 << 8) /* zero beyond +-fmax: Classic
+                                         (functionsImpl.F90:1409-1413, 1474-1477) */
+#define BSAHIP_SUM_EXACT     (1 << 8) /* evaluated: Mesher / BsaCL semantics       */
+
+/* what to compute (flag bits 9-10) */
+#define BSAHIP_NO_PSD        (1 << 9)  /* skip m2 (modal PSD) outputs   */
+#define BSAHIP_NO_BISP       (1 << 10) /* skip m3 (bispectrum) outputs  */
+
+/* Problem description: borrowed host pointers, read during the call only. */
+typedef struct bsahip_problem {
+   int32_t nn;            /* total nodes (packed coherence index base)       */
+   int32_t nnl;           /* loaded nodes                                    */
+   int32_t nm;            /* effective (kept) modes                          */
+   int32_t ntc;           /* turbulence components                           */
+   int32_t ndegw;         /* third extent of psi (6)                         */
+   int32_t nwz;           /* wind zones                                      */
+   int32_t psd_id;        /* 1 von Karman, 2 Kaimal(=0), 3 Davenport-Greisch, 5 Davenport-Uliege */
+   int32_t pulsation;     /* 1: PSD / 4 pi and d = 2 pi df (default convention) */
+   int32_t ncorr_rows;    /* rows of nod_corr ((nn*nn+nn)/2)                  */
+   int32_t corr_col_mode; /* 0: column tc per component, 1: column 1 for all   */
+   const int32_t *n_load;     /* [nnl] 1-based node ids          */
+   const int32_t *tc;         /* [ntc] 1-based components        */
+   const double  *psi;        /* (nm, nnl, ndegw)                */
+   const double  *nod_corr;   /* (ncorr_rows, >= max tc)         */
+   const double  *u_node;     /* [nn]                            */
+   const int32_t *wz_node;    /* [nn] 1-based                    */
+   const double  *turb_scales;/* (3, 3, nwz)                     */
+   const double  *turb_std;   /* (3, nwz)                        */
+   const double  *u_mean_ref; /* [nwz]                           */
+   const double  *Mm;         /* [nm] modal mass, kept-mode order      */
+   const double  *Km;         /* [nm] modal stiffness                  */
+   const double  *Cm;         /* [nm] modal damping (diagonal of Cm)   */
+} bsahip_problem;
+
+/* Library / device. */
+int         bsahip_init(int device);
+int         bsahip_finalise(void);
+const char *bsahip_last_error(void);
+int         bsahip_version(void);
+
+/*
+ * Classic full-grid run: grid f_k = k*df, k = -(nfh-1)..(nfh-1) (NF = 2nfh-1),
+ * the same one computeFreqsVect_ builds (BsaClassicImpl.F90:443-560).
+ * Outputs (host pointers, any may be NULL when not wanted):
+ *   m2mf, m2mr, m2o2mr  (nm*nm)     -- as mainClassic_ (BsaClassicImpl.F90:137-212)
+ *   m3mf, m3mr          (nm^3)
+ * Replaces getBFM_vect_cls/getBRM_vect_cls + intgSpectraVect_ inside
+ * mainClassic_ (BsaClassicImpl.F90:147-210; slots in src/bsa/data/BsaLibData.F90:166-204).
+ */
+int bsahip_classic_run(const bsahip_problem *p, int32_t nfh, double df, int32_t flags,
+                       double *m2mf, double *m2mr, double *m2o2mr,
+                       double *m3mf, double *m3mr);
+
+/*
+ * Bispectra on the Classic grid in the reference layout bisp(NF, NF, NM^3):
+ * bfm = getFM_full_tnm_vect_cls_ output, brm = after getRM_full_vect_cls_
+ * (functionsImpl.F90:1312-1738).  psd_f/psd_r: (NF, NM^2) modal PSDs.
+ * Any output may be NULL.  Meant for parity checks and visual export; the
+ * moments path (bsahip_classic_run) never materialises them.
+ */
+int bsahip_classic_spectra(const bsahip_problem *p, int32_t nfh, double df, int32_t flags,
+                           double *psd_f, double *psd_r, double *bfm, double *brm);
+
+/*
+ * Modal bispectra at an arbitrary list of points (fi[q], fj[q]), S(f1+f2)
+ * evaluated exactly: getBFM_msh / getBRM_msh (functionsImpl.F90:247-465,
+ * 1031-1115).  bfm, brm: (NM^3, npts); brm may be NULL.
+ */
+int bsahip_bfm_points(const bsahip_problem *p, int32_t npts, const double *fi,
+                      const double *fj, int32_t flags, double *bfm, double *brm);
+
+/*
+ * Health check of the matrix-core lane layout the kernels rely on
+ * (v_mfma_f64_16x16x4f64): out512[t*64+l] / out512[256+t*64+l] receive the row /
+ * column index that accumulator register t of lane l holds.  No reference
+ * counterpart.
+ */
+int bsahip_selftest_mfma_layout(double *out512);
+
+/*
+ * Last measured device time (milliseconds, HIP events on the library stream)
+ * of the dominant kernel of the previous call, and its launch count.
+ */
+int bsahip_last_kernel_time(double *ms, int32_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BSAHIP_H_ */

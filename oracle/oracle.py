@@ -421,9 +421,15 @@ def _write_ref_input(prob, path, mode: int, fi=None, fj=None):
             fh.write(np.asarray(fj, dtype=np.float64).tobytes())
 
 
-def reference_run(prob, mode: int = 1, fi=None, fj=None, workdir=None) -> dict:
+def reference_run(prob, mode: int = 1, fi=None, fj=None, workdir=None,
+                  malloc_perturb: int = 255) -> dict:
     """Run the reference's own operators through oracle/_ref/ref_harness.
-    mode bit 1: Classic vect BFM + BRM on the full grid; bit 2: Mesher BFM at (fi, fj)."""
+    mode bit 1: Classic vect BFM + BRM on the full grid; bit 2: Mesher BFM at (fi, fj).
+
+    The Classic BFM reads S(f1+f2) beyond +-fmax from padding it never
+    initialises (functionsImpl.F90:1409-1413); glibc's MALLOC_PERTURB_=255
+    makes fresh allocations zero-filled, which is the intended zero padding.
+    Any other value exposes which outputs depend on that padding."""
     import tempfile
     if not reference_available():
         raise FileNotFoundError(str(REF_HARNESS))
@@ -434,7 +440,8 @@ def reference_run(prob, mode: int = 1, fi=None, fj=None, workdir=None) -> dict:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         _write_ref_input(prob, fin, mode, fi, fj)
         subprocess.run([str(REF_HARNESS), fin, fout], check=True, stdout=subprocess.DEVNULL,
-                       env=dict(os.environ, OMP_NUM_THREADS=os.environ.get("OMP_NUM_THREADS", "8")))
+                       env=dict(os.environ, OMP_NUM_THREADS=os.environ.get("OMP_NUM_THREADS", "8"),
+                                MALLOC_PERTURB_=str(malloc_perturb)))
         raw = np.fromfile(fout, dtype=np.float64)
     out = {}
     o = 0
