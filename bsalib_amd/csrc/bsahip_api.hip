@@ -7,6 +7,7 @@
 // point returns BSAHIP_EDEVICE.
 #include "kernels.hip"
 #include "../../include/bsahip.h"
+#include "bsahip_internal.h"
 
 #include <algorithm>
 #include <cmath>
@@ -342,27 +343,35 @@ int bsahip_last_kernel_time(double *ms, int32_t *launches)
 int bsahip_classic_run(const bsahip_problem *p, int32_t nfh, double df, int32_t flags,
                        double *m2mf, double *m2mr, double *m2o2mr, double *m3mf, double *m3mr)
 {
+   return bsahip__classic_moments(p, nfh, df, flags, 0, m2mf, m2mr, m2o2mr, m3mf, m3mr);
+}
+
+int bsahip__classic_moments(const bsahip_problem *p, int32_t nfh, double df, int32_t flags,
+                            int32_t psd_mode, double *m2mf, double *m2mr, double *m2o2mr,
+                            double *m3mf, double *m3mr)
+{
    std::lock_guard<std::recursive_mutex> lk(g.mu);
    CHK(ensure_init());
    CHK(check_flags(flags, true));
    if (nfh < 2 || !(df > 0.0)) return fail(BSAHIP_EINVAL, "need nfh >= 2 and df > 0");
-   if (flags & BSAHIP_SUM_EXACT)
-      return fail(BSAHIP_EUNSUPPORTED, "bsahip_classic_run: exact S(f1+f2) on the grid is served by bsahip_classic_spectra");
    Prep P;
-   CHK(prepare(p, P));
+   CHK(prepare(p, P, psd_mode));
    DevProb D;
    CHK(upload_prep(P, D));
+   const bool exact = (flags & BSAHIP_SUM_EXACT) != 0;
    const int nh = nfh, nf = 2 * nh - 1;
-   const int nhp = round_up(nh, 16);
+   const int nu = exact ? nf : nh;                 // rows of A per mode
+   const int nhp = round_up(nu, 16);
    const int rows = P.nm * nhp;
    const int ntile = (rows + TILE - 1) / TILE;
    const long ldU = (long)ntile * TILE;
-   std::vector<double> fl(nh);
-   for (int u = 0; u < nh; ++u) fl[u] = (double)u * df;
+   const int nft = 2 * nu - 1;
+   std::vector<double> fl(nu);
+   for (int u = 0; u < nu; ++u) fl[u] = (double)u * df;
    double *U;
    CHK(build_table(P, D, fl, nhp, p->pulsation, &U, ldU));
    double2 *H;
-   CHK(frf_table(P, D, nf, df, &H));
+   CHK(frf_table(P, D, nft, df, &H));
    const double d = p->pulsation ? df * 2.0 * M_PI : df;
    const bool want_psd = !(flags & BSAHIP_NO_PSD);
    const bool want_bisp = !(flags & BSAHIP_NO_BISP);
@@ -381,10 +390,11 @@ int bsahip_classic_run(const bsahip_problem *p, int32_t nfh, double df, int32_t 
       a.nm = P.nm;
       a.nhp = nhp;
       a.nh = nh;
+      a.nu = nu;
       a.ntile = ntile;
       a.npairs = (long)ntile * (ntile + 1) / 2;
       a.H = H;
-      a.nft = nf;
+      a.nft = nft;
       a.d2 = d * d;
       a.rule = (flags & BSAHIP_QUAD_MASK) >> 4;
       a.m3f = m3;
@@ -402,7 +412,7 @@ int bsahip_classic_run(const bsahip_problem *p, int32_t nfh, double df, int32_t 
       double *m2;
       const size_t nm2 = (size_t)P.nm * P.nm;
       CHK(dbuf("m2", 3 * nm2, &m2));
-      k_m2<<<P.nm * P.nm, 256, 0, g.s>>>(U, ldU, D.pa_k, P.K, P.nm, nhp, nh, H, nf, d,
+      k_m2<<<P.nm * P.nm, 256, 0, g.s>>>(U, ldU, D.pa_k, P.K, P.nm, nhp, nh, H, nft, d,
                                          2.0 * M_PI * df, m2, m2 + nm2, m2 + 2 * nm2);
       HIPC(hipGetLastError());
       if (m2mf) HIPC(hipMemcpyAsync(m2mf, m2, nm2 * sizeof(double), hipMemcpyDeviceToHost, g.s));
@@ -416,6 +426,64 @@ int bsahip_classic_run(const bsahip_problem *p, int32_t nfh, double df, int32_t 
       g.last_ms = ms;
       g.last_launches = 1;
    }
+   return 0;
+}
+
+int bsahip__points_sum(const bsahip_problem *p, int32_t npts, const double *fi, const double *fj,
+                       const double *w, int32_t psd_mode, int32_t with_frf, double *m3)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   CHK(ensure_init());
+   if (npts < 0 || (npts > 0 && (!fi || !fj || !w || !m3)))
+      return fail(BSAHIP_EINVAL, "npts < 0 or NULL point/weight/output arrays");
+   Prep P;
+   CHK(prepare(p, P, psd_mode));
+   DevProb D;
+   CHK(upload_prep(P, D));
+   const size_t nm3 = (size_t)P.nm * P.nm * P.nm;
+   if (npts == 0) {
+      std::fill(m3, m3 + nm3, 0.0);
+      return 0;
+   }
+   std::vector<double> all;
+   all.reserve(3 * (size_t)npts);
+   for (int q = 0; q < npts; ++q) {
+      all.push_back(std::fabs(fi[q]));
+      all.push_back(std::fabs(fj[q]));
+      all.push_back(std::fabs(fi[q] + fj[q]));
+   }
+   std::vector<double> uq = all;
+   std::sort(uq.begin(), uq.end());
+   uq.erase(std::unique(uq.begin(), uq.end()), uq.end());
+   auto idx = [&](double v) { return (int)(std::lower_bound(uq.begin(), uq.end(), v) - uq.begin()); };
+   std::vector<int> i1(npts), i2(npts), i3(npts);
+   for (int q = 0; q < npts; ++q) {
+      i1[q] = idx(all[3 * q]);
+      i2[q] = idx(all[3 * q + 1]);
+      i3[q] = idx(all[3 * q + 2]);
+   }
+   const int nu = (int)uq.size();
+   const long ldU = (long)P.nm * nu;
+   double *U;
+   CHK(build_table(P, D, uq, nu, p->pulsation, &U, ldU));
+   std::vector<double> f1(fi, fi + npts), f2(fj, fj + npts), ww(w, w + npts);
+   int *di1, *di2, *di3;
+   double *df1, *df2, *dw, *out;
+   CHK(upload("i1", i1, &di1));
+   CHK(upload("i2", i2, &di2));
+   CHK(upload("i3", i3, &di3));
+   CHK(upload("f1", f1, &df1));
+   CHK(upload("f2", f2, &df2));
+   CHK(upload("wpt", ww, &dw));
+   CHK(dbuf("m3sum", nm3, &out));
+   CHK(fill(out, nm3, 0.0));
+   const int nchunk = (int)std::min<long>(1024, std::max<long>(1, npts / 16));
+   dim3 gs((unsigned)((nm3 + 255) / 256), nchunk);
+   k_bfm_points_sum<<<gs, 256, 0, g.s>>>(U, ldU, nu, D.pb_k, P.K, P.nm, npts, di1, di2, di3,
+                                         df1, df2, dw, D.Mm, D.Km, D.Cm, with_frf, out);
+   HIPC(hipGetLastError());
+   HIPC(hipMemcpyAsync(m3, out, nm3 * sizeof(double), hipMemcpyDeviceToHost, g.s));
+   HIPC(hipStreamSynchronize(g.s));
    return 0;
 }
 

@@ -205,11 +205,13 @@ struct GsymArgs {
    long ldU;
    const double *pb;      // [nm][K]  Psi_b per K index
    int K;                 // multiple of KC
-   int nm, nhp, nh;       // modes, rows per mode, valid half-grid points
+   int nm, nhp, nh;       // modes, rows per mode, half-grid points of the plane
+   int nu;                // table rows holding A (nh, or 2nh-1 when S(f1+f2)
+                          // is evaluated beyond the plane)
    int ntile;             // tiles per side
    long npairs;           // ntile*(ntile+1)/2
    const double2 *H;      // [nm][nft]
-   int nft;               // >= 2*nh - 1
+   int nft;               // >= 2*nu - 1
    double d2;             // (2 pi df)^2 or df^2
    int rule;
    double *m3f, *m3r;     // [nm^3] accumulated (atomic)
@@ -313,14 +315,14 @@ __global__ __launch_bounds__(256) void k_gsym_f64(GsymArgs a)
          double s[12];
 #pragma unroll
          for (int t = 0; t < 12; ++t) s[t] = 0.0;
-         if (v < a.nh) {
+         if (v < a.nu) {
             const double2 hxv = hget(a.H, a.nft, x, v);
             const double2 hyv = hget(a.H, a.nft, y, v);
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                const int u = r0 - y * a.nhp + (lane >> 4) + 4 * t;
                const double g = 2.0 * acc[br][bc][t];   // B = 2 * sum of the three G terms
-               if (u >= a.nh) continue;
+               if (u >= a.nu) continue;
                const double ms = (u == 0 && v == 0) ? 1.0 : 2.0;
                const double mo = (u > 0 && v > 0) ? 2.0 : 0.0;
                const double2 hxu = hget(a.H, a.nft, x, u);
@@ -519,4 +521,54 @@ __global__ void k_mfma_f64_layout(double *__restrict__ out)
       out[t * 64 + l] = rows[t];
       out[256 + t * 64 + l] = cols[t];
    }
+}
+
+// ---------------------------------------------------------------------------
+// Sum over a point list of w_pt * B(m,n,p)(pt) [* FRF]: one thread per mode
+// triple and chunk of points, one atomic per thread.  Used for point sets that
+// are not a symmetric uniform grid (BsaCL computation frequencies).
+// grid (ceil(nm3/256), nchunk).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bfm_points_sum(
+   const double *__restrict__ U, long ldU, int rows, const double *__restrict__ pb, int K,
+   int nm, int npts, const int *__restrict__ i1, const int *__restrict__ i2,
+   const int *__restrict__ i3, const double *__restrict__ f1s, const double *__restrict__ f2s,
+   const double *__restrict__ wpt, const double *__restrict__ Mm, const double *__restrict__ Km,
+   const double *__restrict__ Cm, int with_frf, double *__restrict__ out)
+{
+   const int nm3 = nm * nm * nm;
+   const int mnp = blockIdx.x * blockDim.x + threadIdx.x;
+   if (mnp >= nm3) return;
+   const int m = mnp % nm, n = (mnp / nm) % nm, p = mnp / (nm * nm);
+   const int per = (npts + gridDim.y - 1) / gridDim.y;
+   const int p0 = blockIdx.y * per, p1 = min(npts, p0 + per);
+   double acc = 0.0;
+   for (int pt = p0; pt < p1; ++pt) {
+      const int a1 = i1[pt], a2 = i2[pt], a3 = i3[pt];
+      double s = 0.0;
+      for (int k = 0; k < K; ++k) {
+         const double *row = U + (size_t)k * ldU;
+         const double An1 = row[(size_t)n * rows + a1], Ap2 = row[(size_t)p * rows + a2];
+         double v = pb[(size_t)m * K + k] * An1 * Ap2;
+         if (a3 >= 0) {
+            const double Am3 = row[(size_t)m * rows + a3];
+            v += pb[(size_t)n * K + k] * Am3 * Ap2 + pb[(size_t)p * K + k] * Am3 * An1;
+         }
+         s += v;
+      }
+      s *= 2.0 * wpt[pt];
+      if (with_frf) {
+         const double w1 = 2.0 * M_PI * f1s[pt], w2 = 2.0 * M_PI * f2s[pt], w3 = w1 + w2;
+         double r, im, d;
+         r = -(w1 * w1 * Mm[m]) + Km[m]; im = w1 * Cm[m]; d = r * r + im * im;
+         const double2 h1 = make_double2(r / d, -im / d);
+         r = -(w2 * w2 * Mm[n]) + Km[n]; im = w2 * Cm[n]; d = r * r + im * im;
+         const double2 h2 = make_double2(r / d, -im / d);
+         r = -(w3 * w3 * Mm[p]) + Km[p]; im = w3 * Cm[p]; d = r * r + im * im;
+         const double2 h3 = make_double2(r / d, -im / d);
+         s *= re3(h1, h2, h3);
+      }
+      acc += s;
+   }
+   atomicAdd(out + mnp, acc);
 }

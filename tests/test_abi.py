@@ -30,7 +30,8 @@ def libpath():
 
 def test_headers_declare_the_expected_entry_points():
     names = declared_symbols()
-    for must in ("bsahip_classic_run", "bsahip_bfm_points", "bsahip_classic_spectra"):
+    for must in ("bsahip_classic_run", "bsahip_bfm_points", "bsahip_classic_spectra", "bsaclInit",
+                 "bsaclRun", "bsaclFinalise", "bsaclAcquireComputationFreqs", "bsaclSetKernelID"):
         assert must in names
 
 
