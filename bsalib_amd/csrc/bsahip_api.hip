@@ -269,11 +269,12 @@ int frf_table(const Prep &P, const DevProb &D, int nft, double df, double2 **H)
    return 0;
 }
 
-int check_flags(int32_t flags, bool grid)
+int check_flags(int32_t flags, bool grid, bool allow_mixed)
 {
    const int prec = flags & BSAHIP_PREC_MASK;
    const int quad = (flags & BSAHIP_QUAD_MASK) >> 4;
-   if (prec != BSAHIP_PREC_FP64) return fail(BSAHIP_EUNSUPPORTED, "only BSAHIP_PREC_FP64 in this entry point");
+   if (prec != BSAHIP_PREC_FP64 && !(allow_mixed && prec == BSAHIP_PREC_BF16X3))
+      return fail(BSAHIP_EUNSUPPORTED, "precision not available in this entry point");
    if (grid && quad > 2) return fail(BSAHIP_EINVAL, "unknown quadrature rule");
    return 0;
 }
@@ -333,6 +334,20 @@ int bsahip_selftest_mfma_layout(double *out512)
    return 0;
 }
 
+int bsahip_selftest_mfma_bf16_layout(float *out768)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   CHK(ensure_init());
+   if (!out768) return fail(BSAHIP_EINVAL, "NULL output");
+   float *d;
+   CHK(dbuf("selftest16", 768, &d));
+   k_mfma_bf16_layout<<<1, 64, 0, g.s>>>(d);
+   HIPC(hipGetLastError());
+   HIPC(hipMemcpyAsync(out768, d, 768 * sizeof(float), hipMemcpyDeviceToHost, g.s));
+   HIPC(hipStreamSynchronize(g.s));
+   return 0;
+}
+
 int bsahip_last_kernel_time(double *ms, int32_t *launches)
 {
    if (ms) *ms = g.last_ms;
@@ -352,8 +367,9 @@ int bsahip__classic_moments(const bsahip_problem *p, int32_t nfh, double df, int
 {
    std::lock_guard<std::recursive_mutex> lk(g.mu);
    CHK(ensure_init());
-   CHK(check_flags(flags, true));
+   CHK(check_flags(flags, true, true));
    if (nfh < 2 || !(df > 0.0)) return fail(BSAHIP_EINVAL, "need nfh >= 2 and df > 0");
+   const int prec = flags & BSAHIP_PREC_MASK;
    Prep P;
    CHK(prepare(p, P, psd_mode));
    DevProb D;
@@ -363,8 +379,9 @@ int bsahip__classic_moments(const bsahip_problem *p, int32_t nfh, double df, int
    const int nu = exact ? nf : nh;                 // rows of A per mode
    const int nhp = round_up(nu, 16);
    const int rows = P.nm * nhp;
-   const int ntile = (rows + TILE - 1) / TILE;
-   const long ldU = (long)ntile * TILE;
+   const int tile = (prec == BSAHIP_PREC_BF16X3) ? T2 : TILE;
+   const int ntile = (rows + tile - 1) / tile;
+   const long ldU = (long)round_up(rows, T2);
    const int nft = 2 * nu - 1;
    std::vector<double> fl(nu);
    for (int u = 0; u < nu; ++u) fl[u] = (double)u * df;
@@ -401,8 +418,23 @@ int bsahip__classic_moments(const bsahip_problem *p, int32_t nfh, double df, int
       a.m3r = m3 + nm3;
       const long nwg = a.npairs * P.nm;
       if (nwg > 0x7fffffffL) return fail(BSAHIP_EUNSUPPORTED, "grid too large for one launch");
-      HIPC(hipEventRecord(g.e0, g.s));
-      k_gsym_f64<<<dim3((unsigned)nwg), 256, 0, g.s>>>(a);
+      if (prec == BSAHIP_PREC_BF16X3) {
+         GsymArgs16 b16;
+         float *U32, *pb32;
+         CHK(dbuf("U32", (size_t)P.K * ldU, &U32));
+         std::vector<float> pbf(P.pb_k.begin(), P.pb_k.end());
+         CHK(upload("pb32", pbf, &pb32));
+         const size_t n = (size_t)P.K * ldU;
+         k_to_f32<<<(int)std::min<size_t>((n + 255) / 256, 65536), 256, 0, g.s>>>(U, U32, n);
+         HIPC(hipGetLastError());
+         b16.U32 = U32;
+         b16.pb32 = pb32;
+         HIPC(hipEventRecord(g.e0, g.s));
+         k_gsym_bf16x3<<<dim3((unsigned)nwg), 256, 0, g.s>>>(a, b16);
+      } else {
+         HIPC(hipEventRecord(g.e0, g.s));
+         k_gsym_f64<<<dim3((unsigned)nwg), 256, 0, g.s>>>(a);
+      }
       HIPC(hipGetLastError());
       HIPC(hipEventRecord(g.e1, g.s));
       if (m3mf) HIPC(hipMemcpyAsync(m3mf, m3, nm3 * sizeof(double), hipMemcpyDeviceToHost, g.s));
@@ -492,7 +524,7 @@ int bsahip_classic_spectra(const bsahip_problem *p, int32_t nfh, double df, int3
 {
    std::lock_guard<std::recursive_mutex> lk(g.mu);
    CHK(ensure_init());
-   CHK(check_flags(flags, true));
+   CHK(check_flags(flags, true, false));
    if (nfh < 2 || !(df > 0.0)) return fail(BSAHIP_EINVAL, "need nfh >= 2 and df > 0");
    Prep P;
    CHK(prepare(p, P));
@@ -559,8 +591,7 @@ int bsahip_bfm_points(const bsahip_problem *p, int32_t npts, const double *fi, c
 {
    std::lock_guard<std::recursive_mutex> lk(g.mu);
    CHK(ensure_init());
-   if ((flags & BSAHIP_PREC_MASK) != BSAHIP_PREC_FP64)
-      return fail(BSAHIP_EUNSUPPORTED, "only BSAHIP_PREC_FP64 in this entry point");
+   CHK(check_flags(flags, false, false));
    if (npts < 0 || (npts > 0 && (!fi || !fj || !bfm)))
       return fail(BSAHIP_EINVAL, "npts < 0 or NULL point/output arrays");
    if (npts == 0) return 0;

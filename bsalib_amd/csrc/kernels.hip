@@ -217,6 +217,99 @@ struct GsymArgs {
    double *m3f, *m3r;     // [nm^3] accumulated (atomic)
 };
 
+// Plane epilogue of one 16x16 block of G[x] values (rows r0.., cols q0..).
+// Each lane holds four values of column q0 + lane%16, rows given by the MFMA
+// accumulator layout: interleaved (f64: (l/16) + 4t) or grouped (f32
+// outputs: 4(l/16) + t).  Adds 2*G, weighted by the quadrature weight and
+// the FRF product, to the m3 entries of the six mode permutations it feeds.
+__device__ __forceinline__ void epi_block(const GsymArgs &a, int x, bool diag, int r0, int q0,
+                                          int lane, const double (&gval)[4], bool interleaved)
+{
+   const int nh1 = a.nh - 1;
+   const int nm = a.nm;
+   const size_t nm2 = (size_t)nm * nm;
+   const int y = r0 / a.nhp, z = q0 / a.nhp;
+   if (y >= nm || z >= nm) return;                    // uniform per block
+   const int v = q0 - z * a.nhp + (lane & 15);
+   double s[12];
+#pragma unroll
+   for (int t = 0; t < 12; ++t) s[t] = 0.0;
+   if (v < a.nu) {
+      const double2 hxv = hget(a.H, a.nft, x, v);
+      const double2 hyv = hget(a.H, a.nft, y, v);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+         const int u = r0 - y * a.nhp + (interleaved ? ((lane >> 4) + 4 * t) : (4 * (lane >> 4) + t));
+         const double g = 2.0 * gval[t];   // B = 2 * sum of the three G terms
+         if (u >= a.nu) continue;
+         const double ms = (u == 0 && v == 0) ? 1.0 : 2.0;
+         const double mo = (u > 0 && v > 0) ? 2.0 : 0.0;
+         const double2 hxu = hget(a.H, a.nft, x, u);
+         const double2 hzu = hget(a.H, a.nft, z, u);
+         const double2 hyp = hget(a.H, a.nft, y, u + v);
+         const double2 hym = hget(a.H, a.nft, y, u - v);
+         const double2 hzp = hget(a.H, a.nft, z, u + v);
+         const double2 hzm = hget(a.H, a.nft, z, u - v);
+         const double2 cyv = make_double2(hyv.x, -hyv.y);
+         const double2 cxv = make_double2(hxv.x, -hxv.y);
+         const double2 cxu = make_double2(hxu.x, -hxu.y);
+         const double2 czu = make_double2(hzu.x, -hzu.y);
+         const int dm = u - v;
+         const double Wa = qweight(u, v, nh1, a.rule, a.d2);
+         const double Wb = qweight(dm, v, nh1, a.rule, a.d2);
+         const double Wd = qweight(u + v, v, nh1, a.rule, a.d2);
+         // (m,n,p) = (x,y,z): T1 at (u,v) and (u,-v)
+         s[0] += g * Wa * (ms + mo);
+         s[1] += g * Wa * (ms * re3(hxu, hyv, hzp) + mo * re3(hxu, cyv, hzm));
+         // (y,x,z): T2 at (u-v, v) and (u+v, -v)
+         s[2] += g * (ms * Wb + mo * Wd);
+         s[3] += g * (ms * Wb * re3(hym, hxv, hzu) + mo * Wd * re3(hyp, cxv, hzu));
+         // (y,z,x): T3 at (v, u-v) and (-v, u+v)
+         s[4] += g * (ms * Wb + mo * Wd);
+         s[5] += g * (ms * Wb * re3(hyv, hzm, hxu) + mo * Wd * re3(cyv, hzp, hxu));
+         if (!diag) {
+            const double We = qweight(dm, u, nh1, a.rule, a.d2);
+            const double Wf = qweight(u + v, u, nh1, a.rule, a.d2);
+            const double2 hzmn = make_double2(hzm.x, -hzm.y);   // H_z(v-u)
+            const double2 hymn = make_double2(hym.x, -hym.y);   // H_y(v-u)
+            // (x,z,y): T1 at (v,u) and (v,-u)
+            s[6] += g * Wa * (ms + mo);
+            s[7] += g * Wa * (ms * re3(hxv, hzu, hyp) + mo * re3(hxv, czu, hymn));
+            // (z,x,y): T2 at (v-u, u) and (v+u, -u)
+            s[8] += g * (ms * We + mo * Wf);
+            s[9] += g * (ms * We * re3(hzmn, hxu, hyv) + mo * Wf * re3(hzp, cxu, hyv));
+            // (z,y,x): T3 at (u, v-u) and (-u, u+v)
+            s[10] += g * (ms * We + mo * Wf);
+            s[11] += g * (ms * We * re3(hzu, hymn, hxv) + mo * Wf * re3(czu, hyp, hxv));
+         }
+      }
+   }
+#pragma unroll
+   for (int t = 0; t < 12; ++t) s[t] = wave_sum(s[t]);
+   if (lane == 0) {
+      const size_t i_xyz = x + (size_t)nm * y + nm2 * z;
+      const size_t i_yxz = y + (size_t)nm * x + nm2 * z;
+      const size_t i_yzx = y + (size_t)nm * z + nm2 * x;
+      atomicAdd(a.m3f + i_xyz, s[0]);
+      atomicAdd(a.m3r + i_xyz, s[1]);
+      atomicAdd(a.m3f + i_yxz, s[2]);
+      atomicAdd(a.m3r + i_yxz, s[3]);
+      atomicAdd(a.m3f + i_yzx, s[4]);
+      atomicAdd(a.m3r + i_yzx, s[5]);
+      if (!diag) {
+         const size_t i_xzy = x + (size_t)nm * z + nm2 * y;
+         const size_t i_zxy = z + (size_t)nm * x + nm2 * y;
+         const size_t i_zyx = z + (size_t)nm * y + nm2 * x;
+         atomicAdd(a.m3f + i_xzy, s[6]);
+         atomicAdd(a.m3r + i_xzy, s[7]);
+         atomicAdd(a.m3f + i_zxy, s[8]);
+         atomicAdd(a.m3r + i_zxy, s[9]);
+         atomicAdd(a.m3f + i_zyx, s[10]);
+         atomicAdd(a.m3r + i_zyx, s[11]);
+      }
+   }
+}
+
 __device__ __forceinline__ void pair_from_index(long t, int &R, int &Q)
 {
    long q = (long)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
@@ -300,96 +393,200 @@ __global__ __launch_bounds__(256) void k_gsym_f64(GsymArgs a)
 
    // ---- epilogue: for v_mfma_f64_16x16x4f64, register t of lane l holds
    // D[(l/16) + 4t][l%16] of its 16x16 block (rows interleaved by 4).
-   const int nh1 = a.nh - 1;
-   const int nm = a.nm;
-   const size_t nm2 = (size_t)nm * nm;
 #pragma unroll
    for (int br = 0; br < 2; ++br) {
 #pragma unroll
       for (int bc = 0; bc < 2; ++bc) {
          const int r0 = R * TILE + wr * 32 + br * 16;
          const int q0 = Q * TILE + wc * 32 + bc * 16;
-         const int y = r0 / a.nhp, z = q0 / a.nhp;
-         if (y >= nm || z >= nm) continue;            // uniform per block
-         const int v = q0 - z * a.nhp + (lane & 15);
-         double s[12];
+         double g[4];
 #pragma unroll
-         for (int t = 0; t < 12; ++t) s[t] = 0.0;
-         if (v < a.nu) {
-            const double2 hxv = hget(a.H, a.nft, x, v);
-            const double2 hyv = hget(a.H, a.nft, y, v);
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-               const int u = r0 - y * a.nhp + (lane >> 4) + 4 * t;
-               const double g = 2.0 * acc[br][bc][t];   // B = 2 * sum of the three G terms
-               if (u >= a.nu) continue;
-               const double ms = (u == 0 && v == 0) ? 1.0 : 2.0;
-               const double mo = (u > 0 && v > 0) ? 2.0 : 0.0;
-               const double2 hxu = hget(a.H, a.nft, x, u);
-               const double2 hzu = hget(a.H, a.nft, z, u);
-               const double2 hyp = hget(a.H, a.nft, y, u + v);
-               const double2 hym = hget(a.H, a.nft, y, u - v);
-               const double2 hzp = hget(a.H, a.nft, z, u + v);
-               const double2 hzm = hget(a.H, a.nft, z, u - v);
-               const double2 cyv = make_double2(hyv.x, -hyv.y);
-               const double2 cxv = make_double2(hxv.x, -hxv.y);
-               const double2 cxu = make_double2(hxu.x, -hxu.y);
-               const double2 czu = make_double2(hzu.x, -hzu.y);
-               const int dm = u - v;
-               const double Wa = qweight(u, v, nh1, a.rule, a.d2);
-               const double Wb = qweight(dm, v, nh1, a.rule, a.d2);
-               const double Wd = qweight(u + v, v, nh1, a.rule, a.d2);
-               // (m,n,p) = (x,y,z): T1 at (u,v) and (u,-v)
-               s[0] += g * Wa * (ms + mo);
-               s[1] += g * Wa * (ms * re3(hxu, hyv, hzp) + mo * re3(hxu, cyv, hzm));
-               // (y,x,z): T2 at (u-v, v) and (u+v, -v)
-               s[2] += g * (ms * Wb + mo * Wd);
-               s[3] += g * (ms * Wb * re3(hym, hxv, hzu) + mo * Wd * re3(hyp, cxv, hzu));
-               // (y,z,x): T3 at (v, u-v) and (-v, u+v)
-               s[4] += g * (ms * Wb + mo * Wd);
-               s[5] += g * (ms * Wb * re3(hyv, hzm, hxu) + mo * Wd * re3(cyv, hzp, hxu));
-               if (!diag) {
-                  const double We = qweight(dm, u, nh1, a.rule, a.d2);
-                  const double Wf = qweight(u + v, u, nh1, a.rule, a.d2);
-                  const double2 hzmn = make_double2(hzm.x, -hzm.y);   // H_z(v-u)
-                  const double2 hymn = make_double2(hym.x, -hym.y);   // H_y(v-u)
-                  // (x,z,y): T1 at (v,u) and (v,-u)
-                  s[6] += g * Wa * (ms + mo);
-                  s[7] += g * Wa * (ms * re3(hxv, hzu, hyp) + mo * re3(hxv, czu, hymn));
-                  // (z,x,y): T2 at (v-u, u) and (v+u, -u)
-                  s[8] += g * (ms * We + mo * Wf);
-                  s[9] += g * (ms * We * re3(hzmn, hxu, hyv) + mo * Wf * re3(hzp, cxu, hyv));
-                  // (z,y,x): T3 at (u, v-u) and (-u, u+v)
-                  s[10] += g * (ms * We + mo * Wf);
-                  s[11] += g * (ms * We * re3(hzu, hymn, hxv) + mo * Wf * re3(czu, hyp, hxv));
-               }
-            }
-         }
-#pragma unroll
-         for (int t = 0; t < 12; ++t) s[t] = wave_sum(s[t]);
-         if (lane == 0) {
-            const size_t i_xyz = x + (size_t)nm * y + nm2 * z;
-            const size_t i_yxz = y + (size_t)nm * x + nm2 * z;
-            const size_t i_yzx = y + (size_t)nm * z + nm2 * x;
-            atomicAdd(a.m3f + i_xyz, s[0]);
-            atomicAdd(a.m3r + i_xyz, s[1]);
-            atomicAdd(a.m3f + i_yxz, s[2]);
-            atomicAdd(a.m3r + i_yxz, s[3]);
-            atomicAdd(a.m3f + i_yzx, s[4]);
-            atomicAdd(a.m3r + i_yzx, s[5]);
-            if (!diag) {
-               const size_t i_xzy = x + (size_t)nm * z + nm2 * y;
-               const size_t i_zxy = z + (size_t)nm * x + nm2 * y;
-               const size_t i_zyx = z + (size_t)nm * y + nm2 * x;
-               atomicAdd(a.m3f + i_xzy, s[6]);
-               atomicAdd(a.m3r + i_xzy, s[7]);
-               atomicAdd(a.m3f + i_zxy, s[8]);
-               atomicAdd(a.m3r + i_zxy, s[9]);
-               atomicAdd(a.m3f + i_zyx, s[10]);
-               atomicAdd(a.m3r + i_zyx, s[11]);
-            }
-         }
+         for (int t = 0; t < 4; ++t) g[t] = acc[br][bc][t];
+         epi_block(a, x, diag, r0, q0, lane, g, true);
       }
+   }
+}
+
+// ---------------------------------------------------------------------------
+// G contraction in split-bf16 x3 on v_mfma_f32_16x16x32_bf16 (FP32 accumulate)
+//   a*b ~= a_hi*b_hi + a_hi*b_lo + a_lo*b_hi,  a_hi = bf16(a), a_lo = bf16(a - a_hi)
+// Operands come from the FP32 copy of the A table; the row operand is scaled
+// by Psi_b(x,k) in FP32 before the split.  Same plane epilogue (FP64) as
+// k_gsym_f64.  Workgroup tile 128x128, 4 waves of 64x64 (4x4 blocks of
+// 16x16), K chunk 32 double-buffered in LDS.  LDS rows hold four 16-byte
+// k-groups stored at slot g ^ ((row >> 2) & 3): the 16 lanes of a fragment read
+// touch 16 distinct 16-byte bank groups.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+#define T2 128
+#define KC2 32
+
+__device__ __forceinline__ uint32_t bf16_rne_bits(float f)
+{
+   const uint32_t b = __float_as_uint(f);
+   return (b + 0x7FFFu + ((b >> 16) & 1u)) >> 16;
+}
+
+__device__ __forceinline__ void split_bf16(float v, uint32_t &hi, uint32_t &lo)
+{
+   hi = bf16_rne_bits(v);
+   const float r = v - __uint_as_float(hi << 16);     // exact in FP32
+   lo = bf16_rne_bits(r);
+}
+
+__device__ __forceinline__ int swz(int row, int g) { return row * 4 + (g ^ ((row >> 2) & 3)); }
+
+struct GsymArgs16 {
+   const float *U32;      // [K][ldU] FP32 copy of U
+   const float *pb32;     // [nm][K]
+};
+
+__global__ __launch_bounds__(256, 2) void k_gsym_bf16x3(GsymArgs a, GsymArgs16 b)
+{
+   __shared__ uint4 Ah[2][T2 * 4], Al[2][T2 * 4], Bh[2][T2 * 4], Bl[2][T2 * 4];
+
+   const long nwg = (long)gridDim.x;
+   const long wid = blockIdx.x;
+   long lid = wid;
+   const long per = nwg / 8;
+   if (wid < per * 8) lid = (wid % 8) * per + wid / 8;
+   const int x = (int)(lid % a.nm);
+   const long tp = lid / a.nm;
+   int R, Q;
+   pair_from_index(tp, R, Q);
+   const bool diag = (R == Q);
+
+   const int tid = threadIdx.x;
+   const int lane = tid & 63;
+   const int w = tid >> 6;
+   const int wr = w >> 1, wc = w & 1;
+
+   // staging: thread -> (row, 16 consecutive k)
+   const int srow = tid & 127;
+   const int skh = tid >> 7;
+   const float *gA = b.U32 + (size_t)R * T2 + srow;
+   const float *gB = b.U32 + (size_t)Q * T2 + srow;
+   const float *pbx = b.pb32 + (size_t)x * a.K;
+
+   f4v acc[4][4];
+#pragma unroll
+   for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = (f4v){0.f, 0.f, 0.f, 0.f};
+
+   float ra[16], rb[16];
+   auto gload = [&](int k0) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+         const size_t kk = (size_t)(k0 + skh * 16 + j) * a.ldU;
+         ra[j] = gA[kk] * pbx[k0 + skh * 16 + j];
+         rb[j] = gB[kk];
+      }
+   };
+   auto lstore = [&](int buf) {
+      uint32_t h[16], l[16];
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+#pragma unroll
+         for (int j = 0; j < 8; ++j) split_bf16(half ? rb[j] : ra[j], h[j], l[j]);
+#pragma unroll
+         for (int j = 8; j < 16; ++j) split_bf16(half ? rb[j] : ra[j], h[j], l[j]);
+         const uint4 h0 = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
+         const uint4 h1 = make_uint4(h[8] | (h[9] << 16), h[10] | (h[11] << 16), h[12] | (h[13] << 16), h[14] | (h[15] << 16));
+         const uint4 l0 = make_uint4(l[0] | (l[1] << 16), l[2] | (l[3] << 16), l[4] | (l[5] << 16), l[6] | (l[7] << 16));
+         const uint4 l1 = make_uint4(l[8] | (l[9] << 16), l[10] | (l[11] << 16), l[12] | (l[13] << 16), l[14] | (l[15] << 16));
+         uint4 *H = half ? Bh[buf] : Ah[buf];
+         uint4 *Lo = half ? Bl[buf] : Al[buf];
+         H[swz(srow, 2 * skh)] = h0;
+         H[swz(srow, 2 * skh + 1)] = h1;
+         Lo[swz(srow, 2 * skh)] = l0;
+         Lo[swz(srow, 2 * skh + 1)] = l1;
+      }
+   };
+
+   const int nchunk = a.K / KC2;
+   gload(0);
+   lstore(0);
+   __syncthreads();
+   for (int c = 0; c < nchunk; ++c) {
+      const int buf = c & 1;
+      if (c + 1 < nchunk) gload((c + 1) * KC2);
+      const int g = lane >> 4;
+      bf8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+         const int row = wr * 64 + i * 16 + (lane & 15);
+         const int col = wc * 64 + i * 16 + (lane & 15);
+         ah[i] = __builtin_bit_cast(bf8, Ah[buf][swz(row, g)]);
+         al[i] = __builtin_bit_cast(bf8, Al[buf][swz(row, g)]);
+         bh[i] = __builtin_bit_cast(bf8, Bh[buf][swz(col, g)]);
+         bl[i] = __builtin_bit_cast(bf8, Bl[buf][swz(col, g)]);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+         for (int j = 0; j < 4; ++j) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+         }
+      if (c + 1 < nchunk) lstore(buf ^ 1);
+      __syncthreads();
+   }
+
+   // epilogue: f32-output 16x16 MFMAs hold D[4(l/16)+t][l%16]
+#pragma unroll
+   for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+         const int r0 = R * T2 + wr * 64 + i * 16;
+         const int q0 = Q * T2 + wc * 64 + j * 16;
+         double gv[4];
+#pragma unroll
+         for (int t = 0; t < 4; ++t) gv[t] = (double)acc[i][j][t];
+         epi_block(a, x, diag, r0, q0, lane, gv, false);
+      }
+}
+
+// FP64 -> FP32 copy of the A table
+__global__ void k_to_f32(const double *__restrict__ src, float *__restrict__ dst, size_t n)
+{
+   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+   const size_t st = (size_t)gridDim.x * blockDim.x;
+   for (; i < n; i += st) dst[i] = (float)src[i];
+}
+
+// Lane-layout self test of v_mfma_f32_16x16x32_bf16 (one wave, 768 floats):
+// three products with known answers, see bsahip_selftest_mfma_layout.
+__global__ void k_mfma_bf16_layout(float *__restrict__ out)
+{
+   const int l = threadIdx.x;
+   const int row = l & 15, g = l >> 4;
+   uint32_t a1[8], b1[8], a2[8], b2[8], b3[8];
+#pragma unroll
+   for (int j = 0; j < 8; ++j) {
+      const int k = 8 * g + j;                    // k index this element holds (assumed)
+      a1[j] = bf16_rne_bits(k == 0 ? (float)row : 0.f);
+      b1[j] = bf16_rne_bits(k == 0 ? 1.f : 0.f);
+      a2[j] = bf16_rne_bits((float)k);
+      b2[j] = bf16_rne_bits(k == row ? 1.f : 0.f);          // B[k][col=row-of-lane]
+      b3[j] = bf16_rne_bits(k == row + 16 ? 1.f : 0.f);
+   }
+   auto pack = [](const uint32_t (&v)[8]) {
+      uint4 q = make_uint4(v[0] | (v[1] << 16), v[2] | (v[3] << 16), v[4] | (v[5] << 16), v[6] | (v[7] << 16));
+      return __builtin_bit_cast(bf8, q);
+   };
+   const f4v z = (f4v){0.f, 0.f, 0.f, 0.f};
+   const f4v d1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(pack(a1), pack(b1), z, 0, 0, 0);
+   const f4v d2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(pack(a2), pack(b2), z, 0, 0, 0);
+   const f4v d3 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(pack(a2), pack(b3), z, 0, 0, 0);
+#pragma unroll
+   for (int t = 0; t < 4; ++t) {
+      out[t * 64 + l] = d1[t];
+      out[256 + t * 64 + l] = d2[t];
+      out[512 + t * 64 + l] = d3[t];
    }
 }
 

@@ -129,6 +129,13 @@ int bsahip_bfm_points(const bsahip_problem *p, int32_t npts, const double *fi,
 int bsahip_selftest_mfma_layout(double *out512);
 
 /*
+ * Same for v_mfma_f32_16x16x32_bf16 (split-bf16 path): out768 receives three
+ * 4x64 register images of products with known answers (rows, columns k<16,
+ * columns k>=16).  No reference counterpart.
+ */
+int bsahip_selftest_mfma_bf16_layout(float *out768);
+
+/*
  * Last measured device time (milliseconds, HIP events on the library stream)
  * of the dominant kernel of the previous call, and its launch count.
  */
