@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--config", default="config3", choices=["config1", "config2", "config3"])
+    ap.add_argument("--prec", default="both", choices=["both", "fp64", "bf16x3"],
+                    help="profiling aid: run only one precision (the JSON line needs both)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build targets one GPU; multi-GPU runs are not part of it "
@@ -117,6 +119,11 @@ def main():
 
     evals = prob.nf ** 2
     flops = gsym_flops(prob, nm)
+    if args.prec != "both":
+        r1, t1, k1 = timed(args.prec)
+        log(f"{args.prec}: step {1e3 * t1 / args.steps:.2f} ms, kernel {k1:.2f} ms, "
+            f"{flops / k1 / 1e9:.2f} TFLOP/s (algorithmic)")
+        return
     r64, t64, k64 = timed("fp64")
     log(f"fp64: step {1e3 * t64 / args.steps:.2f} ms, k_gsym_f64 {k64:.2f} ms, "
         f"{flops / k64 / 1e9:.2f} TFLOP/s")

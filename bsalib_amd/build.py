@@ -21,7 +21,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-shared", "-std=c++17",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-munsafe-fp-atomics", "-Wno-unused-result",
+    "-munsafe-fp-atomics", "-Wno-unused-result", "-mllvm", "-disable-promote-alloca-to-lds",
 ]
 
 

@@ -420,15 +420,23 @@ int bsahip__classic_moments(const bsahip_problem *p, int32_t nfh, double df, int
       if (nwg > 0x7fffffffL) return fail(BSAHIP_EUNSUPPORTED, "grid too large for one launch");
       if (prec == BSAHIP_PREC_BF16X3) {
          GsymArgs16 b16;
-         float *U32, *pb32;
-         CHK(dbuf("U32", (size_t)P.K * ldU, &U32));
+         const long ldk = P.K;                       // multiple of KC2
+         const int rowsP = (int)ldU;                 // ntile * 128
+         float *UR, *pb32;
+         uint16_t *Ch, *Cl;
+         CHK(dbuf("UR32", (size_t)rowsP * ldk, &UR));
+         CHK(dbuf("UCh", (size_t)rowsP * ldk, &Ch));
+         CHK(dbuf("UCl", (size_t)rowsP * ldk, &Cl));
          std::vector<float> pbf(P.pb_k.begin(), P.pb_k.end());
          CHK(upload("pb32", pbf, &pb32));
-         const size_t n = (size_t)P.K * ldU;
-         k_to_f32<<<(int)std::min<size_t>((n + 255) / 256, 65536), 256, 0, g.s>>>(U, U32, n);
+         dim3 gs((rowsP + 31) / 32, (unsigned)((ldk + 31) / 32));
+         k_split_table<<<gs, 256, 0, g.s>>>(U, ldU, P.K, rowsP, ldk, UR, Ch, Cl);
          HIPC(hipGetLastError());
-         b16.U32 = U32;
+         b16.UR = UR;
+         b16.UCh = reinterpret_cast<const uint4 *>(Ch);
+         b16.UCl = reinterpret_cast<const uint4 *>(Cl);
          b16.pb32 = pb32;
+         b16.ldk = ldk;
          HIPC(hipEventRecord(g.e0, g.s));
          k_gsym_bf16x3<<<dim3((unsigned)nwg), 256, 0, g.s>>>(a, b16);
       } else {

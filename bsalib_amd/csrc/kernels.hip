@@ -217,75 +217,107 @@ struct GsymArgs {
    double *m3f, *m3r;     // [nm^3] accumulated (atomic)
 };
 
-// Plane epilogue of one 16x16 block of G[x] values (rows r0.., cols q0..).
-// Each lane holds four values of column q0 + lane%16, rows given by the MFMA
-// accumulator layout: interleaved (f64: (l/16) + 4t) or grouped (f32
-// outputs: 4(l/16) + t).  Adds 2*G, weighted by the quadrature weight and
-// the FRF product, to the m3 entries of the six mode permutations it feeds.
-__device__ __forceinline__ void epi_block(const GsymArgs &a, int x, bool diag, int r0, int q0,
-                                          int lane, const double (&gval)[4], bool interleaved)
+// Quadrature weight / d^2 from edge flags of |F1| (p) and |F2| (q) on the
+// half grid [0, n]: outside the plane 0; corner c_corner; edge c_edge; else 1.
+struct QRule {
+   double c_edge, c_corner;
+};
+
+__device__ __forceinline__ QRule qrule(int rule)
 {
-   const int nh1 = a.nh - 1;
-   const int nm = a.nm;
-   const size_t nm2 = (size_t)nm * nm;
-   const int y = r0 / a.nhp, z = q0 / a.nhp;
-   if (y >= nm || z >= nm) return;                    // uniform per block
-   const int v = q0 - z * a.nhp + (lane & 15);
-   double s[12];
+   if (rule == 0) return QRule{0.5, 1.0 - 1.5};   // intgSpectraVect_ (BsaClassicImpl.F90:604-631)
+   if (rule == 1) return QRule{1.0, 1.0};         // Riemann (BsaClassicImpl.F90:331,341)
+   return QRule{0.5, 0.25};                       // tensor trapezoid (_shared_poly2d.fi:827-832)
+}
+
+__device__ __forceinline__ double qw(bool ep, bool op, bool eq, bool oq, const QRule &r)
+{
+   return (op || oq) ? 0.0 : ((ep && eq) ? r.c_corner : ((ep || eq) ? r.c_edge : 1.0));
+}
+
+// Plane epilogue of one 16x16 block of G[x] values (rows r0.., cols q0..),
+// mode rows y and cols z fixed per block.  Each lane holds four values of
+// column q0 + lane%16, rows given by the MFMA accumulator layout: interleaved
+// (f64: (l/16) + 4t) or grouped (f32 outputs: 4(l/16) + t).  Adds 2*G times
+// the quadrature weight / d^2 and the FRF product to nine per-lane sums:
+//   s0 force (x,y,z)=(x,z,y)  s1 resp (x,y,z)  s2 force (y,x,z)=(y,z,x)
+//   s3 resp (y,x,z)  s4 resp (y,z,x)  s5 resp (x,z,y)
+//   s6 force (z,x,y)=(z,y,x)  s7 resp (z,x,y)  s8 resp (z,y,x)
+// (transposed entries s5..s8 only when !diag).
+__device__ __forceinline__ void epi_accum(const GsymArgs &a, int x, int y, int z, bool diag,
+                                          int ur0, int vq0, int lane, const double (&gval)[4],
+                                          bool interleaved, double (&s)[9])
+{
+   const int n = a.nh - 1;
+   const QRule qr = qrule(a.rule);
+   const double2 *Hx = a.H + (size_t)x * a.nft;
+   const double2 *Hy = a.H + (size_t)y * a.nft;
+   const double2 *Hz = a.H + (size_t)z * a.nft;
+   const int v = vq0 + (lane & 15);
+   if (v >= a.nu) return;
+   const double2 hxv = Hx[v], hyv = Hy[v];
+   const double2 cyv = make_double2(hyv.x, -hyv.y);
+   const double2 cxv = make_double2(hxv.x, -hxv.y);
+   const bool ev = (v == n), ov = (v > n);
 #pragma unroll
-   for (int t = 0; t < 12; ++t) s[t] = 0.0;
-   if (v < a.nu) {
-      const double2 hxv = hget(a.H, a.nft, x, v);
-      const double2 hyv = hget(a.H, a.nft, y, v);
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-         const int u = r0 - y * a.nhp + (interleaved ? ((lane >> 4) + 4 * t) : (4 * (lane >> 4) + t));
-         const double g = 2.0 * gval[t];   // B = 2 * sum of the three G terms
-         if (u >= a.nu) continue;
-         const double ms = (u == 0 && v == 0) ? 1.0 : 2.0;
-         const double mo = (u > 0 && v > 0) ? 2.0 : 0.0;
-         const double2 hxu = hget(a.H, a.nft, x, u);
-         const double2 hzu = hget(a.H, a.nft, z, u);
-         const double2 hyp = hget(a.H, a.nft, y, u + v);
-         const double2 hym = hget(a.H, a.nft, y, u - v);
-         const double2 hzp = hget(a.H, a.nft, z, u + v);
-         const double2 hzm = hget(a.H, a.nft, z, u - v);
-         const double2 cyv = make_double2(hyv.x, -hyv.y);
-         const double2 cxv = make_double2(hxv.x, -hxv.y);
-         const double2 cxu = make_double2(hxu.x, -hxu.y);
+   for (int t = 0; t < 4; ++t) {
+      const int u = ur0 + (interleaved ? ((lane >> 4) + 4 * t) : (4 * (lane >> 4) + t));
+      if (u >= a.nu) continue;
+      const double g = 2.0 * gval[t];      // B = 2 * sum of the three G terms
+      const double ms = (u == 0 && v == 0) ? 1.0 : 2.0;
+      const double mo = (u > 0 && v > 0) ? 2.0 : 0.0;
+      const int sp = u + v;
+      const int dm = u - v;
+      const int ad = dm < 0 ? -dm : dm;
+      const double2 hxu = Hx[u], hzu = Hz[u], hyp = Hy[sp], hzp = Hz[sp];
+      double2 hym = Hy[ad], hzm = Hz[ad];
+      if (dm < 0) {
+         hym.y = -hym.y;
+         hzm.y = -hzm.y;
+      }
+      const bool eu = (u == n), ou = (u > n), es = (sp == n), os = (sp > n);
+      const bool ed = (ad == n), od = (ad > n);
+      const double Wa = qw(eu, ou, ev, ov, qr);
+      const double Wb = qw(ed, od, ev, ov, qr);
+      const double Wd = qw(es, os, ev, ov, qr);
+      // FRF products Re[H_m(f1) H_n(f2) conj H_p(f1+f2)]; Re[conj w] = Re[w]
+      // makes four of the twelve contributions reuse these six:
+      const double F1s = re3(hxu, hyv, hzp);      // (x,y,z) at (u,v);    = (z,x,y) at (v+u,-u)
+      const double F1o = re3(hxu, cyv, hzm);      // (x,y,z) at (u,-v)
+      const double F2s = re3(hym, hxv, hzu);      // (y,x,z) at (u-v,v);  = (x,z,y) at (v,-u)
+      const double F2o = re3(hyp, cxv, hzu);      // (y,x,z) at (u+v,-v); = (x,z,y) at (v,u)
+      const double F3s = re3(hyv, hzm, hxu);      // (y,z,x) at (v,u-v);  = (z,x,y) at (v-u,u)
+      const double F3o = re3(cyv, hzp, hxu);      // (y,z,x) at (-v,u+v)
+      const double ga = g * Wa, gbs = ms * g * Wb, gdo = mo * g * Wd;
+      s[0] += ga * (ms + mo);
+      s[1] += ga * (ms * F1s + mo * F1o);
+      s[2] += gbs + gdo;
+      s[3] += gbs * F2s + gdo * F2o;
+      s[4] += gbs * F3s + gdo * F3o;
+      if (!diag) {
+         const double We = qw(ed, od, eu, ou, qr);
+         const double Wf = qw(es, os, eu, ou, qr);
+         const double2 hymn = make_double2(hym.x, -hym.y);   // H_y(v-u)
          const double2 czu = make_double2(hzu.x, -hzu.y);
-         const int dm = u - v;
-         const double Wa = qweight(u, v, nh1, a.rule, a.d2);
-         const double Wb = qweight(dm, v, nh1, a.rule, a.d2);
-         const double Wd = qweight(u + v, v, nh1, a.rule, a.d2);
-         // (m,n,p) = (x,y,z): T1 at (u,v) and (u,-v)
-         s[0] += g * Wa * (ms + mo);
-         s[1] += g * Wa * (ms * re3(hxu, hyv, hzp) + mo * re3(hxu, cyv, hzm));
-         // (y,x,z): T2 at (u-v, v) and (u+v, -v)
-         s[2] += g * (ms * Wb + mo * Wd);
-         s[3] += g * (ms * Wb * re3(hym, hxv, hzu) + mo * Wd * re3(hyp, cxv, hzu));
-         // (y,z,x): T3 at (v, u-v) and (-v, u+v)
-         s[4] += g * (ms * Wb + mo * Wd);
-         s[5] += g * (ms * Wb * re3(hyv, hzm, hxu) + mo * Wd * re3(cyv, hzp, hxu));
-         if (!diag) {
-            const double We = qweight(dm, u, nh1, a.rule, a.d2);
-            const double Wf = qweight(u + v, u, nh1, a.rule, a.d2);
-            const double2 hzmn = make_double2(hzm.x, -hzm.y);   // H_z(v-u)
-            const double2 hymn = make_double2(hym.x, -hym.y);   // H_y(v-u)
-            // (x,z,y): T1 at (v,u) and (v,-u)
-            s[6] += g * Wa * (ms + mo);
-            s[7] += g * Wa * (ms * re3(hxv, hzu, hyp) + mo * re3(hxv, czu, hymn));
-            // (z,x,y): T2 at (v-u, u) and (v+u, -u)
-            s[8] += g * (ms * We + mo * Wf);
-            s[9] += g * (ms * We * re3(hzmn, hxu, hyv) + mo * Wf * re3(hzp, cxu, hyv));
-            // (z,y,x): T3 at (u, v-u) and (-u, u+v)
-            s[10] += g * (ms * We + mo * Wf);
-            s[11] += g * (ms * We * re3(hzu, hymn, hxv) + mo * Wf * re3(czu, hyp, hxv));
-         }
+         const double G3s = re3(hzu, hymn, hxv);  // (z,y,x) at (u,v-u)
+         const double G3o = re3(czu, hyp, hxv);   // (z,y,x) at (-u,u+v)
+         const double ges = ms * g * We, gfo = mo * g * Wf;
+         s[5] += ga * (ms * F2o + mo * F2s);
+         s[6] += ges + gfo;
+         s[7] += ges * F3s + gfo * F1s;
+         s[8] += ges * G3s + gfo * G3o;
       }
    }
+}
+
+// Wave-reduce the nine sums and add them (times d^2) to the m3 entries.
+__device__ __forceinline__ void epi_flush(const GsymArgs &a, int x, int y, int z, bool diag,
+                                          int lane, double (&s)[9])
+{
+   const int nm = a.nm;
+   const size_t nm2 = (size_t)nm * nm;
 #pragma unroll
-   for (int t = 0; t < 12; ++t) s[t] = wave_sum(s[t]);
+   for (int t = 0; t < 9; ++t) s[t] = wave_sum(s[t]) * a.d2;
    if (lane == 0) {
       const size_t i_xyz = x + (size_t)nm * y + nm2 * z;
       const size_t i_yxz = y + (size_t)nm * x + nm2 * z;
@@ -294,18 +326,58 @@ __device__ __forceinline__ void epi_block(const GsymArgs &a, int x, bool diag, i
       atomicAdd(a.m3r + i_xyz, s[1]);
       atomicAdd(a.m3f + i_yxz, s[2]);
       atomicAdd(a.m3r + i_yxz, s[3]);
-      atomicAdd(a.m3f + i_yzx, s[4]);
-      atomicAdd(a.m3r + i_yzx, s[5]);
+      atomicAdd(a.m3f + i_yzx, s[2]);
+      atomicAdd(a.m3r + i_yzx, s[4]);
       if (!diag) {
          const size_t i_xzy = x + (size_t)nm * z + nm2 * y;
          const size_t i_zxy = z + (size_t)nm * x + nm2 * y;
          const size_t i_zyx = z + (size_t)nm * y + nm2 * x;
-         atomicAdd(a.m3f + i_xzy, s[6]);
-         atomicAdd(a.m3r + i_xzy, s[7]);
-         atomicAdd(a.m3f + i_zxy, s[8]);
-         atomicAdd(a.m3r + i_zxy, s[9]);
-         atomicAdd(a.m3f + i_zyx, s[10]);
-         atomicAdd(a.m3r + i_zyx, s[11]);
+         atomicAdd(a.m3f + i_xzy, s[0]);
+         atomicAdd(a.m3r + i_xzy, s[5]);
+         atomicAdd(a.m3f + i_zxy, s[6]);
+         atomicAdd(a.m3r + i_zxy, s[7]);
+         atomicAdd(a.m3f + i_zyx, s[6]);
+         atomicAdd(a.m3r + i_zyx, s[8]);
+      }
+   }
+#pragma unroll
+   for (int t = 0; t < 9; ++t) s[t] = 0.0;
+}
+
+// Epilogue of a wave's BR x BC blocks of 16x16 (rows rw0.., cols cw0..): one
+// accumulation / reduction when all blocks share (y,z), else per block.
+template <int BR, int BC, class GetVal>
+__device__ __forceinline__ void epi_wave(const GsymArgs &a, int x, bool diag, int rw0, int cw0,
+                                         int lane, bool interleaved, GetVal getv)
+{
+   const int nm = a.nm;
+   const int y0 = rw0 / a.nhp, y1 = (rw0 + 16 * BR - 1) / a.nhp;
+   const int z0 = cw0 / a.nhp, z1 = (cw0 + 16 * BC - 1) / a.nhp;
+   double s[9];
+#pragma unroll
+   for (int t = 0; t < 9; ++t) s[t] = 0.0;
+   if (y0 == y1 && z0 == z1) {
+      if (y0 >= nm || z0 >= nm) return;
+#pragma unroll 1
+      for (int b = 0; b < BR * BC; ++b) {
+         const int i = b / BC, j = b % BC;
+         double g[4];
+         getv(b, g);
+         epi_accum(a, x, y0, z0, diag, rw0 + 16 * i - y0 * a.nhp, cw0 + 16 * j - z0 * a.nhp,
+                   lane, g, interleaved, s);
+      }
+      epi_flush(a, x, y0, z0, diag, lane, s);
+   } else {
+#pragma unroll 1
+      for (int b = 0; b < BR * BC; ++b) {
+         const int i = b / BC, j = b % BC;
+         const int r0 = rw0 + 16 * i, q0 = cw0 + 16 * j;
+         const int y = r0 / a.nhp, z = q0 / a.nhp;
+         if (y >= nm || z >= nm) continue;             // uniform per block
+         double g[4];
+         getv(b, g);
+         epi_accum(a, x, y, z, diag, r0 - y * a.nhp, q0 - z * a.nhp, lane, g, interleaved, s);
+         epi_flush(a, x, y, z, diag, lane, s);
       }
    }
 }
@@ -393,18 +465,18 @@ __global__ __launch_bounds__(256) void k_gsym_f64(GsymArgs a)
 
    // ---- epilogue: for v_mfma_f64_16x16x4f64, register t of lane l holds
    // D[(l/16) + 4t][l%16] of its 16x16 block (rows interleaved by 4).
+   epi_wave<2, 2>(a, x, diag, R * TILE + wr * 32, Q * TILE + wc * 32, lane, true,
+                  [&](int b, double (&g)[4]) {
+                     d4 v;
+                     switch (b) {       // uniform branch: constant register indices
+                     case 0: v = acc[0][0]; break;
+                     case 1: v = acc[0][1]; break;
+                     case 2: v = acc[1][0]; break;
+                     default: v = acc[1][1]; break;
+                     }
 #pragma unroll
-   for (int br = 0; br < 2; ++br) {
-#pragma unroll
-      for (int bc = 0; bc < 2; ++bc) {
-         const int r0 = R * TILE + wr * 32 + br * 16;
-         const int q0 = Q * TILE + wc * 32 + bc * 16;
-         double g[4];
-#pragma unroll
-         for (int t = 0; t < 4; ++t) g[t] = acc[br][bc][t];
-         epi_block(a, x, diag, r0, q0, lane, g, true);
-      }
-   }
+                     for (int t = 0; t < 4; ++t) g[t] = v[t];
+                  });
 }
 
 // ---------------------------------------------------------------------------
@@ -418,6 +490,8 @@ __global__ __launch_bounds__(256) void k_gsym_f64(GsymArgs a)
 // touch 16 distinct 16-byte bank groups.
 // ---------------------------------------------------------------------------
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
 typedef float f4v __attribute__((ext_vector_type(4)));
 
 #define T2 128
@@ -439,8 +513,11 @@ __device__ __forceinline__ void split_bf16(float v, uint32_t &hi, uint32_t &lo)
 __device__ __forceinline__ int swz(int row, int g) { return row * 4 + (g ^ ((row >> 2) & 3)); }
 
 struct GsymArgs16 {
-   const float *U32;      // [K][ldU] FP32 copy of U
-   const float *pb32;     // [nm][K]
+   const float *UR;       // [rows][ldk]  A table, FP32, K contiguous (row operand source)
+   const uint4 *UCh;      // [rows][ldk/8] bf16 hi of the A table (column operand)
+   const uint4 *UCl;      // [rows][ldk/8] bf16 lo
+   const float *pb32;     // [nm][ldk]  Psi_b per K index, FP32
+   long ldk;              // K stride (multiple of 32)
 };
 
 __global__ __launch_bounds__(256, 2) void k_gsym_bf16x3(GsymArgs a, GsymArgs16 b)
@@ -463,12 +540,17 @@ __global__ __launch_bounds__(256, 2) void k_gsym_bf16x3(GsymArgs a, GsymArgs16 b
    const int w = tid >> 6;
    const int wr = w >> 1, wc = w & 1;
 
-   // staging: thread -> (row, 16 consecutive k)
-   const int srow = tid & 127;
-   const int skh = tid >> 7;
-   const float *gA = b.U32 + (size_t)R * T2 + srow;
-   const float *gB = b.U32 + (size_t)Q * T2 + srow;
-   const float *pbx = b.pb32 + (size_t)x * a.K;
+   // staging: items tid and tid+256 -> (row = item/4, k-group = item%4 of 8)
+   const int srow0 = tid >> 2, srow1 = srow0 + 64;
+   const int sgrp = tid & 3;
+   const long ldk = b.ldk;
+   const float *gA0 = b.UR + ((size_t)R * T2 + srow0) * ldk + sgrp * 8;
+   const float *gA1 = b.UR + ((size_t)R * T2 + srow1) * ldk + sgrp * 8;
+   const uint4 *gBh0 = b.UCh + (((size_t)Q * T2 + srow0) * ldk + sgrp * 8) / 8;
+   const uint4 *gBh1 = b.UCh + (((size_t)Q * T2 + srow1) * ldk + sgrp * 8) / 8;
+   const uint4 *gBl0 = b.UCl + (((size_t)Q * T2 + srow0) * ldk + sgrp * 8) / 8;
+   const uint4 *gBl1 = b.UCl + (((size_t)Q * T2 + srow1) * ldk + sgrp * 8) / 8;
+   const float *pbx = b.pb32 + (size_t)x * ldk + sgrp * 8;
 
    f4v acc[4][4];
 #pragma unroll
@@ -476,34 +558,45 @@ __global__ __launch_bounds__(256, 2) void k_gsym_bf16x3(GsymArgs a, GsymArgs16 b
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[i][j] = (f4v){0.f, 0.f, 0.f, 0.f};
 
-   float ra[16], rb[16];
+   float4 ra0, ra1, ra2, ra3, sc0, sc1;
+   uint4 rbh0, rbh1, rbl0, rbl1;
    auto gload = [&](int k0) {
-#pragma unroll
-      for (int j = 0; j < 16; ++j) {
-         const size_t kk = (size_t)(k0 + skh * 16 + j) * a.ldU;
-         ra[j] = gA[kk] * pbx[k0 + skh * 16 + j];
-         rb[j] = gB[kk];
-      }
+      ra0 = *(const float4 *)(gA0 + k0);
+      ra1 = *(const float4 *)(gA0 + k0 + 4);
+      ra2 = *(const float4 *)(gA1 + k0);
+      ra3 = *(const float4 *)(gA1 + k0 + 4);
+      sc0 = *(const float4 *)(pbx + k0);
+      sc1 = *(const float4 *)(pbx + k0 + 4);
+      rbh0 = gBh0[k0 / 8];
+      rbh1 = gBh1[k0 / 8];
+      rbl0 = gBl0[k0 / 8];
+      rbl1 = gBl1[k0 / 8];
+   };
+   auto split2 = [](float v0, float v1, uint32_t &h, uint32_t &l) {
+      const f2v vv = (f2v){v0, v1};
+      const bf2v hv = __builtin_convertvector(vv, bf2v);              // v_cvt_pk_bf16_f32 (RNE)
+      const f2v r = vv - __builtin_convertvector(hv, f2v);             // exact in FP32
+      h = __builtin_bit_cast(uint32_t, hv);
+      l = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf2v));
+   };
+   auto split8 = [&](float4 p0, float4 p1, uint4 &hi, uint4 &lo) {
+      split2(p0.x * sc0.x, p0.y * sc0.y, hi.x, lo.x);
+      split2(p0.z * sc0.z, p0.w * sc0.w, hi.y, lo.y);
+      split2(p1.x * sc1.x, p1.y * sc1.y, hi.z, lo.z);
+      split2(p1.z * sc1.z, p1.w * sc1.w, hi.w, lo.w);
    };
    auto lstore = [&](int buf) {
-      uint32_t h[16], l[16];
-#pragma unroll
-      for (int half = 0; half < 2; ++half) {
-#pragma unroll
-         for (int j = 0; j < 8; ++j) split_bf16(half ? rb[j] : ra[j], h[j], l[j]);
-#pragma unroll
-         for (int j = 8; j < 16; ++j) split_bf16(half ? rb[j] : ra[j], h[j], l[j]);
-         const uint4 h0 = make_uint4(h[0] | (h[1] << 16), h[2] | (h[3] << 16), h[4] | (h[5] << 16), h[6] | (h[7] << 16));
-         const uint4 h1 = make_uint4(h[8] | (h[9] << 16), h[10] | (h[11] << 16), h[12] | (h[13] << 16), h[14] | (h[15] << 16));
-         const uint4 l0 = make_uint4(l[0] | (l[1] << 16), l[2] | (l[3] << 16), l[4] | (l[5] << 16), l[6] | (l[7] << 16));
-         const uint4 l1 = make_uint4(l[8] | (l[9] << 16), l[10] | (l[11] << 16), l[12] | (l[13] << 16), l[14] | (l[15] << 16));
-         uint4 *H = half ? Bh[buf] : Ah[buf];
-         uint4 *Lo = half ? Bl[buf] : Al[buf];
-         H[swz(srow, 2 * skh)] = h0;
-         H[swz(srow, 2 * skh + 1)] = h1;
-         Lo[swz(srow, 2 * skh)] = l0;
-         Lo[swz(srow, 2 * skh + 1)] = l1;
-      }
+      uint4 hi, lo;
+      split8(ra0, ra1, hi, lo);
+      Ah[buf][swz(srow0, sgrp)] = hi;
+      Al[buf][swz(srow0, sgrp)] = lo;
+      split8(ra2, ra3, hi, lo);
+      Ah[buf][swz(srow1, sgrp)] = hi;
+      Al[buf][swz(srow1, sgrp)] = lo;
+      Bh[buf][swz(srow0, sgrp)] = rbh0;
+      Bh[buf][swz(srow1, sgrp)] = rbh1;
+      Bl[buf][swz(srow0, sgrp)] = rbl0;
+      Bl[buf][swz(srow1, sgrp)] = rbl1;
    };
 
    const int nchunk = a.K / KC2;
@@ -524,30 +617,68 @@ __global__ __launch_bounds__(256, 2) void k_gsym_bf16x3(GsymArgs a, GsymArgs16 b
          bh[i] = __builtin_bit_cast(bf8, Bh[buf][swz(col, g)]);
          bl[i] = __builtin_bit_cast(bf8, Bl[buf][swz(col, g)]);
       }
+      // small products first; product-major order keeps consecutive MFMAs independent
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-         for (int j = 0; j < 4; ++j) {
+         for (int j = 0; j < 4; ++j)
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+         for (int j = 0; j < 4; ++j)
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+         for (int j = 0; j < 4; ++j)
             acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-         }
       if (c + 1 < nchunk) lstore(buf ^ 1);
       __syncthreads();
    }
 
    // epilogue: f32-output 16x16 MFMAs hold D[4(l/16)+t][l%16]
+   epi_wave<4, 4>(a, x, diag, R * T2 + wr * 64, Q * T2 + wc * 64, lane, false,
+                  [&](int b, double (&g)[4]) {
+                     f4v v;
+                     switch (b) {       // uniform branch: constant register indices
+#define CASE_(k) case k: v = acc[k / 4][k % 4]; break;
+                     CASE_(0) CASE_(1) CASE_(2) CASE_(3) CASE_(4) CASE_(5) CASE_(6) CASE_(7)
+                     CASE_(8) CASE_(9) CASE_(10) CASE_(11) CASE_(12) CASE_(13) CASE_(14)
+#undef CASE_
+                     default: v = acc[3][3]; break;
+                     }
 #pragma unroll
-   for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-         const int r0 = R * T2 + wr * 64 + i * 16;
-         const int q0 = Q * T2 + wc * 64 + j * 16;
-         double gv[4];
-#pragma unroll
-         for (int t = 0; t < 4; ++t) gv[t] = (double)acc[i][j][t];
-         epi_block(a, x, diag, r0, q0, lane, gv, false);
+                     for (int t = 0; t < 4; ++t) g[t] = (double)v[t];
+                  });
+}
+
+// A table [K][ldU] (FP64) -> row operand [rows][ldk] FP32 and column operand
+// [rows][ldk] split into bf16 hi/lo.  32x32 tiles transposed through LDS.
+__global__ void k_split_table(const double *__restrict__ U, long ldU, int K, int rows, long ldk,
+                              float *__restrict__ UR, uint16_t *__restrict__ Ch,
+                              uint16_t *__restrict__ Cl)
+{
+   __shared__ float tile[32][33];
+   const int r0 = blockIdx.x * 32, k0 = blockIdx.y * 32;
+   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;     // 256 threads: 32 x 8
+   for (int j = ty; j < 32; j += 8) {
+      const int k = k0 + j, r = r0 + tx;
+      tile[j][tx] = (k < K && r < rows) ? (float)U[(size_t)k * ldU + r] : 0.f;
+   }
+   __syncthreads();
+   for (int j = ty; j < 32; j += 8) {
+      const int r = r0 + j, k = k0 + tx;
+      if (r < rows && k < ldk) {
+         const float v = tile[tx][j];
+         const __bf16 h = (__bf16)v;
+         const __bf16 l = (__bf16)(v - (float)h);
+         const size_t o = (size_t)r * ldk + k;
+         UR[o] = v;
+         Ch[o] = __builtin_bit_cast(uint16_t, h);
+         Cl[o] = __builtin_bit_cast(uint16_t, l);
       }
+   }
 }
 
 // FP64 -> FP32 copy of the A table
