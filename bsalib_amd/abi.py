@@ -49,7 +49,8 @@ _lib = None
 
 EXPORTS = ["bsahip_init", "bsahip_finalise", "bsahip_last_error", "bsahip_version",
            "bsahip_classic_run", "bsahip_classic_spectra", "bsahip_bfm_points",
-           "bsahip_last_kernel_time"]
+           "bsahip_last_kernel_time", "bsahip_plan_create", "bsahip_plan_run",
+           "bsahip_plan_destroy"]
 
 
 def lib():
@@ -61,6 +62,10 @@ def lib():
                                     "(the HIP library is required, there is no CPU path)")
         L = C.CDLL(str(LIB_PATH))
         L.bsahip_last_error.restype = C.c_char_p
+        L.bsahip_plan_create.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_int32,
+                                         C.POINTER(C.c_void_p)]
+        L.bsahip_plan_run.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 5
+        L.bsahip_plan_destroy.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -168,3 +173,39 @@ def last_kernel_ms():
     n = C.c_int32(0)
     _check(lib().bsahip_last_kernel_time(C.byref(ms), C.byref(n)))
     return ms.value, n.value
+
+
+class Plan:
+    """A Classic run resident on the device (bsahip_plan_*): create once, run
+    many times.  The problem's host arrays are only read by the constructor."""
+
+    def __init__(self, prob, quad: str = "vect", prec: str = "fp64", bound: Bound = None,
+                 psd: bool = True, bisp: bool = True):
+        b = bound or Bound(prob)
+        self.nm = b.nm
+        self.modes = b.modes
+        flags = PREC[prec] | QUAD[quad] | (0 if psd else NO_PSD) | (0 if bisp else NO_BISP)
+        h = C.c_void_p()
+        _check(lib().bsahip_plan_create(C.addressof(b.struct), C.c_int32(prob.nfreqs),
+                                        C.c_double(prob.df), C.c_int32(flags), C.byref(h)))
+        self.h = h
+
+    def run(self) -> dict:
+        nm = self.nm
+        out = {k: np.zeros(nm * nm) for k in ("m2mf", "m2mr", "m2o2mr")}
+        out.update({k: np.zeros(nm ** 3) for k in ("m3mf", "m3mr")})
+        _check(lib().bsahip_plan_run(self.h, *(_ptr(out[k], C.c_double) for k in
+                                               ("m2mf", "m2mr", "m2o2mr", "m3mf", "m3mr"))))
+        out["modes"] = self.modes
+        return out
+
+    def close(self):
+        if self.h:
+            lib().bsahip_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass

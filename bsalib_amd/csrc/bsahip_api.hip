@@ -1,10 +1,11 @@
 // C ABI of libbsahip.so (declarations and layouts: include/bsahip.h).
 //
-// Host side of the MI355X BFM path: validates the reference-layout inputs,
-// builds the device tables (Psi per K index, ln rho, PSD parameters, FRF),
-// launches the kernels of kernels.hip on the library stream and copies the
-// moments back.  There is no CPU fallback: without a usable GPU every entry
-// point returns BSAHIP_EDEVICE.
+// Host side of the MI355X BFM path.  A plan holds one structure resident on
+// the device (Psi per K index, ln rho, PSD parameters, modal data) and every
+// buffer its Classic run needs; bsahip_plan_run is the per-step hot path
+// (nodal PSD -> projection A(f) -> [bf16 split] -> G contraction with the
+// plane epilogue -> modal PSD moments -> copy of the moments).  There is no
+// CPU fallback: without a usable GPU every entry point returns BSAHIP_EDEVICE.
 #include "kernels.hip"
 #include "../../include/bsahip.h"
 #include "bsahip_internal.h"
@@ -14,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -41,9 +43,38 @@ int fail(int code, const std::string &msg)
       if (r_ != 0) return r_;                                                                \
    } while (0)
 
-struct Buf {
-   void *p = nullptr;
-   size_t n = 0;
+// Named, grow-only device buffers.
+struct Bufs {
+   struct Buf {
+      void *p = nullptr;
+      size_t n = 0;
+   };
+   std::map<std::string, Buf> m;
+   ~Bufs() { release(); }
+   void release()
+   {
+      for (auto &kv : m)
+         if (kv.second.p) (void)hipFree(kv.second.p);
+      m.clear();
+   }
+   template <class T>
+   int get(const char *name, size_t count, T **out)
+   {
+      Buf &b = m[name];
+      const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+      if (b.n < bytes) {
+         if (b.p) (void)hipFree(b.p);
+         b.p = nullptr;
+         b.n = 0;
+         hipError_t e = hipMalloc(&b.p, bytes);
+         if (e != hipSuccess)
+            return fail(BSAHIP_ENOMEM, std::string("hipMalloc(") + name + ", " +
+                                           std::to_string(bytes) + " B): " + hipGetErrorString(e));
+         b.n = bytes;
+      }
+      *out = static_cast<T *>(b.p);
+      return 0;
+   }
 };
 
 struct Ctx {
@@ -51,7 +82,7 @@ struct Ctx {
    int dev = 0;
    hipStream_t s = nullptr;
    hipEvent_t e0 = nullptr, e1 = nullptr;
-   std::map<std::string, Buf> bufs;
+   Bufs *bufs = nullptr;                 // scratch of the one-shot entry points
    double last_ms = 0.0;
    int last_launches = 0;
    std::recursive_mutex mu;
@@ -60,29 +91,11 @@ struct Ctx {
 Ctx g;
 
 template <class T>
-int dbuf(const char *name, size_t count, T **out)
+int upload(Bufs &B, const char *name, const std::vector<T> &h, T **out)
 {
-   Buf &b = g.bufs[name];
-   const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
-   if (b.n < bytes) {
-      if (b.p) (void)hipFree(b.p);
-      b.p = nullptr;
-      b.n = 0;
-      hipError_t e = hipMalloc(&b.p, bytes);
-      if (e != hipSuccess)
-         return fail(BSAHIP_ENOMEM, std::string("hipMalloc(") + name + ", " +
-                                        std::to_string(bytes) + " B): " + hipGetErrorString(e));
-      b.n = bytes;
-   }
-   *out = static_cast<T *>(b.p);
-   return 0;
-}
-
-template <class T>
-int upload(const char *name, const std::vector<T> &h, T **out)
-{
-   CHK(dbuf(name, h.size(), out));
-   if (!h.empty()) HIPC(hipMemcpyAsync(*out, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, g.s));
+   CHK(B.get(name, h.size(), out));
+   if (!h.empty())
+      HIPC(hipMemcpyAsync(*out, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, g.s));
    return 0;
 }
 
@@ -101,25 +114,7 @@ int ensure_init()
    return bsahip_init(0) == 0 ? 0 : BSAHIP_EDEVICE;
 }
 
-inline long corr_index(long ni, long nj, long tot)
-{
-   if (nj >= ni) return (ni - 1) * tot + nj - (ni * ni - ni) / 2;
-   return (nj - 1) * tot + ni - (nj * nj - nj) / 2;
-}
-
 inline int round_up(int a, int b) { return (a + b - 1) / b * b; }
-
-// Host-prepared, device-resident description of one structure.
-struct Prep {
-   int nn, nnl, nnlp, nm, ntc, K;
-   std::vector<double> pa_t;   // [tc][m][nnl]
-   std::vector<double> pa_k;   // [m][K]
-   std::vector<double> pb_k;   // [m][K]
-   std::vector<double> lnr;    // [tc][nnl(j)][nnl(i)]
-   std::vector<double> prm;    // [tc][nnl][4]
-   std::vector<int> formula;   // [tc]
-   std::vector<double> Mm, Km, Cm;
-};
 
 int validate(const bsahip_problem *p)
 {
@@ -146,108 +141,120 @@ int validate(const bsahip_problem *p)
    return 0;
 }
 
+// Device-resident description of one structure.
+struct Prob {
+   int nn = 0, nnl = 0, nnlp = 0, nm = 0, ntc = 0, K = 0, pulsation = 1;
+   double *pa_t = nullptr;   // [tc][m][nnl]
+   double *pa_k = nullptr;   // [m][K]
+   double *pb_k = nullptr;   // [m][K]
+   float *pb32 = nullptr;    // [m][K]
+   double *lnr = nullptr;    // [tc][nnl(j)][nnl(i)]
+   double *prm = nullptr;    // [tc][nnl][4]
+   int *formula = nullptr;   // [tc]
+   double *Mm = nullptr, *Km = nullptr, *Cm = nullptr;
+};
+
 // psd_mode 0: reference Classic/Mesher semantics (evalPSD per component);
 // psd_mode 1: BsaCL semantics (u formula for every component, zone 1, rho clamped).
-int prepare(const bsahip_problem *p, Prep &P, int psd_mode = 0)
+int load_problem(const bsahip_problem *p, int psd_mode, Bufs &B, Prob &D)
 {
    CHK(validate(p));
-   P.nn = p->nn;
-   P.nnl = p->nnl;
-   P.nm = p->nm;
-   P.ntc = p->ntc;
-   P.nnlp = round_up(p->nnl, KC);
-   P.K = P.ntc * P.nnlp;
-   const int nm = P.nm, nnl = P.nnl, K = P.K;
-   P.pa_t.assign((size_t)P.ntc * nm * nnl, 0.0);
-   P.pa_k.assign((size_t)nm * K, 0.0);
-   P.pb_k.assign((size_t)nm * K, 0.0);
-   for (int c = 0; c < P.ntc; ++c) {
+   D.nn = p->nn;
+   D.nnl = p->nnl;
+   D.nm = p->nm;
+   D.ntc = p->ntc;
+   D.nnlp = round_up(p->nnl, KC);
+   D.K = D.ntc * D.nnlp;
+   D.pulsation = p->pulsation;
+   const int nm = D.nm, nnl = D.nnl, K = D.K;
+   std::vector<double> pa_t((size_t)D.ntc * nm * nnl, 0.0), pa_k((size_t)nm * K, 0.0),
+      pb_k((size_t)nm * K, 0.0);
+   for (int c = 0; c < D.ntc; ++c) {
       const int t = p->tc[c];
       for (int i = 0; i < nnl; ++i)
          for (int m = 0; m < nm; ++m) {
             const double a = p->psi[m + (size_t)nm * (i + (size_t)nnl * (t - 1))];
             const double b = p->psi[m + (size_t)nm * (i + (size_t)nnl * (t + 2))];
-            P.pa_t[((size_t)c * nm + m) * nnl + i] = a;
-            P.pa_k[(size_t)m * K + (size_t)c * P.nnlp + i] = a;
-            P.pb_k[(size_t)m * K + (size_t)c * P.nnlp + i] = b;
+            pa_t[((size_t)c * nm + m) * nnl + i] = a;
+            pa_k[(size_t)m * K + (size_t)c * D.nnlp + i] = a;
+            pb_k[(size_t)m * K + (size_t)c * D.nnlp + i] = b;
          }
    }
-   P.lnr.assign((size_t)P.ntc * nnl * nnl, 0.0);
-   for (int c = 0; c < P.ntc; ++c) {
-      const int col = (p->corr_col_mode == 1 || psd_mode == 1) ? 1 : p->tc[c];
-      const double *rc = p->nod_corr + (size_t)(col - 1) * p->ncorr_rows;
-      double *out = P.lnr.data() + (size_t)c * nnl * nnl;
-      for (int j = 0; j < nnl; ++j)
-         for (int i = 0; i < nnl; ++i) {
-            double r = rc[corr_index(p->n_load[j], p->n_load[i], p->nn) - 1];
-            if (psd_mode == 1 && r < 2.2250738585072014e-308) r = 2.2250738585072014e-308;
-            out[(size_t)j * nnl + i] = std::log(r);
-         }
-   }
-   P.prm.assign((size_t)P.ntc * nnl * 4, 0.0);
-   P.formula.assign(P.ntc, 0);
-   for (int c = 0; c < P.ntc; ++c) {
+   std::vector<float> pb32(pb_k.begin(), pb_k.end());
+   // coherence columns actually read (nod_corr_(:, tc), or column 1 for all)
+   std::vector<int> col(D.ntc);
+   for (int c = 0; c < D.ntc; ++c)
+      col[c] = (p->corr_col_mode == 1 || psd_mode == 1) ? 0 : p->tc[c] - 1;
+   std::vector<int> ucol = col;
+   std::sort(ucol.begin(), ucol.end());
+   ucol.erase(std::unique(ucol.begin(), ucol.end()), ucol.end());
+   std::vector<int> colmap(D.ntc);
+   for (int c = 0; c < D.ntc; ++c)
+      colmap[c] = (int)(std::lower_bound(ucol.begin(), ucol.end(), col[c]) - ucol.begin());
+   std::vector<double> corr((size_t)ucol.size() * p->ncorr_rows);
+   for (size_t k = 0; k < ucol.size(); ++k)
+      std::memcpy(corr.data() + k * p->ncorr_rows, p->nod_corr + (size_t)ucol[k] * p->ncorr_rows,
+                  sizeof(double) * p->ncorr_rows);
+   std::vector<int> nl(p->n_load, p->n_load + nnl);
+   std::vector<double> prm((size_t)D.ntc * nnl * 4, 0.0);
+   std::vector<int> formula(D.ntc, 0);
+   for (int c = 0; c < D.ntc; ++c) {
       const int t = p->tc[c];
       int f = 0;
       if (psd_mode == 1) f = (p->psd_id == 5) ? 5 : (p->psd_id == 1 ? 1 : 0);
       else if (p->psd_id == 1) f = (t == 1) ? 1 : 2;
       else f = (p->psd_id == 2) ? 0 : p->psd_id;
-      P.formula[c] = f;
+      formula[c] = f;
       for (int i = 0; i < nnl; ++i) {
          const int node = p->n_load[i] - 1;
          const int wz = (psd_mode == 1) ? 0 : p->wz_node[node] - 1;
          // L(component, direction 1, zone): von Karman u reads component 1
          // (WindPSDImpl.F90:167), the other formulas component tc.
          const int lc = (psd_mode == 0 && f == 1) ? 1 : t;
-         double *q = P.prm.data() + ((size_t)c * nnl + i) * 4;
+         double *q = prm.data() + ((size_t)c * nnl + i) * 4;
          q[0] = p->turb_scales[(lc - 1) + 9 * (size_t)wz];
          q[1] = p->turb_std[(t - 1) + 3 * (size_t)wz];
          q[2] = p->u_node[node];
          q[3] = p->u_mean_ref ? p->u_mean_ref[wz] : 1.0;
       }
    }
-   P.Mm.assign(p->Mm, p->Mm + nm);
-   P.Km.assign(p->Km, p->Km + nm);
-   P.Cm.assign(p->Cm, p->Cm + nm);
-   return 0;
-}
-
-struct DevProb {
-   double *pa_t, *pa_k, *pb_k, *lnr, *prm, *Mm, *Km, *Cm;
-   int *formula;
-};
-
-int upload_prep(const Prep &P, DevProb &D)
-{
-   CHK(upload("pa_t", P.pa_t, &D.pa_t));
-   CHK(upload("pa_k", P.pa_k, &D.pa_k));
-   CHK(upload("pb_k", P.pb_k, &D.pb_k));
-   CHK(upload("lnr", P.lnr, &D.lnr));
-   CHK(upload("prm", P.prm, &D.prm));
-   CHK(upload("formula", P.formula, &D.formula));
-   CHK(upload("Mm", P.Mm, &D.Mm));
-   CHK(upload("Km", P.Km, &D.Km));
-   CHK(upload("Cm", P.Cm, &D.Cm));
-   return 0;
-}
-
-// A table on a list of |f| values: U[K][ldU], rows y*rows_per_mode + q.
-int build_table(const Prep &P, const DevProb &D, const std::vector<double> &fabs_list,
-                int rows_per_mode, int pulsation, double **U_out, long ldU)
-{
-   const int nfl = (int)fabs_list.size();
-   if (nfl > 65535) return fail(BSAHIP_EUNSUPPORTED, "more than 65535 distinct frequencies in one table");
-   double *fl, *sq, *U;
-   CHK(upload("fabs", fabs_list, &fl));
-   CHK(dbuf("sq", (size_t)P.ntc * nfl * P.nnl, &sq));
-   CHK(dbuf("U", (size_t)P.K * ldU, &U));
-   CHK(fill(U, (size_t)P.K * ldU, 0.0));
-   dim3 gp((P.nnl + 63) / 64, nfl, P.ntc);
-   k_psd_sqrt<<<gp, 64, 0, g.s>>>(fl, nfl, D.prm, D.formula, P.nnl, P.ntc, pulsation, sq);
+   std::vector<double> Mm(p->Mm, p->Mm + nm), Km(p->Km, p->Km + nm), Cm(p->Cm, p->Cm + nm);
+   double *dcorr;
+   int *dcol, *dnl;
+   CHK(upload(B, "pa_t", pa_t, &D.pa_t));
+   CHK(upload(B, "pa_k", pa_k, &D.pa_k));
+   CHK(upload(B, "pb_k", pb_k, &D.pb_k));
+   CHK(upload(B, "pb32", pb32, &D.pb32));
+   CHK(upload(B, "corr", corr, &dcorr));
+   CHK(upload(B, "colmap", colmap, &dcol));
+   CHK(upload(B, "n_load", nl, &dnl));
+   CHK(upload(B, "prm", prm, &D.prm));
+   CHK(upload(B, "formula", formula, &D.formula));
+   CHK(upload(B, "Mm", Mm, &D.Mm));
+   CHK(upload(B, "Km", Km, &D.Km));
+   CHK(upload(B, "Cm", Cm, &D.Cm));
+   CHK(B.get("lnr", (size_t)D.ntc * nnl * nnl, &D.lnr));
+   dim3 gl((nnl + 255) / 256, nnl, D.ntc);
+   k_lnrho<<<gl, 256, 0, g.s>>>(dcorr, p->ncorr_rows, dcol, dnl, nnl, p->nn,
+                                psd_mode == 1 ? 2.2250738585072014e-308 : 0.0, D.lnr);
    HIPC(hipGetLastError());
-   const int nq = (P.nm + 3) / 4;
+   return 0;
+}
+
+// A table on a list of |f| values (device array fl): U[K][ldU], rows y*rows_per_mode + q.
+int build_table(Bufs &B, const Prob &D, const double *fl, int nfl, int rows_per_mode, double *U,
+                long ldU)
+{
+   if (nfl > 65535) return fail(BSAHIP_EUNSUPPORTED, "more than 65535 distinct frequencies in one table");
+   double *sq;
+   CHK(B.get("sq", (size_t)D.ntc * nfl * D.nnl, &sq));
+   CHK(fill(U, (size_t)D.K * ldU, 0.0));
+   dim3 gp((D.nnl + 63) / 64, nfl, D.ntc);
+   k_psd_sqrt<<<gp, 64, 0, g.s>>>(fl, nfl, D.prm, D.formula, D.nnl, D.ntc, D.pulsation, sq);
+   HIPC(hipGetLastError());
+   const int nq = (D.nm + 3) / 4;
 #define LAUNCH_PROJ(NQ)                                                                       \
-   k_project<NQ><<<gp, 256, 0, g.s>>>(D.pa_t, D.lnr, sq, fl, nfl, P.nnl, P.nm, P.nnlp,        \
+   k_project<NQ><<<gp, 256, 0, g.s>>>(D.pa_t, D.lnr, sq, fl, nfl, D.nnl, D.nm, D.nnlp,        \
                                       rows_per_mode, U, ldU)
    if (nq <= 1) LAUNCH_PROJ(1);
    else if (nq <= 2) LAUNCH_PROJ(2);
@@ -256,15 +263,13 @@ int build_table(const Prep &P, const DevProb &D, const std::vector<double> &fabs
    else LAUNCH_PROJ(16);
 #undef LAUNCH_PROJ
    HIPC(hipGetLastError());
-   *U_out = U;
    return 0;
 }
 
-int frf_table(const Prep &P, const DevProb &D, int nft, double df, double2 **H)
+int launch_frf(const Prob &D, int nft, double df, double2 *H)
 {
-   CHK(dbuf("H", (size_t)P.nm * nft, H));
-   dim3 gh((nft + 255) / 256, P.nm);
-   k_frf_table<<<gh, 256, 0, g.s>>>(D.Mm, D.Km, D.Cm, P.nm, nft, df, *H);
+   dim3 gh((nft + 255) / 256, D.nm);
+   k_frf_table<<<gh, 256, 0, g.s>>>(D.Mm, D.Km, D.Cm, D.nm, nft, df, H);
    HIPC(hipGetLastError());
    return 0;
 }
@@ -279,11 +284,172 @@ int check_flags(int32_t flags, bool grid, bool allow_mixed)
    return 0;
 }
 
+// Distinct |f| over f1, f2, f1+f2 of a point list and the index maps into it.
+void point_tables(int npts, const double *fi, const double *fj, std::vector<double> &uq,
+                  std::vector<int> &i1, std::vector<int> &i2, std::vector<int> &i3)
+{
+   std::vector<double> all;
+   all.reserve(3 * (size_t)npts);
+   for (int q = 0; q < npts; ++q) {
+      all.push_back(std::fabs(fi[q]));
+      all.push_back(std::fabs(fj[q]));
+      all.push_back(std::fabs(fi[q] + fj[q]));
+   }
+   uq = all;
+   std::sort(uq.begin(), uq.end());
+   uq.erase(std::unique(uq.begin(), uq.end()), uq.end());
+   auto idx = [&](double v) { return (int)(std::lower_bound(uq.begin(), uq.end(), v) - uq.begin()); };
+   i1.resize(npts);
+   i2.resize(npts);
+   i3.resize(npts);
+   for (int q = 0; q < npts; ++q) {
+      i1[q] = idx(all[3 * q]);
+      i2[q] = idx(all[3 * q + 1]);
+      i3[q] = idx(all[3 * q + 2]);
+   }
+}
+
+}  // namespace
+
+// A Classic run of one structure on one grid, resident on the device.
+struct bsahip_plan {
+   Bufs B;
+   Prob D;
+   int32_t flags = 0, prec = 0;
+   int nh = 0, nf = 0, nu = 0, nhp = 0, rows = 0, ntile = 0, nft = 0;
+   long ldU = 0;
+   double df = 0.0, d = 0.0;
+   double *fl = nullptr, *U = nullptr, *m3 = nullptr, *m2 = nullptr;
+   double2 *H = nullptr;
+   float *UR = nullptr;
+   uint16_t *Ch = nullptr, *Cl = nullptr;
+   double last_ms = 0.0;
+};
+
+namespace {
+
+int plan_setup(bsahip_plan *P, const bsahip_problem *p, int32_t nfh, double df, int32_t flags,
+               int psd_mode)
+{
+   CHK(check_flags(flags, true, true));
+   if (nfh < 2 || !(df > 0.0)) return fail(BSAHIP_EINVAL, "need nfh >= 2 and df > 0");
+   CHK(load_problem(p, psd_mode, P->B, P->D));
+   P->flags = flags;
+   P->prec = flags & BSAHIP_PREC_MASK;
+   P->df = df;
+   P->nh = nfh;
+   P->nf = 2 * nfh - 1;
+   P->nu = (flags & BSAHIP_SUM_EXACT) ? P->nf : P->nh;          // rows of A per mode
+   P->nhp = round_up(P->nu, 16);
+   P->rows = P->D.nm * P->nhp;
+   const int tile = (P->prec == BSAHIP_PREC_BF16X3) ? T2 : TILE;
+   P->ntile = (P->rows + tile - 1) / tile;
+   P->ldU = (long)round_up(P->rows, T2);
+   P->nft = 2 * P->nu - 1;
+   P->d = p->pulsation ? df * 2.0 * M_PI : df;
+   const long npairs = (long)P->ntile * (P->ntile + 1) / 2;
+   if (npairs * P->D.nm > 0x7fffffffL) return fail(BSAHIP_EUNSUPPORTED, "grid too large for one launch");
+   std::vector<double> fl(P->nu);
+   for (int u = 0; u < P->nu; ++u) fl[u] = (double)u * df;
+   CHK(upload(P->B, "fl", fl, &P->fl));
+   CHK(P->B.get("U", (size_t)P->D.K * P->ldU, &P->U));
+   CHK(P->B.get("H", (size_t)P->D.nm * P->nft, &P->H));
+   const size_t nm3 = (size_t)P->D.nm * P->D.nm * P->D.nm, nm2 = (size_t)P->D.nm * P->D.nm;
+   CHK(P->B.get("m3", 2 * nm3, &P->m3));
+   CHK(P->B.get("m2", 3 * nm2, &P->m2));
+   if (P->prec == BSAHIP_PREC_BF16X3) {
+      const size_t n = (size_t)P->ldU * P->D.K;
+      CHK(P->B.get("UR32", n, &P->UR));
+      CHK(P->B.get("UCh", n, &P->Ch));
+      CHK(P->B.get("UCl", n, &P->Cl));
+   }
+   HIPC(hipStreamSynchronize(g.s));
+   return 0;
+}
+
+int plan_exec(bsahip_plan *P, double *m2mf, double *m2mr, double *m2o2mr, double *m3mf, double *m3mr)
+{
+   Prob &D = P->D;
+   const size_t nm3 = (size_t)D.nm * D.nm * D.nm, nm2 = (size_t)D.nm * D.nm;
+   const bool want_psd = !(P->flags & BSAHIP_NO_PSD);
+   const bool want_bisp = !(P->flags & BSAHIP_NO_BISP);
+   CHK(build_table(P->B, D, P->fl, P->nu, P->nhp, P->U, P->ldU));
+   CHK(launch_frf(D, P->nft, P->df, P->H));
+   P->last_ms = 0.0;
+   if (want_bisp) {
+      CHK(fill(P->m3, 2 * nm3, 0.0));
+      GsymArgs a;
+      a.U = P->U;
+      a.ldU = P->ldU;
+      a.pb = D.pb_k;
+      a.K = D.K;
+      a.nm = D.nm;
+      a.nhp = P->nhp;
+      a.nh = P->nh;
+      a.nu = P->nu;
+      a.ntile = P->ntile;
+      a.npairs = (long)P->ntile * (P->ntile + 1) / 2;
+      a.H = P->H;
+      a.nft = P->nft;
+      a.d2 = P->d * P->d;
+      a.rule = (P->flags & BSAHIP_QUAD_MASK) >> 4;
+      a.m3f = P->m3;
+      a.m3r = P->m3 + nm3;
+      const long nwg = a.npairs * D.nm;
+      if (P->prec == BSAHIP_PREC_BF16X3) {
+         dim3 gs((unsigned)(P->ldU / 32), (unsigned)((D.K + 31) / 32));
+         k_split_table<<<gs, 256, 0, g.s>>>(P->U, P->ldU, D.K, (int)P->ldU, D.K, P->UR, P->Ch, P->Cl);
+         HIPC(hipGetLastError());
+         GsymArgs16 b16;
+         b16.UR = P->UR;
+         b16.UCh = reinterpret_cast<const uint4 *>(P->Ch);
+         b16.UCl = reinterpret_cast<const uint4 *>(P->Cl);
+         b16.pb32 = D.pb32;
+         b16.ldk = D.K;
+         HIPC(hipEventRecord(g.e0, g.s));
+         k_gsym_bf16x3<<<dim3((unsigned)nwg), 256, 0, g.s>>>(a, b16);
+      } else {
+         HIPC(hipEventRecord(g.e0, g.s));
+         k_gsym_f64<<<dim3((unsigned)nwg), 256, 0, g.s>>>(a);
+      }
+      HIPC(hipGetLastError());
+      HIPC(hipEventRecord(g.e1, g.s));
+      if (m3mf) HIPC(hipMemcpyAsync(m3mf, P->m3, nm3 * sizeof(double), hipMemcpyDeviceToHost, g.s));
+      if (m3mr) HIPC(hipMemcpyAsync(m3mr, P->m3 + nm3, nm3 * sizeof(double), hipMemcpyDeviceToHost, g.s));
+   }
+   if (want_psd) {
+      k_m2<<<D.nm * D.nm, 256, 0, g.s>>>(P->U, P->ldU, D.pa_k, D.K, D.nm, P->nhp, P->nh, P->H,
+                                          P->nft, P->d, 2.0 * M_PI * P->df, P->m2, P->m2 + nm2,
+                                          P->m2 + 2 * nm2);
+      HIPC(hipGetLastError());
+      if (m2mf) HIPC(hipMemcpyAsync(m2mf, P->m2, nm2 * sizeof(double), hipMemcpyDeviceToHost, g.s));
+      if (m2mr) HIPC(hipMemcpyAsync(m2mr, P->m2 + nm2, nm2 * sizeof(double), hipMemcpyDeviceToHost, g.s));
+      if (m2o2mr) HIPC(hipMemcpyAsync(m2o2mr, P->m2 + 2 * nm2, nm2 * sizeof(double), hipMemcpyDeviceToHost, g.s));
+   }
+   HIPC(hipStreamSynchronize(g.s));
+   g.last_ms = 0.0;
+   g.last_launches = 0;
+   if (want_bisp) {
+      float ms = 0.f;
+      HIPC(hipEventElapsedTime(&ms, g.e0, g.e1));
+      P->last_ms = ms;
+      g.last_ms = ms;
+      g.last_launches = 1;
+   }
+   return 0;
+}
+
+Bufs &scratch()
+{
+   if (!g.bufs) g.bufs = new Bufs();
+   return *g.bufs;
+}
+
 }  // namespace
 
 extern "C" {
 
-int bsahip_version(void) { return 100; }
+int bsahip_version(void) { return 110; }
 
 const char *bsahip_last_error(void) { return g_err.c_str(); }
 
@@ -310,9 +476,8 @@ int bsahip_finalise(void)
    std::lock_guard<std::recursive_mutex> lk(g.mu);
    if (!g.ready) return 0;
    (void)hipStreamSynchronize(g.s);
-   for (auto &kv : g.bufs)
-      if (kv.second.p) (void)hipFree(kv.second.p);
-   g.bufs.clear();
+   delete g.bufs;
+   g.bufs = nullptr;
    (void)hipEventDestroy(g.e0);
    (void)hipEventDestroy(g.e1);
    (void)hipStreamDestroy(g.s);
@@ -320,38 +485,42 @@ int bsahip_finalise(void)
    return 0;
 }
 
-int bsahip_selftest_mfma_layout(double *out512)
-{
-   std::lock_guard<std::recursive_mutex> lk(g.mu);
-   CHK(ensure_init());
-   if (!out512) return fail(BSAHIP_EINVAL, "NULL output");
-   double *d;
-   CHK(dbuf("selftest", 512, &d));
-   k_mfma_f64_layout<<<1, 64, 0, g.s>>>(d);
-   HIPC(hipGetLastError());
-   HIPC(hipMemcpyAsync(out512, d, 512 * sizeof(double), hipMemcpyDeviceToHost, g.s));
-   HIPC(hipStreamSynchronize(g.s));
-   return 0;
-}
-
-int bsahip_selftest_mfma_bf16_layout(float *out768)
-{
-   std::lock_guard<std::recursive_mutex> lk(g.mu);
-   CHK(ensure_init());
-   if (!out768) return fail(BSAHIP_EINVAL, "NULL output");
-   float *d;
-   CHK(dbuf("selftest16", 768, &d));
-   k_mfma_bf16_layout<<<1, 64, 0, g.s>>>(d);
-   HIPC(hipGetLastError());
-   HIPC(hipMemcpyAsync(out768, d, 768 * sizeof(float), hipMemcpyDeviceToHost, g.s));
-   HIPC(hipStreamSynchronize(g.s));
-   return 0;
-}
-
 int bsahip_last_kernel_time(double *ms, int32_t *launches)
 {
    if (ms) *ms = g.last_ms;
    if (launches) *launches = g.last_launches;
+   return 0;
+}
+
+int bsahip_plan_create(const bsahip_problem *p, int32_t nfh, double df, int32_t flags,
+                       bsahip_plan **out)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   if (!out) return fail(BSAHIP_EINVAL, "NULL plan output");
+   *out = nullptr;
+   CHK(ensure_init());
+   std::unique_ptr<bsahip_plan> P(new bsahip_plan());
+   CHK(plan_setup(P.get(), p, nfh, df, flags, 0));
+   *out = P.release();
+   return 0;
+}
+
+int bsahip_plan_run(bsahip_plan *plan, double *m2mf, double *m2mr, double *m2o2mr,
+                    double *m3mf, double *m3mr)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   if (!plan) return fail(BSAHIP_EINVAL, "NULL plan");
+   CHK(ensure_init());
+   return plan_exec(plan, m2mf, m2mr, m2o2mr, m3mf, m3mr);
+}
+
+int bsahip_plan_destroy(bsahip_plan *plan)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   if (plan) {
+      if (g.ready) (void)hipStreamSynchronize(g.s);
+      delete plan;
+   }
    return 0;
 }
 
@@ -367,106 +536,9 @@ int bsahip__classic_moments(const bsahip_problem *p, int32_t nfh, double df, int
 {
    std::lock_guard<std::recursive_mutex> lk(g.mu);
    CHK(ensure_init());
-   CHK(check_flags(flags, true, true));
-   if (nfh < 2 || !(df > 0.0)) return fail(BSAHIP_EINVAL, "need nfh >= 2 and df > 0");
-   const int prec = flags & BSAHIP_PREC_MASK;
-   Prep P;
-   CHK(prepare(p, P, psd_mode));
-   DevProb D;
-   CHK(upload_prep(P, D));
-   const bool exact = (flags & BSAHIP_SUM_EXACT) != 0;
-   const int nh = nfh, nf = 2 * nh - 1;
-   const int nu = exact ? nf : nh;                 // rows of A per mode
-   const int nhp = round_up(nu, 16);
-   const int rows = P.nm * nhp;
-   const int tile = (prec == BSAHIP_PREC_BF16X3) ? T2 : TILE;
-   const int ntile = (rows + tile - 1) / tile;
-   const long ldU = (long)round_up(rows, T2);
-   const int nft = 2 * nu - 1;
-   std::vector<double> fl(nu);
-   for (int u = 0; u < nu; ++u) fl[u] = (double)u * df;
-   double *U;
-   CHK(build_table(P, D, fl, nhp, p->pulsation, &U, ldU));
-   double2 *H;
-   CHK(frf_table(P, D, nft, df, &H));
-   const double d = p->pulsation ? df * 2.0 * M_PI : df;
-   const bool want_psd = !(flags & BSAHIP_NO_PSD);
-   const bool want_bisp = !(flags & BSAHIP_NO_BISP);
-   const size_t nm3 = (size_t)P.nm * P.nm * P.nm;
-   g.last_ms = 0.0;
-   g.last_launches = 0;
-   if (want_bisp) {
-      double *m3;
-      CHK(dbuf("m3", 2 * nm3, &m3));
-      CHK(fill(m3, 2 * nm3, 0.0));
-      GsymArgs a;
-      a.U = U;
-      a.ldU = ldU;
-      a.pb = D.pb_k;
-      a.K = P.K;
-      a.nm = P.nm;
-      a.nhp = nhp;
-      a.nh = nh;
-      a.nu = nu;
-      a.ntile = ntile;
-      a.npairs = (long)ntile * (ntile + 1) / 2;
-      a.H = H;
-      a.nft = nft;
-      a.d2 = d * d;
-      a.rule = (flags & BSAHIP_QUAD_MASK) >> 4;
-      a.m3f = m3;
-      a.m3r = m3 + nm3;
-      const long nwg = a.npairs * P.nm;
-      if (nwg > 0x7fffffffL) return fail(BSAHIP_EUNSUPPORTED, "grid too large for one launch");
-      if (prec == BSAHIP_PREC_BF16X3) {
-         GsymArgs16 b16;
-         const long ldk = P.K;                       // multiple of KC2
-         const int rowsP = (int)ldU;                 // ntile * 128
-         float *UR, *pb32;
-         uint16_t *Ch, *Cl;
-         CHK(dbuf("UR32", (size_t)rowsP * ldk, &UR));
-         CHK(dbuf("UCh", (size_t)rowsP * ldk, &Ch));
-         CHK(dbuf("UCl", (size_t)rowsP * ldk, &Cl));
-         std::vector<float> pbf(P.pb_k.begin(), P.pb_k.end());
-         CHK(upload("pb32", pbf, &pb32));
-         dim3 gs((rowsP + 31) / 32, (unsigned)((ldk + 31) / 32));
-         k_split_table<<<gs, 256, 0, g.s>>>(U, ldU, P.K, rowsP, ldk, UR, Ch, Cl);
-         HIPC(hipGetLastError());
-         b16.UR = UR;
-         b16.UCh = reinterpret_cast<const uint4 *>(Ch);
-         b16.UCl = reinterpret_cast<const uint4 *>(Cl);
-         b16.pb32 = pb32;
-         b16.ldk = ldk;
-         HIPC(hipEventRecord(g.e0, g.s));
-         k_gsym_bf16x3<<<dim3((unsigned)nwg), 256, 0, g.s>>>(a, b16);
-      } else {
-         HIPC(hipEventRecord(g.e0, g.s));
-         k_gsym_f64<<<dim3((unsigned)nwg), 256, 0, g.s>>>(a);
-      }
-      HIPC(hipGetLastError());
-      HIPC(hipEventRecord(g.e1, g.s));
-      if (m3mf) HIPC(hipMemcpyAsync(m3mf, m3, nm3 * sizeof(double), hipMemcpyDeviceToHost, g.s));
-      if (m3mr) HIPC(hipMemcpyAsync(m3mr, m3 + nm3, nm3 * sizeof(double), hipMemcpyDeviceToHost, g.s));
-   }
-   if (want_psd) {
-      double *m2;
-      const size_t nm2 = (size_t)P.nm * P.nm;
-      CHK(dbuf("m2", 3 * nm2, &m2));
-      k_m2<<<P.nm * P.nm, 256, 0, g.s>>>(U, ldU, D.pa_k, P.K, P.nm, nhp, nh, H, nft, d,
-                                         2.0 * M_PI * df, m2, m2 + nm2, m2 + 2 * nm2);
-      HIPC(hipGetLastError());
-      if (m2mf) HIPC(hipMemcpyAsync(m2mf, m2, nm2 * sizeof(double), hipMemcpyDeviceToHost, g.s));
-      if (m2mr) HIPC(hipMemcpyAsync(m2mr, m2 + nm2, nm2 * sizeof(double), hipMemcpyDeviceToHost, g.s));
-      if (m2o2mr) HIPC(hipMemcpyAsync(m2o2mr, m2 + 2 * nm2, nm2 * sizeof(double), hipMemcpyDeviceToHost, g.s));
-   }
-   HIPC(hipStreamSynchronize(g.s));
-   if (want_bisp) {
-      float ms = 0.f;
-      HIPC(hipEventElapsedTime(&ms, g.e0, g.e1));
-      g.last_ms = ms;
-      g.last_launches = 1;
-   }
-   return 0;
+   bsahip_plan P;
+   CHK(plan_setup(&P, p, nfh, df, flags, psd_mode));
+   return plan_exec(&P, m2mf, m2mr, m2o2mr, m3mf, m3mr);
 }
 
 int bsahip__points_sum(const bsahip_problem *p, int32_t npts, const double *fi, const double *fj,
@@ -476,50 +548,37 @@ int bsahip__points_sum(const bsahip_problem *p, int32_t npts, const double *fi, 
    CHK(ensure_init());
    if (npts < 0 || (npts > 0 && (!fi || !fj || !w || !m3)))
       return fail(BSAHIP_EINVAL, "npts < 0 or NULL point/weight/output arrays");
-   Prep P;
-   CHK(prepare(p, P, psd_mode));
-   DevProb D;
-   CHK(upload_prep(P, D));
-   const size_t nm3 = (size_t)P.nm * P.nm * P.nm;
+   Bufs &B = scratch();
+   Prob D;
+   CHK(load_problem(p, psd_mode, B, D));
+   const size_t nm3 = (size_t)D.nm * D.nm * D.nm;
    if (npts == 0) {
       std::fill(m3, m3 + nm3, 0.0);
       return 0;
    }
-   std::vector<double> all;
-   all.reserve(3 * (size_t)npts);
-   for (int q = 0; q < npts; ++q) {
-      all.push_back(std::fabs(fi[q]));
-      all.push_back(std::fabs(fj[q]));
-      all.push_back(std::fabs(fi[q] + fj[q]));
-   }
-   std::vector<double> uq = all;
-   std::sort(uq.begin(), uq.end());
-   uq.erase(std::unique(uq.begin(), uq.end()), uq.end());
-   auto idx = [&](double v) { return (int)(std::lower_bound(uq.begin(), uq.end(), v) - uq.begin()); };
-   std::vector<int> i1(npts), i2(npts), i3(npts);
-   for (int q = 0; q < npts; ++q) {
-      i1[q] = idx(all[3 * q]);
-      i2[q] = idx(all[3 * q + 1]);
-      i3[q] = idx(all[3 * q + 2]);
-   }
+   std::vector<double> uq;
+   std::vector<int> i1, i2, i3;
+   point_tables(npts, fi, fj, uq, i1, i2, i3);
    const int nu = (int)uq.size();
-   const long ldU = (long)P.nm * nu;
-   double *U;
-   CHK(build_table(P, D, uq, nu, p->pulsation, &U, ldU));
+   const long ldU = (long)D.nm * nu;
+   double *fl, *U;
+   CHK(upload(B, "fl", uq, &fl));
+   CHK(B.get("U", (size_t)D.K * ldU, &U));
+   CHK(build_table(B, D, fl, nu, nu, U, ldU));
    std::vector<double> f1(fi, fi + npts), f2(fj, fj + npts), ww(w, w + npts);
    int *di1, *di2, *di3;
    double *df1, *df2, *dw, *out;
-   CHK(upload("i1", i1, &di1));
-   CHK(upload("i2", i2, &di2));
-   CHK(upload("i3", i3, &di3));
-   CHK(upload("f1", f1, &df1));
-   CHK(upload("f2", f2, &df2));
-   CHK(upload("wpt", ww, &dw));
-   CHK(dbuf("m3sum", nm3, &out));
+   CHK(upload(B, "i1", i1, &di1));
+   CHK(upload(B, "i2", i2, &di2));
+   CHK(upload(B, "i3", i3, &di3));
+   CHK(upload(B, "f1", f1, &df1));
+   CHK(upload(B, "f2", f2, &df2));
+   CHK(upload(B, "wpt", ww, &dw));
+   CHK(B.get("m3sum", nm3, &out));
    CHK(fill(out, nm3, 0.0));
    const int nchunk = (int)std::min<long>(1024, std::max<long>(1, npts / 16));
    dim3 gs((unsigned)((nm3 + 255) / 256), nchunk);
-   k_bfm_points_sum<<<gs, 256, 0, g.s>>>(U, ldU, nu, D.pb_k, P.K, P.nm, npts, di1, di2, di3,
+   k_bfm_points_sum<<<gs, 256, 0, g.s>>>(U, ldU, nu, D.pb_k, D.K, D.nm, npts, di1, di2, di3,
                                          df1, df2, dw, D.Mm, D.Km, D.Cm, with_frf, out);
    HIPC(hipGetLastError());
    HIPC(hipMemcpyAsync(m3, out, nm3 * sizeof(double), hipMemcpyDeviceToHost, g.s));
@@ -534,27 +593,29 @@ int bsahip_classic_spectra(const bsahip_problem *p, int32_t nfh, double df, int3
    CHK(ensure_init());
    CHK(check_flags(flags, true, false));
    if (nfh < 2 || !(df > 0.0)) return fail(BSAHIP_EINVAL, "need nfh >= 2 and df > 0");
-   Prep P;
-   CHK(prepare(p, P));
-   DevProb D;
-   CHK(upload_prep(P, D));
+   Bufs &B = scratch();
+   Prob D;
+   CHK(load_problem(p, 0, B, D));
    const int nh = nfh, nf = 2 * nh - 1, c = nh - 1;
    const bool exact = (flags & BSAHIP_SUM_EXACT) != 0;
    const int nu = exact ? nf : nh;
-   const long ldU = (long)P.nm * nu;
+   const long ldU = (long)D.nm * nu;
    std::vector<double> fl(nu);
    for (int u = 0; u < nu; ++u) fl[u] = (double)u * df;
-   double *U;
-   CHK(build_table(P, D, fl, nu, p->pulsation, &U, ldU));
+   double *dfl, *U;
    double2 *H;
-   CHK(frf_table(P, D, nf, df, &H));
-   const size_t nm2 = (size_t)P.nm * P.nm, nm3 = nm2 * P.nm;
+   CHK(upload(B, "fl", fl, &dfl));
+   CHK(B.get("U", (size_t)D.K * ldU, &U));
+   CHK(build_table(B, D, dfl, nu, nu, U, ldU));
+   CHK(B.get("H", (size_t)D.nm * nf, &H));
+   CHK(launch_frf(D, nf, df, H));
+   const size_t nm2 = (size_t)D.nm * D.nm, nm3 = nm2 * D.nm;
    if (psd_f || psd_r) {
       double *pf, *pr;
-      CHK(dbuf("psd_f", nm2 * nf, &pf));
-      CHK(dbuf("psd_r", nm2 * nf, &pr));
+      CHK(B.get("psd_f", nm2 * nf, &pf));
+      CHK(B.get("psd_r", nm2 * nf, &pr));
       dim3 gq((nf + 255) / 256, (unsigned)nm2);
-      k_psd_grid<<<gq, 256, 0, g.s>>>(U, ldU, D.pa_k, P.K, P.nm, nu, nf, c, H, nf, pf, pr);
+      k_psd_grid<<<gq, 256, 0, g.s>>>(U, ldU, D.pa_k, D.K, D.nm, nu, nf, c, H, nf, pf, pr);
       HIPC(hipGetLastError());
       if (psd_f) HIPC(hipMemcpyAsync(psd_f, pf, nm2 * nf * sizeof(double), hipMemcpyDeviceToHost, g.s));
       if (psd_r) HIPC(hipMemcpyAsync(psd_r, pr, nm2 * nf * sizeof(double), hipMemcpyDeviceToHost, g.s));
@@ -575,16 +636,16 @@ int bsahip_classic_spectra(const bsahip_problem *p, int32_t nfh, double df, int3
          }
       int *di1, *di2, *di3;
       double *df1, *df2, *ob, *orr;
-      CHK(upload("i1", i1, &di1));
-      CHK(upload("i2", i2, &di2));
-      CHK(upload("i3", i3, &di3));
-      CHK(upload("f1", f1, &df1));
-      CHK(upload("f2", f2, &df2));
-      CHK(dbuf("bfm_out", nm3 * npts, &ob));
-      CHK(dbuf("brm_out", nm3 * npts, &orr));
+      CHK(upload(B, "i1", i1, &di1));
+      CHK(upload(B, "i2", i2, &di2));
+      CHK(upload(B, "i3", i3, &di3));
+      CHK(upload(B, "f1", f1, &df1));
+      CHK(upload(B, "f2", f2, &df2));
+      CHK(B.get("bfm_out", nm3 * npts, &ob));
+      CHK(B.get("brm_out", nm3 * npts, &orr));
       const long tot = (long)nm3 * npts;
       k_bfm_points<<<(unsigned)((tot + 255) / 256), 256, 0, g.s>>>(
-         U, ldU, nu, D.pb_k, P.K, P.nm, (int)npts, di1, di2, di3, df1, df2, D.Mm, D.Km, D.Cm,
+         U, ldU, nu, D.pb_k, D.K, D.nm, (int)npts, di1, di2, di3, df1, df2, D.Mm, D.Km, D.Cm,
          ob, orr, 1, npts);
       HIPC(hipGetLastError());
       if (bfm) HIPC(hipMemcpyAsync(bfm, ob, nm3 * npts * sizeof(double), hipMemcpyDeviceToHost, g.s));
@@ -603,50 +664,64 @@ int bsahip_bfm_points(const bsahip_problem *p, int32_t npts, const double *fi, c
    if (npts < 0 || (npts > 0 && (!fi || !fj || !bfm)))
       return fail(BSAHIP_EINVAL, "npts < 0 or NULL point/output arrays");
    if (npts == 0) return 0;
-   Prep P;
-   CHK(prepare(p, P));
-   DevProb D;
-   CHK(upload_prep(P, D));
-   // distinct |f| over f1, f2, f1+f2 of every point
-   std::vector<double> all;
-   all.reserve(3 * (size_t)npts);
-   for (int q = 0; q < npts; ++q) {
-      all.push_back(std::fabs(fi[q]));
-      all.push_back(std::fabs(fj[q]));
-      all.push_back(std::fabs(fi[q] + fj[q]));
-   }
-   std::vector<double> uq = all;
-   std::sort(uq.begin(), uq.end());
-   uq.erase(std::unique(uq.begin(), uq.end()), uq.end());
-   auto idx = [&](double v) { return (int)(std::lower_bound(uq.begin(), uq.end(), v) - uq.begin()); };
-   std::vector<int> i1(npts), i2(npts), i3(npts);
-   std::vector<double> f1(fi, fi + npts), f2(fj, fj + npts);
-   for (int q = 0; q < npts; ++q) {
-      i1[q] = idx(all[3 * q]);
-      i2[q] = idx(all[3 * q + 1]);
-      i3[q] = idx(all[3 * q + 2]);
-   }
+   Bufs &B = scratch();
+   Prob D;
+   CHK(load_problem(p, 0, B, D));
+   std::vector<double> uq;
+   std::vector<int> i1, i2, i3;
+   point_tables(npts, fi, fj, uq, i1, i2, i3);
    const int nu = (int)uq.size();
-   const long ldU = (long)P.nm * nu;
-   double *U;
-   CHK(build_table(P, D, uq, nu, p->pulsation, &U, ldU));
+   const long ldU = (long)D.nm * nu;
+   double *dfl, *U;
+   CHK(upload(B, "fl", uq, &dfl));
+   CHK(B.get("U", (size_t)D.K * ldU, &U));
+   CHK(build_table(B, D, dfl, nu, nu, U, ldU));
+   std::vector<double> f1(fi, fi + npts), f2(fj, fj + npts);
    int *di1, *di2, *di3;
    double *df1, *df2, *ob, *orr = nullptr;
-   CHK(upload("i1", i1, &di1));
-   CHK(upload("i2", i2, &di2));
-   CHK(upload("i3", i3, &di3));
-   CHK(upload("f1", f1, &df1));
-   CHK(upload("f2", f2, &df2));
-   const size_t nm3 = (size_t)P.nm * P.nm * P.nm;
-   CHK(dbuf("bfm_out", nm3 * npts, &ob));
-   if (brm) CHK(dbuf("brm_out", nm3 * npts, &orr));
+   CHK(upload(B, "i1", i1, &di1));
+   CHK(upload(B, "i2", i2, &di2));
+   CHK(upload(B, "i3", i3, &di3));
+   CHK(upload(B, "f1", f1, &df1));
+   CHK(upload(B, "f2", f2, &df2));
+   const size_t nm3 = (size_t)D.nm * D.nm * D.nm;
+   CHK(B.get("bfm_out", nm3 * npts, &ob));
+   if (brm) CHK(B.get("brm_out", nm3 * npts, &orr));
    const long tot = (long)nm3 * npts;
    k_bfm_points<<<(unsigned)((tot + 255) / 256), 256, 0, g.s>>>(
-      U, ldU, nu, D.pb_k, P.K, P.nm, npts, di1, di2, di3, df1, df2, D.Mm, D.Km, D.Cm, ob, orr,
+      U, ldU, nu, D.pb_k, D.K, D.nm, npts, di1, di2, di3, df1, df2, D.Mm, D.Km, D.Cm, ob, orr,
       (long)nm3, 1);
    HIPC(hipGetLastError());
    HIPC(hipMemcpyAsync(bfm, ob, nm3 * npts * sizeof(double), hipMemcpyDeviceToHost, g.s));
    if (brm) HIPC(hipMemcpyAsync(brm, orr, nm3 * npts * sizeof(double), hipMemcpyDeviceToHost, g.s));
+   HIPC(hipStreamSynchronize(g.s));
+   return 0;
+}
+
+int bsahip_selftest_mfma_layout(double *out512)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   CHK(ensure_init());
+   if (!out512) return fail(BSAHIP_EINVAL, "NULL output");
+   double *d;
+   CHK(scratch().get("selftest", 512, &d));
+   k_mfma_f64_layout<<<1, 64, 0, g.s>>>(d);
+   HIPC(hipGetLastError());
+   HIPC(hipMemcpyAsync(out512, d, 512 * sizeof(double), hipMemcpyDeviceToHost, g.s));
+   HIPC(hipStreamSynchronize(g.s));
+   return 0;
+}
+
+int bsahip_selftest_mfma_bf16_layout(float *out768)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   CHK(ensure_init());
+   if (!out768) return fail(BSAHIP_EINVAL, "NULL output");
+   float *d;
+   CHK(scratch().get("selftest16", 768, &d));
+   k_mfma_bf16_layout<<<1, 64, 0, g.s>>>(d);
+   HIPC(hipGetLastError());
+   HIPC(hipMemcpyAsync(out768, d, 768 * sizeof(float), hipMemcpyDeviceToHost, g.s));
    HIPC(hipStreamSynchronize(g.s));
    return 0;
 }

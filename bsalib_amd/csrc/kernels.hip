@@ -79,6 +79,30 @@ __global__ void k_psd_sqrt(const double *__restrict__ fabs_list, int nfl,
    sq[((size_t)tc * nfl + q) * nnl + i] = sqrt(S);
 }
 
+// ln rho_ji for the loaded nodes from the packed coherence vector
+// (util_getCorrVectIndex, src/utils/utility.F90:60-79): lnr[c][j][i].
+// col[c] is the 0-based nod_corr column of component c; clamp > 0 replaces
+// rho < clamp by clamp (BsaCL convention, bsacl.cl:243-245).
+__global__ void k_lnrho(const double *__restrict__ corr, long ncorr_rows, const int *__restrict__ col,
+                        const int *__restrict__ n_load, int nnl, long nn, double clamp,
+                        double *__restrict__ lnr)
+{
+   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+   const int j = blockIdx.y;
+   const int c = blockIdx.z;
+   if (i >= nnl || j >= nnl) return;
+   long a = n_load[j], b = n_load[i];
+   if (a > b) {
+      const long t = a;
+      a = b;
+      b = t;
+   }
+   const long id = (a - 1) * nn + b - (a * a - a) / 2;
+   double r = corr[(size_t)col[c] * ncorr_rows + (id - 1)];
+   if (clamp > 0.0 && r < clamp) r = clamp;
+   lnr[((size_t)c * nnl + j) * nnl + i] = log(r);
+}
+
 // Modal FRF table H[m*nft + q] = 1/(K - w^2 M + i w C), w = 2 pi q df, q >= 0
 // (functionsImpl.F90:1641-1651).  Negative frequencies use conj.
 __global__ void k_frf_table(const double *__restrict__ Mm, const double *__restrict__ Km,

@@ -103,6 +103,22 @@ int bsahip_classic_run(const bsahip_problem *p, int32_t nfh, double df, int32_t 
                        double *m3mf, double *m3mr);
 
 /*
+ * Plans: the same Classic run with the structure resident on the device.
+ * bsahip_plan_create validates and uploads the problem (the host arrays may
+ * be released afterwards), computes ln rho on the device and allocates every
+ * buffer of the run; bsahip_plan_run executes one full pass (nodal PSD,
+ * projection, node contraction and plane quadrature, modal PSD moments) and
+ * copies the moments back; outputs as bsahip_classic_run.  No reference
+ * counterpart: the reference recomputes its tables inside every bsa_Run.
+ */
+typedef struct bsahip_plan bsahip_plan;
+int bsahip_plan_create(const bsahip_problem *p, int32_t nfh, double df, int32_t flags,
+                       bsahip_plan **plan);
+int bsahip_plan_run(bsahip_plan *plan, double *m2mf, double *m2mr, double *m2o2mr,
+                    double *m3mf, double *m3mr);
+int bsahip_plan_destroy(bsahip_plan *plan);
+
+/*
  * Bispectra on the Classic grid in the reference layout bisp(NF, NF, NM^3):
  * bfm = getFM_full_tnm_vect_cls_ output, brm = after getRM_full_vect_cls_
  * (functionsImpl.F90:1312-1738).  psd_f/psd_r: (NF, NM^2) modal PSDs.

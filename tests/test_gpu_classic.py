@@ -109,3 +109,16 @@ def test_mode_subset_consistency_full_size(gpu_lib):
     # force third moment is symmetric under (n,p) swap: B_mnp(f1,f2) = B_mpn(f2,f1)
     assert rel_err(Ff, np.transpose(Ff, (0, 2, 1))) < TOL
     assert np.all(np.isfinite(full["m3mr"])) and np.all(np.isfinite(full["m2mr"]))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+def test_plan_runs_repeat_and_match_one_shot(gpu_lib, prec):
+    p = CASES["config1"]()
+    one = gpu_lib.classic_run(p, quad="vect", prec=prec)
+    plan = gpu_lib.Plan(p, quad="vect", prec=prec)
+    a = plan.run()
+    b = plan.run()
+    plan.close()
+    for k in ("m2mf", "m2mr", "m2o2mr", "m3mf", "m3mr"):
+        assert rel_err(a[k], one[k]) < 1e-14, k
+        assert rel_err(b[k], a[k]) < 1e-14, k
