@@ -1,0 +1,86 @@
+!! Thin iso_c_binding layer over libbsahip.so (C declarations: include/bsahip.h).
+!!
+!! bsahip_problem_t is interoperable with the C struct bsahip_problem; the
+!! interfaces bind the C entry points by name.  Nothing here computes.
+module BsaHip_C
+   use, intrinsic :: iso_c_binding
+   implicit none
+   private
+
+   integer(c_int32_t), parameter, public :: BSAHIP_PREC_FP64 = 0
+   integer(c_int32_t), parameter, public :: BSAHIP_PREC_BF16X3 = 2
+   integer(c_int32_t), parameter, public :: BSAHIP_QUAD_VECT = 0
+   integer(c_int32_t), parameter, public :: BSAHIP_QUAD_RIEMANN = 16
+   integer(c_int32_t), parameter, public :: BSAHIP_QUAD_TRAPZ = 32
+   integer(c_int32_t), parameter, public :: BSAHIP_SUM_EXACT = 256
+   integer(c_int32_t), parameter, public :: BSAHIP_NO_PSD = 512
+   integer(c_int32_t), parameter, public :: BSAHIP_NO_BISP = 1024
+
+   type, bind(c), public :: bsahip_problem_t
+      integer(c_int32_t) :: nn = 0, nnl = 0, nm = 0, ntc = 0, ndegw = 0, nwz = 0
+      integer(c_int32_t) :: psd_id = 0, pulsation = 1, ncorr_rows = 0, corr_col_mode = 0
+      type(c_ptr) :: n_load = c_null_ptr, tc = c_null_ptr, psi = c_null_ptr
+      type(c_ptr) :: nod_corr = c_null_ptr, u_node = c_null_ptr, wz_node = c_null_ptr
+      type(c_ptr) :: turb_scales = c_null_ptr, turb_std = c_null_ptr, u_mean_ref = c_null_ptr
+      type(c_ptr) :: Mm = c_null_ptr, Km = c_null_ptr, Cm = c_null_ptr
+   end type bsahip_problem_t
+
+   public :: bsahip_init, bsahip_finalise, bsahip_classic_run, bsahip_bfm_points
+   public :: bsahip_last_error_f
+
+   interface
+      integer(c_int) function bsahip_init(device) bind(c, name="bsahip_init")
+         import c_int
+         integer(c_int), value :: device
+      end function
+
+      integer(c_int) function bsahip_finalise() bind(c, name="bsahip_finalise")
+         import c_int
+      end function
+
+      integer(c_int) function bsahip_classic_run(p, nfh, df, flags, m2mf, m2mr, m2o2mr, m3mf, m3mr) &
+            bind(c, name="bsahip_classic_run")
+         import c_int, c_int32_t, c_double, c_ptr, bsahip_problem_t
+         type(bsahip_problem_t), intent(in) :: p
+         integer(c_int32_t), value :: nfh, flags
+         real(c_double), value :: df
+         type(c_ptr), value :: m2mf, m2mr, m2o2mr, m3mf, m3mr
+      end function
+
+      integer(c_int) function bsahip_bfm_points(p, npts, fi, fj, flags, bfm, brm) &
+            bind(c, name="bsahip_bfm_points")
+         import c_int, c_int32_t, c_ptr, bsahip_problem_t
+         type(bsahip_problem_t), intent(in) :: p
+         integer(c_int32_t), value :: npts, flags
+         type(c_ptr), value :: fi, fj, bfm, brm
+      end function
+
+      type(c_ptr) function bsahip_last_error_c() bind(c, name="bsahip_last_error")
+         import c_ptr
+      end function
+   end interface
+
+contains
+
+   !> Message of the last failed libbsahip call on this thread.
+   function bsahip_last_error_f() result(msg)
+      character(len=:), allocatable :: msg
+      character(kind=c_char), pointer :: s(:)
+      type(c_ptr) :: p
+      integer :: n
+      p = bsahip_last_error_c()
+      msg = ''
+      if (.not. c_associated(p)) return
+      call c_f_pointer(p, s, [4096])
+      n = 0
+      do while (n < 4096)
+         if (s(n + 1) == c_null_char) exit
+         n = n + 1
+      end do
+      allocate(character(len=n) :: msg)
+      do n = 1, len(msg)
+         msg(n:n) = s(n)
+      end do
+   end function
+
+end module BsaHip_C

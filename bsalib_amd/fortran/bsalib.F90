@@ -1,0 +1,630 @@
+!! BsaLib front for the GPU BFM path: module BsaLib with the reference's
+!! bsa_Init / bsa_Run / bsa_Finalise and the setters that feed the Classic
+!! third-moment computation, with the reference's names, argument meanings and
+!! abort behaviour (src/BsaLib.F90, src/bsa/BsaLibImpl.F90).  All O(NF^2)
+!! work runs in libbsahip.so on the GPU (bsahip_classic_run); this module only
+!! collects the inputs, does the reference's O(NDOFS*NM) host preparation
+!! (mode validation, Phi*C projection, frequency-grid checks) and hands plain
+!! arrays to the C ABI.  There is no CPU path: without a GPU bsa_Run aborts.
+!!
+!! Differences from the reference, all deliberate:
+!!  * Inputs are copied at the setter call (the reference keeps pointers to
+!!    the caller's arrays); L(3,3,*) and sigma(3,*) are assumed-size in the
+!!    reference interface, so those two are kept as addresses and copied at
+!!    bsa_Run once the number of wind zones is known.
+!!  * The number of wind zones is taken from bsa_setWindZoneLimits when it
+!!    was called (as the reference), else from bsa_setWZMeanWindVel, else
+!!    from max(NodWZ).
+!!  * Only the Classic approach is computed.  bsa_enableGPU forces it, as
+!!    the reference's GPU path does (BsaLibImpl.F90:347-353); without it an
+!!    analysis type other than 1 aborts.  Unlike that GPU path, PSDs are
+!!    still computed when requested.
+!!  * bsa_setComputePrecision is an addition: BSA_PREC_BF16X3 selects the
+!!    split-bf16x3 node contraction (FP32 accumulation on matrix cores,
+!!    FP64 everywhere else); the default is FP64.
+module BsaLib
+   use, intrinsic :: iso_fortran_env, only: int32, real64, output_unit
+   use, intrinsic :: iso_c_binding
+   use BsaHip_C
+   implicit none
+   private
+
+   integer, parameter, public :: bsa_int_t = int32
+   integer, parameter, public :: bsa_real_t = real64
+
+   integer(bsa_int_t), parameter, public :: BSA_SPATIAL_SYM_NONE = 0
+   integer(bsa_int_t), parameter, public :: BSA_SPATIAL_SYM_HALF = 2
+   integer(bsa_int_t), parameter, public :: BSA_SPATIAL_SYM_FOUR = 4
+   integer(bsa_int_t), parameter, public :: BSA_CLASSIC_MODE_VECTOR = 0
+   integer(bsa_int_t), parameter, public :: BSA_CLASSIC_MODE_SCALAR = 1
+   integer(bsa_int_t), parameter, public :: BSA_PSD_CONVENTION_FREQ = 0
+   integer(bsa_int_t), parameter, public :: BSA_PSD_CONVENTION_PULS = 1
+   integer(bsa_int_t), parameter, public :: BSA_WIND_PSD_VONKARMAN = 1
+   integer(bsa_int_t), parameter, public :: BSA_WIND_PSD_KAIMAL = 2
+   integer(bsa_int_t), parameter, public :: BSA_WIND_PSD_DAVENPORT = 5
+   integer(bsa_int_t), parameter, public :: BSA_PREC_FP64 = 0
+   integer(bsa_int_t), parameter, public :: BSA_PREC_BF16X3 = 2
+
+   character(len=*), parameter :: INFOMSG = '  --[info]   '
+   character(len=*), parameter :: WARNMSG = '  --[warn]   '
+   character(len=*), parameter :: ERRMSG = '  --[error]  '
+
+   public :: bsa_Init, bsa_Run, bsa_Finalise, bsa_isCleaned, bsa_Abort
+   public :: bsa_enableGPU, bsa_setAnalysisType, bsa_setClassicMode, bsa_setScalingConv
+   public :: bsa_setSpectraComputation, bsa_setSpectraExtension, bsa_setTestMode
+   public :: bsa_setSpatialSymmetry, bsa_doValidateModalData, bsa_setupClassic
+   public :: bsa_setWindDirections, bsa_setWindTurbComps, bsa_setPSDType, bsa_setWindZoneLimits
+   public :: bsa_setWZMeanWindVel, bsa_setTurbWindScales, bsa_setTurbWindSDT
+   public :: bsa_setNodalVel, bsa_setNodalWindZones, bsa_setSpatialNodalCorr
+   public :: bsa_setWindFCoeffs, bsa_setPhitimesC
+   public :: bsa_setTotalNumOfDOFs, bsa_setNumOfNodalDOFs, bsa_setTotalNOfNodes
+   public :: bsa_setLoadedNodalDOFs, bsa_setLoadedNodes, bsa_setModalInfo
+   public :: bsa_setKeptModalShapes, bsa_setModalMatrices, bsa_setTotDamping
+   public :: bsa_getUsedModeShapes, bsa_setComputePrecision
+
+   ! ---- settings (defaults as src/settings/Settings.F90:33-84) ----
+   integer(bsa_int_t) :: isuban_ = 2, iclassic_ = 0, iconv_ = 1
+   integer(bsa_int_t) :: icomp_psd_ = 1, icomp_bisp_ = 1, ionly_diag_ = 0, itest_ = 0
+   integer(bsa_int_t) :: isym_ = 0, nfreqs_ = 0, iprec_ = BSA_PREC_FP64
+   real(bsa_real_t) :: df_ = 0
+   logical :: validate_modal_ = .true., gpu_enabled_ = .false.
+   logical :: cleaned_ = .true., device_ready_ = .false.
+
+   ! ---- structure ----
+   integer(bsa_int_t) :: nn_ = 0, nlibs_ = 0, ndofs_ = 0, nm_ = 0
+   integer(bsa_int_t), allocatable, target :: n_load_(:), libs_load_(:), modes_user_(:), modes_(:)
+   real(bsa_real_t), allocatable :: phi_(:, :), natf_(:), xsi_(:), Cg_(:, :)
+   real(bsa_real_t), allocatable :: Mg_(:), Kg_(:)
+
+   ! ---- wind ----
+   integer(bsa_int_t) :: ipsd_ = 0, nz_lim_ = 0
+   integer(bsa_int_t), allocatable, target :: tc_(:), dirs_(:), wz_node_(:)
+   real(bsa_real_t), allocatable, target :: u_node_(:), umref_(:), corr_(:, :), wfc_(:, :, :)
+   real(bsa_real_t), allocatable, target :: psi_user_(:, :, :)
+   type(c_ptr) :: tscl_addr_ = c_null_ptr, tstd_addr_ = c_null_ptr
+
+contains
+
+   ! ======================================================================
+   !  general
+   ! ======================================================================
+
+   !> Prints emsg and stops (src/bsa/data/BsaLibDataImpl.F90:132).
+   subroutine bsa_Abort(emsg)
+      character(len=*), intent(in), optional :: emsg
+      if (present(emsg)) then
+         write(output_unit, '(1x, 2a)') ERRMSG, emsg
+      else
+         write(output_unit, '(1x, 2a)') ERRMSG, 'BsaLib aborted.'
+      end if
+      flush(output_unit)
+      error stop 1
+   end subroutine
+
+   !> src/bsa/BsaLibImpl.F90:61-87 (allocates the global singletons there).
+   subroutine bsa_Init()
+      cleaned_ = .false.
+   end subroutine
+
+   !> src/bsa/BsaLibImpl.F90:683-685: releases every input and the device.
+   subroutine bsa_Finalise()
+      integer(c_int) :: rc
+      if (allocated(n_load_)) deallocate(n_load_)
+      if (allocated(libs_load_)) deallocate(libs_load_)
+      if (allocated(modes_user_)) deallocate(modes_user_)
+      if (allocated(modes_)) deallocate(modes_)
+      if (allocated(phi_)) deallocate(phi_)
+      if (allocated(natf_)) deallocate(natf_)
+      if (allocated(xsi_)) deallocate(xsi_)
+      if (allocated(Mg_)) deallocate(Mg_)
+      if (allocated(Kg_)) deallocate(Kg_)
+      if (allocated(Cg_)) deallocate(Cg_)
+      if (allocated(tc_)) deallocate(tc_)
+      if (allocated(dirs_)) deallocate(dirs_)
+      if (allocated(wz_node_)) deallocate(wz_node_)
+      if (allocated(u_node_)) deallocate(u_node_)
+      if (allocated(umref_)) deallocate(umref_)
+      if (allocated(corr_)) deallocate(corr_)
+      if (allocated(wfc_)) deallocate(wfc_)
+      if (allocated(psi_user_)) deallocate(psi_user_)
+      tscl_addr_ = c_null_ptr
+      tstd_addr_ = c_null_ptr
+      nn_ = 0; nlibs_ = 0; ndofs_ = 0; nm_ = 0; nz_lim_ = 0; ipsd_ = 0
+      if (device_ready_) rc = bsahip_finalise()
+      device_ready_ = .false.
+      cleaned_ = .true.
+   end subroutine
+
+   logical pure function bsa_isCleaned()
+      bsa_isCleaned = cleaned_
+   end function
+
+   !> Marks the run as GPU-offloaded (src/bsa/BsaLibImpl.F90:54-56); as in
+   !> the reference's GPU path it forces the Classic approach.
+   subroutine bsa_enableGPU()
+      gpu_enabled_ = .true.
+   end subroutine
+
+   !> Extension (no counterpart in the reference): node-contraction precision,
+   !> BSA_PREC_FP64 (default) or BSA_PREC_BF16X3.
+   subroutine bsa_setComputePrecision(iprec)
+      integer(bsa_int_t), value :: iprec
+      if (iprec /= BSA_PREC_FP64 .and. iprec /= BSA_PREC_BF16X3) &
+         call bsa_Abort('Unknown compute precision (use BSA_PREC_FP64 or BSA_PREC_BF16X3).')
+      iprec_ = iprec
+   end subroutine
+
+   ! ======================================================================
+   !  settings (src/settings/SettingsImpl.F90)
+   ! ======================================================================
+
+   subroutine bsa_setAnalysisType(isuban)
+      integer(bsa_int_t), value :: isuban
+      isuban_ = isuban
+   end subroutine
+
+   subroutine bsa_setClassicMode(i_mode)
+      integer(bsa_int_t), value :: i_mode
+      iclassic_ = i_mode
+   end subroutine
+
+   subroutine bsa_setScalingConv(iconv)
+      integer(bsa_int_t), value :: iconv
+      iconv_ = iconv
+   end subroutine
+
+   subroutine bsa_setSpectraComputation(ipsd, ibisp)
+      integer(bsa_int_t), value :: ipsd, ibisp
+      icomp_psd_ = ipsd
+      icomp_bisp_ = ibisp
+   end subroutine
+
+   subroutine bsa_setSpectraExtension(ionlydiag)
+      integer(bsa_int_t), value :: ionlydiag
+      ionly_diag_ = ionlydiag
+   end subroutine
+
+   subroutine bsa_setTestMode(itest)
+      integer(bsa_int_t), value :: itest
+      itest_ = itest
+   end subroutine
+
+   !> Accepted values as src/bsa/BsaLibImpl.F90 (bsa_setSpatialSymmetry); the
+   !> Classic approach does not use it.
+   subroutine bsa_setSpatialSymmetry(isym)
+      integer(bsa_int_t), value :: isym
+      if (isym == BSA_SPATIAL_SYM_NONE .or. isym == BSA_SPATIAL_SYM_HALF .or. &
+          isym == BSA_SPATIAL_SYM_FOUR) then
+         isym_ = isym
+      else
+         write(output_unit, '(1x, 2a, i0)') WARNMSG, 'Unsupported spatial symmetry value ', isym
+         isym_ = BSA_SPATIAL_SYM_NONE
+      end if
+   end subroutine
+
+   subroutine bsa_doValidateModalData(bool)
+      logical, value :: bool
+      validate_modal_ = bool
+   end subroutine
+
+   subroutine bsa_setupClassic(nfreqs, df)
+      integer(bsa_int_t), value :: nfreqs
+      real(bsa_real_t), value :: df
+      nfreqs_ = nfreqs
+      df_ = df
+   end subroutine
+
+   ! ======================================================================
+   !  wind (src/wind/WindSetImpl.F90)
+   ! ======================================================================
+
+   subroutine bsa_setWindDirections(dirs, ndirs)
+      integer(bsa_int_t), intent(in) :: dirs(:)
+      integer(bsa_int_t), value, optional :: ndirs
+      integer(bsa_int_t) :: n
+      n = size(dirs)
+      if (present(ndirs)) n = ndirs
+      dirs_ = dirs(1:n)
+   end subroutine
+
+   subroutine bsa_setWindTurbComps(tc, ntc)
+      integer(bsa_int_t), intent(in) :: tc(:)
+      integer(bsa_int_t), value, optional :: ntc
+      integer(bsa_int_t) :: n
+      n = size(tc)
+      if (present(ntc)) n = ntc
+      tc_ = tc(1:n)
+   end subroutine
+
+   subroutine bsa_setPSDType(ipsd)
+      integer(bsa_int_t), value :: ipsd
+      ipsd_ = ipsd
+   end subroutine
+
+   subroutine bsa_setWindZoneLimits(lim, ilim)
+      real(bsa_real_t), intent(in), target :: lim(:)
+      integer(bsa_int_t), intent(in), optional :: ilim(:)
+      if (nz_lim_ /= 0 .and. nz_lim_ /= size(lim) - 1) &
+         call bsa_Abort('N. of wind zones differs from the one previously set.')
+      nz_lim_ = size(lim) - 1
+   end subroutine
+
+   subroutine bsa_setWZMeanWindVel(mat)
+      real(bsa_real_t), target, intent(in) :: mat(:)
+      umref_ = mat
+   end subroutine
+
+   !> L(3, 3, NWZ): turbulence length scales per wind zone.
+   subroutine bsa_setTurbWindScales(L)
+      real(bsa_real_t), target, intent(in) :: L(3, 3, *)
+      tscl_addr_ = c_loc(L(1, 1, 1))
+   end subroutine
+
+   !> sigma(3, NWZ): turbulence standard deviations per wind zone.
+   subroutine bsa_setTurbWindSDT(sigma)
+      real(bsa_real_t), target, intent(in) :: sigma(3, *)
+      tstd_addr_ = c_loc(sigma(1, 1))
+   end subroutine
+
+   subroutine bsa_setNodalVel(Unod)
+      real(bsa_real_t), target, intent(in) :: Unod(:)
+      u_node_ = Unod
+   end subroutine
+
+   subroutine bsa_setNodalWindZones(NodWZ)
+      integer(bsa_int_t), target, intent(in) :: NodWZ(:)
+      wz_node_ = NodWZ
+   end subroutine
+
+   !> Packed lower triangle ((NN^2+NN)/2, 3) of the nodal coherence, one column
+   !> per turbulence component.
+   subroutine bsa_setSpatialNodalCorr(nodCorr)
+      real(bsa_real_t), target, intent(in) :: nodCorr(:, :)
+      corr_ = nodCorr
+   end subroutine
+
+   !> wfc(NLIBSL, NDEGW, NNL): wind force coefficients.
+   subroutine bsa_setWindFCoeffs(wfc)
+      real(bsa_real_t), target, intent(in) :: wfc(:, :, :)
+      wfc_ = wfc
+   end subroutine
+
+   !> phiTc(NM_EFF, NNL, NDEGW): projected coefficients given by the caller
+   !> (otherwise bsa_Run builds them from Phi and wfc).
+   subroutine bsa_setPhitimesC(phiTc)
+      real(bsa_real_t), target, intent(in) :: phiTc(:, :, :)
+      psi_user_ = phiTc
+   end subroutine
+
+   ! ======================================================================
+   !  structure (src/structure/StructureImpl.F90)
+   ! ======================================================================
+
+   subroutine bsa_setTotalNumOfDOFs(ndofs)
+      integer(bsa_int_t), value :: ndofs
+      ndofs_ = ndofs
+   end subroutine
+
+   subroutine bsa_setNumOfNodalDOFs(nndofs)
+      integer(bsa_int_t), value :: nndofs
+      nlibs_ = nndofs
+   end subroutine
+
+   subroutine bsa_setTotalNOfNodes(nn)
+      integer(bsa_int_t), value :: nn
+      nn_ = nn
+   end subroutine
+
+   subroutine bsa_setLoadedNodalDOFs(libs_l, nlibs_l)
+      integer(bsa_int_t), intent(in), target, allocatable :: libs_l(:)
+      integer(bsa_int_t), value, optional :: nlibs_l
+      integer(bsa_int_t) :: n
+      n = size(libs_l)
+      if (present(nlibs_l)) n = nlibs_l
+      libs_load_ = libs_l(1:n)
+   end subroutine
+
+   subroutine bsa_setLoadedNodes(nodes_l, nn_l)
+      integer(bsa_int_t), intent(in), target, allocatable :: nodes_l(:)
+      integer(bsa_int_t), value, optional :: nn_l
+      integer(bsa_int_t) :: n
+      n = size(nodes_l)
+      if (present(nn_l)) n = nn_l
+      n_load_ = nodes_l(1:n)
+   end subroutine
+
+   subroutine bsa_setModalInfo(ndofs, nm, Phi, natf)
+      integer(bsa_int_t), value :: ndofs, nm
+      real(bsa_real_t), intent(in), target :: Phi(ndofs, nm), natf(nm)
+      if (nm_ /= 0 .and. nm_ /= nm) &
+         call bsa_Abort('You passed a value of "nm" which differs from previously set.')
+      nm_ = nm
+      phi_ = Phi
+      natf_ = natf
+   end subroutine
+
+   subroutine bsa_setKeptModalShapes(modes)
+      integer(bsa_int_t), intent(in) :: modes(:)
+      modes_user_ = modes
+   end subroutine
+
+   !> Generalised mass, stiffness (NM) and damping (NM, NM) matrices; the FRF
+   !> uses the diagonal of Cgen.
+   subroutine bsa_setModalMatrices(nm, Mgen, Kgen, Cgen)
+      integer(bsa_int_t), value :: nm
+      real(bsa_real_t), intent(in), target :: Mgen(nm), Kgen(nm), Cgen(nm, nm)
+      if (nm_ /= 0 .and. nm_ /= nm) &
+         call bsa_Abort('You passed a value of "nm" which differs from previously set.')
+      nm_ = nm
+      Mg_ = Mgen
+      Kg_ = Kgen
+      Cg_ = Cgen
+   end subroutine
+
+   subroutine bsa_setTotDamping(xsi)
+      real(bsa_real_t), intent(in) :: xsi(:)
+      if (nm_ == 0) call bsa_Abort('NM == 0 when setting Damping info.')
+      xsi_ = xsi(1:nm_)
+   end subroutine
+
+   !> Modes used by the last run (after validation).
+   pure function bsa_getUsedModeShapes() result(modes)
+      integer(bsa_int_t), allocatable :: modes(:)
+      if (allocated(modes_)) then
+         modes = modes_
+      else
+         allocate(modes(0))
+      end if
+   end function
+
+   ! ======================================================================
+   !  run
+   ! ======================================================================
+
+   !> Kept modes as validateModalInfo_ (src/bsa/BsaLibImpl.F90:545-599):
+   !> with validation on, modes whose max|phi| is not exactly 1 are dropped
+   !> and, if any is dropped, the kept list is every normalised mode.
+   subroutine selectModes_()
+      real(bsa_real_t) :: mx
+      integer(bsa_int_t) :: i, j, nk
+      logical :: keep(nm_)
+      if (validate_modal_) then
+         do i = 1, nm_
+            mx = maxval(abs(phi_(:, i)))
+            keep(i) = (mx == 1.0_bsa_real_t)
+            if (.not. keep(i)) write(output_unit, '(1x, 2a, i3, a)') &
+               WARNMSG, 'Mode  ', i, '  is not 1-normalised. Ignoring it..'
+         end do
+         nk = count(keep)
+         if (nk < nm_) then
+            allocate(modes_(nk))
+            j = 0
+            do i = 1, nm_
+               if (keep(i)) then
+                  j = j + 1
+                  modes_(j) = i
+               end if
+            end do
+            return
+         end if
+      end if
+      if (allocated(modes_user_)) then
+         modes_ = modes_user_
+      else
+         modes_ = [(i, i = 1, nm_)]
+      end if
+   end subroutine
+
+   !> Phi*C projection as setPhitimesCLocalInstance_ (src/bsa/BsaLibImpl.F90:611-671):
+   !> psi(im, in, id) = sum_l wfc(l, id, in) * phi((n_load(in)-1)*NLIBS + libs_load(l), modes(im)).
+   subroutine buildPsi_(psi)
+      real(bsa_real_t), allocatable, intent(out) :: psi(:, :, :)
+      integer(bsa_int_t) :: nlibl, ndegw, nnl, nme, in, id, im, skip
+      nlibl = size(wfc_, 1)
+      ndegw = size(wfc_, 2)
+      nnl = size(wfc_, 3)
+      nme = size(modes_)
+      if (nnl /= size(n_load_)) call bsa_Abort('N. of loaded nodes does not match.')
+      if (nlibl /= size(libs_load_)) call bsa_Abort('N. of loaded libs  does not match.')
+      if (nme == 0) call bsa_Abort('Effective number of modes to be kept is 0. Validate modal info before this step.')
+      allocate(psi(nme, nnl, ndegw))
+      do in = 1, nnl
+         skip = (n_load_(in) - 1) * nlibs_
+         do id = 1, ndegw
+            do im = 1, nme
+               psi(im, in, id) = sum(wfc_(:, id, in) * phi_(skip + libs_load_, modes_(im)))
+            end do
+         end do
+      end do
+   end subroutine
+
+   !> Frequency grid as computeFreqsVect_ (src/bsa/classic/BsaClassicImpl.F90:437-560):
+   !> outside test mode, df and the number of points are raised so that the
+   !> grid reaches the highest natural frequency with df <= min(xsi*f_nat)/5.
+   !> Returns the number of non-negative frequencies (pulsation convention).
+   subroutine classicGrid_(nfh, df)
+      integer(bsa_int_t), intent(out) :: nfh
+      real(bsa_real_t), intent(out) :: df
+      real(bsa_real_t) :: df_ref, max_freq, max_freq_ref
+      integer(bsa_int_t) :: nf1
+      logical :: df_big
+      if (nfreqs_ == 0 .or. df_ == 0.0_bsa_real_t) call bsa_Abort('Either NFREQs or DF are == 0.')
+      df = df_
+      nf1 = nfreqs_ - 1
+      if (itest_ == 0) then
+         if (.not. allocated(xsi_)) &
+            call bsa_Abort('Damping ratios are needed to check the frequency grid (bsa_setTotDamping).')
+         df_ref = minval(xsi_ * natf_) / 5
+         df_big = df > df_ref
+         max_freq = df * nf1
+         max_freq_ref = maxval(natf_)
+         if (max_freq < max_freq_ref) then
+            if (df_big) then
+               max_freq = df_ref * nf1
+               if (max_freq < max_freq_ref) nf1 = ceiling(max_freq_ref / df_ref)
+               df = df_ref
+            else
+               nf1 = ceiling(max_freq_ref / df)
+            end if
+            write(output_unit, '(1x, 2a, i0, a, es12.5)') INFOMSG, 'Classic grid adjusted: nfreqs= ', &
+               nf1 + 1, '  df= ', df
+         end if
+      end if
+      nfh = nf1 + 1
+   end subroutine
+
+   !> Classic moments on the GPU (the Classic branch of src/bsa/BsaLibImpl.F90:153-462).
+   !> m2* have NM_EFF^2 entries, m3* NM_EFF^3 (first index fastest); the
+   !> Mesher outputs m3mf_msh, m3mr_msh are left unallocated.
+   subroutine bsa_Run(m2mf_cls, m2mr_cls, m2o2mr_cls, m3mf_msh, m3mr_msh, m3mf_cls, m3mr_cls)
+      real(bsa_real_t), target, allocatable, dimension(:) :: &
+         m2mf_cls, m2mr_cls, m2o2mr_cls, m3mf_msh, m3mr_msh, m3mf_cls, m3mr_cls
+      type(bsahip_problem_t) :: p
+      real(bsa_real_t), allocatable, target :: psi(:, :, :), Mk(:), Kk(:), Ck(:)
+      real(bsa_real_t), allocatable, target :: tscl(:, :, :), tstd(:, :), umref(:)
+      integer(bsa_int_t), allocatable, target :: tcs(:)
+      real(bsa_real_t), pointer :: fp3(:, :, :), fp2(:, :)
+      real(bsa_real_t) :: df
+      integer(bsa_int_t) :: nfh, nme, nwz, i
+      integer(c_int32_t) :: flags
+      integer(c_int) :: rc
+      type(c_ptr) :: o_m2mf, o_m2mr, o_m2o2mr, o_m3mf, o_m3mr
+
+      if (icomp_psd_ == 0 .and. icomp_bisp_ == 0) then
+         write(output_unit, '(1x, 2a)') WARNMSG, 'Both  PSD  and  BISP  computation are disabled.'
+         return
+      end if
+      if (isuban_ /= 1) then
+         if (.not. gpu_enabled_) call bsa_Abort('Only the Classic approach (analysis type 1) is '// &
+            'available in this library; call bsa_setAnalysisType(1) or bsa_enableGPU().')
+         isuban_ = 1
+      end if
+      if (ionly_diag_ /= 0) call bsa_Abort('Diagonal-only spectra are not available in this library.')
+      if (iconv_ /= BSA_PSD_CONVENTION_PULS) &
+         call bsa_Abort('Only the pulsation convention (BSA_PSD_CONVENTION_PULS) is available in this library.')
+
+      ! ---- structure checks (validateAll_, BsaLibImpl.F90:468-541) ----
+      if (.not. allocated(phi_)) call bsa_Abort('Modal matrix was not acquired. Aborting.')
+      if (ndofs_ == 0) ndofs_ = nn_ * nlibs_
+      if (ndofs_ /= size(phi_, 1)) call bsa_Abort('NDOFs does not match modal matrix size.')
+      if (.not. allocated(n_load_)) n_load_ = [(i, i = 1, nn_)]
+      if (.not. allocated(libs_load_)) libs_load_ = [(i, i = 1, nlibs_)]
+      if (.not. (allocated(Mg_) .and. allocated(Kg_) .and. allocated(Cg_))) &
+         call bsa_Abort('Modal matrices were not acquired (bsa_setModalMatrices).')
+      if (allocated(modes_)) deallocate(modes_)
+      call selectModes_()
+      nme = size(modes_)
+      if (nme == 0) call bsa_Abort('Effective number of modes to be kept is 0.')
+      if (any(modes_ < 1) .or. any(modes_ > nm_)) call bsa_Abort('Kept mode index out of range.')
+      Mk = Mg_(modes_)
+      Kk = Kg_(modes_)
+      allocate(Ck(nme))
+      do i = 1, nme
+         Ck(i) = Cg_(modes_(i), modes_(i))
+      end do
+
+      ! ---- wind (turbulence components default as SetTurbCompsAndDirsDefault) ----
+      if (allocated(tc_) .and. allocated(dirs_)) then
+         tcs = tc_
+      else
+         tcs = [1]
+      end if
+      if (.not. allocated(wfc_)) call bsa_Abort('Wind force coefficients were not acquired. Aborting.')
+      if (allocated(psi_user_)) then
+         if (size(psi_user_, 1) /= nme .or. size(psi_user_, 2) /= size(n_load_)) &
+            call bsa_Abort('Phi*C sizes do not match the kept modes and loaded nodes.')
+         psi = psi_user_
+      else
+         call buildPsi_(psi)
+      end if
+      if (.not. (allocated(u_node_) .and. allocated(wz_node_) .and. allocated(corr_))) &
+         call bsa_Abort('Nodal wind data missing (velocities, wind zones or nodal correlation).')
+      if (.not. (c_associated(tscl_addr_) .and. c_associated(tstd_addr_))) &
+         call bsa_Abort('Turbulence scales and standard deviations were not acquired.')
+      if (size(corr_, 2) < maxval(tcs)) call bsa_Abort('Nodal correlation needs one column per turbulence component.')
+      if (nz_lim_ > 0) then
+         nwz = nz_lim_
+      else if (allocated(umref_)) then
+         nwz = size(umref_)
+      else
+         nwz = maxval(wz_node_)
+      end if
+      call c_f_pointer(tscl_addr_, fp3, [3, 3, nwz])
+      call c_f_pointer(tstd_addr_, fp2, [3, nwz])
+      tscl = fp3
+      tstd = fp2
+      if (allocated(umref_)) then
+         umref = umref_
+      else
+         allocate(umref(nwz), source=0.0_bsa_real_t)
+      end if
+
+      call classicGrid_(nfh, df)
+
+      p%nn = nn_
+      p%nnl = size(n_load_)
+      p%nm = nme
+      p%ntc = size(tcs)
+      p%ndegw = size(psi, 3)
+      p%nwz = nwz
+      p%psd_id = ipsd_
+      p%pulsation = 1
+      p%ncorr_rows = size(corr_, 1)
+      p%corr_col_mode = 0
+      p%n_load = c_loc(n_load_)
+      p%tc = c_loc(tcs)
+      p%psi = c_loc(psi)
+      p%nod_corr = c_loc(corr_)
+      p%u_node = c_loc(u_node_)
+      p%wz_node = c_loc(wz_node_)
+      p%turb_scales = c_loc(tscl)
+      p%turb_std = c_loc(tstd)
+      p%u_mean_ref = c_loc(umref)
+      p%Mm = c_loc(Mk)
+      p%Km = c_loc(Kk)
+      p%Cm = c_loc(Ck)
+
+      flags = int(iprec_, c_int32_t) + BSAHIP_QUAD_VECT
+      o_m2mf = c_null_ptr; o_m2mr = c_null_ptr; o_m2o2mr = c_null_ptr
+      o_m3mf = c_null_ptr; o_m3mr = c_null_ptr
+      if (allocated(m3mf_msh)) deallocate(m3mf_msh)
+      if (allocated(m3mr_msh)) deallocate(m3mr_msh)
+      if (icomp_psd_ /= 0) then
+         call realloc_(m2mf_cls, nme**2)
+         call realloc_(m2mr_cls, nme**2)
+         call realloc_(m2o2mr_cls, nme**2)
+         o_m2mf = c_loc(m2mf_cls)
+         o_m2mr = c_loc(m2mr_cls)
+         o_m2o2mr = c_loc(m2o2mr_cls)
+      else
+         flags = ior(flags, BSAHIP_NO_PSD)
+      end if
+      if (icomp_bisp_ /= 0) then
+         call realloc_(m3mf_cls, nme**3)
+         call realloc_(m3mr_cls, nme**3)
+         o_m3mf = c_loc(m3mf_cls)
+         o_m3mr = c_loc(m3mr_cls)
+      else
+         flags = ior(flags, BSAHIP_NO_BISP)
+      end if
+
+      if (.not. device_ready_) then
+         rc = bsahip_init(0_c_int)
+         if (rc /= 0) call bsa_Abort('GPU initialisation failed: '//bsahip_last_error_f())
+         device_ready_ = .true.
+      end if
+      rc = bsahip_classic_run(p, int(nfh, c_int32_t), real(df, c_double), flags, &
+                              o_m2mf, o_m2mr, o_m2o2mr, o_m3mf, o_m3mr)
+      if (rc /= 0) call bsa_Abort('bsahip_classic_run failed: '//bsahip_last_error_f())
+      write(output_unit, '(1x, 2a, i0, a, i0, a)') INFOMSG, 'Classic run on the GPU: ', &
+         nme, ' modes, ', 2 * nfh - 1, '^2 frequency points.'
+   contains
+      subroutine realloc_(a, n)
+         real(bsa_real_t), allocatable, intent(inout) :: a(:)
+         integer(bsa_int_t), intent(in) :: n
+         if (allocated(a)) deallocate(a)
+         allocate(a(n), source=0.0_bsa_real_t)
+      end subroutine
+   end subroutine bsa_Run
+
+end module BsaLib

@@ -1,0 +1,46 @@
+"""Runs the Fortran BsaLib front through its test driver (tests/fortran/bsalib_driver.F90,
+built to bsalib_amd/_lib/bsalib_fdriver) and reads back what bsa_Run returned."""
+from __future__ import annotations
+
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+from bsalib_amd import build
+from oracle import oracle as O
+
+
+def ensure_built():
+    if build.needs_build():
+        build.build(verbose=False)
+    if build.fortran_needs_build():
+        build.build_fortran(verbose=False)
+    return build.FDRIVER
+
+
+def run(prob, *opts, check=True, timeout=300):
+    """Returns (outputs dict or None, completed process)."""
+    drv = ensure_built()
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        O._write_ref_input(prob, fin, 1)
+        r = subprocess.run([str(drv), fin, fout, *opts], capture_output=True, text=True,
+                           timeout=timeout)
+        if r.returncode != 0:
+            if check:
+                raise RuntimeError(f"bsalib_fdriver failed ({r.returncode}):\n{r.stdout}\n{r.stderr}")
+            return None, r
+        raw = open(fout, "rb").read()
+    nme = int(np.frombuffer(raw[:4], dtype=np.int32)[0])
+    modes = np.frombuffer(raw[4:4 + 4 * nme], dtype=np.int32).copy()
+    vals = np.frombuffer(raw[4 + 4 * nme:], dtype=np.float64)
+    out = {"modes": modes}
+    o = 0
+    for k, n in (("m2mf", nme ** 2), ("m2mr", nme ** 2), ("m2o2mr", nme ** 2),
+                 ("m3mf", nme ** 3), ("m3mr", nme ** 3)):
+        out[k] = vals[o:o + n].copy()
+        o += n
+    assert o == vals.size
+    return out, r

@@ -1,0 +1,96 @@
+!! Test driver of the Fortran BsaLib front (module BsaLib in bsalib_amd/fortran):
+!! a program written the way a user of the reference writes one (setters,
+!! bsa_Run, bsa_Finalise), reading its inputs from a raw stream file.
+!!
+!! Usage:  bsalib_driver <input.bin> <output.bin> [fp64|bf16x3] [grid]
+!! The input layout is the one tests write for the reference harness (header
+!! of 14 int32, then integer and real64 arrays in Fortran order).  "grid"
+!! leaves test mode off and passes damping ratios, so bsa_Run applies the
+!! frequency-grid checks.  Output: nme (int32), the used modes, then m2mf,
+!! m2mr, m2o2mr (NME^2 each) and m3mf, m3mr (NME^3 each) as real64.
+program bsalib_driver
+   use, intrinsic :: iso_fortran_env, only: int32, real64
+   use BsaLib
+   implicit none
+
+   integer(int32) :: hdr(14)
+   integer(int32) :: nn, nlibs, nnl, nlibsl, nm, nme, ntc, ndegw, nwz, psdid, nfh, ncorr, npts, i
+   integer(int32), allocatable :: n_load(:), libs_load(:), modes(:), tcs(:), wz(:), used(:)
+   real(real64) :: df
+   real(real64), allocatable, target :: phi(:, :), natf(:), Mm(:), Km(:), Cm(:, :), unod(:)
+   real(real64), allocatable, target :: tscl(:, :, :), tstd(:, :), umref(:), corr(:, :)
+   real(real64), allocatable, target :: wfc(:, :, :), psi(:, :, :), xsi(:)
+   real(real64), allocatable :: m2mf(:), m2mr(:), m2o2mr(:), m3mf_msh(:), m3mr_msh(:)
+   real(real64), allocatable :: m3mf(:), m3mr(:)
+   character(len=512) :: fin, fout, arg
+   logical :: grid_checks
+   integer :: uin, uout
+
+   call get_command_argument(1, fin)
+   call get_command_argument(2, fout)
+   grid_checks = .false.
+
+   open(newunit=uin, file=trim(fin), access='stream', form='unformatted', status='old')
+   read(uin) hdr
+   nn = hdr(1); nlibs = hdr(2); nnl = hdr(3); nlibsl = hdr(4); nm = hdr(5); nme = hdr(6)
+   ntc = hdr(7); ndegw = hdr(8); nwz = hdr(9); psdid = hdr(10); nfh = hdr(11)
+   npts = hdr(13); ncorr = hdr(14)
+   allocate(n_load(nnl), libs_load(nlibsl), modes(nme), tcs(ntc), wz(nn))
+   read(uin) n_load, libs_load, modes, tcs, wz
+   allocate(phi(nn*nlibs, nm), natf(nm), Mm(nm), Km(nm), Cm(nm, nm), unod(nn))
+   allocate(tscl(3, 3, nwz), tstd(3, nwz), umref(nwz), corr(ncorr, 3))
+   allocate(wfc(nlibsl, ndegw, nnl), psi(nme, nnl, ndegw))
+   read(uin) df
+   read(uin) phi, natf, Mm, Km, Cm, unod, tscl, tstd, umref, corr, wfc, psi
+   close(uin)
+
+   call bsa_Init()
+   call bsa_setAnalysisType(1)
+   call bsa_setClassicMode(BSA_CLASSIC_MODE_VECTOR)
+   call bsa_setScalingConv(BSA_PSD_CONVENTION_PULS)
+   call bsa_setSpectraComputation(1, 1)
+   call bsa_setupClassic(nfh, df)
+   call bsa_setTestMode(1)
+   do i = 3, command_argument_count()
+      call get_command_argument(i, arg)
+      if (trim(arg) == 'bf16x3') call bsa_setComputePrecision(BSA_PREC_BF16X3)
+      if (trim(arg) == 'grid') grid_checks = .true.
+   end do
+
+   call bsa_setTotalNOfNodes(nn)
+   call bsa_setNumOfNodalDOFs(nlibs)
+   call bsa_setLoadedNodes(n_load)
+   call bsa_setLoadedNodalDOFs(libs_load)
+   call bsa_setModalInfo(nn*nlibs, nm, phi, natf)
+   call bsa_setModalMatrices(nm, Mm, Km, Cm)
+   if (grid_checks) then
+      ! modal damping ratios of the diagonal model: xsi = c / (2 sqrt(k m))
+      allocate(xsi(nm))
+      do i = 1, nm
+         xsi(i) = Cm(i, i) / (2.0_real64 * sqrt(Km(i) * Mm(i)))
+      end do
+      call bsa_setTotDamping(xsi)
+      call bsa_setTestMode(0)
+   end if
+
+   call bsa_setPSDType(psdid)
+   call bsa_setWindDirections([1])
+   call bsa_setWindTurbComps(tcs)
+   call bsa_setWZMeanWindVel(umref)
+   call bsa_setTurbWindScales(tscl)
+   call bsa_setTurbWindSDT(tstd)
+   call bsa_setNodalVel(unod)
+   call bsa_setNodalWindZones(wz)
+   call bsa_setSpatialNodalCorr(corr)
+   call bsa_setWindFCoeffs(wfc)
+
+   call bsa_Run(m2mf, m2mr, m2o2mr, m3mf_msh, m3mr_msh, m3mf, m3mr)
+   used = bsa_getUsedModeShapes()
+
+   open(newunit=uout, file=trim(fout), access='stream', form='unformatted', status='replace')
+   write(uout) int(size(used), int32), used, m2mf, m2mr, m2o2mr, m3mf, m3mr
+   close(uout)
+   if (allocated(m3mf_msh) .or. allocated(m3mr_msh)) error stop 'Mesher outputs must stay unallocated'
+   call bsa_Finalise()
+   if (.not. bsa_isCleaned()) error stop 'bsa_Finalise did not clean up'
+end program bsalib_driver

@@ -1,0 +1,73 @@
+"""GPU parity of the Fortran BsaLib front: a user-style program (setters,
+bsa_Run, bsa_Finalise through module BsaLib) against the reference's golden
+moments (1e-12 normwise relative, FP64; 1e-5 for split-bf16x3 third moments)
+and, for the frequency-grid checks of computeFreqsVect_
+(src/bsa/classic/BsaClassicImpl.F90:437-560), against the C ABI run on the
+grid those checks must produce."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import fdriver
+from tests.cases import CASES
+from tests.conftest import load_golden, rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", ["tiny_u", "tiny_uvw", "tiny_dav", "config1"])
+def test_fortran_front_matches_reference(name):
+    g = load_golden(name)
+    p = CASES[name]()
+    out, _ = fdriver.run(p)
+    np.testing.assert_array_equal(out["modes"], O.kept_modes(p))
+    for k in ("m2mf", "m2mr", "m3mf", "m3mr"):
+        assert rel_err(out[k], g[k]) < 1e-12, (k, rel_err(out[k], g[k]))
+    o = O.modal_psd(p)
+    assert rel_err(out["m2o2mr"], o["m2o2mr"]) < 1e-12
+
+
+def test_fortran_front_bf16x3():
+    g = load_golden("config1")
+    out, _ = fdriver.run(CASES["config1"](), "bf16x3")
+    assert rel_err(out["m3mf"], g["m3mf"]) < 1e-5
+    assert rel_err(out["m3mr"], g["m3mr"]) < 1e-5
+    assert rel_err(out["m2mr"], g["m2mr"]) < 1e-12
+
+
+def _expected_grid(p):
+    """The two branches of the grid check, worked by hand for this tower:
+    natural frequencies 0.2/0.6/1.2 Hz, damping ratio 0.01 -> the reference
+    spacing is min(xsi f) / 5 = 0.002 / 5 = 4e-4 Hz."""
+    xsi = np.diag(p.Cm) / (2.0 * np.sqrt(p.Km * p.Mm))
+    df_ref = float(np.min(xsi * p.natf) / 5)
+    assert abs(df_ref - 4e-4) < 1e-15
+    nf1, df = p.nfreqs - 1, p.df
+    fmax_ref = float(np.max(p.natf))
+    if df * nf1 < fmax_ref:
+        if df > df_ref:
+            if df_ref * nf1 < fmax_ref:
+                nf1 = math.ceil(fmax_ref / df_ref)
+            df = df_ref
+        else:
+            nf1 = math.ceil(fmax_ref / df)
+    return nf1 + 1, df
+
+
+@pytest.mark.parametrize("nfreqs,df,branch", [(17, 0.03125, "df_too_big"),
+                                              (17, 0.0002, "df_ok")])
+def test_fortran_front_grid_checks(gpu_lib, nfreqs, df, branch):
+    p = CASES["config1"]()
+    p.nfreqs, p.df = nfreqs, df
+    nfh, dfx = _expected_grid(p)
+    assert (dfx == 4e-4) == (branch == "df_too_big")
+    assert nfh > 2000
+    out, r = fdriver.run(p, "fp64", "grid")
+    assert f"nfreqs= {nfh}" in r.stdout, r.stdout
+    q = CASES["config1"]()
+    q.nfreqs, q.df = nfh, dfx
+    want = gpu_lib.classic_run(q, quad="vect")
+    for k in ("m2mf", "m2mr", "m2o2mr", "m3mf", "m3mr"):
+        assert rel_err(out[k], want[k]) < 1e-13, k
