@@ -23,7 +23,7 @@
 !!    split-bf16x3 node contraction (FP32 accumulation on matrix cores,
 !!    FP64 everywhere else); the default is FP64.
 module BsaLib
-   use, intrinsic :: iso_fortran_env, only: int32, real64, output_unit
+   use, intrinsic :: iso_fortran_env, only: int32, int64, real64, output_unit
    use, intrinsic :: iso_c_binding
    use BsaHip_C
    implicit none
@@ -485,7 +485,7 @@ contains
       integer(bsa_int_t), allocatable, target :: tcs(:)
       real(bsa_real_t), pointer :: fp3(:, :, :), fp2(:, :)
       real(bsa_real_t) :: df
-      integer(bsa_int_t) :: nfh, nme, nwz, i
+      integer(bsa_int_t) :: nfh, nme, nwz, i, iclassic
       integer(c_int32_t) :: flags
       integer(c_int) :: rc
       type(c_ptr) :: o_m2mf, o_m2mr, o_m2o2mr, o_m3mf, o_m3mr
@@ -561,6 +561,20 @@ contains
 
       call classicGrid_(nfh, df)
 
+      ! Classic version as checkMaxAllocation_ (BsaClassicImpl.F90:384-434): outside
+      ! test mode a vectorised bispectrum above 1e9 elements switches to the scalar
+      ! version, whose third moments are plain Riemann sums (every point times
+      ! dw^2, BsaClassicImpl.F90:331,341); the vectorised version uses the
+      ! edge/corner-corrected rule of intgSpectraVect_.  The element count is
+      ! formed in 64-bit here (the reference forms it in 32-bit, which wraps).
+      iclassic = iclassic_
+      if (itest_ == 0 .and. icomp_bisp_ /= 0 .and. iclassic == BSA_CLASSIC_MODE_VECTOR) then
+         if (int(2 * nfh - 1, int64)**2 * int(nme, int64)**3 > 1000000000_int64) then
+            write(output_unit, '(1x, 2a)') WARNMSG, 'Too high allocation size for VECTORISED BSA version: using SCALAR.'
+            iclassic = BSA_CLASSIC_MODE_SCALAR
+         end if
+      end if
+
       p%nn = nn_
       p%nnl = size(n_load_)
       p%nm = nme
@@ -584,7 +598,11 @@ contains
       p%Km = c_loc(Kk)
       p%Cm = c_loc(Ck)
 
-      flags = int(iprec_, c_int32_t) + BSAHIP_QUAD_VECT
+      if (iclassic == BSA_CLASSIC_MODE_SCALAR) then
+         flags = int(iprec_, c_int32_t) + BSAHIP_QUAD_RIEMANN
+      else
+         flags = int(iprec_, c_int32_t) + BSAHIP_QUAD_VECT
+      end if
       o_m2mf = c_null_ptr; o_m2mr = c_null_ptr; o_m2o2mr = c_null_ptr
       o_m3mf = c_null_ptr; o_m3mr = c_null_ptr
       if (allocated(m3mf_msh)) deallocate(m3mf_msh)
@@ -616,8 +634,9 @@ contains
       rc = bsahip_classic_run(p, int(nfh, c_int32_t), real(df, c_double), flags, &
                               o_m2mf, o_m2mr, o_m2o2mr, o_m3mf, o_m3mr)
       if (rc /= 0) call bsa_Abort('bsahip_classic_run failed: '//bsahip_last_error_f())
-      write(output_unit, '(1x, 2a, i0, a, i0, a)') INFOMSG, 'Classic run on the GPU: ', &
-         nme, ' modes, ', 2 * nfh - 1, '^2 frequency points.'
+      write(output_unit, '(1x, 2a, i0, a, i0, a, a)') INFOMSG, 'Classic run on the GPU: ', &
+         nme, ' modes, ', 2 * nfh - 1, '^2 frequency points, ', &
+         merge('SCALAR    ', 'VECTORISED', iclassic == BSA_CLASSIC_MODE_SCALAR)
    contains
       subroutine realloc_(a, n)
          real(bsa_real_t), allocatable, intent(inout) :: a(:)

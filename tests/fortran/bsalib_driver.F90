@@ -2,12 +2,13 @@
 !! a program written the way a user of the reference writes one (setters,
 !! bsa_Run, bsa_Finalise), reading its inputs from a raw stream file.
 !!
-!! Usage:  bsalib_driver <input.bin> <output.bin> [fp64|bf16x3] [grid]
+!! Usage:  bsalib_driver <input.bin> <output.bin> [fp64|bf16x3] [grid] [scalar]
 !! The input layout is the one tests write for the reference harness (header
 !! of 14 int32, then integer and real64 arrays in Fortran order).  "grid"
 !! leaves test mode off and passes damping ratios, so bsa_Run applies the
-!! frequency-grid checks.  Output: nme (int32), the used modes, then m2mf,
-!! m2mr, m2o2mr (NME^2 each) and m3mf, m3mr (NME^3 each) as real64.
+!! frequency-grid checks; "scalar" selects the scalar Classic version.
+!! Output: nme (int32), the used modes, then m2mf, m2mr, m2o2mr (NME^2 each)
+!! and m3mf, m3mr (NME^3 each) as real64.
 program bsalib_driver
    use, intrinsic :: iso_fortran_env, only: int32, real64
    use BsaLib
@@ -55,6 +56,7 @@ program bsalib_driver
       call get_command_argument(i, arg)
       if (trim(arg) == 'bf16x3') call bsa_setComputePrecision(BSA_PREC_BF16X3)
       if (trim(arg) == 'grid') grid_checks = .true.
+      if (trim(arg) == 'scalar') call bsa_setClassicMode(BSA_CLASSIC_MODE_SCALAR)
    end do
 
    call bsa_setTotalNOfNodes(nn)

@@ -37,6 +37,20 @@ def test_fortran_front_bf16x3():
     assert rel_err(out["m2mr"], g["m2mr"]) < 1e-12
 
 
+@pytest.mark.parametrize("name", ["tiny_uvw", "config1"])
+def test_fortran_front_scalar_version(gpu_lib, name):
+    """The scalar Classic version integrates third moments by plain Riemann
+    sums (BsaClassicImpl.F90:331,341); second moments keep the vectorised rule."""
+    p = CASES[name]()
+    out, r = fdriver.run(p, "scalar")
+    assert "SCALAR" in r.stdout
+    o = O.classic_factorised(p, rule="riemann", truncate=True, nthreads=8)
+    assert rel_err(out["m3mf"], o["m3mf"]) < 1e-12
+    assert rel_err(out["m3mr"], o["m3mr"]) < 1e-12
+    g = load_golden(name)
+    assert rel_err(out["m2mr"], g["m2mr"]) < 1e-12
+
+
 def _expected_grid(p):
     """The two branches of the grid check, worked by hand for this tower:
     natural frequencies 0.2/0.6/1.2 Hz, damping ratio 0.01 -> the reference
@@ -66,8 +80,13 @@ def test_fortran_front_grid_checks(gpu_lib, nfreqs, df, branch):
     assert nfh > 2000
     out, r = fdriver.run(p, "fp64", "grid")
     assert f"nfreqs= {nfh}" in r.stdout, r.stdout
+    # outside test mode a bispectrum above 1e9 elements runs the scalar version
+    # (plain Riemann third moments): 12001^2 * 27 does, 6001^2 * 27 does not
+    scalar = (2 * nfh - 1) ** 2 * 27 > 10 ** 9
+    assert scalar == (branch == "df_ok")
+    assert ("SCALAR" in r.stdout) == scalar
     q = CASES["config1"]()
     q.nfreqs, q.df = nfh, dfx
-    want = gpu_lib.classic_run(q, quad="vect")
+    want = gpu_lib.classic_run(q, quad="riemann" if scalar else "vect")
     for k in ("m2mf", "m2mr", "m2o2mr", "m3mf", "m3mr"):
         assert rel_err(out[k], want[k]) < 1e-13, k
