@@ -3,6 +3,11 @@
 # place under /root/reference/src) plus the test-only harness ref_harness.F90
 # into oracle/_ref/ref_harness.  Test infrastructure only.
 #
+# Second product: oracle/_ref/bsacl_module_driver, the reference's own Fortran
+# binding of BsaCL (BsaCL/BsaCL_Fortran/*.F90, unmodified) plus the test-only
+# bsacl_module_driver.F90, linked against this repository's libbsahip.so
+# (built first by bsalib_amd/build.py) in place of the reference BsaCL runtime.
+#
 # Units compiled (dependency order): CONSTANTS, Timer, BsaIO, Settings,
 # StructureData, WindType, utility, BsaLibData(+Impl), IO/Settings/Structure/
 # Wind implementations, BsaLib_Functions(+Impl).  The reference's own build
@@ -34,3 +39,22 @@ done
 $FC $FFLAGS -c "$HERE/ref_harness.F90" -o ref_harness.o
 $FC -fopenmp -o "$OUT/ref_harness" ref_harness.o $OBJS -Wl,--unresolved-symbols=ignore-all
 echo "built $OUT/ref_harness"
+
+CLF=$REF/BsaCL/BsaCL_Fortran
+LIBDIR=$(cd "$OUT/../.." && pwd)/bsalib_amd/_lib
+if [ -f "$LIBDIR/libbsahip.so" ]; then
+  mkdir -p "$OUT/obj/bsacl"
+  cd "$OUT/obj/bsacl"
+  CLFLAGS="-cpp -O2 -I$REF/BsaCL/include"
+  CLOBJS="bsacl_c.o bsacl.o bsacl_impl.o"
+  # dependency order: the submodule bsacl_impl uses module BsaCL_c
+  $FC $CLFLAGS -c "$CLF/bsacl_c.F90" -o bsacl_c.o
+  $FC $CLFLAGS -c "$CLF/bsacl.F90" -o bsacl.o
+  $FC $CLFLAGS -c "$CLF/bsacl_impl.F90" -o bsacl_impl.o
+  $FC $CLFLAGS -c "$HERE/bsacl_module_driver.F90" -o bsacl_module_driver.o
+  $FC -o "$OUT/bsacl_module_driver" bsacl_module_driver.o $CLOBJS -L"$LIBDIR" -lbsahip \
+      -Wl,-rpath,'$ORIGIN/../../bsalib_amd/_lib'
+  echo "built $OUT/bsacl_module_driver"
+else
+  echo "libbsahip.so not built yet: skipping bsacl_module_driver" >&2
+fi

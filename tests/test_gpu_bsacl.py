@@ -88,3 +88,26 @@ def test_bsacl_run_error_codes_on_gpu(gpu_lib):
     assert "NOFREQ -20" in out
     assert "NULLRES -123" in out
     assert "AFTER -1 -999" in out
+
+
+REF_CL_DRIVER = ROOT / "oracle" / "_ref" / "bsacl_module_driver"
+
+
+@pytest.mark.skipif(not REF_CL_DRIVER.exists(),
+                    reason="reference BsaCL Fortran binding not built (needs the reference sources)")
+@pytest.mark.parametrize("psd_id,comps", [(5, (1,)), (5, (1, 3))])
+def test_reference_fortran_bsacl_module_binds_unchanged(gpu_lib, tmp_path, psd_id, comps):
+    """The reference's own module BsaCL (BsaCL/BsaCL_Fortran, compiled unmodified)
+    linked against libbsahip.so, called in bsa_Run's order."""
+    p = _problem(psd_id, comps)
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    O._write_ref_input(p, str(fin), 1)
+    r = subprocess.run([str(REF_CL_DRIVER), str(fin), str(fout)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = fout.read_bytes()
+    rc = np.frombuffer(raw[:16], dtype=np.int32)
+    assert rc.tolist() == [0, 0, 0, 0], (rc, r.stdout)
+    got = np.frombuffer(raw[16:], dtype=np.float64)
+    f = p.freqs()
+    assert rel_err(got, O.bsacl_run(p, f, f)) < 1e-12
