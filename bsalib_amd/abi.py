@@ -50,7 +50,9 @@ _lib = None
 EXPORTS = ["bsahip_init", "bsahip_finalise", "bsahip_last_error", "bsahip_version",
            "bsahip_classic_run", "bsahip_classic_spectra", "bsahip_bfm_points",
            "bsahip_last_kernel_time", "bsahip_plan_create", "bsahip_plan_run",
-           "bsahip_plan_destroy"]
+           "bsahip_plan_destroy", "bsahip_mesher_create", "bsahip_mesher_layout", "bsahip_mesher_ntiles",
+           "bsahip_mesher_tiles", "bsahip_mesher_features", "bsahip_mesher_run",
+           "bsahip_mesher_destroy"]
 
 
 def lib():
@@ -66,6 +68,15 @@ def lib():
                                          C.POINTER(C.c_void_p)]
         L.bsahip_plan_run.argtypes = [C.c_void_p] + [C.POINTER(C.c_double)] * 5
         L.bsahip_plan_destroy.argtypes = [C.c_void_p]
+        L.bsahip_mesher_create.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]
+        L.bsahip_mesher_ntiles.argtypes = [C.c_void_p, C.POINTER(C.c_int32)]
+        L.bsahip_mesher_layout.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_double),
+                                           C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.bsahip_mesher_tiles.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+        L.bsahip_mesher_features.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
+        L.bsahip_mesher_run.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                        C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.bsahip_mesher_destroy.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -122,13 +133,15 @@ def init(device: int = 0):
 
 
 def classic_run(prob, quad: str = "vect", prec: str = "fp64", bound: Bound = None,
-                psd: bool = True, bisp: bool = True) -> dict:
-    """Classic full-grid moments (bsahip_classic_run)."""
+                psd: bool = True, bisp: bool = True, exact: bool = False) -> dict:
+    """Classic full-grid moments (bsahip_classic_run); exact=True evaluates
+    S(f1+f2) beyond the grid instead of taking it as zero."""
     b = bound or Bound(prob)
     nm = b.nm
     out = {k: np.zeros(nm * nm) for k in ("m2mf", "m2mr", "m2o2mr")}
     out.update({k: np.zeros(nm ** 3) for k in ("m3mf", "m3mr")})
-    flags = PREC[prec] | QUAD[quad] | (0 if psd else NO_PSD) | (0 if bisp else NO_BISP)
+    flags = PREC[prec] | QUAD[quad] | (0 if psd else NO_PSD) | (0 if bisp else NO_BISP) | \
+        (SUM_EXACT if exact else SUM_TRUNCATED)
     D = C.POINTER(C.c_double)
     _check(lib().bsahip_classic_run(C.byref(b.struct), C.c_int32(prob.nfreqs), C.c_double(prob.df),
                                     C.c_int32(flags), _ptr(out["m2mf"], C.c_double),
@@ -202,6 +215,109 @@ class Plan:
     def close(self):
         if self.h:
             lib().bsahip_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+
+class MesherCfg(C.Structure):
+    """Mirror of ``bsahip_mesher_cfg`` (include/bsahip.h)."""
+    _fields_ = [("bkg_base_rfmnt", C.c_int32), ("tile_cells", C.c_int32),
+                ("diag_crest", C.c_int32), ("reserved", C.c_int32), ("bkg_peakw", C.c_double),
+                ("bkg_area_ext", C.c_double), ("peak_area_ext", C.c_double),
+                ("max_area_ext", C.c_double),
+                ("natf", C.POINTER(C.c_double)), ("xsi", C.POINTER(C.c_double))]
+
+
+def mesher_cfg(prob, bkg_base_rfmnt: int = 20, tile_cells: int = 32, bkg_peakw=None,
+               bkg_area_ext: float = 2.0, peak_area_ext: float = 2.0, max_area_ext: float = 2.0,
+               modes=None, diag_crest: bool = False, natf=None, xsi=None):
+    """bsahip_mesher_cfg for a problem; returns (cfg, arrays kept alive).  natf
+    and xsi (kept modes) default to the problem's; they only shape the layout."""
+    modes = host.kept_modes(prob) if modes is None else np.asarray(modes, np.int32)
+    mo = modes - 1
+    natf = np.ascontiguousarray(np.asarray(prob.natf, dtype=np.float64)[mo] if natf is None
+                                else np.asarray(natf, dtype=np.float64))
+    xsi = np.ascontiguousarray(host.damping_ratios(prob, modes) if xsi is None
+                               else np.asarray(xsi, dtype=np.float64))
+    cfg = MesherCfg()
+    cfg.bkg_base_rfmnt, cfg.tile_cells = bkg_base_rfmnt, tile_cells
+    cfg.diag_crest, cfg.reserved = int(diag_crest), 0
+    cfg.bkg_peakw = host.bkg_peak_width(prob) if bkg_peakw is None else float(bkg_peakw)
+    cfg.bkg_area_ext, cfg.peak_area_ext, cfg.max_area_ext = bkg_area_ext, peak_area_ext, max_area_ext
+    cfg.natf = _ptr(natf, C.c_double)
+    cfg.xsi = _ptr(xsi, C.c_double)
+    return cfg, (natf, xsi)
+
+
+def mesher_layout(cfg, nm: int):
+    """Tile layout only (host computation): geom (ntiles, 10), rule levels."""
+    n = C.c_int32(0)
+    _check(lib().bsahip_mesher_layout(C.addressof(cfg), C.c_int32(nm), C.c_int32(0), None, None,
+                                      C.byref(n)))
+    geom = np.zeros((n.value, 10))
+    rule = np.zeros(n.value, dtype=np.int32)
+    _check(lib().bsahip_mesher_layout(C.addressof(cfg), C.c_int32(nm), n, _ptr(geom, C.c_double),
+                                      _ptr(rule, C.c_int32), C.byref(n)))
+    return geom, rule
+
+
+class Mesher:
+    """Zone Mesher resident on the device (bsahip_mesher_*).  The constructor
+    lays out the tiles and evaluates every tile's level-0 BFM (pre-mesh);
+    run() integrates with the given per-tile refinement levels (post-mesh)."""
+
+    def __init__(self, prob, bkg_base_rfmnt: int = 20, tile_cells: int = 32, bkg_peakw=None,
+                 bkg_area_ext: float = 2.0, peak_area_ext: float = 2.0, max_area_ext: float = 2.0,
+                 bound: Bound = None, diag_crest: bool = False, natf=None, xsi=None):
+        b = bound or Bound(prob)
+        self.nm = b.nm
+        self.modes = b.modes
+        cfg, self._keep = mesher_cfg(prob, bkg_base_rfmnt, tile_cells, bkg_peakw, bkg_area_ext,
+                                     peak_area_ext, max_area_ext, modes=b.modes,
+                                     diag_crest=diag_crest, natf=natf, xsi=xsi)
+        self.cfg = cfg
+        self.bkg_peakw = cfg.bkg_peakw
+        h = C.c_void_p()
+        self.h = None
+        _check(lib().bsahip_mesher_create(C.addressof(b.struct), C.addressof(cfg), C.c_int32(0),
+                                          C.byref(h)))
+        self.h = h
+        n = C.c_int32(0)
+        _check(lib().bsahip_mesher_ntiles(self.h, C.byref(n)))
+        self.ntiles = n.value
+
+    def tiles(self):
+        geom = np.zeros((self.ntiles, 10))
+        rule = np.zeros(self.ntiles, dtype=np.int32)
+        _check(lib().bsahip_mesher_tiles(self.h, _ptr(geom, C.c_double), _ptr(rule, C.c_int32)))
+        return geom, rule
+
+    def features(self, ns: int = 16) -> np.ndarray:
+        out = np.zeros((self.ntiles, ns * ns))
+        _check(lib().bsahip_mesher_features(self.h, C.c_int32(ns), _ptr(out, C.c_double)))
+        return out
+
+    def run(self, levels=None) -> dict:
+        nm3 = self.nm ** 3
+        m3mf, m3mr = np.zeros(nm3), np.zeros(nm3)
+        counts = np.zeros(3, dtype=np.int64)
+        lv = None
+        if levels is not None:
+            lv = np.ascontiguousarray(np.broadcast_to(np.asarray(levels, dtype=np.int32),
+                                                      (self.ntiles,)))
+        _check(lib().bsahip_mesher_run(self.h, _ptr(lv, C.c_int32), _ptr(m3mf, C.c_double),
+                                       _ptr(m3mr, C.c_double), _ptr(counts, C.c_int64)))
+        return {"m3mf": m3mf, "m3mr": m3mr, "evals_pre": int(counts[0]),
+                "evals_post": int(counts[1]), "brm_points": int(counts[2]), "modes": self.modes}
+
+    def close(self):
+        if self.h:
+            lib().bsahip_mesher_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -727,3 +727,5 @@ int bsahip_selftest_mfma_bf16_layout(float *out768)
 }
 
 }  // extern "C"
+
+#include "mesher.hip"

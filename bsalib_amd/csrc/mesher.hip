@@ -1,0 +1,765 @@
+// Zone Mesher on the device: adaptive (f1,f2) quadrature of the modal force and
+// response bispectra over rectangular tiles, with all zone data resident in HBM.
+//
+// The reference's Mesher (src/bsa/meshing/BsaMesherImpl.F90, zones/*, policy/MPolicy.F90)
+// samples the smooth modal force bispectrum (BFM) on coarse zone grids, refines
+// the sampling where its policy asks for it, and integrates the response
+// bispectrum (BRM = BFM * FRF product) on a finer grid where the BFM is
+// interpolated bilinearly and the FRF evaluated exactly, with trapezoid weights
+// (vertex 1/4, edge 1/2, interior 1; zones/_shared_poly2d.fi:827-832).  Here:
+//
+//  * 1D zone limits per axis as prefetchZoneLimits_ (BsaMesherImpl.F90:1638-1859):
+//    background half-width b = bkg_peakw * bkg_area_ext / 2, peak zones
+//    f_n +- xsi_n f_n peak_area_ext (getActualPeakZoneExtensionLimits_, :1864-1890),
+//    pre-peak gaps between them, external padding up to max f_n * max_area_ext
+//    (getMaxSpaceExtension_, :1626-1631); mirrored to negative frequencies.
+//  * Rectangular zones are the tensor products of those 1D intervals, cut into
+//    tiles of at most tile_cells x tile_cells level-0 cells.  Level-0 spacing is
+//    the background spacing df_ref = 2b/(bkg_base_rfmnt-1) times the interval's
+//    delta factor (BKG 1, PEAK 1, PRE_PEAK 2, PAD 4).
+//  * A tile's BFM is sampled at refinement level l in {1,2,4,8} (l cells per
+//    level-0 cell and axis).  The BRM grid of a tile does not depend on l: its
+//    spacing follows the FRF (msh_brm_sub: 16 BRM cells per peak width at a
+//    resonance line, growing with the distance to it), so the level only
+//    decides how densely the smooth BFM is sampled.
+//  * The rule-based level (the reference's delta validation,
+//    _shared_poly2d.fi:735-795) asks for BFM spacing <= w/fct per axis
+//    (w = narrowest interest-mode peak width 2 xsi f peak_area_ext in a peak
+//    interval, bkg_peakw elsewhere; fct = the policy's BFM factor), capped at 8.
+//
+// Each tile's BFM points are evaluated by k_bfm_batch (FP64 MFMA, S(f1+f2)
+// evaluated exactly, the semantics of getFM_full_tnm_scalar_msh_,
+// functionsImpl.F90:247-465); k_msh_integrate interpolates, applies the FRF
+// and the trapezoid weights and adds into m3mf, m3mr.
+
+// ---------------------------------------------------------------------------
+// V table for point evaluation: V[(u*K + k)*NMP + y] = A(y, k; |f_u|), zero for
+// y >= nm.  Built from the U table U[k*ldU + y*nu + u].
+// grid (ceil(nu/64), K), block 256
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_vtable(const double *__restrict__ U, long ldU, int nu,
+                                                int nm, int K, int nmp, double *__restrict__ V)
+{
+   __shared__ double s[64][65];
+   const int k = blockIdx.y;
+   const int u0 = blockIdx.x * 64;
+   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+   for (int y0 = 0; y0 < nmp; y0 += 64) {
+      for (int y = y0 + ty; y < y0 + 64; y += 4) {
+         const int u = u0 + tx;
+         double v = 0.0;
+         if (y < nm && u < nu) v = U[(size_t)k * ldU + (size_t)y * nu + u];
+         s[y - y0][tx] = v;
+      }
+      __syncthreads();
+      const int ny = min(64, nmp - y0);
+      for (int t = threadIdx.x; t < 64 * ny; t += 256) {
+         const int i = t / ny, y = t % ny;
+         const int u = u0 + i;
+         if (u < nu) V[((size_t)u * K + k) * nmp + y0 + y] = s[y][i];
+      }
+      __syncthreads();
+   }
+}
+
+// Psi_b transposed and padded: PB[k*NMP + y]
+__global__ void k_pbt(const double *__restrict__ pb, int nm, int K, int nmp, double *__restrict__ PB)
+{
+   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (gid >= (long)K * nmp) return;
+   const int k = (int)(gid / nmp), y = (int)(gid % nmp);
+   PB[gid] = (y < nm) ? pb[(size_t)y * K + k] : 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// BFM at a batch of points on FP64 MFMA (v_mfma_f64_16x16x4f64).
+// For one point, with A1 = A(f1), A2 = A(f2), A3 = A(f1+f2):
+//   B(m,n,p)/2 = sum_k Psi_b(m,k) A1(n,k) A2(p,k)
+//              + sum_k A3(m,k) [Psi_b(n,k) A2(p,k) + Psi_b(p,k) A1(n,k)]
+// i.e. one NM x NM^2 product with a 2K-long contraction whose right operand
+// is formed on the fly.  Rows m are padded to MT*16, columns (n,p) (col =
+// n + p*nm) tiled by 16 and dealt to the 8 waves (TPW tiles each).
+// One workgroup per point; out[slot[pt]*nm3 + m + col*nm] = B.
+// ---------------------------------------------------------------------------
+#define BKC 16   // K chunk staged in LDS
+
+template <int MT, int TPW>
+__global__ __launch_bounds__(512) void k_bfm_batch(
+   const double *__restrict__ V, const double *__restrict__ PB, int K, int nm, int npts,
+   const int *__restrict__ ia1, const int *__restrict__ ia2, const int *__restrict__ ia3,
+   const long *__restrict__ slot, double *__restrict__ out)
+{
+   constexpr int NMP = MT * 16;
+   __shared__ double s1[BKC][NMP], s2[BKC][NMP], s3[BKC][NMP], sp[BKC][NMP];
+   const int pt = blockIdx.x;
+   if (pt >= npts) return;
+   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+   const int ncol = nm * nm, ct = (ncol + 15) / 16;
+   const double *v1 = V + (size_t)ia1[pt] * K * NMP;
+   const double *v2 = V + (size_t)ia2[pt] * K * NMP;
+   const double *v3 = V + (size_t)ia3[pt] * K * NMP;
+
+   // columns beyond nm^2 compute garbage that is never stored (MFMA columns
+   // are independent), so they just read mode 0
+   int cn[TPW], cp[TPW];
+#pragma unroll
+   for (int j = 0; j < TPW; ++j) {
+      const int col = (wv + 8 * j) * 16 + (lane & 15);
+      const bool ok = col < ncol;
+      cn[j] = ok ? col % nm : 0;
+      cp[j] = ok ? col / nm : 0;
+   }
+   d4 acc[MT][TPW];
+#pragma unroll
+   for (int r = 0; r < MT; ++r)
+#pragma unroll
+      for (int j = 0; j < TPW; ++j) acc[r][j] = (d4){0.0, 0.0, 0.0, 0.0};
+
+   const int kr = lane >> 4, rr = lane & 15;
+   for (int k0 = 0; k0 < K; k0 += BKC) {
+      for (int t = threadIdx.x; t < BKC * NMP; t += 512) {
+         const int kk = t / NMP, y = t % NMP;
+         const size_t o = (size_t)(k0 + kk) * NMP + y;
+         s1[kk][y] = v1[o];
+         s2[kk][y] = v2[o];
+         s3[kk][y] = v3[o];
+         sp[kk][y] = PB[o];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int k4 = 0; k4 < BKC; k4 += 4) {
+         const int kk = k4 + kr;
+         double la[MT], lb[MT];
+#pragma unroll
+         for (int r = 0; r < MT; ++r) {
+            la[r] = sp[kk][r * 16 + rr];
+            lb[r] = s3[kk][r * 16 + rr];
+         }
+#pragma unroll
+         for (int j = 0; j < TPW; ++j) {
+            if (wv + 8 * j < ct) {
+               const int n = cn[j], p = cp[j];
+               const double a1n = s1[kk][n], a2p = s2[kk][p];
+               const double pbn = sp[kk][n], pbp = sp[kk][p];
+               const double ra = a1n * a2p;
+               const double rb = pbn * a2p + pbp * a1n;
+#pragma unroll
+               for (int r = 0; r < MT; ++r) {
+                  acc[r][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(la[r], ra, acc[r][j], 0, 0, 0);
+                  acc[r][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(lb[r], rb, acc[r][j], 0, 0, 0);
+               }
+            }
+         }
+      }
+      __syncthreads();
+   }
+   const long nm3 = (long)nm * ncol;
+   double *o = out + slot[pt] * nm3;
+#pragma unroll
+   for (int j = 0; j < TPW; ++j) {
+      const int col = (wv + 8 * j) * 16 + (lane & 15);
+      if (wv + 8 * j >= ct || col >= ncol) continue;
+#pragma unroll
+      for (int r = 0; r < MT; ++r)
+#pragma unroll
+         for (int t = 0; t < 4; ++t) {
+            const int m = r * 16 + (lane >> 4) + 4 * t;
+            if (m < nm) o[m + (size_t)col * nm] = 2.0 * acc[r][j][t];
+         }
+   }
+}
+
+// ---------------------------------------------------------------------------
+// Tile integration: bilinear BFM on the BRM grid, exact FRF, trapezoid weights.
+// BFM grid nb1 x nb2 (point q = i + j*nb1, values bf[q*nm3 + mnp]); each BFM
+// cell holds s1 x s2 BRM cells.  BRM point (a, b): f1 = f1a + a*h1,
+// f2 = f2a + b*h2.  Adds sum w*B into m3f and sum w*B*Re[H_m(f1)H_n(f2)H_p*(f1+f2)]
+// into m3r.  grid (ceil(nm3/256), nchunk), block 256.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_msh_integrate(
+   const double *__restrict__ bf, int nm, int nb1, int nb2, int s1, int s2, double f1a,
+   double f2a, double h1, double h2, double wscale, const double *__restrict__ Mm,
+   const double *__restrict__ Km, const double *__restrict__ Cm, double *__restrict__ m3f,
+   double *__restrict__ m3r)
+{
+   const int nm3 = nm * nm * nm;
+   const int mnp = blockIdx.x * blockDim.x + threadIdx.x;
+   if (mnp >= nm3) return;
+   const int m = mnp % nm, n = (mnp / nm) % nm, p = mnp / (nm * nm);
+   const int nc1 = nb1 - 1, nc2 = nb2 - 1, ncell = nc1 * nc2;
+   const int per = (ncell + gridDim.y - 1) / gridDim.y;
+   const int c0 = blockIdx.y * per, c1 = min(ncell, c0 + per);
+   const int NA = nc1 * s1, NB = nc2 * s2;          // last BRM index per axis
+   const double Mmm = Mm[m], Kmm = Km[m], Cmm = Cm[m];
+   const double Mn = Mm[n], Kn = Km[n], Cn = Cm[n];
+   const double Mp = Mm[p], Kp = Km[p], Cp = Cm[p];
+   const double inv1 = 1.0 / s1, inv2 = 1.0 / s2;
+   double sf = 0.0, sr = 0.0;
+   for (int c = c0; c < c1; ++c) {
+      const int ci = c % nc1, cj = c / nc1;
+      const size_t q00 = (size_t)ci + (size_t)cj * nb1;
+      const double b00 = bf[q00 * nm3 + mnp], b10 = bf[(q00 + 1) * nm3 + mnp];
+      const double b01 = bf[(q00 + nb1) * nm3 + mnp], b11 = bf[(q00 + nb1 + 1) * nm3 + mnp];
+      const int ea = (ci == nc1 - 1) ? s1 : s1 - 1;   // owned BRM points per axis
+      const int eb = (cj == nc2 - 1) ? s2 : s2 - 1;
+      for (int bb = 0; bb <= eb; ++bb) {
+         const int b = cj * s2 + bb;
+         const double f2 = f2a + b * h2, w2 = 2.0 * M_PI * f2;
+         const double wb = (b == 0 || b == NB) ? 0.5 : 1.0;
+         const double sv = bb * inv2;
+         double r = Kn - w2 * w2 * Mn, im = w2 * Cn, d = r * r + im * im;
+         const double2 h2v = make_double2(r / d, -im / d);
+         for (int aa = 0; aa <= ea; ++aa) {
+            const int a = ci * s1 + aa;
+            const double f1 = f1a + a * h1, w1 = 2.0 * M_PI * f1, w3 = w1 + w2;
+            const double wa = (a == 0 || a == NA) ? 0.5 : 1.0;
+            const double t = aa * inv1;
+            const double bv = (1.0 - t) * (1.0 - sv) * b00 + t * (1.0 - sv) * b10 +
+                              (1.0 - t) * sv * b01 + t * sv * b11;
+            r = Kmm - w1 * w1 * Mmm; im = w1 * Cmm; d = r * r + im * im;
+            const double2 h1v = make_double2(r / d, -im / d);
+            r = Kp - w3 * w3 * Mp; im = w3 * Cp; d = r * r + im * im;
+            const double2 h3v = make_double2(r / d, -im / d);
+            const double w = wa * wb;
+            sf += w * bv;
+            sr += w * bv * re3(h1v, h2v, h3v);
+         }
+      }
+   }
+   atomicAdd(m3f + mnp, sf * wscale);
+   atomicAdd(m3r + mnp, sr * wscale);
+}
+
+// Feature stencil of a tile's level-0 BFM: out[s] = sqrt(sum_mnp B^2) at the
+// level-0 point nearest to stencil node s (ns x ns nodes over the tile).
+__global__ void k_bfm_stencil(const double *__restrict__ bf, int nm3, int nb1, int nb2, int ns,
+                              double *__restrict__ out)
+{
+   const int s = blockIdx.x;
+   const int si = s % ns, sj = s / ns;
+   const int i = (int)lround((double)si * (nb1 - 1) / (ns - 1));
+   const int j = (int)lround((double)sj * (nb2 - 1) / (ns - 1));
+   const double *b = bf + ((size_t)i + (size_t)j * nb1) * nm3;
+   double acc = 0.0;
+   for (int t = threadIdx.x; t < nm3; t += 64) acc += b[t] * b[t];
+   acc = wave_sum(acc);
+   if (threadIdx.x == 0) out[s] = sqrt(acc);
+}
+
+// ===========================================================================
+// host side
+// ===========================================================================
+namespace {
+
+// policy ids of the reference (src/bsa/meshing/policy/MPolicy.F90:10-19)
+enum { POL_PRE_PEAK_1 = 3, POL_PRE_PEAK_2 = 4, POL_PEAK = 5, POL_PAD_EXTERN = 9, POL_BKG = 100 };
+
+struct MshInterval {
+   double a, b;      // [a, b] in Hz
+   int pol;
+   double w;         // narrowest interest-mode peak width of a PEAK interval, else
+                     // bkg_peakw (pre-peak gaps validate against the background
+                     // width, _shared_poly2d.fi:748-750)
+};
+
+// BFM factor of a policy (MPolicy_fromID bfm_pol_%i_fct_; the background zone
+// carries MPolicy_PEAK, BsaMesherImpl.F90:390)
+int msh_bfm_fct(int pol)
+{
+   switch (pol) {
+   case POL_PRE_PEAK_1: return 4;
+   case POL_PRE_PEAK_2: return 2;
+   case POL_PEAK: return 4;
+   case POL_PAD_EXTERN: return 1;
+   default: return 4;   // background
+   }
+}
+
+// level-0 spacing factor of an interval relative to the background spacing
+double msh_delta_fct(int pol)
+{
+   switch (pol) {
+   case POL_PRE_PEAK_1:
+   case POL_PRE_PEAK_2: return 2.0;
+   case POL_PAD_EXTERN: return 4.0;
+   default: return 1.0;
+   }
+}
+
+// Positive-axis limits as prefetchZoneLimits_ (BsaMesherImpl.F90:1638-1859) for
+// modes sorted by natural frequency; returns intervals from b up to the
+// maximum extension.
+int msh_axis(const bsahip_mesher_cfg *c, int nm, std::vector<MshInterval> &out, double &b,
+             double &fmax)
+{
+   std::vector<int> ord(nm);
+   for (int i = 0; i < nm; ++i) ord[i] = i;
+   std::sort(ord.begin(), ord.end(), [&](int x, int y) { return c->natf[x] < c->natf[y]; });
+   b = 0.5 * c->bkg_peakw * c->bkg_area_ext;
+   double fn_max = 0.0;
+   for (int i = 0; i < nm; ++i) fn_max = std::max(fn_max, c->natf[i]);
+   fmax = fn_max * c->max_area_ext;
+   out.clear();
+   std::vector<double> lo(nm), hi(nm), wd(nm);
+   for (int q = 0; q < nm; ++q) {
+      const int i = ord[q];
+      const double e = c->xsi[i] * c->natf[i] * c->peak_area_ext;
+      lo[q] = c->natf[i] - e;
+      hi[q] = c->natf[i] + e;
+      wd[q] = 2.0 * e;
+   }
+   int q = 0;
+   while (q < nm && hi[q] <= b) ++q;              // peaks entirely inside the background zone
+   if (q < nm) {
+      double cur_hi;
+      if (b < lo[q]) {
+         out.push_back({b, lo[q], POL_PRE_PEAK_1, c->bkg_peakw});
+         out.push_back({lo[q], hi[q], POL_PEAK, wd[q]});
+      } else {
+         out.push_back({b, hi[q], POL_PEAK, wd[q]});
+      }
+      cur_hi = hi[q];
+      for (++q; q < nm; ++q) {
+         if (lo[q] < cur_hi) {                     // overlap: extend the current peak zone
+            MshInterval &pk = out.back();
+            pk.b = std::max(pk.b, hi[q]);
+            pk.w = std::min(pk.w, wd[q]);
+            cur_hi = pk.b;
+         } else {
+            out.push_back({cur_hi, lo[q], POL_PRE_PEAK_2, c->bkg_peakw});
+            out.push_back({lo[q], hi[q], POL_PEAK, wd[q]});
+            cur_hi = hi[q];
+         }
+      }
+      if (fmax > cur_hi) out.push_back({cur_hi, fmax, POL_PAD_EXTERN, c->bkg_peakw});
+   } else if (fmax > b) {
+      out.push_back({b, fmax, POL_PAD_EXTERN, c->bkg_peakw});
+   }
+   for (auto &iv : out)
+      if (!(iv.b > iv.a)) return fail(BSAHIP_EINVAL, "degenerate Mesher zone limits");
+   return 0;
+}
+
+struct MshAxisCell {   // one interval of the full axis, cut into level-0 cells
+   double a, b;
+   int pol, ncell, sub;  // sub = BRM cells per level-0 cell
+   double w;
+};
+
+int msh_full_axis(const bsahip_mesher_cfg *c, int nm, std::vector<MshAxisCell> &ax, double &dfref)
+{
+   std::vector<MshInterval> pos;
+   double b, fmax;
+   CHK(msh_axis(c, nm, pos, b, fmax));
+   if (c->bkg_base_rfmnt < 2) return fail(BSAHIP_EINVAL, "bkg_base_rfmnt must be >= 2");
+   dfref = 2.0 * b / (c->bkg_base_rfmnt - 1);
+   auto cells = [&](double a, double bb, int pol, double w) {
+      const double h = dfref * msh_delta_fct(pol);
+      const int n = std::max(1, (int)std::ceil((bb - a) / h - 1e-9));
+      return MshAxisCell{a, bb, pol, n, 8, w};
+   };
+   ax.clear();
+   for (int i = (int)pos.size() - 1; i >= 0; --i)
+      ax.push_back(cells(-pos[i].b, -pos[i].a, pos[i].pol, pos[i].w));
+   ax.push_back(MshAxisCell{-b, b, POL_BKG, c->bkg_base_rfmnt - 1, 8, c->bkg_peakw});
+   for (auto &iv : pos) ax.push_back(cells(iv.a, iv.b, iv.pol, iv.w));
+   return 0;
+}
+
+constexpr int MSH_MAX_SUB = 256;   // BRM cells per level-0 cell, at most
+
+struct MshTile {
+   double a1, b1, a2, b2;   // f1 in [a1,b1], f2 in [a2,b2]
+   int n1, n2;              // level-0 cells
+   int pol1, pol2, sub1, sub2;
+   int rule;                // rule-based level
+   size_t lv0_off;          // offset (points) of the level-0 BFM in the resident store
+};
+
+// rule-based level on one axis: smallest l in {1,2,4,8} with h0/l <= w/fct
+int msh_rule_level(double h0, double w, int pol)
+{
+   const double lim = w / msh_bfm_fct(pol) * (1.0 + 1e-9);   // spacing ties count as met
+   int l = 1;
+   while (l < 8 && h0 / l > lim) l *= 2;
+   return l;
+}
+
+// BRM cells per level-0 cell of a tile along each axis: a power of two >= 8
+// (every refinement level divides it) such that the BRM spacing is at most
+// max(w_n/16, d_n/16) for every mode n, where d_n is the distance from the
+// tile's range on that axis to +-f_n (resonance of H_m(f1) or H_n(f2)) and,
+// with diag_crest, also from its range of f1+f2 (resonance of H_p(f1+f2),
+// the reference's diagonal crest, BSA_PREMESH_TYPE_DIAG_CREST_YES).  w_n is the
+// peak width 2 xsi_n f_n peak_area_ext: 16 BRM cells per peak width, fewer as
+// the FRF tail flattens with distance.
+void msh_brm_sub(MshTile &t, const bsahip_mesher_cfg *c, int nm)
+{
+   auto dist = [](double a, double b, double f) { return f < a ? a - f : (f > b ? f - b : 0.0); };
+   double r1 = 1e300, r2 = 1e300;
+   for (int i = 0; i < nm; ++i) {
+      const double w = 2.0 * c->xsi[i] * c->natf[i] * c->peak_area_ext;
+      for (int sg = -1; sg <= 1; sg += 2) {
+         const double f = sg * c->natf[i];
+         r1 = std::min(r1, std::max(w, dist(t.a1, t.b1, f)) / 16.0);
+         r2 = std::min(r2, std::max(w, dist(t.a2, t.b2, f)) / 16.0);
+         if (c->diag_crest) {
+            const double rd = std::max(w, dist(t.a1 + t.a2, t.b1 + t.b2, f)) / 16.0;
+            r1 = std::min(r1, rd);
+            r2 = std::min(r2, rd);
+         }
+      }
+   }
+   const double h1 = (t.b1 - t.a1) / t.n1, h2 = (t.b2 - t.a2) / t.n2;
+   t.sub1 = 8;
+   while (h1 / t.sub1 > r1 * (1.0 + 1e-9) && t.sub1 < MSH_MAX_SUB) t.sub1 *= 2;
+   t.sub2 = 8;
+   while (h2 / t.sub2 > r2 * (1.0 + 1e-9) && t.sub2 < MSH_MAX_SUB) t.sub2 *= 2;
+}
+
+void msh_tiles(const std::vector<MshAxisCell> &ax, const bsahip_mesher_cfg *cfg, int nm,
+               std::vector<MshTile> &tiles)
+{
+   const int tile_cells = cfg->tile_cells;
+   struct Seg { double a, b; int n, pol, sub; double w; };
+   std::vector<Seg> segs;
+   for (const auto &c : ax) {
+      const int nt = (c.ncell + tile_cells - 1) / tile_cells;
+      const double h = (c.b - c.a) / c.ncell;
+      int done = 0;
+      for (int t = 0; t < nt; ++t) {
+         const int n = (c.ncell - done + (nt - t) - 1) / (nt - t);   // balanced split
+         const double a = c.a + done * h, bb = (done + n == c.ncell) ? c.b : c.a + (done + n) * h;
+         segs.push_back({a, bb, n, c.pol, c.sub, c.w});
+         done += n;
+      }
+   }
+   tiles.clear();
+   size_t off = 0;
+   for (const auto &s2 : segs)
+      for (const auto &s1 : segs) {
+         MshTile t;
+         t.a1 = s1.a; t.b1 = s1.b; t.a2 = s2.a; t.b2 = s2.b;
+         t.n1 = s1.n; t.n2 = s2.n;
+         t.pol1 = s1.pol; t.pol2 = s2.pol; t.sub1 = s1.sub; t.sub2 = s2.sub;
+         t.rule = std::max(msh_rule_level((s1.b - s1.a) / s1.n, s1.w, s1.pol),
+                           msh_rule_level((s2.b - s2.a) / s2.n, s2.w, s2.pol));
+         msh_brm_sub(t, cfg, nm);
+         t.lv0_off = off;
+         off += (size_t)(t.n1 + 1) * (t.n2 + 1);
+         tiles.push_back(t);
+      }
+}
+
+}  // namespace
+
+struct bsahip_mesher {
+   Bufs B;
+   Prob D;
+   int nmp = 0;
+   std::vector<MshTile> tiles;
+   double dfref = 0.0;
+   double *lv0 = nullptr;     // resident level-0 BFM: [point][nm3]
+   double *PB = nullptr;      // [K][NMP]
+   long evals_lv0 = 0;
+};
+
+namespace {
+
+// BFM at the points (f1[q], f2[q]) into out + slot[q]*nm3, in chunks of at
+// most MSH_CHUNK points so the A tables stay small (3 distinct |f| per point
+// at most).
+constexpr int MSH_CHUNK = 8192;
+
+int msh_eval_chunk(bsahip_mesher *M, int npts, const double *f1, const double *f2,
+                   const long *slot_p, double *out);
+
+int msh_eval(bsahip_mesher *M, const std::vector<double> &f1, const std::vector<double> &f2,
+             const std::vector<long> &slot, double *out)
+{
+   for (size_t q0 = 0; q0 < f1.size(); q0 += MSH_CHUNK) {
+      const int n = (int)std::min<size_t>(MSH_CHUNK, f1.size() - q0);
+      CHK(msh_eval_chunk(M, n, f1.data() + q0, f2.data() + q0, slot.data() + q0, out));
+   }
+   return 0;
+}
+
+int msh_eval_chunk(bsahip_mesher *M, int npts, const double *f1p, const double *f2p,
+                   const long *slot_p, double *out)
+{
+   if (npts == 0) return 0;
+   Prob &D = M->D;
+   std::vector<double> uq;
+   std::vector<int> i1, i2, i3;
+   const std::vector<long> slot(slot_p, slot_p + npts);
+   point_tables(npts, f1p, f2p, uq, i1, i2, i3);
+   const int nu = (int)uq.size();
+   const long ldU = (long)D.nm * nu;
+   double *fl, *U, *V;
+   CHK(upload(M->B, "m_fl", uq, &fl));
+   CHK(M->B.get("m_U", (size_t)D.K * ldU, &U));
+   CHK(build_table(M->B, D, fl, nu, nu, U, ldU));
+   CHK(M->B.get("m_V", (size_t)nu * D.K * M->nmp, &V));
+   dim3 gv((nu + 63) / 64, D.K);
+   k_vtable<<<gv, 256, 0, g.s>>>(U, ldU, nu, D.nm, D.K, M->nmp, V);
+   HIPC(hipGetLastError());
+   int *d1, *d2, *d3;
+   long *ds;
+   CHK(upload(M->B, "m_i1", i1, &d1));
+   CHK(upload(M->B, "m_i2", i2, &d2));
+   CHK(upload(M->B, "m_i3", i3, &d3));
+   CHK(upload(M->B, "m_slot", slot, &ds));
+   const int ncol = D.nm * D.nm, ct = (ncol + 15) / 16, tpw = (ct + 7) / 8;
+#define LAUNCH_BATCH(MT, TPW)                                                                 \
+   k_bfm_batch<MT, TPW><<<npts, 512, 0, g.s>>>(V, M->PB, D.K, D.nm, npts, d1, d2, d3, ds, out)
+   if (M->nmp == 16) {
+      if (tpw <= 1) LAUNCH_BATCH(1, 1);
+      else LAUNCH_BATCH(1, 2);
+   } else if (M->nmp == 32) {
+      if (tpw <= 2) LAUNCH_BATCH(2, 2);
+      else if (tpw <= 4) LAUNCH_BATCH(2, 4);
+      else LAUNCH_BATCH(2, 8);
+   } else {
+      return fail(BSAHIP_EUNSUPPORTED, "point evaluation supports at most 32 modes");
+   }
+#undef LAUNCH_BATCH
+   HIPC(hipGetLastError());
+   return 0;
+}
+
+// Points of a tile's BFM grid at level l (q = i + j*nb1).
+void msh_grid(const MshTile &t, int l, std::vector<double> &f1, std::vector<double> &f2,
+              int &nb1, int &nb2)
+{
+   nb1 = t.n1 * l + 1;
+   nb2 = t.n2 * l + 1;
+   const double h1 = (t.b1 - t.a1) / (nb1 - 1), h2 = (t.b2 - t.a2) / (nb2 - 1);
+   f1.resize((size_t)nb1 * nb2);
+   f2.resize((size_t)nb1 * nb2);
+   for (int j = 0; j < nb2; ++j)
+      for (int i = 0; i < nb1; ++i) {
+         f1[i + (size_t)j * nb1] = (i == nb1 - 1) ? t.b1 : t.a1 + i * h1;
+         f2[i + (size_t)j * nb1] = (j == nb2 - 1) ? t.b2 : t.a2 + j * h2;
+      }
+}
+
+int msh_integrate(bsahip_mesher *M, const MshTile &t, int l, const double *bf, int nb1, int nb2,
+                  double *m3f, double *m3r, long &brm_pts)
+{
+   Prob &D = M->D;
+   const int s1 = std::max(1, t.sub1 / l), s2 = std::max(1, t.sub2 / l);
+   const double h1 = (t.b1 - t.a1) / ((nb1 - 1) * s1), h2 = (t.b2 - t.a2) / ((nb2 - 1) * s2);
+   const double ws = (2.0 * M_PI * h1) * (2.0 * M_PI * h2);
+   const int nm3 = D.nm * D.nm * D.nm;
+   const int ncell = (nb1 - 1) * (nb2 - 1);
+   const int bx = (nm3 + 255) / 256;
+   const int nchunk = std::max(1, std::min(ncell, std::max(1, 4096 / bx)));
+   dim3 gs(bx, nchunk);
+   k_msh_integrate<<<gs, 256, 0, g.s>>>(bf, D.nm, nb1, nb2, s1, s2, t.a1, t.a2, h1, h2, ws, D.Mm,
+                                         D.Km, D.Cm, m3f, m3r);
+   HIPC(hipGetLastError());
+   brm_pts += (long)((nb1 - 1) * s1 + 1) * ((nb2 - 1) * s2 + 1);
+   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bsahip_mesher_create(const bsahip_problem *p, const bsahip_mesher_cfg *cfg, int32_t flags,
+                         bsahip_mesher **out)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   if (!out || !cfg || !cfg->natf || !cfg->xsi) return fail(BSAHIP_EINVAL, "NULL Mesher argument");
+   *out = nullptr;
+   CHK(ensure_init());
+   if ((flags & BSAHIP_PREC_MASK) != BSAHIP_PREC_FP64)
+      return fail(BSAHIP_EUNSUPPORTED, "the Mesher runs in FP64");
+   if (!(cfg->bkg_peakw > 0.0) || !(cfg->bkg_area_ext > 0.0) || !(cfg->peak_area_ext > 0.0) ||
+       !(cfg->max_area_ext > 0.0) || cfg->tile_cells < 1)
+      return fail(BSAHIP_EINVAL, "Mesher extensions, bkg_peakw and tile_cells must be > 0");
+   std::unique_ptr<bsahip_mesher> M(new bsahip_mesher());
+   CHK(load_problem(p, 0, M->B, M->D));
+   Prob &D = M->D;
+   for (int i = 0; i < D.nm; ++i)
+      if (!(cfg->natf[i] > 0.0) || !(cfg->xsi[i] > 0.0))
+         return fail(BSAHIP_EINVAL, "natural frequencies and damping ratios must be > 0");
+   if (D.nm > 32) return fail(BSAHIP_EUNSUPPORTED, "the Mesher supports at most 32 modes");
+   M->nmp = D.nm <= 16 ? 16 : 32;
+   std::vector<MshAxisCell> ax;
+   CHK(msh_full_axis(cfg, D.nm, ax, M->dfref));
+   msh_tiles(ax, cfg, D.nm, M->tiles);
+   CHK(M->B.get("m_PB", (size_t)D.K * M->nmp, &M->PB));
+   k_pbt<<<(unsigned)((D.K * M->nmp + 255) / 256), 256, 0, g.s>>>(D.pb_k, D.nm, D.K, M->nmp, M->PB);
+   HIPC(hipGetLastError());
+   // pre-mesh: level-0 BFM of every tile, kept resident
+   const size_t nm3 = (size_t)D.nm * D.nm * D.nm;
+   size_t npts = 0;
+   for (const auto &t : M->tiles) npts += (size_t)(t.n1 + 1) * (t.n2 + 1);
+   CHK(M->B.get("m_lv0", npts * nm3, &M->lv0));
+   std::vector<double> f1, f2, a1, a2;
+   std::vector<long> slot;
+   const size_t batch = MSH_CHUNK;
+   for (size_t ti = 0; ti < M->tiles.size(); ++ti) {
+      int nb1, nb2;
+      msh_grid(M->tiles[ti], 1, a1, a2, nb1, nb2);
+      for (size_t q = 0; q < a1.size(); ++q) {
+         f1.push_back(a1[q]);
+         f2.push_back(a2[q]);
+         slot.push_back((long)(M->tiles[ti].lv0_off + q));
+      }
+      if (f1.size() >= batch || ti + 1 == M->tiles.size()) {
+         CHK(msh_eval(M.get(), f1, f2, slot, M->lv0));
+         HIPC(hipStreamSynchronize(g.s));
+         f1.clear();
+         f2.clear();
+         slot.clear();
+      }
+   }
+   M->evals_lv0 = (long)npts;
+   *out = M.release();
+   return 0;
+}
+
+int bsahip_mesher_layout(const bsahip_mesher_cfg *cfg, int32_t nm, int32_t max_tiles,
+                         double *geom, int32_t *rule_level, int32_t *ntiles)
+{
+   if (!cfg || !cfg->natf || !cfg->xsi || !ntiles || nm < 1)
+      return fail(BSAHIP_EINVAL, "NULL Mesher argument or nm < 1");
+   if (!(cfg->bkg_peakw > 0.0) || !(cfg->bkg_area_ext > 0.0) || !(cfg->peak_area_ext > 0.0) ||
+       !(cfg->max_area_ext > 0.0) || cfg->tile_cells < 1)
+      return fail(BSAHIP_EINVAL, "Mesher extensions, bkg_peakw and tile_cells must be > 0");
+   std::vector<MshAxisCell> ax;
+   double dfref;
+   CHK(msh_full_axis(cfg, nm, ax, dfref));
+   std::vector<MshTile> tiles;
+   msh_tiles(ax, cfg, nm, tiles);
+   *ntiles = (int32_t)tiles.size();
+   if (!geom && !rule_level) return 0;
+   if ((int32_t)tiles.size() > max_tiles) return fail(BSAHIP_EINVAL, "max_tiles too small");
+   for (size_t i = 0; i < tiles.size(); ++i) {
+      const MshTile &t = tiles[i];
+      if (geom) {
+         double *gq = geom + 10 * i;
+         gq[0] = t.a1; gq[1] = t.b1; gq[2] = t.a2; gq[3] = t.b2;
+         gq[4] = t.n1; gq[5] = t.n2; gq[6] = t.pol1; gq[7] = t.pol2;
+         gq[8] = t.sub1; gq[9] = t.sub2;
+      }
+      if (rule_level) rule_level[i] = t.rule;
+   }
+   return 0;
+}
+
+int bsahip_mesher_ntiles(const bsahip_mesher *m, int32_t *ntiles)
+{
+   if (!m || !ntiles) return fail(BSAHIP_EINVAL, "NULL argument");
+   *ntiles = (int32_t)m->tiles.size();
+   return 0;
+}
+
+int bsahip_mesher_tiles(const bsahip_mesher *m, double *geom, int32_t *rule_level)
+{
+   if (!m) return fail(BSAHIP_EINVAL, "NULL Mesher");
+   for (size_t i = 0; i < m->tiles.size(); ++i) {
+      const MshTile &t = m->tiles[i];
+      if (geom) {
+         double *gq = geom + 10 * i;
+         gq[0] = t.a1; gq[1] = t.b1; gq[2] = t.a2; gq[3] = t.b2;
+         gq[4] = t.n1; gq[5] = t.n2; gq[6] = t.pol1; gq[7] = t.pol2;
+         gq[8] = t.sub1; gq[9] = t.sub2;
+      }
+      if (rule_level) rule_level[i] = t.rule;
+   }
+   return 0;
+}
+
+int bsahip_mesher_features(bsahip_mesher *m, int32_t ns, double *stencil)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   if (!m || !stencil || ns < 2) return fail(BSAHIP_EINVAL, "need a Mesher, ns >= 2 and an output");
+   const int nm3 = m->D.nm * m->D.nm * m->D.nm;
+   const size_t nt = m->tiles.size(), per = (size_t)ns * ns;
+   double *d;
+   CHK(m->B.get("m_stencil", nt * per, &d));
+   for (size_t i = 0; i < nt; ++i) {
+      const MshTile &t = m->tiles[i];
+      k_bfm_stencil<<<(unsigned)per, 64, 0, g.s>>>(m->lv0 + t.lv0_off * nm3, nm3, t.n1 + 1,
+                                                   t.n2 + 1, ns, d + i * per);
+      HIPC(hipGetLastError());
+   }
+   HIPC(hipMemcpyAsync(stencil, d, nt * per * sizeof(double), hipMemcpyDeviceToHost, g.s));
+   HIPC(hipStreamSynchronize(g.s));
+   return 0;
+}
+
+int bsahip_mesher_run(bsahip_mesher *m, const int32_t *levels, double *m3mf, double *m3mr,
+                      int64_t *counts)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   if (!m) return fail(BSAHIP_EINVAL, "NULL Mesher");
+   Prob &D = m->D;
+   const size_t nm3 = (size_t)D.nm * D.nm * D.nm;
+   for (size_t i = 0; i < m->tiles.size(); ++i) {
+      const int l = levels ? levels[i] : m->tiles[i].rule;
+      if (!(l == 1 || l == 2 || l == 4 || l == 8))
+         return fail(BSAHIP_EINVAL, "tile levels must be 1, 2, 4 or 8");
+   }
+   double *acc;
+   CHK(m->B.get("m_acc", 2 * nm3, &acc));
+   CHK(fill(acc, 2 * nm3, 0.0));
+   long evals_post = 0, brm = 0;
+   std::vector<double> f1, f2, e1, e2;
+   std::vector<long> slot;
+   for (size_t i = 0; i < m->tiles.size(); ++i) {
+      const MshTile &t = m->tiles[i];
+      const int l = levels ? levels[i] : t.rule;
+      int nb1, nb2;
+      const double *bf = m->lv0 + t.lv0_off * nm3;
+      if (l > 1) {
+         msh_grid(t, l, f1, f2, nb1, nb2);
+         double *fine;
+         CHK(m->B.get("m_fine", (size_t)nb1 * nb2 * nm3, &fine));
+         // level-0 points are reused; only the new points are evaluated
+         e1.clear();
+         e2.clear();
+         slot.clear();
+         for (int j = 0; j < nb2; ++j)
+            for (int ii = 0; ii < nb1; ++ii)
+               if ((ii % l) || (j % l)) {
+                  const size_t q = ii + (size_t)j * nb1;
+                  e1.push_back(f1[q]);
+                  e2.push_back(f2[q]);
+                  slot.push_back((long)q);
+               }
+         CHK(msh_eval(m, e1, e2, slot, fine));
+         evals_post += (long)e1.size();
+         // copy the level-0 points into the fine grid
+         for (int j = 0; j <= t.n2; ++j)
+            HIPC(hipMemcpy2DAsync(fine + ((size_t)j * l * nb1) * nm3, (size_t)l * nm3 * sizeof(double),
+                                  bf + ((size_t)j * (t.n1 + 1)) * nm3, nm3 * sizeof(double),
+                                  nm3 * sizeof(double), t.n1 + 1, hipMemcpyDeviceToDevice, g.s));
+         CHK(msh_integrate(m, t, l, fine, nb1, nb2, acc, acc + nm3, brm));
+         HIPC(hipStreamSynchronize(g.s));
+      } else {
+         CHK(msh_integrate(m, t, 1, bf, t.n1 + 1, t.n2 + 1, acc, acc + nm3, brm));
+      }
+   }
+   if (m3mf) HIPC(hipMemcpyAsync(m3mf, acc, nm3 * sizeof(double), hipMemcpyDeviceToHost, g.s));
+   if (m3mr) HIPC(hipMemcpyAsync(m3mr, acc + nm3, nm3 * sizeof(double), hipMemcpyDeviceToHost, g.s));
+   HIPC(hipStreamSynchronize(g.s));
+   if (counts) {
+      counts[0] = m->evals_lv0;
+      counts[1] = evals_post;
+      counts[2] = brm;
+   }
+   return 0;
+}
+
+int bsahip_mesher_destroy(bsahip_mesher *m)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   delete m;
+   return 0;
+}
+
+}  // extern "C"

@@ -5,6 +5,10 @@
 * build_psi     setPhitimesCLocalInstance_ (src/bsa/BsaLibImpl.F90:611-671):
                 Psi(im, in, id) = sum_lib wfc(lib, id, in) * phi(dof(n, lib), modes(im)).
 * skewness      computeSkewness_ (src/bsa/BsaLibImpl.F90:1630-1707).
+* bkg_peak_width  validateAll_ (BsaLibImpl.F90:513-527) + computeBKGPeakWidths
+                (src/structure/StructureImpl.F90:225-260): max over (component,
+                direction) of U_ref/L, with L/U_ref maximised over wind zones first.
+* damping_ratios  xsi_m = C_mm / (2 sqrt(K_m M_m)) of the kept modes.
 
 These are O(NDOFS*NM) host loops (vectorised with numpy); everything O(NF^2)
 runs in libbsahip.
@@ -46,3 +50,19 @@ def skewness(m2, m3, nm: int) -> np.ndarray:
     sig = np.sqrt(np.diagonal(m2))
     den = sig[:, None, None] * sig[None, :, None] * sig[None, None, :]
     return (np.asarray(m3).reshape(nm, nm, nm, order="F") / den).reshape(-1, order="F")
+
+
+def bkg_peak_width(prob, restrict: bool = False) -> float:
+    L = np.asarray(prob.turb_scales, dtype=np.float64)            # (3, 3, NWZ)
+    U = np.asarray(prob.u_mean_ref, dtype=np.float64)             # (NWZ,)
+    v = (L / U[None, None, :]).max(axis=2)                        # (3, 3)
+    w = np.where(v != 0.0, 1.0 / np.where(v != 0.0, v, 1.0), 0.0)
+    # bsa_setMaxBkgPeakRestriction: only the first column (BsaMesherImpl.F90:298-302)
+    return float(w[:, 0].max() if restrict else w.max())
+
+
+def damping_ratios(prob, modes=None) -> np.ndarray:
+    modes = kept_modes(prob) if modes is None else np.asarray(modes)
+    mo = modes - 1
+    c = np.diag(np.asarray(prob.Cm))[mo]
+    return c / (2.0 * np.sqrt(np.asarray(prob.Km)[mo] * np.asarray(prob.Mm)[mo]))

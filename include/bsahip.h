@@ -157,6 +157,57 @@ int bsahip_selftest_mfma_bf16_layout(float *out768);
  */
 int bsahip_last_kernel_time(double *ms, int32_t *launches);
 
+/* ---- Mesher: zone quadrature with the zone data resident on the device ----
+ * Zone layout (src/bsa/meshing/BsaMesherImpl.F90:266-1393, 1638-1890): a
+ * background zone |f| <= b (b = bkg_peakw*bkg_area_ext/2), peak zones
+ * f_n +- xsi_n f_n peak_area_ext, pre-peak gaps, external padding up to
+ * max(f_n)*max_area_ext, on both axes; rectangular zones are the tensor
+ * products of these intervals, cut into tiles of at most tile_cells^2
+ * level-0 cells.  Each tile's force bispectrum is sampled at a refinement
+ * level (1, 2, 4 or 8 cells per level-0 cell), interpolated bilinearly onto
+ * the tile's BRM grid, multiplied by the FRF product and integrated with
+ * trapezoid weights (zones/_shared_poly2d.fi:827-832).  S(f1+f2) is evaluated
+ * exactly (functionsImpl.F90:247-465). */
+typedef struct bsahip_mesher_cfg {
+   int32_t bkg_base_rfmnt;    /* level-0 points per side of the background zone (reference default 20) */
+   int32_t tile_cells;        /* max level-0 cells per tile side (e.g. 32) */
+   int32_t diag_crest;        /* 1: BRM grids also resolve the f1+f2 = +-f_n crests
+                                 (BSA_PREMESH_TYPE_DIAG_CREST_YES); reference default 0 */
+   int32_t reserved;          /* 0 */
+   double bkg_peakw;          /* background peak width [Hz] (max over L/U ratios: U/L) */
+   double bkg_area_ext;       /* reference default 2 */
+   double peak_area_ext;      /* reference default 2 */
+   double max_area_ext;       /* reference default 2 */
+   const double *natf;        /* (nm) natural frequencies of the kept modes [Hz] */
+   const double *xsi;         /* (nm) damping ratios of the kept modes */
+} bsahip_mesher_cfg;
+
+typedef struct bsahip_mesher bsahip_mesher;
+
+/* Builds the layout and evaluates the level-0 BFM of every tile (pre-mesh),
+ * kept resident until bsahip_mesher_destroy.  FP64 only; at most 32 modes. */
+int bsahip_mesher_create(const bsahip_problem *p, const bsahip_mesher_cfg *cfg, int32_t flags,
+                         bsahip_mesher **out);
+/* The tile layout alone (host computation, no device): *ntiles always; geom
+ * and rule_level as bsahip_mesher_tiles when given (max_tiles entries). */
+int bsahip_mesher_layout(const bsahip_mesher_cfg *cfg, int32_t nm, int32_t max_tiles,
+                         double *geom, int32_t *rule_level, int32_t *ntiles);
+int bsahip_mesher_ntiles(const bsahip_mesher *m, int32_t *ntiles);
+/* geom (ntiles x 10): f1a f1b f2a f2b n1 n2 policy1 policy2 sub1 sub2 -- the tile
+ * rectangle, its level-0 cells per axis, the interval policies (reference MPolicy
+ * ids, 100 = background) and the BRM cells per level-0 cell per axis;
+ * rule_level (ntiles): the rule-based refinement level. */
+int bsahip_mesher_tiles(const bsahip_mesher *m, double *geom, int32_t *rule_level);
+/* stencil (ntiles x ns^2): ||B(f1,f2)||_2 over all mode triples at the level-0
+ * point nearest to each node of an ns x ns stencil over the tile. */
+int bsahip_mesher_features(bsahip_mesher *m, int32_t ns, double *stencil);
+/* Post-mesh: levels (ntiles, values 1/2/4/8) or NULL for the rule-based levels;
+ * m3mf, m3mr (NM^3); counts[3] = {level-0 BFM evaluations, refinement BFM
+ * evaluations of this run, BRM points of this run}. */
+int bsahip_mesher_run(bsahip_mesher *m, const int32_t *levels, double *m3mf, double *m3mr,
+                      int64_t *counts);
+int bsahip_mesher_destroy(bsahip_mesher *m);
+
 #ifdef __cplusplus
 }
 #endif

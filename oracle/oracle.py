@@ -475,3 +475,62 @@ def reference_time(prob, threads: int) -> float:
         if line.startswith("REFTIME"):
             return float(line.split()[1])
     raise RuntimeError("reference timing line missing:\n" + r.stdout[-2000:])
+
+
+def mesher_integrate(prob, geom, levels, modes=None):
+    """Zone quadrature of the Mesher on given tiles (CPU restatement of
+    bsahip_mesher_run): per tile, the force bispectrum at the level-l grid by
+    getFM_full_tnm_scalar_msh_ (msh_points), bilinear interpolation onto the
+    BRM grid (16 BRM cells per level-0 cell across a PEAK interval, 8
+    otherwise), the FRF product Re[H_m(f1) H_n(f2) conj(H_p(f1+f2))] and
+    trapezoid weights (zones/_shared_poly2d.fi:827-832: vertex 1/4, edge 1/2,
+    interior 1, times dw1 dw2).  geom rows: f1a f1b f2a f2b n1 n2 pol1 pol2
+    sub1 sub2 (sub = BRM cells per level-0 cell).  Returns m3mf, m3mr and the
+    BFM evaluation counts (level-0, refinement)."""
+    modes = kept_modes(prob) if modes is None else np.asarray(modes)
+    nm = len(modes)
+    nm3 = nm ** 3
+    m3f = np.zeros(nm3)
+    m3r = np.zeros(nm3)
+    ev0 = ev1 = 0
+    for t, lv in zip(np.asarray(geom), np.broadcast_to(np.asarray(levels), (len(geom),))):
+        a1, b1, a2, b2 = t[:4]
+        n1, n2 = int(t[4]), int(t[5])
+        sub1, sub2 = int(t[8]), int(t[9])
+        lv = int(lv)
+        nb1, nb2 = n1 * lv + 1, n2 * lv + 1
+        g1 = a1 + np.arange(nb1) * (b1 - a1) / (nb1 - 1)
+        g2 = a2 + np.arange(nb2) * (b2 - a2) / (nb2 - 1)
+        g1[-1], g2[-1] = b1, b2
+        F1, F2 = np.meshgrid(g1, g2, indexing="ij")
+        B = msh_points(prob, F1.ravel(), F2.ravel()).reshape(nm3, nb1, nb2)
+        ev0 += (n1 + 1) * (n2 + 1)
+        if lv > 1:
+            ev1 += nb1 * nb2 - (n1 + 1) * (n2 + 1)
+        s1, s2 = max(1, sub1 // lv), max(1, sub2 // lv)
+        NA, NB = (nb1 - 1) * s1, (nb2 - 1) * s2
+        h1, h2 = (b1 - a1) / NA, (b2 - a2) / NB
+
+        def interp(nb, s, N):
+            P = np.zeros((N + 1, nb))
+            for a in range(N + 1):
+                c = min(a // s, nb - 2)
+                u = (a - c * s) / s
+                P[a, c] += 1.0 - u
+                P[a, c + 1] += u
+            return P
+        P1, P2 = interp(nb1, s1, NA), interp(nb2, s2, NB)
+        Bb = np.einsum("ai,xij,bj->xab", P1, B, P2)                    # (nm3, NA+1, NB+1)
+        f1 = a1 + np.arange(NA + 1) * h1
+        f2 = a2 + np.arange(NB + 1) * h2
+        w1 = np.ones(NA + 1); w1[[0, -1]] = 0.5
+        w2 = np.ones(NB + 1); w2[[0, -1]] = 0.5
+        W = np.outer(w1, w2) * (TWO_PI * h1) * (TWO_PI * h2)
+        H1 = frf(prob, f1, modes)                                       # (NA+1, nm)
+        H2 = frf(prob, f2, modes)                                       # (NB+1, nm)
+        H3 = frf(prob, f1[:, None] + f2[None, :], modes)               # (NA+1, NB+1, nm)
+        R = np.einsum("am,bn,abp->mnpab", H1, H2, np.conj(H3)).real.reshape(nm3, NA + 1, NB + 1,
+                                                                           order="F")
+        m3f += np.einsum("xab,ab->x", Bb, W)
+        m3r += np.einsum("xab,xab,ab->x", Bb, R, W)
+    return {"m3mf": m3f, "m3mr": m3r, "evals_pre": ev0, "evals_post": ev1}
