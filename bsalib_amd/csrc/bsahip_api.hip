@@ -729,3 +729,4 @@ int bsahip_selftest_mfma_bf16_layout(float *out768)
 }  // extern "C"
 
 #include "mesher.hip"
+#include "mlp.hip"

@@ -208,6 +208,33 @@ int bsahip_mesher_run(bsahip_mesher *m, const int32_t *levels, double *m3mf, dou
                       int64_t *counts);
 int bsahip_mesher_destroy(bsahip_mesher *m);
 
+/* ---- Learned Mesher proposer: a multilayer perceptron trained and evaluated
+ * on the device (no counterpart in the reference; it proposes the per-tile
+ * refinement levels that the rule-based delta validation would otherwise set).
+ * depth linear layers: nin -> width -> ... -> width -> 1, GELU between them,
+ * FP32 parameters; inputs are normalised as (x - mean) * inv_scale.
+ * Parameter vector layout: for each layer l, W_l [out][in] then b_l [out]. */
+typedef struct bsahip_mlp bsahip_mlp;
+int bsahip_mlp_create(int32_t nin, int32_t width, int32_t depth, uint64_t seed, bsahip_mlp **out);
+int bsahip_mlp_nparams(const bsahip_mlp *mlp, int64_t *n);
+/* any of params (nparams), mean, inv_scale (nin) may be NULL */
+int bsahip_mlp_set_params(bsahip_mlp *mlp, const float *params, const float *mean,
+                          const float *inv_scale);
+int bsahip_mlp_get_params(bsahip_mlp *mlp, float *params, float *grads);
+/* logits (n) for rows x (n x nin, row-major) */
+int bsahip_mlp_forward(bsahip_mlp *mlp, const float *x, int32_t n, float *logits);
+/* weighted binary cross-entropy on logits and its gradient (left in the MLP,
+ * copied to grads when non-NULL) */
+int bsahip_mlp_loss_grad(bsahip_mlp *mlp, const float *x, const float *y, const float *w,
+                         int32_t n, float *loss, float *grads);
+/* minibatch Adam (learning rate lr; weight_decay * p is added to each gradient)
+ * over `epochs` seeded permutations of the rows; losses (epochs) = mean batch
+ * loss per epoch, may be NULL */
+int bsahip_mlp_train(bsahip_mlp *mlp, const float *x, const float *y, const float *w, int32_t n,
+                     int32_t epochs, int32_t batch, float lr, float weight_decay, uint64_t seed,
+                     float *losses);
+int bsahip_mlp_destroy(bsahip_mlp *mlp);
+
 #ifdef __cplusplus
 }
 #endif
