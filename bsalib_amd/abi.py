@@ -52,6 +52,7 @@ EXPORTS = ["bsahip_init", "bsahip_finalise", "bsahip_last_error", "bsahip_versio
            "bsahip_last_kernel_time", "bsahip_plan_create", "bsahip_plan_run",
            "bsahip_plan_destroy", "bsahip_mesher_create", "bsahip_mesher_layout", "bsahip_mesher_ntiles",
            "bsahip_mesher_tiles", "bsahip_mesher_features", "bsahip_mesher_run",
+           "bsahip_mesher_run_tiles",
            "bsahip_mesher_destroy"]
 
 
@@ -76,6 +77,9 @@ def lib():
         L.bsahip_mesher_features.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]
         L.bsahip_mesher_run.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double),
                                         C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.bsahip_mesher_run_tiles.argtypes = [C.c_void_p, C.POINTER(C.c_int32),
+                                              C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                              C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         L.bsahip_mesher_destroy.argtypes = [C.c_void_p]
         _lib = L
     return _lib
@@ -302,7 +306,9 @@ class Mesher:
         _check(lib().bsahip_mesher_features(self.h, C.c_int32(ns), _ptr(out, C.c_double)))
         return out
 
-    def run(self, levels=None) -> dict:
+    def run(self, levels=None, per_tile: bool = False) -> dict:
+        """Post-mesh with per-tile levels (None: rule-based); per_tile adds
+        each tile's m3mr contribution as "tile_m3mr" (ntiles, NM^3)."""
         nm3 = self.nm ** 3
         m3mf, m3mr = np.zeros(nm3), np.zeros(nm3)
         counts = np.zeros(3, dtype=np.int64)
@@ -310,10 +316,15 @@ class Mesher:
         if levels is not None:
             lv = np.ascontiguousarray(np.broadcast_to(np.asarray(levels, dtype=np.int32),
                                                       (self.ntiles,)))
-        _check(lib().bsahip_mesher_run(self.h, _ptr(lv, C.c_int32), _ptr(m3mf, C.c_double),
-                                       _ptr(m3mr, C.c_double), _ptr(counts, C.c_int64)))
-        return {"m3mf": m3mf, "m3mr": m3mr, "evals_pre": int(counts[0]),
-                "evals_post": int(counts[1]), "brm_points": int(counts[2]), "modes": self.modes}
+        tiles = np.zeros((self.ntiles, nm3)) if per_tile else None
+        _check(lib().bsahip_mesher_run_tiles(self.h, _ptr(lv, C.c_int32), _ptr(tiles, C.c_double),
+                                             _ptr(m3mf, C.c_double), _ptr(m3mr, C.c_double),
+                                             _ptr(counts, C.c_int64)))
+        out = {"m3mf": m3mf, "m3mr": m3mr, "evals_pre": int(counts[0]),
+               "evals_post": int(counts[1]), "brm_points": int(counts[2]), "modes": self.modes}
+        if per_tile:
+            out["tile_m3mr"] = tiles
+        return out
 
     def close(self):
         if self.h:

@@ -63,7 +63,11 @@ struct Bufs {
       Buf &b = m[name];
       const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
       if (b.n < bytes) {
-         if (b.p) (void)hipFree(b.p);
+         // work queued earlier may still read the old buffer
+         if (b.p) {
+            (void)hipDeviceSynchronize();
+            (void)hipFree(b.p);
+         }
          b.p = nullptr;
          b.n = 0;
          hipError_t e = hipMalloc(&b.p, bytes);

@@ -171,17 +171,33 @@ __global__ __launch_bounds__(512) void k_bfm_batch(
 
 // ---------------------------------------------------------------------------
 // Tile integration: bilinear BFM on the BRM grid, exact FRF, trapezoid weights.
-// BFM grid nb1 x nb2 (point q = i + j*nb1, values bf[q*nm3 + mnp]); each BFM
+// BFM grid nb1 x nb2 (point q = i + j*ldq, values bf[q*nm3 + mnp]); each BFM
 // cell holds s1 x s2 BRM cells.  BRM point (a, b): f1 = f1a + a*h1,
 // f2 = f2a + b*h2.  Adds sum w*B into m3f and sum w*B*Re[H_m(f1)H_n(f2)H_p*(f1+f2)]
 // into m3r.  grid (ceil(nm3/256), nchunk), block 256.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_msh_integrate(
-   const double *__restrict__ bf, int nm, int nb1, int nb2, int s1, int s2, double f1a,
-   double f2a, double h1, double h2, double wscale, const double *__restrict__ Mm,
-   const double *__restrict__ Km, const double *__restrict__ Cm, double *__restrict__ m3f,
-   double *__restrict__ m3r)
+// FRF table H[i*nm + m] = 1/(K_m - w^2 M_m + i w C_mm), w = 2 pi (f0 + i h)
+__global__ void k_frf_axis(double f0, double h, int n, const double *__restrict__ Mm,
+                           const double *__restrict__ Km, const double *__restrict__ Cm, int nm,
+                           double2 *__restrict__ H)
 {
+   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= n * nm) return;
+   const int i = t / nm, m = t % nm;
+   const double w = 2.0 * M_PI * (f0 + i * h);
+   const double r = Km[m] - w * w * Mm[m], im = w * Cm[m], d = r * r + im * im;
+   H[t] = make_double2(r / d, -im / d);
+}
+
+__global__ __launch_bounds__(256) void k_msh_integrate(
+   const double *__restrict__ bf, long ldq, int nm, int nb1, int nb2, int s1, int s2,
+   const double2 *__restrict__ H1, const double2 *__restrict__ H2,
+   const double2 *__restrict__ H3, double f3a, double h1, double h2, double wscale,
+   const double *__restrict__ Mm, const double *__restrict__ Km, const double *__restrict__ Cm,
+   double *__restrict__ m3f, double *__restrict__ m3r)
+{
+   // H1[a*nm + m] at f1 = f1a + a h1, H2[b*nm + n] at f2 = f2a + b h2; H3[(a+b)*nm + p]
+   // at f1 + f2 when h1 == h2 (H3 != NULL), else H_p(f1+f2) is evaluated here
    const int nm3 = nm * nm * nm;
    const int mnp = blockIdx.x * blockDim.x + threadIdx.x;
    if (mnp >= nm3) return;
@@ -190,36 +206,36 @@ __global__ __launch_bounds__(256) void k_msh_integrate(
    const int per = (ncell + gridDim.y - 1) / gridDim.y;
    const int c0 = blockIdx.y * per, c1 = min(ncell, c0 + per);
    const int NA = nc1 * s1, NB = nc2 * s2;          // last BRM index per axis
-   const double Mmm = Mm[m], Kmm = Km[m], Cmm = Cm[m];
-   const double Mn = Mm[n], Kn = Km[n], Cn = Cm[n];
    const double Mp = Mm[p], Kp = Km[p], Cp = Cm[p];
    const double inv1 = 1.0 / s1, inv2 = 1.0 / s2;
    double sf = 0.0, sr = 0.0;
    for (int c = c0; c < c1; ++c) {
       const int ci = c % nc1, cj = c / nc1;
-      const size_t q00 = (size_t)ci + (size_t)cj * nb1;
+      const size_t q00 = (size_t)ci + (size_t)cj * ldq;
       const double b00 = bf[q00 * nm3 + mnp], b10 = bf[(q00 + 1) * nm3 + mnp];
-      const double b01 = bf[(q00 + nb1) * nm3 + mnp], b11 = bf[(q00 + nb1 + 1) * nm3 + mnp];
+      const double b01 = bf[(q00 + ldq) * nm3 + mnp], b11 = bf[(q00 + ldq + 1) * nm3 + mnp];
       const int ea = (ci == nc1 - 1) ? s1 : s1 - 1;   // owned BRM points per axis
       const int eb = (cj == nc2 - 1) ? s2 : s2 - 1;
       for (int bb = 0; bb <= eb; ++bb) {
          const int b = cj * s2 + bb;
-         const double f2 = f2a + b * h2, w2 = 2.0 * M_PI * f2;
          const double wb = (b == 0 || b == NB) ? 0.5 : 1.0;
          const double sv = bb * inv2;
-         double r = Kn - w2 * w2 * Mn, im = w2 * Cn, d = r * r + im * im;
-         const double2 h2v = make_double2(r / d, -im / d);
+         const double2 h2v = H2[(size_t)b * nm + n];
+         const double e0 = (1.0 - sv) * b00 + sv * b01, e1 = (1.0 - sv) * b10 + sv * b11;
          for (int aa = 0; aa <= ea; ++aa) {
             const int a = ci * s1 + aa;
-            const double f1 = f1a + a * h1, w1 = 2.0 * M_PI * f1, w3 = w1 + w2;
             const double wa = (a == 0 || a == NA) ? 0.5 : 1.0;
             const double t = aa * inv1;
-            const double bv = (1.0 - t) * (1.0 - sv) * b00 + t * (1.0 - sv) * b10 +
-                              (1.0 - t) * sv * b01 + t * sv * b11;
-            r = Kmm - w1 * w1 * Mmm; im = w1 * Cmm; d = r * r + im * im;
-            const double2 h1v = make_double2(r / d, -im / d);
-            r = Kp - w3 * w3 * Mp; im = w3 * Cp; d = r * r + im * im;
-            const double2 h3v = make_double2(r / d, -im / d);
+            const double bv = (1.0 - t) * e0 + t * e1;
+            const double2 h1v = H1[(size_t)a * nm + m];
+            double2 h3v;
+            if (H3) {
+               h3v = H3[(size_t)(a + b) * nm + p];
+            } else {
+               const double w3 = 2.0 * M_PI * (f3a + a * h1 + b * h2);
+               const double r = Kp - w3 * w3 * Mp, im = w3 * Cp, d = r * r + im * im;
+               h3v = make_double2(r / d, -im / d);
+            }
             const double w = wa * wb;
             sf += w * bv;
             sr += w * bv * re3(h1v, h2v, h3v);
@@ -230,16 +246,28 @@ __global__ __launch_bounds__(256) void k_msh_integrate(
    atomicAdd(m3r + mnp, sr * wscale);
 }
 
+// Level-0 points of a tile into its level-l grid:
+// fine[(i*l + j*l*nb1)] = lv0[i + j*ldq].  grid (ceil(nm3/256), (n1+1)*(n2+1))
+__global__ void k_lv0_to_fine(const double *__restrict__ lv0, long ldq, int n1, int l, int nb1,
+                              int nm3, double *__restrict__ fine)
+{
+   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= nm3) return;
+   const int q = blockIdx.y, i = q % (n1 + 1), j = q / (n1 + 1);
+   fine[((size_t)i * l + (size_t)j * l * nb1) * nm3 + t] =
+      lv0[((size_t)i + (size_t)j * ldq) * nm3 + t];
+}
+
 // Feature stencil of a tile's level-0 BFM: out[s] = sqrt(sum_mnp B^2) at the
 // level-0 point nearest to stencil node s (ns x ns nodes over the tile).
-__global__ void k_bfm_stencil(const double *__restrict__ bf, int nm3, int nb1, int nb2, int ns,
-                              double *__restrict__ out)
+__global__ void k_bfm_stencil(const double *__restrict__ bf, long ldq, int nm3, int nb1, int nb2,
+                              int ns, double *__restrict__ out)
 {
    const int s = blockIdx.x;
    const int si = s % ns, sj = s / ns;
    const int i = (int)lround((double)si * (nb1 - 1) / (ns - 1));
    const int j = (int)lround((double)sj * (nb2 - 1) / (ns - 1));
-   const double *b = bf + ((size_t)i + (size_t)j * nb1) * nm3;
+   const double *b = bf + ((size_t)i + (size_t)j * ldq) * nm3;
    double acc = 0.0;
    for (int t = threadIdx.x; t < nm3; t += 64) acc += b[t] * b[t];
    acc = wave_sum(acc);
@@ -373,7 +401,7 @@ struct MshTile {
    int n1, n2;              // level-0 cells
    int pol1, pol2, sub1, sub2;
    int rule;                // rule-based level
-   size_t lv0_off;          // offset (points) of the level-0 BFM in the resident store
+   int g1, g2;              // first level-0 node of the tile on each axis in the global grid
 };
 
 // rule-based level on one axis: smallest l in {1,2,4,8} with h0/l <= w/fct
@@ -421,8 +449,9 @@ void msh_tiles(const std::vector<MshAxisCell> &ax, const bsahip_mesher_cfg *cfg,
                std::vector<MshTile> &tiles)
 {
    const int tile_cells = cfg->tile_cells;
-   struct Seg { double a, b; int n, pol, sub; double w; };
+   struct Seg { double a, b; int n, pol, sub; double w; int g; };
    std::vector<Seg> segs;
+   int g = 0;
    for (const auto &c : ax) {
       const int nt = (c.ncell + tile_cells - 1) / tile_cells;
       const double h = (c.b - c.a) / c.ncell;
@@ -430,25 +459,38 @@ void msh_tiles(const std::vector<MshAxisCell> &ax, const bsahip_mesher_cfg *cfg,
       for (int t = 0; t < nt; ++t) {
          const int n = (c.ncell - done + (nt - t) - 1) / (nt - t);   // balanced split
          const double a = c.a + done * h, bb = (done + n == c.ncell) ? c.b : c.a + (done + n) * h;
-         segs.push_back({a, bb, n, c.pol, c.sub, c.w});
+         segs.push_back({a, bb, n, c.pol, c.sub, c.w, g});
          done += n;
+         g += n;
       }
    }
    tiles.clear();
-   size_t off = 0;
    for (const auto &s2 : segs)
       for (const auto &s1 : segs) {
          MshTile t;
          t.a1 = s1.a; t.b1 = s1.b; t.a2 = s2.a; t.b2 = s2.b;
          t.n1 = s1.n; t.n2 = s2.n;
+         t.g1 = s1.g; t.g2 = s2.g;
          t.pol1 = s1.pol; t.pol2 = s2.pol; t.sub1 = s1.sub; t.sub2 = s2.sub;
          t.rule = std::max(msh_rule_level((s1.b - s1.a) / s1.n, s1.w, s1.pol),
                            msh_rule_level((s2.b - s2.a) / s2.n, s2.w, s2.pol));
          msh_brm_sub(t, cfg, nm);
-         t.lv0_off = off;
-         off += (size_t)(t.n1 + 1) * (t.n2 + 1);
          tiles.push_back(t);
       }
+}
+
+// Level-0 nodes of the whole axis (the tiles' level-0 grids are sub-grids of
+// the tensor product of these nodes).
+std::vector<double> msh_axis_nodes(const std::vector<MshTile> &tiles)
+{
+   std::vector<double> x;
+   for (const auto &t : tiles) {
+      if (t.a2 != tiles[0].a2) break;               // the first row of tiles spans axis 1
+      const double h = (t.b1 - t.a1) / t.n1;
+      if (x.empty()) x.push_back(t.a1);
+      for (int i = 1; i <= t.n1; ++i) x.push_back(i == t.n1 ? t.b1 : t.a1 + i * h);
+   }
+   return x;
 }
 
 }  // namespace
@@ -459,12 +501,19 @@ struct bsahip_mesher {
    int nmp = 0;
    std::vector<MshTile> tiles;
    double dfref = 0.0;
-   double *lv0 = nullptr;     // resident level-0 BFM: [point][nm3]
+   double *lv0 = nullptr;     // resident level-0 BFM on the global grid: [i + j*nx][nm3]
+   long nx = 0;               // level-0 nodes per axis
    double *PB = nullptr;      // [K][NMP]
    long evals_lv0 = 0;
 };
 
 namespace {
+
+// first point of a tile's level-0 grid in the global grid
+size_t msh_lv0_at(const bsahip_mesher *M, const MshTile &t)
+{
+   return (size_t)t.g1 + (size_t)t.g2 * (size_t)M->nx;
+}
 
 // BFM at the points (f1[q], f2[q]) into out + slot[q]*nm3, in chunks of at
 // most MSH_CHUNK points so the A tables stay small (3 distinct |f| per point
@@ -543,22 +592,38 @@ void msh_grid(const MshTile &t, int l, std::vector<double> &f1, std::vector<doub
       }
 }
 
-int msh_integrate(bsahip_mesher *M, const MshTile &t, int l, const double *bf, int nb1, int nb2,
-                  double *m3f, double *m3r, long &brm_pts)
+int msh_integrate(bsahip_mesher *M, const MshTile &t, int l, const double *bf, long ldq, int nb1,
+                  int nb2, double *m3f, double *m3r, long &brm_pts)
 {
    Prob &D = M->D;
    const int s1 = std::max(1, t.sub1 / l), s2 = std::max(1, t.sub2 / l);
-   const double h1 = (t.b1 - t.a1) / ((nb1 - 1) * s1), h2 = (t.b2 - t.a2) / ((nb2 - 1) * s2);
+   const int NA = (nb1 - 1) * s1, NB = (nb2 - 1) * s2;
+   const double h1 = (t.b1 - t.a1) / NA, h2 = (t.b2 - t.a2) / NB;
    const double ws = (2.0 * M_PI * h1) * (2.0 * M_PI * h2);
    const int nm3 = D.nm * D.nm * D.nm;
+   // FRF tables along the two axes and, with equal spacings, along f1 + f2
+   const bool diag = std::fabs(h1 - h2) <= 1e-13 * std::max(std::fabs(h1), std::fabs(h2));
+   const size_t n3 = diag ? (size_t)(NA + NB + 1) : 0;
+   double2 *H;
+   CHK(M->B.get("m_H", ((size_t)(NA + 1) + (NB + 1) + n3) * D.nm, &H));
+   double2 *H1 = H, *H2 = H + (size_t)(NA + 1) * D.nm, *H3 = diag ? H2 + (size_t)(NB + 1) * D.nm : nullptr;
+   auto table = [&](double f0, double h, int n, double2 *out) -> int {
+      const int tot = n * D.nm;
+      k_frf_axis<<<(tot + 255) / 256, 256, 0, g.s>>>(f0, h, n, D.Mm, D.Km, D.Cm, D.nm, out);
+      HIPC(hipGetLastError());
+      return 0;
+   };
+   CHK(table(t.a1, h1, NA + 1, H1));
+   CHK(table(t.a2, h2, NB + 1, H2));
+   if (diag) CHK(table(t.a1 + t.a2, h1, NA + NB + 1, H3));
    const int ncell = (nb1 - 1) * (nb2 - 1);
    const int bx = (nm3 + 255) / 256;
    const int nchunk = std::max(1, std::min(ncell, std::max(1, 4096 / bx)));
    dim3 gs(bx, nchunk);
-   k_msh_integrate<<<gs, 256, 0, g.s>>>(bf, D.nm, nb1, nb2, s1, s2, t.a1, t.a2, h1, h2, ws, D.Mm,
-                                         D.Km, D.Cm, m3f, m3r);
+   k_msh_integrate<<<gs, 256, 0, g.s>>>(bf, ldq, D.nm, nb1, nb2, s1, s2, H1, H2, H3,
+                                         t.a1 + t.a2, h1, h2, ws, D.Mm, D.Km, D.Cm, m3f, m3r);
    HIPC(hipGetLastError());
-   brm_pts += (long)((nb1 - 1) * s1 + 1) * ((nb2 - 1) * s2 + 1);
+   brm_pts += (long)(NA + 1) * (NB + 1);
    return 0;
 }
 
@@ -576,8 +641,8 @@ int bsahip_mesher_create(const bsahip_problem *p, const bsahip_mesher_cfg *cfg, 
    if ((flags & BSAHIP_PREC_MASK) != BSAHIP_PREC_FP64)
       return fail(BSAHIP_EUNSUPPORTED, "the Mesher runs in FP64");
    if (!(cfg->bkg_peakw > 0.0) || !(cfg->bkg_area_ext > 0.0) || !(cfg->peak_area_ext > 0.0) ||
-       !(cfg->max_area_ext > 0.0) || cfg->tile_cells < 1)
-      return fail(BSAHIP_EINVAL, "Mesher extensions, bkg_peakw and tile_cells must be > 0");
+       !(cfg->max_area_ext > 0.0) || cfg->tile_cells < 1 || cfg->tile_cells > 128)
+      return fail(BSAHIP_EINVAL, "Mesher extensions and bkg_peakw must be > 0, tile_cells in 1..128");
    std::unique_ptr<bsahip_mesher> M(new bsahip_mesher());
    CHK(load_problem(p, 0, M->B, M->D));
    Prob &D = M->D;
@@ -592,30 +657,23 @@ int bsahip_mesher_create(const bsahip_problem *p, const bsahip_mesher_cfg *cfg, 
    CHK(M->B.get("m_PB", (size_t)D.K * M->nmp, &M->PB));
    k_pbt<<<(unsigned)((D.K * M->nmp + 255) / 256), 256, 0, g.s>>>(D.pb_k, D.nm, D.K, M->nmp, M->PB);
    HIPC(hipGetLastError());
-   // pre-mesh: level-0 BFM of every tile, kept resident
+   // pre-mesh: the level-0 BFM on the global grid (every tile's level-0 points
+   // once, shared edges and corners included once), kept resident
    const size_t nm3 = (size_t)D.nm * D.nm * D.nm;
-   size_t npts = 0;
-   for (const auto &t : M->tiles) npts += (size_t)(t.n1 + 1) * (t.n2 + 1);
+   const std::vector<double> x = msh_axis_nodes(M->tiles);
+   M->nx = (long)x.size();
+   const size_t npts = x.size() * x.size();
    CHK(M->B.get("m_lv0", npts * nm3, &M->lv0));
-   std::vector<double> f1, f2, a1, a2;
+   std::vector<double> f1, f2;
    std::vector<long> slot;
-   const size_t batch = MSH_CHUNK;
-   for (size_t ti = 0; ti < M->tiles.size(); ++ti) {
-      int nb1, nb2;
-      msh_grid(M->tiles[ti], 1, a1, a2, nb1, nb2);
-      for (size_t q = 0; q < a1.size(); ++q) {
-         f1.push_back(a1[q]);
-         f2.push_back(a2[q]);
-         slot.push_back((long)(M->tiles[ti].lv0_off + q));
+   for (size_t j = 0; j < x.size(); ++j)
+      for (size_t i = 0; i < x.size(); ++i) {
+         f1.push_back(x[i]);
+         f2.push_back(x[j]);
+         slot.push_back((long)(i + j * x.size()));
       }
-      if (f1.size() >= batch || ti + 1 == M->tiles.size()) {
-         CHK(msh_eval(M.get(), f1, f2, slot, M->lv0));
-         HIPC(hipStreamSynchronize(g.s));
-         f1.clear();
-         f2.clear();
-         slot.clear();
-      }
-   }
+   CHK(msh_eval(M.get(), f1, f2, slot, M->lv0));
+   HIPC(hipStreamSynchronize(g.s));
    M->evals_lv0 = (long)npts;
    *out = M.release();
    return 0;
@@ -627,8 +685,8 @@ int bsahip_mesher_layout(const bsahip_mesher_cfg *cfg, int32_t nm, int32_t max_t
    if (!cfg || !cfg->natf || !cfg->xsi || !ntiles || nm < 1)
       return fail(BSAHIP_EINVAL, "NULL Mesher argument or nm < 1");
    if (!(cfg->bkg_peakw > 0.0) || !(cfg->bkg_area_ext > 0.0) || !(cfg->peak_area_ext > 0.0) ||
-       !(cfg->max_area_ext > 0.0) || cfg->tile_cells < 1)
-      return fail(BSAHIP_EINVAL, "Mesher extensions, bkg_peakw and tile_cells must be > 0");
+       !(cfg->max_area_ext > 0.0) || cfg->tile_cells < 1 || cfg->tile_cells > 128)
+      return fail(BSAHIP_EINVAL, "Mesher extensions and bkg_peakw must be > 0, tile_cells in 1..128");
    std::vector<MshAxisCell> ax;
    double dfref;
    CHK(msh_full_axis(cfg, nm, ax, dfref));
@@ -683,8 +741,8 @@ int bsahip_mesher_features(bsahip_mesher *m, int32_t ns, double *stencil)
    CHK(m->B.get("m_stencil", nt * per, &d));
    for (size_t i = 0; i < nt; ++i) {
       const MshTile &t = m->tiles[i];
-      k_bfm_stencil<<<(unsigned)per, 64, 0, g.s>>>(m->lv0 + t.lv0_off * nm3, nm3, t.n1 + 1,
-                                                   t.n2 + 1, ns, d + i * per);
+      k_bfm_stencil<<<(unsigned)per, 64, 0, g.s>>>(m->lv0 + msh_lv0_at(m, t) * nm3, m->nx, nm3,
+                                                   t.n1 + 1, t.n2 + 1, ns, d + i * per);
       HIPC(hipGetLastError());
    }
    HIPC(hipMemcpyAsync(stencil, d, nt * per * sizeof(double), hipMemcpyDeviceToHost, g.s));
@@ -692,67 +750,112 @@ int bsahip_mesher_features(bsahip_mesher *m, int32_t ns, double *stencil)
    return 0;
 }
 
-int bsahip_mesher_run(bsahip_mesher *m, const int32_t *levels, double *m3mf, double *m3mr,
-                      int64_t *counts)
+int bsahip_mesher_run_tiles(bsahip_mesher *m, const int32_t *levels, double *tile_m3mr,
+                            double *m3mf, double *m3mr, int64_t *counts)
 {
    std::lock_guard<std::recursive_mutex> lk(g.mu);
    if (!m) return fail(BSAHIP_EINVAL, "NULL Mesher");
    Prob &D = m->D;
-   const size_t nm3 = (size_t)D.nm * D.nm * D.nm;
-   for (size_t i = 0; i < m->tiles.size(); ++i) {
+   const size_t nm3 = (size_t)D.nm * D.nm * D.nm, nt = m->tiles.size();
+   for (size_t i = 0; i < nt; ++i) {
       const int l = levels ? levels[i] : m->tiles[i].rule;
       if (!(l == 1 || l == 2 || l == 4 || l == 8))
          return fail(BSAHIP_EINVAL, "tile levels must be 1, 2, 4 or 8");
    }
+   // accumulators: m3f, m3r, then one m3r slot per tile when asked for
    double *acc;
-   CHK(m->B.get("m_acc", 2 * nm3, &acc));
-   CHK(fill(acc, 2 * nm3, 0.0));
+   const size_t nacc = 2 * nm3 + (tile_m3mr ? nt * nm3 : 0);
+   CHK(m->B.get("m_acc", nacc, &acc));
+   CHK(fill(acc, nacc, 0.0));
    long evals_post = 0, brm = 0;
+   // level-1 tiles integrate straight from the resident level-0 BFM; refined
+   // tiles are evaluated in groups whose fine grids fit the staging buffer
+   const size_t cap = std::max<size_t>(66049, ((size_t)1 << 29) / nm3);   // points (4 GiB)
    std::vector<double> f1, f2, e1, e2;
    std::vector<long> slot;
-   for (size_t i = 0; i < m->tiles.size(); ++i) {
+   std::vector<size_t> grp, goff;
+   size_t used = 0;
+   auto slot_r = [&](size_t k) { return tile_m3mr ? acc + 2 * nm3 + k * nm3 : acc + nm3; };
+   auto flush = [&]() -> int {
+      if (grp.empty()) return 0;
+      double *fine;
+      CHK(m->B.get("m_fine", used * nm3, &fine));
+      CHK(msh_eval(m, e1, e2, slot, fine));
+      evals_post += (long)e1.size();
+      for (size_t k = 0; k < grp.size(); ++k) {
+         const MshTile &t = m->tiles[grp[k]];
+         const int l = levels ? levels[grp[k]] : t.rule;
+         const int nb1 = t.n1 * l + 1, nb2 = t.n2 * l + 1;
+         double *fg = fine + goff[k] * nm3;
+         dim3 gc((unsigned)((nm3 + 255) / 256), (unsigned)((t.n1 + 1) * (t.n2 + 1)));
+         k_lv0_to_fine<<<gc, 256, 0, g.s>>>(m->lv0 + msh_lv0_at(m, t) * nm3, m->nx, t.n1, l, nb1,
+                                            (int)nm3, fg);
+         HIPC(hipGetLastError());
+         CHK(msh_integrate(m, t, l, fg, nb1, nb1, nb2, acc, slot_r(grp[k]), brm));
+      }
+      HIPC(hipStreamSynchronize(g.s));
+      grp.clear();
+      goff.clear();
+      e1.clear();
+      e2.clear();
+      slot.clear();
+      used = 0;
+      return 0;
+   };
+   for (size_t i = 0; i < nt; ++i) {
       const MshTile &t = m->tiles[i];
       const int l = levels ? levels[i] : t.rule;
-      int nb1, nb2;
-      const double *bf = m->lv0 + t.lv0_off * nm3;
-      if (l > 1) {
-         msh_grid(t, l, f1, f2, nb1, nb2);
-         double *fine;
-         CHK(m->B.get("m_fine", (size_t)nb1 * nb2 * nm3, &fine));
-         // level-0 points are reused; only the new points are evaluated
-         e1.clear();
-         e2.clear();
-         slot.clear();
-         for (int j = 0; j < nb2; ++j)
-            for (int ii = 0; ii < nb1; ++ii)
-               if ((ii % l) || (j % l)) {
-                  const size_t q = ii + (size_t)j * nb1;
-                  e1.push_back(f1[q]);
-                  e2.push_back(f2[q]);
-                  slot.push_back((long)q);
-               }
-         CHK(msh_eval(m, e1, e2, slot, fine));
-         evals_post += (long)e1.size();
-         // copy the level-0 points into the fine grid
-         for (int j = 0; j <= t.n2; ++j)
-            HIPC(hipMemcpy2DAsync(fine + ((size_t)j * l * nb1) * nm3, (size_t)l * nm3 * sizeof(double),
-                                  bf + ((size_t)j * (t.n1 + 1)) * nm3, nm3 * sizeof(double),
-                                  nm3 * sizeof(double), t.n1 + 1, hipMemcpyDeviceToDevice, g.s));
-         CHK(msh_integrate(m, t, l, fine, nb1, nb2, acc, acc + nm3, brm));
-         HIPC(hipStreamSynchronize(g.s));
-      } else {
-         CHK(msh_integrate(m, t, 1, bf, t.n1 + 1, t.n2 + 1, acc, acc + nm3, brm));
+      if (l == 1) {
+         CHK(msh_integrate(m, t, 1, m->lv0 + msh_lv0_at(m, t) * nm3, m->nx, t.n1 + 1, t.n2 + 1, acc,
+                           slot_r(i), brm));
+         continue;
       }
+      const size_t np = (size_t)(t.n1 * l + 1) * (t.n2 * l + 1);
+      if (!grp.empty() && used + np > cap) CHK(flush());
+      int nb1, nb2;
+      msh_grid(t, l, f1, f2, nb1, nb2);
+      for (int j = 0; j < nb2; ++j)
+         for (int ii = 0; ii < nb1; ++ii)
+            if ((ii % l) || (j % l)) {
+               const size_t q = ii + (size_t)j * nb1;
+               e1.push_back(f1[q]);
+               e2.push_back(f2[q]);
+               slot.push_back((long)(used + q));
+            }
+      grp.push_back(i);
+      goff.push_back(used);
+      used += np;
+   }
+   CHK(flush());
+   if (tile_m3mr) {
+      HIPC(hipMemcpyAsync(tile_m3mr, acc + 2 * nm3, nt * nm3 * sizeof(double),
+                          hipMemcpyDeviceToHost, g.s));
    }
    if (m3mf) HIPC(hipMemcpyAsync(m3mf, acc, nm3 * sizeof(double), hipMemcpyDeviceToHost, g.s));
-   if (m3mr) HIPC(hipMemcpyAsync(m3mr, acc + nm3, nm3 * sizeof(double), hipMemcpyDeviceToHost, g.s));
    HIPC(hipStreamSynchronize(g.s));
+   if (m3mr) {
+      if (tile_m3mr) {
+         for (size_t q = 0; q < nm3; ++q) {
+            double sum = 0.0;
+            for (size_t k = 0; k < nt; ++k) sum += tile_m3mr[k * nm3 + q];
+            m3mr[q] = sum;
+         }
+      } else {
+         HIPC(hipMemcpy(m3mr, acc + nm3, nm3 * sizeof(double), hipMemcpyDeviceToHost));
+      }
+   }
    if (counts) {
       counts[0] = m->evals_lv0;
       counts[1] = evals_post;
       counts[2] = brm;
    }
    return 0;
+}
+
+int bsahip_mesher_run(bsahip_mesher *m, const int32_t *levels, double *m3mf, double *m3mr,
+                      int64_t *counts)
+{
+   return bsahip_mesher_run_tiles(m, levels, nullptr, m3mf, m3mr, counts);
 }
 
 int bsahip_mesher_destroy(bsahip_mesher *m)

@@ -206,6 +206,10 @@ int bsahip_mesher_features(bsahip_mesher *m, int32_t ns, double *stencil);
  * evaluations of this run, BRM points of this run}. */
 int bsahip_mesher_run(bsahip_mesher *m, const int32_t *levels, double *m3mf, double *m3mr,
                       int64_t *counts);
+/* As bsahip_mesher_run, plus each tile's own m3mr contribution in tile_m3mr
+ * (ntiles x NM^3; m3mr is then their sum). */
+int bsahip_mesher_run_tiles(bsahip_mesher *m, const int32_t *levels, double *tile_m3mr,
+                            double *m3mf, double *m3mr, int64_t *counts);
 int bsahip_mesher_destroy(bsahip_mesher *m);
 
 /* ---- Learned Mesher proposer: a multilayer perceptron trained and evaluated

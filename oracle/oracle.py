@@ -481,18 +481,18 @@ def mesher_integrate(prob, geom, levels, modes=None):
     """Zone quadrature of the Mesher on given tiles (CPU restatement of
     bsahip_mesher_run): per tile, the force bispectrum at the level-l grid by
     getFM_full_tnm_scalar_msh_ (msh_points), bilinear interpolation onto the
-    BRM grid (16 BRM cells per level-0 cell across a PEAK interval, 8
-    otherwise), the FRF product Re[H_m(f1) H_n(f2) conj(H_p(f1+f2))] and
+    BRM grid, the FRF product Re[H_m(f1) H_n(f2) conj(H_p(f1+f2))] and
     trapezoid weights (zones/_shared_poly2d.fi:827-832: vertex 1/4, edge 1/2,
     interior 1, times dw1 dw2).  geom rows: f1a f1b f2a f2b n1 n2 pol1 pol2
     sub1 sub2 (sub = BRM cells per level-0 cell).  Returns m3mf, m3mr and the
-    BFM evaluation counts (level-0, refinement)."""
+    BFM evaluation counts: level-0 points of all tiles counted once (they lie on
+    one tensor grid), refinement points per tile."""
     modes = kept_modes(prob) if modes is None else np.asarray(modes)
     nm = len(modes)
     nm3 = nm ** 3
     m3f = np.zeros(nm3)
     m3r = np.zeros(nm3)
-    ev0 = ev1 = 0
+    ev1 = 0
     for t, lv in zip(np.asarray(geom), np.broadcast_to(np.asarray(levels), (len(geom),))):
         a1, b1, a2, b2 = t[:4]
         n1, n2 = int(t[4]), int(t[5])
@@ -504,7 +504,6 @@ def mesher_integrate(prob, geom, levels, modes=None):
         g1[-1], g2[-1] = b1, b2
         F1, F2 = np.meshgrid(g1, g2, indexing="ij")
         B = msh_points(prob, F1.ravel(), F2.ravel()).reshape(nm3, nb1, nb2)
-        ev0 += (n1 + 1) * (n2 + 1)
         if lv > 1:
             ev1 += nb1 * nb2 - (n1 + 1) * (n2 + 1)
         s1, s2 = max(1, sub1 // lv), max(1, sub2 // lv)
@@ -533,4 +532,10 @@ def mesher_integrate(prob, geom, levels, modes=None):
                                                                            order="F")
         m3f += np.einsum("xab,ab->x", Bb, W)
         m3r += np.einsum("xab,xab,ab->x", Bb, R, W)
+    g = np.asarray(geom)
+    nodes = set()
+    for t in g[g[:, 2] == g[0, 2]]:
+        n = int(t[4])
+        nodes.update(np.round(t[0] + np.arange(n + 1) * (t[1] - t[0]) / n, 12).tolist())
+    ev0 = len(nodes) ** 2
     return {"m3mf": m3f, "m3mr": m3r, "evals_pre": ev0, "evals_post": ev1}

@@ -1,0 +1,38 @@
+"""The learned Mesher proposer on the evaluation deck (NN=500, NM=20, seeded
+deck of configuration 5, not in the training set): with the committed network
+it must keep the third moment within 1e-3 of the converged Mesher (every tile
+at level 8) while evaluating the BFM at >= 3x fewer points than the rule-based
+Mesher and >= 10x fewer than the converged one.  The rule-based Mesher itself
+misses the 1e-3 target on this deck, which the test records."""
+import numpy as np
+import pytest
+
+from bsalib_amd import abi, proposer as PR, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def test_proposer_on_evaluation_deck(gpu_lib):
+    p = synth.config5()
+    P = PR.Proposer()
+    r = PR.evaluate(p, P)
+    print(r)
+    assert r["err_mlp"] <= 1e-3
+    assert r["evals_rule"] >= 3 * r["evals_mlp"]
+    assert r["evals_conv"] >= 10 * r["evals_mlp"]
+    assert r["err_rule"] > 1e-3
+    P.net.close()
+
+
+def test_proposer_features_shape_and_levels(gpu_lib):
+    p = synth.deck(nn=60, nm=6, seed=77)
+    M = abi.Mesher(p, **PR.mesher_settings(p))
+    X = PR.tile_features(p, M)
+    assert X.shape == (M.ntiles, PR.NFEAT) and np.all(np.isfinite(X))
+    P = PR.Proposer()
+    lv = P.levels(p, M)
+    assert set(np.unique(lv)) <= {1, 2, 4, 8}
+    r = M.run(levels=lv)
+    assert np.all(np.isfinite(r["m3mr"]))
+    M.close()
+    P.net.close()
