@@ -25,7 +25,14 @@ module BsaHip_C
       type(c_ptr) :: Mm = c_null_ptr, Km = c_null_ptr, Cm = c_null_ptr
    end type bsahip_problem_t
 
+   type, bind(c), public :: bsahip_mesher_cfg_t
+      integer(c_int32_t) :: bkg_base_rfmnt = 20, tile_cells = 32, diag_crest = 0, reserved = 0
+      real(c_double) :: bkg_peakw = 0, bkg_area_ext = 2, peak_area_ext = 2, max_area_ext = 2
+      type(c_ptr) :: natf = c_null_ptr, xsi = c_null_ptr
+   end type bsahip_mesher_cfg_t
+
    public :: bsahip_init, bsahip_finalise, bsahip_classic_run, bsahip_bfm_points
+   public :: bsahip_mesher_create, bsahip_mesher_run, bsahip_mesher_destroy
    public :: bsahip_last_error_f
 
    interface
@@ -53,6 +60,26 @@ module BsaHip_C
          type(bsahip_problem_t), intent(in) :: p
          integer(c_int32_t), value :: npts, flags
          type(c_ptr), value :: fi, fj, bfm, brm
+      end function
+
+      integer(c_int) function bsahip_mesher_create(p, cfg, flags, mesher) &
+            bind(c, name="bsahip_mesher_create")
+         import c_int, c_int32_t, c_ptr, bsahip_problem_t, bsahip_mesher_cfg_t
+         type(bsahip_problem_t), intent(in) :: p
+         type(bsahip_mesher_cfg_t), intent(in) :: cfg
+         integer(c_int32_t), value :: flags
+         type(c_ptr), intent(out) :: mesher
+      end function
+
+      integer(c_int) function bsahip_mesher_run(mesher, levels, m3mf, m3mr, counts) &
+            bind(c, name="bsahip_mesher_run")
+         import c_int, c_ptr
+         type(c_ptr), value :: mesher, levels, m3mf, m3mr, counts
+      end function
+
+      integer(c_int) function bsahip_mesher_destroy(mesher) bind(c, name="bsahip_mesher_destroy")
+         import c_int, c_ptr
+         type(c_ptr), value :: mesher
       end function
 
       type(c_ptr) function bsahip_last_error_c() bind(c, name="bsahip_last_error")

@@ -15,10 +15,13 @@
 !!  * The number of wind zones is taken from bsa_setWindZoneLimits when it
 !!    was called (as the reference), else from bsa_setWZMeanWindVel, else
 !!    from max(NodWZ).
-!!  * Only the Classic approach is computed.  bsa_enableGPU forces it, as
-!!    the reference's GPU path does (BsaLibImpl.F90:347-353); without it an
-!!    analysis type other than 1 aborts.  Unlike that GPU path, PSDs are
-!!    still computed when requested.
+!!  * Analysis type 1 runs the Classic approach, 2 the Mesher, 3 both (as
+!!    BsaLibImpl.F90:412-452).  bsa_enableGPU forces Classic, as the
+!!    reference's GPU path does (BsaLibImpl.F90:347-353); unlike that path,
+!!    PSDs are still computed when requested.  The Mesher is the device zone
+!!    Mesher of libbsahip (zone limits and policies of BsaMesherImpl.F90,
+!!    tiles of at most 32 x 32 level-0 cells, rule-based refinement levels);
+!!    its results are m3mf_msh, m3mr_msh.
 !!  * bsa_setComputePrecision is an addition: BSA_PREC_BF16X3 selects the
 !!    split-bf16x3 node contraction (FP32 accumulation on matrix cores,
 !!    FP64 everywhere else); the default is FP64.
@@ -61,12 +64,19 @@ module BsaLib
    public :: bsa_setLoadedNodalDOFs, bsa_setLoadedNodes, bsa_setModalInfo
    public :: bsa_setKeptModalShapes, bsa_setModalMatrices, bsa_setTotDamping
    public :: bsa_getUsedModeShapes, bsa_setComputePrecision
+   public :: bsa_setupMesher, bsa_setPremeshScheme, bsa_setMaxBkgPeakRestriction
+   integer(bsa_int_t), parameter, public :: BSA_PREMESH_TYPE_DIAG_CREST_NO = 0
+   integer(bsa_int_t), parameter, public :: BSA_PREMESH_TYPE_DIAG_CREST_YES = 1
 
    ! ---- settings (defaults as src/settings/Settings.F90:33-84) ----
    integer(bsa_int_t) :: isuban_ = 2, iclassic_ = 0, iconv_ = 1
    integer(bsa_int_t) :: icomp_psd_ = 1, icomp_bisp_ = 1, ionly_diag_ = 0, itest_ = 0
    integer(bsa_int_t) :: isym_ = 0, nfreqs_ = 0, iprec_ = BSA_PREC_FP64
    real(bsa_real_t) :: df_ = 0
+   ! Mesher settings (src/settings/Settings.F90:86-116)
+   integer(bsa_int_t) :: bkgrfmt_ = 20, premesh_type_ = 0
+   real(bsa_real_t) :: bkgaext_ = 2, genpaext_ = 2, maxaext_ = 2
+   logical :: restrict_bkg_ = .false.
    logical :: validate_modal_ = .true., gpu_enabled_ = .false.
    logical :: cleaned_ = .true., device_ready_ = .false.
 
@@ -200,6 +210,30 @@ contains
          write(output_unit, '(1x, 2a, i0)') WARNMSG, 'Unsupported spatial symmetry value ', isym
          isym_ = BSA_SPATIAL_SYM_NONE
       end if
+   end subroutine
+
+   !> src/settings/SettingsImpl.F90:107-119; isvd, ifcov and idumpmod are
+   !> accepted for interface compatibility (the device Mesher evaluates the
+   !> BFM without POD, covers the full domain and keeps its zones in memory).
+   subroutine bsa_setupMesher(isvd, bkgrfmt, bkgaext, genpaext, maxaext, ifcov, idumpmod)
+      integer(bsa_int_t), value :: isvd, bkgrfmt, ifcov, idumpmod
+      real(bsa_real_t), value :: bkgaext, genpaext, maxaext
+      bkgrfmt_ = bkgrfmt
+      bkgaext_ = bkgaext
+      genpaext_ = genpaext
+      maxaext_ = maxaext
+   end subroutine
+
+   !> BSA_PREMESH_TYPE_DIAG_CREST_YES also resolves the f1+f2 = +-f_n crests.
+   subroutine bsa_setPremeshScheme(itype)
+      integer(bsa_int_t), value :: itype
+      premesh_type_ = itype
+   end subroutine
+
+   !> Background peak width from the first direction only (BsaMesherImpl.F90:298-302).
+   subroutine bsa_setMaxBkgPeakRestriction(bool)
+      logical, intent(in) :: bool
+      restrict_bkg_ = bool
    end subroutine
 
    subroutine bsa_doValidateModalData(bool)
@@ -486,6 +520,7 @@ contains
       real(bsa_real_t), pointer :: fp3(:, :, :), fp2(:, :)
       real(bsa_real_t) :: df
       integer(bsa_int_t) :: nfh, nme, nwz, i, iclassic
+      logical :: do_cls, do_msh
       integer(c_int32_t) :: flags
       integer(c_int) :: rc
       type(c_ptr) :: o_m2mf, o_m2mr, o_m2o2mr, o_m3mf, o_m3mr
@@ -494,11 +529,10 @@ contains
          write(output_unit, '(1x, 2a)') WARNMSG, 'Both  PSD  and  BISP  computation are disabled.'
          return
       end if
-      if (isuban_ /= 1) then
-         if (.not. gpu_enabled_) call bsa_Abort('Only the Classic approach (analysis type 1) is '// &
-            'available in this library; call bsa_setAnalysisType(1) or bsa_enableGPU().')
-         isuban_ = 1
-      end if
+      if (gpu_enabled_) isuban_ = 1
+      if (isuban_ < 1 .or. isuban_ > 3) call bsa_Abort('Analysis type must be 1, 2 or 3.')
+      do_cls = isuban_ == 1 .or. isuban_ == 3
+      do_msh = isuban_ == 2 .or. isuban_ == 3
       if (ionly_diag_ /= 0) call bsa_Abort('Diagonal-only spectra are not available in this library.')
       if (iconv_ /= BSA_PSD_CONVENTION_PULS) &
          call bsa_Abort('Only the pulsation convention (BSA_PSD_CONVENTION_PULS) is available in this library.')
@@ -559,22 +593,6 @@ contains
          allocate(umref(nwz), source=0.0_bsa_real_t)
       end if
 
-      call classicGrid_(nfh, df)
-
-      ! Classic version as checkMaxAllocation_ (BsaClassicImpl.F90:384-434): outside
-      ! test mode a vectorised bispectrum above 1e9 elements switches to the scalar
-      ! version, whose third moments are plain Riemann sums (every point times
-      ! dw^2, BsaClassicImpl.F90:331,341); the vectorised version uses the
-      ! edge/corner-corrected rule of intgSpectraVect_.  The element count is
-      ! formed in 64-bit here (the reference forms it in 32-bit, which wraps).
-      iclassic = iclassic_
-      if (itest_ == 0 .and. icomp_bisp_ /= 0 .and. iclassic == BSA_CLASSIC_MODE_VECTOR) then
-         if (int(2 * nfh - 1, int64)**2 * int(nme, int64)**3 > 1000000000_int64) then
-            write(output_unit, '(1x, 2a)') WARNMSG, 'Too high allocation size for VECTORISED BSA version: using SCALAR.'
-            iclassic = BSA_CLASSIC_MODE_SCALAR
-         end if
-      end if
-
       p%nn = nn_
       p%nnl = size(n_load_)
       p%nm = nme
@@ -598,6 +616,32 @@ contains
       p%Km = c_loc(Kk)
       p%Cm = c_loc(Ck)
 
+      if (.not. device_ready_) then
+         rc = bsahip_init(0_c_int)
+         if (rc /= 0) call bsa_Abort('GPU initialisation failed: '//bsahip_last_error_f())
+         device_ready_ = .true.
+      end if
+      if (allocated(m3mf_msh)) deallocate(m3mf_msh)
+      if (allocated(m3mr_msh)) deallocate(m3mr_msh)
+      if (do_msh) call runMesher_(p, m3mf_msh, m3mr_msh, tscl, umref)
+      if (.not. do_cls) return
+
+      call classicGrid_(nfh, df)
+
+      ! Classic version as checkMaxAllocation_ (BsaClassicImpl.F90:384-434): outside
+      ! test mode a vectorised bispectrum above 1e9 elements switches to the scalar
+      ! version, whose third moments are plain Riemann sums (every point times
+      ! dw^2, BsaClassicImpl.F90:331,341); the vectorised version uses the
+      ! edge/corner-corrected rule of intgSpectraVect_.  The element count is
+      ! formed in 64-bit here (the reference forms it in 32-bit, which wraps).
+      iclassic = iclassic_
+      if (itest_ == 0 .and. icomp_bisp_ /= 0 .and. iclassic == BSA_CLASSIC_MODE_VECTOR) then
+         if (int(2 * nfh - 1, int64)**2 * int(nme, int64)**3 > 1000000000_int64) then
+            write(output_unit, '(1x, 2a)') WARNMSG, 'Too high allocation size for VECTORISED BSA version: using SCALAR.'
+            iclassic = BSA_CLASSIC_MODE_SCALAR
+         end if
+      end if
+
       if (iclassic == BSA_CLASSIC_MODE_SCALAR) then
          flags = int(iprec_, c_int32_t) + BSAHIP_QUAD_RIEMANN
       else
@@ -605,8 +649,6 @@ contains
       end if
       o_m2mf = c_null_ptr; o_m2mr = c_null_ptr; o_m2o2mr = c_null_ptr
       o_m3mf = c_null_ptr; o_m3mr = c_null_ptr
-      if (allocated(m3mf_msh)) deallocate(m3mf_msh)
-      if (allocated(m3mr_msh)) deallocate(m3mr_msh)
       if (icomp_psd_ /= 0) then
          call realloc_(m2mf_cls, nme**2)
          call realloc_(m2mr_cls, nme**2)
@@ -626,11 +668,6 @@ contains
          flags = ior(flags, BSAHIP_NO_BISP)
       end if
 
-      if (.not. device_ready_) then
-         rc = bsahip_init(0_c_int)
-         if (rc /= 0) call bsa_Abort('GPU initialisation failed: '//bsahip_last_error_f())
-         device_ready_ = .true.
-      end if
       rc = bsahip_classic_run(p, int(nfh, c_int32_t), real(df, c_double), flags, &
                               o_m2mf, o_m2mr, o_m2o2mr, o_m3mf, o_m3mr)
       if (rc /= 0) call bsa_Abort('bsahip_classic_run failed: '//bsahip_last_error_f())
@@ -638,6 +675,54 @@ contains
          nme, ' modes, ', 2 * nfh - 1, '^2 frequency points, ', &
          merge('SCALAR    ', 'VECTORISED', iclassic == BSA_CLASSIC_MODE_SCALAR)
    contains
+      subroutine runMesher_(p, m3mf, m3mr, tscl, umref)
+         type(bsahip_problem_t), intent(in) :: p
+         real(bsa_real_t), allocatable, target, intent(inout) :: m3mf(:), m3mr(:)
+         real(bsa_real_t), intent(in) :: tscl(:, :, :), umref(:)
+         type(bsahip_mesher_cfg_t) :: cfg
+         type(c_ptr) :: msh
+         real(bsa_real_t), allocatable, target :: fk(:), xk(:)
+         real(bsa_real_t) :: v(3, 3), bw
+         integer(c_int64_t), target :: counts(3)
+         integer(bsa_int_t) :: z, nmk
+         integer(c_int) :: rc2
+         if (.not. allocated(xsi_)) &
+            call bsa_Abort('The Mesher needs the modal damping ratios (bsa_setTotDamping).')
+         ! background peak width: 1 / max_z(L/U), maximised over (component, direction)
+         ! (validateAll_, BsaLibImpl.F90:513-527; StructureImpl.F90:225-260)
+         v = 0
+         do z = 1, size(umref)
+            if (umref(z) > 0) v = max(v, tscl(:, :, z) / umref(z))
+         end do
+         where (v > 0) v = 1 / v
+         if (restrict_bkg_) then
+            bw = maxval(v(:, 1))
+         else
+            bw = maxval(v)
+         end if
+         if (.not. bw > 0) call bsa_Abort('Background peak width is 0 (turbulence scales or mean speeds).')
+         nmk = size(modes_)
+         fk = natf_(modes_)
+         xk = xsi_(modes_)
+         cfg%bkg_base_rfmnt = bkgrfmt_
+         cfg%tile_cells = 32
+         cfg%diag_crest = merge(1, 0, premesh_type_ == BSA_PREMESH_TYPE_DIAG_CREST_YES)
+         cfg%bkg_peakw = bw
+         cfg%bkg_area_ext = bkgaext_
+         cfg%peak_area_ext = genpaext_
+         cfg%max_area_ext = maxaext_
+         cfg%natf = c_loc(fk)
+         cfg%xsi = c_loc(xk)
+         rc2 = bsahip_mesher_create(p, cfg, 0_c_int32_t, msh)
+         if (rc2 /= 0) call bsa_Abort('bsahip_mesher_create failed: '//bsahip_last_error_f())
+         allocate(m3mf(nmk**3), m3mr(nmk**3))
+         rc2 = bsahip_mesher_run(msh, c_null_ptr, c_loc(m3mf), c_loc(m3mr), c_loc(counts))
+         if (rc2 /= 0) call bsa_Abort('bsahip_mesher_run failed: '//bsahip_last_error_f())
+         rc2 = bsahip_mesher_destroy(msh)
+         write(output_unit, '(1x, 2a, i0, a, i0, a, i0, a)') INFOMSG, 'Mesher on the GPU: ', &
+            counts(1) + counts(2), ' BFM points (', counts(1), ' level-0), ', counts(3), ' BRM points.'
+      end subroutine
+
       subroutine realloc_(a, n)
          real(bsa_real_t), allocatable, intent(inout) :: a(:)
          integer(bsa_int_t), intent(in) :: n

@@ -35,12 +35,12 @@ def run(prob, *opts, check=True, timeout=300):
         raw = open(fout, "rb").read()
     nme = int(np.frombuffer(raw[:4], dtype=np.int32)[0])
     modes = np.frombuffer(raw[4:4 + 4 * nme], dtype=np.int32).copy()
-    vals = np.frombuffer(raw[4 + 4 * nme:], dtype=np.float64)
+    o = 4 + 4 * nme
     out = {"modes": modes}
-    o = 0
-    for k, n in (("m2mf", nme ** 2), ("m2mr", nme ** 2), ("m2o2mr", nme ** 2),
-                 ("m3mf", nme ** 3), ("m3mr", nme ** 3)):
-        out[k] = vals[o:o + n].copy()
-        o += n
-    assert o == vals.size
+    for k in ("m2mf", "m2mr", "m2o2mr", "m3mf", "m3mr", "m3mf_msh", "m3mr_msh"):
+        n = int(np.frombuffer(raw[o:o + 4], dtype=np.int32)[0])
+        o += 4
+        out[k] = np.frombuffer(raw[o:o + 8 * n], dtype=np.float64).copy() if n else None
+        o += 8 * n
+    assert o == len(raw)
     return out, r

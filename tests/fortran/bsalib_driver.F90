@@ -2,13 +2,15 @@
 !! a program written the way a user of the reference writes one (setters,
 !! bsa_Run, bsa_Finalise), reading its inputs from a raw stream file.
 !!
-!! Usage:  bsalib_driver <input.bin> <output.bin> [fp64|bf16x3] [grid] [scalar]
+!! Usage:  bsalib_driver <input.bin> <output.bin> [fp64|bf16x3] [grid] [scalar] [mesher|both]
 !! The input layout is the one tests write for the reference harness (header
 !! of 14 int32, then integer and real64 arrays in Fortran order).  "grid"
 !! leaves test mode off and passes damping ratios, so bsa_Run applies the
-!! frequency-grid checks; "scalar" selects the scalar Classic version.
-!! Output: nme (int32), the used modes, then m2mf, m2mr, m2o2mr (NME^2 each)
-!! and m3mf, m3mr (NME^3 each) as real64.
+!! frequency-grid checks; "scalar" selects the scalar Classic version;
+!! "mesher" runs analysis type 2 (Mesher only), "both" type 3.
+!! Output: nme (int32), the used modes, then for each of m2mf, m2mr, m2o2mr,
+!! m3mf_cls, m3mr_cls, m3mf_msh, m3mr_msh: its size (int32, 0 when bsa_Run
+!! left it unallocated) and its values (real64).
 program bsalib_driver
    use, intrinsic :: iso_fortran_env, only: int32, real64
    use BsaLib
@@ -25,11 +27,12 @@ program bsalib_driver
    real(real64), allocatable :: m3mf(:), m3mr(:)
    character(len=512) :: fin, fout, arg
    logical :: grid_checks
-   integer :: uin, uout
+   integer :: uin, uout, isuban
 
    call get_command_argument(1, fin)
    call get_command_argument(2, fout)
    grid_checks = .false.
+   isuban = 1
 
    open(newunit=uin, file=trim(fin), access='stream', form='unformatted', status='old')
    read(uin) hdr
@@ -46,7 +49,6 @@ program bsalib_driver
    close(uin)
 
    call bsa_Init()
-   call bsa_setAnalysisType(1)
    call bsa_setClassicMode(BSA_CLASSIC_MODE_VECTOR)
    call bsa_setScalingConv(BSA_PSD_CONVENTION_PULS)
    call bsa_setSpectraComputation(1, 1)
@@ -57,7 +59,11 @@ program bsalib_driver
       if (trim(arg) == 'bf16x3') call bsa_setComputePrecision(BSA_PREC_BF16X3)
       if (trim(arg) == 'grid') grid_checks = .true.
       if (trim(arg) == 'scalar') call bsa_setClassicMode(BSA_CLASSIC_MODE_SCALAR)
+      if (trim(arg) == 'mesher') isuban = 2
+      if (trim(arg) == 'both') isuban = 3
    end do
+   call bsa_setAnalysisType(isuban)
+   if (isuban /= 1) call bsa_setupMesher(0, 20, 2.0_real64, 2.0_real64, 2.0_real64, 1, 0)
 
    call bsa_setTotalNOfNodes(nn)
    call bsa_setNumOfNodalDOFs(nlibs)
@@ -65,15 +71,13 @@ program bsalib_driver
    call bsa_setLoadedNodalDOFs(libs_load)
    call bsa_setModalInfo(nn*nlibs, nm, phi, natf)
    call bsa_setModalMatrices(nm, Mm, Km, Cm)
-   if (grid_checks) then
-      ! modal damping ratios of the diagonal model: xsi = c / (2 sqrt(k m))
-      allocate(xsi(nm))
-      do i = 1, nm
-         xsi(i) = Cm(i, i) / (2.0_real64 * sqrt(Km(i) * Mm(i)))
-      end do
-      call bsa_setTotDamping(xsi)
-      call bsa_setTestMode(0)
-   end if
+   ! modal damping ratios of the diagonal model: xsi = c / (2 sqrt(k m))
+   allocate(xsi(nm))
+   do i = 1, nm
+      xsi(i) = Cm(i, i) / (2.0_real64 * sqrt(Km(i) * Mm(i)))
+   end do
+   call bsa_setTotDamping(xsi)
+   if (grid_checks) call bsa_setTestMode(0)
 
    call bsa_setPSDType(psdid)
    call bsa_setWindDirections([1])
@@ -90,9 +94,24 @@ program bsalib_driver
    used = bsa_getUsedModeShapes()
 
    open(newunit=uout, file=trim(fout), access='stream', form='unformatted', status='replace')
-   write(uout) int(size(used), int32), used, m2mf, m2mr, m2o2mr, m3mf, m3mr
+   write(uout) int(size(used), int32), used
+   call put(m2mf)
+   call put(m2mr)
+   call put(m2o2mr)
+   call put(m3mf)
+   call put(m3mr)
+   call put(m3mf_msh)
+   call put(m3mr_msh)
    close(uout)
-   if (allocated(m3mf_msh) .or. allocated(m3mr_msh)) error stop 'Mesher outputs must stay unallocated'
    call bsa_Finalise()
    if (.not. bsa_isCleaned()) error stop 'bsa_Finalise did not clean up'
+contains
+   subroutine put(a)
+      real(real64), allocatable, intent(in) :: a(:)
+      if (allocated(a)) then
+         write(uout) int(size(a), int32), a
+      else
+         write(uout) 0_int32
+      end if
+   end subroutine
 end program bsalib_driver

@@ -90,3 +90,24 @@ def test_fortran_front_grid_checks(gpu_lib, nfreqs, df, branch):
     want = gpu_lib.classic_run(q, quad="riemann" if scalar else "vect")
     for k in ("m2mf", "m2mr", "m2o2mr", "m3mf", "m3mr"):
         assert rel_err(out[k], want[k]) < 1e-13, k
+
+
+@pytest.mark.parametrize("mode", ["mesher", "both"])
+def test_fortran_front_mesher(gpu_lib, mode):
+    """Analysis type 2 (Mesher) and 3 (both) through bsa_Run: the Mesher moments
+    equal the C ABI Mesher with the same settings (bsa_setupMesher(0, 20, 2, 2, 2,
+    ...), background width 1/max(L/U), rule-based levels)."""
+    from bsalib_amd import abi, host
+    p = CASES["config1"]()
+    out, r = fdriver.run(p, mode)
+    assert "Mesher on the GPU" in r.stdout
+    M = abi.Mesher(p, bkg_base_rfmnt=20, tile_cells=32, bkg_peakw=host.bkg_peak_width(p))
+    want = M.run()
+    M.close()
+    assert rel_err(out["m3mf_msh"], want["m3mf"]) < 1e-13
+    assert rel_err(out["m3mr_msh"], want["m3mr"]) < 1e-13
+    if mode == "mesher":
+        assert out["m3mr"] is None and out["m2mr"] is None
+    else:
+        g = load_golden("config1")
+        assert rel_err(out["m3mr"], g["m3mr"]) < 1e-12
