@@ -52,7 +52,8 @@ EXPORTS = ["bsahip_init", "bsahip_finalise", "bsahip_last_error", "bsahip_versio
            "bsahip_last_kernel_time", "bsahip_plan_create", "bsahip_plan_run",
            "bsahip_plan_destroy", "bsahip_mesher_create", "bsahip_mesher_layout", "bsahip_mesher_ntiles",
            "bsahip_mesher_tiles", "bsahip_mesher_features", "bsahip_mesher_run",
-           "bsahip_mesher_run_tiles",
+           "bsahip_mesher_run_tiles", "bsahip_batch_create", "bsahip_batch_autotune",
+           "bsahip_batch_run", "bsahip_batch_info", "bsahip_batch_destroy",
            "bsahip_mesher_destroy"]
 
 
@@ -329,6 +330,70 @@ class Mesher:
     def close(self):
         if self.h:
             lib().bsahip_mesher_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001
+            pass
+
+
+class Batch:
+    """Structures of uneven size resident on the device (bsahip_batch_*):
+    run(mode=0) is one persistent G launch over all of them replayed from a
+    HIP graph, run(mode=1) one G launch per structure."""
+
+    def __init__(self, probs, quad: str = "vect", prec: str = "fp64", psd: bool = True):
+        L = lib()
+        if not getattr(L, "_batch_bound", False):
+            L.bsahip_batch_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(C.c_int32),
+                                              C.POINTER(C.c_double), C.c_int32, C.c_int32,
+                                              C.POINTER(C.c_void_p)]
+            L.bsahip_batch_autotune.argtypes = [C.c_void_p, C.POINTER(C.c_int32),
+                                                C.POINTER(C.c_int32)]
+            L.bsahip_batch_run.argtypes = [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p),
+                                           C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
+            L.bsahip_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64),
+                                            C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+            L.bsahip_batch_destroy.argtypes = [C.c_void_p]
+            L._batch_bound = True
+        self.bounds = [Bound(p) for p in probs]
+        self.nm = [b.nm for b in self.bounds]
+        self.evals = int(sum((2 * p.nfreqs - 1) ** 2 for p in probs))
+        n = len(probs)
+        ptrs = (C.c_void_p * n)(*[C.addressof(b.struct) for b in self.bounds])
+        nfh = np.array([p.nfreqs for p in probs], dtype=np.int32)
+        df = np.array([p.df for p in probs], dtype=np.float64)
+        flags = PREC[prec] | QUAD[quad] | (0 if psd else NO_PSD)
+        h = C.c_void_p()
+        self.h = None
+        _check(L.bsahip_batch_create(ptrs, _ptr(nfh, C.c_int32), _ptr(df, C.c_double), n, flags,
+                                     C.byref(h)))
+        self.h = h
+
+    def autotune(self):
+        m, c = C.c_int32(0), C.c_int32(0)
+        _check(lib().bsahip_batch_autotune(self.h, C.byref(m), C.byref(c)))
+        return m.value, c.value
+
+    def run(self, mode: int = 0) -> dict:
+        n = len(self.nm)
+        out = {k: [np.zeros(nm ** 3) for nm in self.nm] for k in ("m3mf", "m3mr")}
+        out["m2mr"] = [np.zeros(nm ** 2) for nm in self.nm]
+        arr = {k: (C.c_void_p * n)(*[v.ctypes.data for v in out[k]]) for k in out}
+        _check(lib().bsahip_batch_run(self.h, C.c_int32(mode), arr["m3mf"], arr["m3mr"], arr["m2mr"]))
+        return out
+
+    def info(self) -> dict:
+        ms, items = C.c_double(0), C.c_int64(0)
+        m, c = C.c_int32(0), C.c_int32(0)
+        _check(lib().bsahip_batch_info(self.h, C.byref(ms), C.byref(items), C.byref(m), C.byref(c)))
+        return {"ms_step": ms.value, "items": items.value, "grid_mult": m.value, "chunk": c.value}
+
+    def close(self):
+        if self.h:
+            lib().bsahip_batch_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -371,65 +371,113 @@ int plan_setup(bsahip_plan *P, const bsahip_problem *p, int32_t nfh, double df, 
    return 0;
 }
 
-int plan_exec(bsahip_plan *P, double *m2mf, double *m2mr, double *m2o2mr, double *m3mf, double *m3mr)
+// Phases of a plan run (all stream-ordered on g.s, no host synchronisation):
+// tables (PSD, projection, FRF, split planes), G contraction, modal PSD moments.
+int plan_prepare(bsahip_plan *P)
 {
    Prob &D = P->D;
-   const size_t nm3 = (size_t)D.nm * D.nm * D.nm, nm2 = (size_t)D.nm * D.nm;
-   const bool want_psd = !(P->flags & BSAHIP_NO_PSD);
-   const bool want_bisp = !(P->flags & BSAHIP_NO_BISP);
+   const size_t nm3 = (size_t)D.nm * D.nm * D.nm;
    CHK(build_table(P->B, D, P->fl, P->nu, P->nhp, P->U, P->ldU));
    CHK(launch_frf(D, P->nft, P->df, P->H));
-   P->last_ms = 0.0;
-   if (want_bisp) {
+   if (!(P->flags & BSAHIP_NO_BISP)) {
       CHK(fill(P->m3, 2 * nm3, 0.0));
-      GsymArgs a;
-      a.U = P->U;
-      a.ldU = P->ldU;
-      a.pb = D.pb_k;
-      a.K = D.K;
-      a.nm = D.nm;
-      a.nhp = P->nhp;
-      a.nh = P->nh;
-      a.nu = P->nu;
-      a.ntile = P->ntile;
-      a.npairs = (long)P->ntile * (P->ntile + 1) / 2;
-      a.H = P->H;
-      a.nft = P->nft;
-      a.d2 = P->d * P->d;
-      a.rule = (P->flags & BSAHIP_QUAD_MASK) >> 4;
-      a.m3f = P->m3;
-      a.m3r = P->m3 + nm3;
-      const long nwg = a.npairs * D.nm;
       if (P->prec == BSAHIP_PREC_BF16X3) {
          dim3 gs((unsigned)(P->ldU / 32), (unsigned)((D.K + 31) / 32));
          k_split_table<<<gs, 256, 0, g.s>>>(P->U, P->ldU, D.K, (int)P->ldU, D.K, P->UR, P->Ch, P->Cl);
          HIPC(hipGetLastError());
-         GsymArgs16 b16;
-         b16.UR = P->UR;
-         b16.UCh = reinterpret_cast<const uint4 *>(P->Ch);
-         b16.UCl = reinterpret_cast<const uint4 *>(P->Cl);
-         b16.pb32 = D.pb32;
-         b16.ldk = D.K;
-         HIPC(hipEventRecord(g.e0, g.s));
-         k_gsym_bf16x3<<<dim3((unsigned)nwg), 256, 0, g.s>>>(a, b16);
-      } else {
-         HIPC(hipEventRecord(g.e0, g.s));
-         k_gsym_f64<<<dim3((unsigned)nwg), 256, 0, g.s>>>(a);
       }
-      HIPC(hipGetLastError());
-      HIPC(hipEventRecord(g.e1, g.s));
+   }
+   return 0;
+}
+
+void plan_args(const bsahip_plan *P, GsymArgs &a, GsymArgs16 &b16)
+{
+   const Prob &D = P->D;
+   const size_t nm3 = (size_t)D.nm * D.nm * D.nm;
+   a.U = P->U;
+   a.ldU = P->ldU;
+   a.pb = D.pb_k;
+   a.K = D.K;
+   a.nm = D.nm;
+   a.nhp = P->nhp;
+   a.nh = P->nh;
+   a.nu = P->nu;
+   a.ntile = P->ntile;
+   a.npairs = (long)P->ntile * (P->ntile + 1) / 2;
+   a.H = P->H;
+   a.nft = P->nft;
+   a.d2 = P->d * P->d;
+   a.rule = (P->flags & BSAHIP_QUAD_MASK) >> 4;
+   a.m3f = P->m3;
+   a.m3r = P->m3 + nm3;
+   b16.UR = P->UR;
+   b16.UCh = reinterpret_cast<const uint4 *>(P->Ch);
+   b16.UCl = reinterpret_cast<const uint4 *>(P->Cl);
+   b16.pb32 = D.pb32;
+   b16.ldk = D.K;
+}
+
+long plan_items(const bsahip_plan *P)
+{
+   return (long)P->ntile * (P->ntile + 1) / 2 * P->D.nm;
+}
+
+int plan_gsym(bsahip_plan *P)
+{
+   GsymArgs a;
+   GsymArgs16 b16;
+   plan_args(P, a, b16);
+   const long nwg = plan_items(P);
+   if (P->prec == BSAHIP_PREC_BF16X3)
+      k_gsym_bf16x3<<<dim3((unsigned)nwg), 256, 0, g.s>>>(a, b16);
+   else
+      k_gsym_f64<<<dim3((unsigned)nwg), 256, 0, g.s>>>(a);
+   HIPC(hipGetLastError());
+   return 0;
+}
+
+int plan_m2(bsahip_plan *P)
+{
+   Prob &D = P->D;
+   const size_t nm2 = (size_t)D.nm * D.nm;
+   CHK(fill(P->m2, 3 * nm2, 0.0));
+   dim3 gm((unsigned)(D.nm * D.nm), (unsigned)((P->nh + 63) / 64));
+   k_m2<<<gm, 256, 0, g.s>>>(P->U, P->ldU, D.pa_k, D.K, D.nm, P->nhp, P->nh, P->H, P->nft, P->d,
+                             2.0 * M_PI * P->df, P->m2, P->m2 + nm2, P->m2 + 2 * nm2);
+   HIPC(hipGetLastError());
+   return 0;
+}
+
+int plan_copyout(bsahip_plan *P, double *m2mf, double *m2mr, double *m2o2mr, double *m3mf,
+                 double *m3mr)
+{
+   const Prob &D = P->D;
+   const size_t nm3 = (size_t)D.nm * D.nm * D.nm, nm2 = (size_t)D.nm * D.nm;
+   if (!(P->flags & BSAHIP_NO_BISP)) {
       if (m3mf) HIPC(hipMemcpyAsync(m3mf, P->m3, nm3 * sizeof(double), hipMemcpyDeviceToHost, g.s));
       if (m3mr) HIPC(hipMemcpyAsync(m3mr, P->m3 + nm3, nm3 * sizeof(double), hipMemcpyDeviceToHost, g.s));
    }
-   if (want_psd) {
-      k_m2<<<D.nm * D.nm, 256, 0, g.s>>>(P->U, P->ldU, D.pa_k, D.K, D.nm, P->nhp, P->nh, P->H,
-                                          P->nft, P->d, 2.0 * M_PI * P->df, P->m2, P->m2 + nm2,
-                                          P->m2 + 2 * nm2);
-      HIPC(hipGetLastError());
+   if (!(P->flags & BSAHIP_NO_PSD)) {
       if (m2mf) HIPC(hipMemcpyAsync(m2mf, P->m2, nm2 * sizeof(double), hipMemcpyDeviceToHost, g.s));
       if (m2mr) HIPC(hipMemcpyAsync(m2mr, P->m2 + nm2, nm2 * sizeof(double), hipMemcpyDeviceToHost, g.s));
       if (m2o2mr) HIPC(hipMemcpyAsync(m2o2mr, P->m2 + 2 * nm2, nm2 * sizeof(double), hipMemcpyDeviceToHost, g.s));
    }
+   return 0;
+}
+
+int plan_exec(bsahip_plan *P, double *m2mf, double *m2mr, double *m2o2mr, double *m3mf, double *m3mr)
+{
+   const bool want_psd = !(P->flags & BSAHIP_NO_PSD);
+   const bool want_bisp = !(P->flags & BSAHIP_NO_BISP);
+   CHK(plan_prepare(P));
+   P->last_ms = 0.0;
+   if (want_bisp) {
+      HIPC(hipEventRecord(g.e0, g.s));
+      CHK(plan_gsym(P));
+      HIPC(hipEventRecord(g.e1, g.s));
+   }
+   if (want_psd) CHK(plan_m2(P));
+   CHK(plan_copyout(P, m2mf, m2mr, m2o2mr, m3mf, m3mr));
    HIPC(hipStreamSynchronize(g.s));
    g.last_ms = 0.0;
    g.last_launches = 0;
@@ -734,3 +782,4 @@ int bsahip_selftest_mfma_bf16_layout(float *out768)
 
 #include "mesher.hip"
 #include "mlp.hip"
+#include "batch.hip"

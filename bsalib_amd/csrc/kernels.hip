@@ -415,21 +415,23 @@ __device__ __forceinline__ void pair_from_index(long t, int &R, int &Q)
    R = (int)(t - q * (q + 1) / 2);
 }
 
-__global__ __launch_bounds__(256) void k_gsym_f64(GsymArgs a)
+// XCD-aware order: workgroups dispatched to one XCD take consecutive logical
+// ids, so the nm modes x that share one (R,Q) tile pair run on the same L2.
+__device__ __forceinline__ long xcd_lid()
 {
-   __shared__ double As[KC][TILE];
-   __shared__ double Bs[KC][TILE];
-
-   // XCD-aware order: workgroups dispatched to one XCD take consecutive
-   // logical ids, so the nm modes x that share one (R,Q) tile pair run on the
-   // same L2.
    const long nwg = (long)gridDim.x;
    const long wid = blockIdx.x;
    long lid = wid;
    const long per = nwg / 8;
    if (wid < per * 8) lid = (wid % 8) * per + wid / 8;
-   const int x = (int)(lid % a.nm);
-   const long tp = lid / a.nm;
+   return lid;
+}
+
+// One work item of the FP64 G contraction: mode x, tile pair tp.
+__device__ __forceinline__ void gsym_f64_item(const GsymArgs &a, int x, long tp)
+{
+   __shared__ double As[KC][TILE];
+   __shared__ double Bs[KC][TILE];
    int R, Q;
    pair_from_index(tp, R, Q);
    const bool diag = (R == Q);
@@ -503,6 +505,12 @@ __global__ __launch_bounds__(256) void k_gsym_f64(GsymArgs a)
                   });
 }
 
+__global__ __launch_bounds__(256) void k_gsym_f64(GsymArgs a)
+{
+   const long lid = xcd_lid();
+   gsym_f64_item(a, (int)(lid % a.nm), lid / a.nm);
+}
+
 // ---------------------------------------------------------------------------
 // G contraction in split-bf16 x3 on v_mfma_f32_16x16x32_bf16 (FP32 accumulate)
 //   a*b ~= a_hi*b_hi + a_hi*b_lo + a_lo*b_hi,  a_hi = bf16(a), a_lo = bf16(a - a_hi)
@@ -544,17 +552,11 @@ struct GsymArgs16 {
    long ldk;              // K stride (multiple of 32)
 };
 
-__global__ __launch_bounds__(256, 2) void k_gsym_bf16x3(GsymArgs a, GsymArgs16 b)
+// One work item of the split-bf16x3 G contraction: mode x, tile pair tp.
+__device__ __forceinline__ void gsym_bf16x3_item(const GsymArgs &a, const GsymArgs16 &b, int x,
+                                                 long tp)
 {
    __shared__ uint4 Ah[2][T2 * 4], Al[2][T2 * 4], Bh[2][T2 * 4], Bl[2][T2 * 4];
-
-   const long nwg = (long)gridDim.x;
-   const long wid = blockIdx.x;
-   long lid = wid;
-   const long per = nwg / 8;
-   if (wid < per * 8) lid = (wid % 8) * per + wid / 8;
-   const int x = (int)(lid % a.nm);
-   const long tp = lid / a.nm;
    int R, Q;
    pair_from_index(tp, R, Q);
    const bool diag = (R == Q);
@@ -677,6 +679,12 @@ __global__ __launch_bounds__(256, 2) void k_gsym_bf16x3(GsymArgs a, GsymArgs16 b
                   });
 }
 
+__global__ __launch_bounds__(256, 2) void k_gsym_bf16x3(GsymArgs a, GsymArgs16 b)
+{
+   const long lid = xcd_lid();
+   gsym_bf16x3_item(a, b, (int)(lid % a.nm), lid / a.nm);
+}
+
 // A table [K][ldU] (FP64) -> row operand [rows][ldk] FP32 and column operand
 // [rows][ldk] split into bf16 hi/lo.  32x32 tiles transposed through LDS.
 __global__ void k_split_table(const double *__restrict__ U, long ldU, int K, int rows, long ldk,
@@ -750,6 +758,10 @@ __global__ void k_mfma_bf16_layout(float *__restrict__ out)
 // BsaClassicImpl.F90:197-210, 567-589).  One block per (n,p); deterministic.
 // psd(n,p;u) = sum_k U[k][n*nhp+u] Psi_a(p,k); the grid holds +-u.
 // ---------------------------------------------------------------------------
+// Modal PSD moments: PSD(n,p;u) = sum_k A(n,k;u) Psi_a(p,k) on the half grid,
+// weighted with the vect rule (+u and -u), times Re[H_p conj(H_n)] and w^2.
+// grid (nm*nm, ceil(nh/64)), block 256 = 64 frequencies x 4 slices of K;
+// partial sums are added atomically (the outputs start at zero).
 __global__ __launch_bounds__(256) void k_m2(const double *__restrict__ U, long ldU,
                                             const double *__restrict__ pa, int K, int nm,
                                             int nhp, int nh, const double2 *__restrict__ H,
@@ -757,37 +769,41 @@ __global__ __launch_bounds__(256) void k_m2(const double *__restrict__ U, long l
                                             double *__restrict__ m2f, double *__restrict__ m2r,
                                             double *__restrict__ m2o)
 {
-   __shared__ double red[3][256];
+   __shared__ double part[4][64];
+   __shared__ double red[3][64];
    const int n = blockIdx.x % nm, p = blockIdx.x / nm;
-   double sf = 0.0, sr = 0.0, so = 0.0;
-   for (int u = threadIdx.x; u < nh; u += blockDim.x) {
-      double s = 0.0;
+   const int tu = threadIdx.x & 63, tk = threadIdx.x >> 6;
+   const int u = blockIdx.y * 64 + tu;
+   double s = 0.0;
+   if (u < nh) {
       const double *col = U + (size_t)n * nhp + u;
       const double *pp = pa + (size_t)p * K;
-      for (int k = 0; k < K; ++k) s = fma(col[(size_t)k * ldU], pp[k], s);
-      double w = (u == nh - 1) ? 0.5 * d : d;
-      if (u > 0) w *= 2.0;                                 // +u and -u
-      const double2 hp = H[(size_t)p * nft + u], hn = H[(size_t)n * nft + u];
-      const double fr = hp.x * hn.x + hp.y * hn.y;          // Re[H_p conj(H_n)]
-      const double om = om_step * u;
-      sf += w * s;
-      sr += w * s * fr;
-      so += w * s * fr * om * om;
+      for (int k = tk; k < K; k += 4) s = fma(col[(size_t)k * ldU], pp[k], s);
    }
-   red[0][threadIdx.x] = sf;
-   red[1][threadIdx.x] = sr;
-   red[2][threadIdx.x] = so;
+   part[tk][tu] = s;
    __syncthreads();
-   for (int o = 128; o > 0; o >>= 1) {
-      if (threadIdx.x < o)
-         for (int c = 0; c < 3; ++c) red[c][threadIdx.x] += red[c][threadIdx.x + o];
-      __syncthreads();
-   }
-   if (threadIdx.x == 0) {
-      const int id = n + nm * p;
-      if (m2f) m2f[id] = red[0][0];
-      if (m2r) m2r[id] = red[1][0];
-      if (m2o) m2o[id] = red[2][0];
+   if (tk == 0) {
+      double sf = 0.0, sr = 0.0, so = 0.0;
+      if (u < nh) {
+         const double v = part[0][tu] + part[1][tu] + part[2][tu] + part[3][tu];
+         double w = (u == nh - 1) ? 0.5 * d : d;
+         if (u > 0) w *= 2.0;                                 // +u and -u
+         const double2 hp = H[(size_t)p * nft + u], hn = H[(size_t)n * nft + u];
+         const double fr = hp.x * hn.x + hp.y * hn.y;          // Re[H_p conj(H_n)]
+         const double om = om_step * u;
+         sf = w * v;
+         sr = w * v * fr;
+         so = w * v * fr * om * om;
+      }
+      red[0][tu] = wave_sum(sf);
+      red[1][tu] = wave_sum(sr);
+      red[2][tu] = wave_sum(so);
+      if (tu == 0) {
+         const int id = n + nm * p;
+         if (m2f) atomicAdd(m2f + id, red[0][0]);
+         if (m2r) atomicAdd(m2r + id, red[1][0]);
+         if (m2o) atomicAdd(m2o + id, red[2][0]);
+      }
    }
 }
 

@@ -239,6 +239,32 @@ int bsahip_mlp_train(bsahip_mlp *mlp, const float *x, const float *y, const floa
                      float *losses);
 int bsahip_mlp_destroy(bsahip_mlp *mlp);
 
+/* ---- Batches of structures of uneven size (no counterpart in the reference) ----
+ * Each structure keeps a resident plan (as bsahip_plan_create, truncated Classic
+ * sum).  A step runs every structure's tables, then ONE persistent G launch
+ * whose workgroups take (structure, tile pair, mode) items from a device queue
+ * ordered by decreasing item cost, then the modal PSD moments; mode 0 replays
+ * that step from a HIP graph, mode 1 launches the G contraction once per
+ * structure instead (the comparison point).  The item list is split into one
+ * queue per XCD (workgroup blockIdx % 8) with work stealing at the end. */
+typedef struct bsahip_batch bsahip_batch;
+int bsahip_batch_create(const bsahip_problem *const *probs, const int32_t *nfh, const double *df,
+                        int32_t n, int32_t flags, bsahip_batch **out);
+/* times the persistent launch for grid sizes {1,2,4} x CUs x occupancy and
+ * {1,4,16} items per queue ticket on a quarter of the queue; keeps the fastest */
+int bsahip_batch_autotune(bsahip_batch *b, int32_t *grid_mult, int32_t *chunk);
+/* mode 0: the captured graph of the step; 1: one G launch per structure, all on
+ * the library stream; 2: the step of mode 0 launched directly (the worker
+ * streams then run concurrently on the hardware queues).  Per-structure outputs
+ * (arrays of n host pointers, each NM_s^3 or NM_s^2; any may be NULL). */
+int bsahip_batch_run(bsahip_batch *b, int32_t mode, double *const *m3mf, double *const *m3mr,
+                     double *const *m2mr);
+/* device time of the last step (tables + contraction + PSD moments), queue size,
+ * current launch shape */
+int bsahip_batch_info(const bsahip_batch *b, double *ms_step, int64_t *items, int32_t *grid_mult,
+                      int32_t *chunk);
+int bsahip_batch_destroy(bsahip_batch *b);
+
 #ifdef __cplusplus
 }
 #endif
