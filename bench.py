@@ -82,12 +82,63 @@ def cpu_factorised_baseline(prob, threads: int, npts: int = 64):
                       f"({t:.2f} s)"}
 
 
+def batch_leg(log):
+    """Batches of uneven structures (bsahip_batch_*): the 256-structure mix of
+    configuration 4 (NN 50-800, NM 4-24, 1-3 components, grids 257-1025) and 256
+    small towers (NN 20-60, NM 3-6, grids 65-129), split-bf16x3.  evals/s from
+    the device time of a step (tables + contraction + PSD moments) for one G
+    launch per structure (mode 1), the persistent launch replayed from a HIP
+    graph (mode 0) and the persistent launch with tables on worker streams
+    (mode 2)."""
+    from bsalib_amd import abi, synth
+    out = {}
+    rng = np.random.default_rng(5)
+    small = []
+    for s in range(256):
+        nm = int(rng.integers(3, 7))
+        small.append(synth.tower(nn=int(rng.integers(20, 61)), nm=nm,
+                                 natf=np.sort(rng.uniform(0.15, 2.0, nm)),
+                                 xi=rng.uniform(0.005, 0.02, nm), nfreqs=int(rng.choice([33, 65])),
+                                 df=1 / 16, seed=s, zone_per_node=False))
+    for name, probs in (("c4_mix_256", synth.batch(256, seed=0)), ("small_towers_256", small)):
+        B = abi.Batch(probs, prec="bf16x3")
+        grid_mult, chunk = B.autotune()
+        res = {"structures": len(probs), "evals_per_step": B.evals, "items": B.info()["items"],
+               "autotuned": {"grid_mult": grid_mult, "chunk": chunk}}
+        for mode, key in ((1, "per_structure"), (0, "graph_persistent"), (2, "streams_persistent")):
+            B.run(mode)
+            ms = []
+            for _ in range(2):
+                B.run(mode)
+                ms.append(B.info()["ms_step"])
+            res[key] = {"ms_per_step": min(ms), "evals_per_s": B.evals / (min(ms) * 1e-3)}
+        res["speedup_best_vs_per_structure"] = res["per_structure"]["ms_per_step"] / min(
+            res["graph_persistent"]["ms_per_step"], res["streams_persistent"]["ms_per_step"])
+        B.close()
+        log(f"batch {name}: {res}")
+        out[name] = res
+    return out
+
+
+def mesher_leg(log):
+    """Learned refinement proposer on the deck of configuration 5 (NN=500,
+    NM=20): BFM evaluations and third-moment error against the converged Mesher
+    (every tile at level 8), next to the rule-based Mesher."""
+    from bsalib_amd import proposer, synth
+    P = proposer.Proposer()
+    r = proposer.evaluate(synth.config5(), P, log=log)
+    P.net.close()
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the structure-batch leg")
+    ap.add_argument("--no-mesher", action="store_true", help="skip the learned-Mesher leg")
     ap.add_argument("--config", default="config3", choices=["config1", "config2", "config3"])
     ap.add_argument("--prec", default="both", choices=["both", "fp64", "bf16x3"],
                     help="profiling aid: run only one precision (the JSON line needs both)")
@@ -142,6 +193,8 @@ def main():
     log(f"bf16x3: step {ms_step:.2f} ms, k_gsym_bf16x3 {k_ms:.2f} ms, {achieved:.2f} TFLOP/s, "
         f"speedup vs fp64 {value / value64:.2f}x, skewness err {sk_err:.2e}")
 
+    batch = None if args.no_batch else batch_leg(log)
+    mesher = None if args.no_mesher else mesher_leg(log)
     cpu = None
     cpu2 = None
     if not args.no_cpu:
@@ -188,6 +241,8 @@ def main():
         "m3mr_rel_err_vs_fp64": m3_err,
         "cpu_baseline": cpu,
         "cpu_baseline_factorised": cpu2,
+        "batch": batch,
+        "mesher_proposer": mesher,
     }
     print(json.dumps(out), flush=True)
 
