@@ -142,7 +142,18 @@ def main():
     ap.add_argument("--config", default="config3", choices=["config1", "config2", "config3"])
     ap.add_argument("--prec", default="both", choices=["both", "fp64", "bf16x3"],
                     help="profiling aid: run only one precision (the JSON line needs both)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="profiling aid: one 1 GiB device-to-device copy (HBM counter scale)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(2 ** 28, device="cuda", dtype=torch.float32)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log("copied 1 GiB")
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build targets one GPU; multi-GPU runs are not part of it "
               "(the (f1,f2) plane would be sharded by tile pairs with one all-reduce of m3).",

@@ -277,17 +277,22 @@ __device__ __forceinline__ void epi_accum(const GsymArgs &a, int x, int y, int z
    const double2 *Hx = a.H + (size_t)x * a.nft;
    const double2 *Hy = a.H + (size_t)y * a.nft;
    const double2 *Hz = a.H + (size_t)z * a.nft;
-   const int v = vq0 + (lane & 15);
-   if (v >= a.nu) return;
+   // branch-free over the lanes: out-of-range rows/columns read row/column 0
+   // and contribute with weight 0 (divergent exits would make the compiler
+   // copy the nine accumulators at every join)
+   const int vr = vq0 + (lane & 15);
+   const bool vok = vr < a.nu;
+   const int v = vok ? vr : 0;
    const double2 hxv = Hx[v], hyv = Hy[v];
    const double2 cyv = make_double2(hyv.x, -hyv.y);
    const double2 cxv = make_double2(hxv.x, -hxv.y);
    const bool ev = (v == n), ov = (v > n);
 #pragma unroll
    for (int t = 0; t < 4; ++t) {
-      const int u = ur0 + (interleaved ? ((lane >> 4) + 4 * t) : (4 * (lane >> 4) + t));
-      if (u >= a.nu) continue;
-      const double g = 2.0 * gval[t];      // B = 2 * sum of the three G terms
+      const int ur = ur0 + (interleaved ? ((lane >> 4) + 4 * t) : (4 * (lane >> 4) + t));
+      const bool ok = vok && ur < a.nu;
+      const int u = ok ? ur : 0;
+      const double g = ok ? 2.0 * gval[t] : 0.0;   // B = 2 * sum of the three G terms
       const double ms = (u == 0 && v == 0) ? 1.0 : 2.0;
       const double mo = (u > 0 && v > 0) ? 2.0 : 0.0;
       const int sp = u + v;
@@ -438,7 +443,7 @@ __device__ __forceinline__ void gsym_f64_item(const GsymArgs &a, int x, long tp)
 
    const int tid = threadIdx.x;
    const int lane = tid & 63;
-   const int w = tid >> 6;
+   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform
    const int wr = w >> 1, wc = w & 1;
 
    // staging: each thread moves 8 consecutive doubles of one k row per tile
@@ -563,7 +568,7 @@ __device__ __forceinline__ void gsym_bf16x3_item(const GsymArgs &a, const GsymAr
 
    const int tid = threadIdx.x;
    const int lane = tid & 63;
-   const int w = tid >> 6;
+   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform
    const int wr = w >> 1, wc = w & 1;
 
    // staging: items tid and tid+256 -> (row = item/4, k-group = item%4 of 8)
