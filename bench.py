@@ -29,6 +29,11 @@ sys.path.insert(0, str(ROOT))
 METRIC = "modal-bispectrum evals/s (NN=1000,NM=30) at skew err≤1e-4: mixed vs FP64 GPU"
 FP64_PEAK_TFLOPS = 78.6
 BF16_PEAK_TFLOPS = 2500.0
+# HBM bytes per k_gsym_bf16x3 launch on the deck workload (1025^2, NM=30):
+# rocprofv3 --pmc FETCH_SIZE x 2 + WRITE_SIZE x 1 (KiB), the scales read off a
+# 1 GiB device copy (bench.py --copy-calibration); commands and counts in
+# profiles/gsym_bf16x3_traffic_deck_nn1000_nm30.md
+GSYM_BF16X3_TRAFFIC_B = (2 * 4303276 + 1241578) * 1024
 
 
 def log(msg):
@@ -239,7 +244,8 @@ def main():
                    "nn": prob.nn, "nm": nm, "ntc": prob.ntc, "grid": prob.nf,
                    "precision": "bf16x3"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / BF16_PEAK_TFLOPS,
+                     "traffic": GSYM_BF16X3_TRAFFIC_B if args.config == "config3" else None,
                      "kernel": "k_gsym_bf16x3", "kernel_ms": k_ms, "flops_per_launch": flops,
                      "note": "algorithmic flops counted once; split-bf16x3 issues 3 MFMA "
                              "products per term, so its attainable fraction is <= 1/3"},

@@ -36,6 +36,7 @@ class Problem_t(C.Structure):
         ("nn", C.c_int32), ("nnl", C.c_int32), ("nm", C.c_int32), ("ntc", C.c_int32),
         ("ndegw", C.c_int32), ("nwz", C.c_int32), ("psd_id", C.c_int32),
         ("pulsation", C.c_int32), ("ncorr_rows", C.c_int32), ("corr_col_mode", C.c_int32),
+        ("pod_nmodes", C.c_int32), ("reserved", C.c_int32), ("pod_trunc_lim", C.c_double),
         ("n_load", C.POINTER(C.c_int32)), ("tc", C.POINTER(C.c_int32)),
         ("psi", C.POINTER(C.c_double)), ("nod_corr", C.POINTER(C.c_double)),
         ("u_node", C.POINTER(C.c_double)), ("wz_node", C.POINTER(C.c_int32)),
@@ -98,7 +99,10 @@ def _ptr(a, ct):
 class Bound:
     """A problem packed into the C struct, keeping the numpy buffers alive."""
 
-    def __init__(self, prob, modes=None, corr_col_mode: int = 0, psd_id=None):
+    def __init__(self, prob, modes=None, corr_col_mode: int = 0, psd_id=None,
+                 pod_nmodes: int = 0, pod_trunc_lim: float = 0.0):
+        """pod_nmodes / pod_trunc_lim: truncated POD of S(f) in the point BFM
+        (bsahip_bfm_points and the Mesher); either needs corr_col_mode=1."""
         self.modes = host.kept_modes(prob) if modes is None else np.asarray(modes, np.int32)
         psi = host.build_psi(prob, self.modes)
         mo = self.modes - 1
@@ -124,6 +128,8 @@ class Bound:
         s.pulsation = 1
         s.ncorr_rows = prob.nod_corr.shape[0]
         s.corr_col_mode = corr_col_mode
+        s.pod_nmodes = pod_nmodes
+        s.pod_trunc_lim = pod_trunc_lim
         for name in ("n_load", "tc", "wz_node"):
             setattr(s, name, _ptr(k[name], C.c_int32))
         for name in ("psi", "nod_corr", "u_node", "turb_scales", "turb_std", "u_mean_ref",

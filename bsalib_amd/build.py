@@ -24,6 +24,7 @@ OUT_DIR = PKG / "_lib"
 LIB = OUT_DIR / "libbsahip.so"
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ROCM_LIB = "/opt/rocm/lib"      # rocSOLVER + rocBLAS: the POD eigendecomposition
 FC = os.environ.get("FC", "/opt/rocm/bin/amdflang")
 FDIR = PKG / "fortran"
 FOUT = OUT_DIR / "fortran"
@@ -50,7 +51,8 @@ def deps():
 def build(verbose: bool = True) -> Path:
     OUT_DIR.mkdir(exist_ok=True)
     srcs = [s for s in sources() if s.exists()]
-    cmd = [HIPCC, *HIP_FLAGS, "-I", str(ROOT / "include"), *map(str, srcs), "-o", str(LIB)]
+    cmd = [HIPCC, *HIP_FLAGS, "-I", str(ROOT / "include"), *map(str, srcs), "-o", str(LIB),
+           "-L", ROCM_LIB, "-lrocsolver", "-lrocblas"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)

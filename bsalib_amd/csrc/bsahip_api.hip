@@ -7,8 +7,12 @@
 // plane epilogue -> modal PSD moments -> copy of the moments).  There is no
 // CPU fallback: without a usable GPU every entry point returns BSAHIP_EDEVICE.
 #include "kernels.hip"
+#include "pod.hip"
 #include "../../include/bsahip.h"
 #include "bsahip_internal.h"
+
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <cmath>
@@ -89,6 +93,7 @@ struct Ctx {
    Bufs *bufs = nullptr;                 // scratch of the one-shot entry points
    double last_ms = 0.0;
    int last_launches = 0;
+   rocblas_handle blas = nullptr;        // created on first use (truncated POD)
    std::recursive_mutex mu;
 };
 
@@ -156,6 +161,9 @@ struct Prob {
    double *prm = nullptr;    // [tc][nnl][4]
    int *formula = nullptr;   // [tc]
    double *Mm = nullptr, *Km = nullptr, *Cm = nullptr;
+   // truncated POD of S(f) in the point tables (0/0: none, direct projection)
+   int pod_nmodes = 0;
+   double pod_lim = 0.0;
 };
 
 // psd_mode 0: reference Classic/Mesher semantics (evalPSD per component);
@@ -245,7 +253,90 @@ int load_problem(const bsahip_problem *p, int psd_mode, Bufs &B, Prob &D)
    return 0;
 }
 
+// Truncated POD for the point paths (pod_nmodes / pod_trunc_lim of the problem).
+int set_pod(const bsahip_problem *p, Prob &D)
+{
+   D.pod_nmodes = 0;
+   D.pod_lim = 0.0;
+   if (p->pod_nmodes < 0 || !std::isfinite(p->pod_trunc_lim) || p->pod_trunc_lim < 0.0)
+      return fail(BSAHIP_EINVAL, "pod_nmodes must be >= 0 and pod_trunc_lim finite and >= 0");
+   if (p->pod_nmodes == 0 && p->pod_trunc_lim == 0.0) return 0;
+   if (p->corr_col_mode != 1)
+      return fail(BSAHIP_EINVAL, "a truncated POD reads coherence column 1 (corr_col_mode must be 1)");
+   D.pod_nmodes = p->pod_nmodes;
+   D.pod_lim = p->pod_trunc_lim;
+   return 0;
+}
+
 // A table on a list of |f| values (device array fl): U[K][ldU], rows y*rows_per_mode + q.
+#define BLASC(x)                                                                             \
+   do {                                                                                      \
+      rocblas_status st_ = (x);                                                              \
+      if (st_ != rocblas_status_success)                                                     \
+         return fail(BSAHIP_EDEVICE, std::string(#x) + ": " + rocblas_status_to_string(st_)); \
+   } while (0)
+
+// A_K(y, i; f) = (Psi_a V) diag(w) V^T for the truncated POD of S(f) (see
+// pod.hip), written into the table layout of k_project.  sq: sqrt(S_i) at fl.
+int pod_project(Bufs &B, const Prob &D, const double *sq, const double *fl, int nfl,
+                int rows_per_mode, double *U, long ldU)
+{
+   if (!g.blas) {
+      BLASC(rocblas_create_handle(&g.blas));
+      BLASC(rocblas_set_stream(g.blas, g.s));
+   }
+   const int n = D.nnl, nm = D.nm;
+   const size_t nn2 = (size_t)n * n, mn = (size_t)nm * n;
+   const long total = (long)D.ntc * nfl;
+   // at most 2^28 doubles (2 GiB) of matrices and 65535 matrices per chunk
+   const long cap = std::max<long>(1, std::min<long>({total, 65535L, (1L << 28) / (long)nn2}));
+   double *S, *lam, *E, *w, *T, *A;
+   int *info, *kept;
+   CHK(B.get("pod_S", (size_t)cap * nn2, &S));
+   CHK(B.get("pod_lam", (size_t)cap * n, &lam));
+   CHK(B.get("pod_E", (size_t)cap * n, &E));
+   CHK(B.get("pod_w", (size_t)cap * n, &w));
+   CHK(B.get("pod_T", (size_t)cap * mn, &T));
+   CHK(B.get("pod_A", (size_t)cap * mn, &A));
+   CHK(B.get("pod_info", (size_t)cap, &info));
+   CHK(B.get("pod_kept", (size_t)cap, &kept));
+   std::vector<int> hk((size_t)cap);
+   const double one = 1.0, zero = 0.0;
+   for (long b0 = 0; b0 < total; b0 += cap) {
+      const int nb = (int)std::min<long>(cap, total - b0);
+      k_pod_smat<<<dim3((unsigned)((nn2 + 255) / 256), nb), 256, 0, g.s>>>(D.lnr, sq, fl, nfl, n, b0, S);
+      HIPC(hipGetLastError());
+      BLASC(rocsolver_dsyevd_strided_batched(g.blas, rocblas_evect_original, rocblas_fill_lower, n, S,
+                                             n, (rocblas_stride)nn2, lam, n, E, n, info, nb));
+      k_pod_weights<<<nb, 64, 0, g.s>>>(lam, info, n, D.pod_nmodes, D.pod_lim, w, kept);
+      HIPC(hipGetLastError());
+      // T = Psi_a(c) V: Psi_a(c) is pa_t[c] read as the column-major n x nm
+      // matrix P, so T = P^T V; one strided batch per component in the chunk
+      for (long bb = b0; bb < b0 + nb;) {
+         const int c = (int)(bb / nfl);
+         const long be = std::min<long>(b0 + nb, (long)(c + 1) * nfl);
+         BLASC(rocblas_dgemm_strided_batched(
+            g.blas, rocblas_operation_transpose, rocblas_operation_none, nm, n, n, &one,
+            D.pa_t + (size_t)c * mn, n, 0, S + (size_t)(bb - b0) * nn2, n, (rocblas_stride)nn2, &zero,
+            T + (size_t)(bb - b0) * mn, nm, (rocblas_stride)mn, (int)(be - bb)));
+         bb = be;
+      }
+      k_pod_scale<<<dim3((unsigned)((mn + 255) / 256), nb), 256, 0, g.s>>>(T, w, nm, n);
+      HIPC(hipGetLastError());
+      BLASC(rocblas_dgemm_strided_batched(g.blas, rocblas_operation_none, rocblas_operation_transpose,
+                                          nm, n, n, &one, T, nm, (rocblas_stride)mn, S, n,
+                                          (rocblas_stride)nn2, &zero, A, nm, (rocblas_stride)mn, nb));
+      k_pod_scatter<<<dim3((unsigned)((mn + 255) / 256), nb), 256, 0, g.s>>>(A, nm, n, nfl, b0, D.nnlp,
+                                                                         rows_per_mode, U, ldU);
+      HIPC(hipGetLastError());
+      HIPC(hipMemcpyAsync(hk.data(), kept, (size_t)nb * sizeof(int), hipMemcpyDeviceToHost, g.s));
+      HIPC(hipStreamSynchronize(g.s));
+      for (int b = 0; b < nb; ++b)
+         if (hk[b] < 0) return fail(BSAHIP_EDEVICE, "POD eigendecomposition of S(f) did not converge");
+   }
+   return 0;
+}
+
 int build_table(Bufs &B, const Prob &D, const double *fl, int nfl, int rows_per_mode, double *U,
                 long ldU)
 {
@@ -256,6 +347,8 @@ int build_table(Bufs &B, const Prob &D, const double *fl, int nfl, int rows_per_
    dim3 gp((D.nnl + 63) / 64, nfl, D.ntc);
    k_psd_sqrt<<<gp, 64, 0, g.s>>>(fl, nfl, D.prm, D.formula, D.nnl, D.ntc, D.pulsation, sq);
    HIPC(hipGetLastError());
+   if (D.pod_nmodes > 0 || D.pod_lim > 0.0)
+      return pod_project(B, D, sq, fl, nfl, rows_per_mode, U, ldU);
    const int nq = (D.nm + 3) / 4;
 #define LAUNCH_PROJ(NQ)                                                                       \
    k_project<NQ><<<gp, 256, 0, g.s>>>(D.pa_t, D.lnr, sq, fl, nfl, D.nnl, D.nm, D.nnlp,        \
@@ -530,6 +623,8 @@ int bsahip_finalise(void)
    (void)hipStreamSynchronize(g.s);
    delete g.bufs;
    g.bufs = nullptr;
+   if (g.blas) (void)rocblas_destroy_handle(g.blas);
+   g.blas = nullptr;
    (void)hipEventDestroy(g.e0);
    (void)hipEventDestroy(g.e1);
    (void)hipStreamDestroy(g.s);
@@ -719,6 +814,7 @@ int bsahip_bfm_points(const bsahip_problem *p, int32_t npts, const double *fi, c
    Bufs &B = scratch();
    Prob D;
    CHK(load_problem(p, 0, B, D));
+   CHK(set_pod(p, D));
    std::vector<double> uq;
    std::vector<int> i1, i2, i3;
    point_tables(npts, fi, fj, uq, i1, i2, i3);

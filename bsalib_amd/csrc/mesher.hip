@@ -645,6 +645,7 @@ int bsahip_mesher_create(const bsahip_problem *p, const bsahip_mesher_cfg *cfg, 
       return fail(BSAHIP_EINVAL, "Mesher extensions and bkg_peakw must be > 0, tile_cells in 1..128");
    std::unique_ptr<bsahip_mesher> M(new bsahip_mesher());
    CHK(load_problem(p, 0, M->B, M->D));
+   CHK(set_pod(p, M->D));
    Prob &D = M->D;
    for (int i = 0; i < D.nm; ++i)
       if (!(cfg->natf[i] > 0.0) || !(cfg->xsi[i] > 0.0))

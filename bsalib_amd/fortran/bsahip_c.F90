@@ -19,6 +19,8 @@ module BsaHip_C
    type, bind(c), public :: bsahip_problem_t
       integer(c_int32_t) :: nn = 0, nnl = 0, nm = 0, ntc = 0, ndegw = 0, nwz = 0
       integer(c_int32_t) :: psd_id = 0, pulsation = 1, ncorr_rows = 0, corr_col_mode = 0
+      integer(c_int32_t) :: pod_nmodes = 0, reserved = 0
+      real(c_double) :: pod_trunc_lim = 0
       type(c_ptr) :: n_load = c_null_ptr, tc = c_null_ptr, psi = c_null_ptr
       type(c_ptr) :: nod_corr = c_null_ptr, u_node = c_null_ptr, wz_node = c_null_ptr
       type(c_ptr) :: turb_scales = c_null_ptr, turb_std = c_null_ptr, u_mean_ref = c_null_ptr

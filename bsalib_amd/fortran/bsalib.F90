@@ -21,7 +21,13 @@
 !!    PSDs are still computed when requested.  The Mesher is the device zone
 !!    Mesher of libbsahip (zone limits and policies of BsaMesherImpl.F90,
 !!    tiles of at most 32 x 32 level-0 cells, rule-based refinement levels);
-!!    its results are m3mf_msh, m3mr_msh.
+!!    its results are m3mf_msh, m3mr_msh.  With isvd = 1 its BFM is the POD
+!!    one; without truncation that is evaluated as the direct projection
+!!    with coherence column 1 (equal up to rounding), with truncation through
+!!    eigendecompositions on the GPU.  A threshold counts the modes as the
+!!    reference's dgesvd build does (largest first); its default dsyev build
+!!    scans its ascending eigenvalues from the small end and so keeps a single
+!!    mode unless the second-smallest eigenvalue passes the threshold.
 !!  * bsa_setComputePrecision is an addition: BSA_PREC_BF16X3 selects the
 !!    split-bf16x3 node contraction (FP32 accumulation on matrix cores,
 !!    FP64 everywhere else); the default is FP64.
@@ -65,6 +71,7 @@ module BsaLib
    public :: bsa_setKeptModalShapes, bsa_setModalMatrices, bsa_setTotDamping
    public :: bsa_getUsedModeShapes, bsa_setComputePrecision
    public :: bsa_setupMesher, bsa_setPremeshScheme, bsa_setMaxBkgPeakRestriction
+   public :: bsa_setPODTruncationThreshold, bsa_setPODNOfModesKept
    integer(bsa_int_t), parameter, public :: BSA_PREMESH_TYPE_DIAG_CREST_NO = 0
    integer(bsa_int_t), parameter, public :: BSA_PREMESH_TYPE_DIAG_CREST_YES = 1
 
@@ -74,9 +81,13 @@ module BsaLib
    integer(bsa_int_t) :: isym_ = 0, nfreqs_ = 0, iprec_ = BSA_PREC_FP64
    real(bsa_real_t) :: df_ = 0
    ! Mesher settings (src/settings/Settings.F90:86-116)
-   integer(bsa_int_t) :: bkgrfmt_ = 20, premesh_type_ = 0
+   integer(bsa_int_t) :: bkgrfmt_ = 20, premesh_type_ = 0, isvd_ = 1
    real(bsa_real_t) :: bkgaext_ = 2, genpaext_ = 2, maxaext_ = 2
    logical :: restrict_bkg_ = .false.
+   ! POD truncation (src/bsa/data/BsaLibData.F90:276-285)
+   logical :: do_trunc_pod_ = .false., npod_set_ = .false.
+   real(bsa_real_t) :: pod_trunc_lim_ = 0
+   integer(bsa_int_t) :: npod_ = 0
    logical :: validate_modal_ = .true., gpu_enabled_ = .false.
    logical :: cleaned_ = .true., device_ready_ = .false.
 
@@ -212,16 +223,37 @@ contains
       end if
    end subroutine
 
-   !> src/settings/SettingsImpl.F90:107-119; isvd, ifcov and idumpmod are
-   !> accepted for interface compatibility (the device Mesher evaluates the
-   !> BFM without POD, covers the full domain and keeps its zones in memory).
+   !> src/settings/SettingsImpl.F90:107-119.  isvd = 1 (the default) selects the
+   !> POD BFM (getFM_full_tm_scalar_msh_POD_, BsaLibImpl.F90:364-368), which
+   !> reads coherence column 1 for every turbulence component; 0 the direct
+   !> BFM with column tc.  ifcov and idumpmod are accepted for interface
+   !> compatibility (the device Mesher covers the full domain and keeps its
+   !> zones in memory).
    subroutine bsa_setupMesher(isvd, bkgrfmt, bkgaext, genpaext, maxaext, ifcov, idumpmod)
       integer(bsa_int_t), value :: isvd, bkgrfmt, ifcov, idumpmod
       real(bsa_real_t), value :: bkgaext, genpaext, maxaext
+      isvd_ = isvd
       bkgrfmt_ = bkgrfmt
       bkgaext_ = bkgaext
       genpaext_ = genpaext
       maxaext_ = maxaext
+   end subroutine
+
+   !> src/bsa/BsaLibImpl.F90:100-107: keep the POD modes whose eigenvalue is at
+   !> least rval % of the largest (Mesher, POD BFM only).
+   subroutine bsa_setPODTruncationThreshold(rval)
+      real(bsa_real_t), value :: rval
+      if (rval > 0.0_bsa_real_t) then
+         do_trunc_pod_ = .true.
+         pod_trunc_lim_ = rval / 100.0_bsa_real_t
+      end if
+   end subroutine
+
+   !> src/bsa/BsaLibImpl.F90:110-115: keep the nmodes largest POD modes.
+   subroutine bsa_setPODNOfModesKept(nmodes)
+      integer(bsa_int_t), value :: nmodes
+      npod_set_ = .true.
+      npod_ = nmodes
    end subroutine
 
    !> BSA_PREMESH_TYPE_DIAG_CREST_YES also resolves the f1+f2 = +-f_n crests.
@@ -680,6 +712,7 @@ contains
          real(bsa_real_t), allocatable, target, intent(inout) :: m3mf(:), m3mr(:)
          real(bsa_real_t), intent(in) :: tscl(:, :, :), umref(:)
          type(bsahip_mesher_cfg_t) :: cfg
+         type(bsahip_problem_t) :: pm
          type(c_ptr) :: msh
          real(bsa_real_t), allocatable, target :: fk(:), xk(:)
          real(bsa_real_t) :: v(3, 3), bw
@@ -713,7 +746,20 @@ contains
          cfg%max_area_ext = maxaext_
          cfg%natf = c_loc(fk)
          cfg%xsi = c_loc(xk)
-         rc2 = bsahip_mesher_create(p, cfg, 0_c_int32_t, msh)
+         ! POD BFM (isvd = 1): coherence column 1 for every component (the POD
+         ! path passes idir = 1, functionsImpl.F90:628-629, 770); truncation
+         ! settings validated as validateAll_ (BsaLibImpl.F90:479-486)
+         pm = p
+         if (isvd_ == 1) then
+            pm%corr_col_mode = 1
+            if (do_trunc_pod_ .and. (pod_trunc_lim_ == 0.0_bsa_real_t .or. &
+                                     pod_trunc_lim_ == 1.0_bsa_real_t)) do_trunc_pod_ = .false.
+            if (npod_set_ .and. (npod_ <= 0 .or. npod_ >= p%nnl)) npod_set_ = .false.
+            if (npod_set_) write(output_unit, '(/ 1x, 2a, i0, a /)') INFOMSG, 'Using  ', npod_, '  POD modes.'
+            if (do_trunc_pod_) pm%pod_trunc_lim = pod_trunc_lim_
+            if (npod_set_) pm%pod_nmodes = npod_
+         end if
+         rc2 = bsahip_mesher_create(pm, cfg, 0_c_int32_t, msh)
          if (rc2 /= 0) call bsa_Abort('bsahip_mesher_create failed: '//bsahip_last_error_f())
          allocate(m3mf(nmk**3), m3mr(nmk**3))
          rc2 = bsahip_mesher_run(msh, c_null_ptr, c_loc(m3mf), c_loc(m3mr), c_loc(counts))

@@ -69,6 +69,14 @@ typedef struct bsahip_problem {
    int32_t pulsation;     /* 1: PSD / 4 pi and d = 2 pi df (default convention) */
    int32_t ncorr_rows;    /* rows of nod_corr ((nn*nn+nn)/2)                  */
    int32_t corr_col_mode; /* 0: column tc per component, 1: column 1 for all   */
+   int32_t pod_nmodes;    /* point BFM (bsahip_bfm_points, Mesher): project     */
+                          /* through the pod_nmodes largest eigenpairs of S(f)  */
+                          /* (bsa_setPODNOfModesKept); 0: no truncation         */
+   int32_t reserved;      /* 0                                                  */
+   double  pod_trunc_lim; /* point BFM: keep the eigenpairs >= lim * largest    */
+                          /* (bsa_setPODTruncationThreshold / 100); 0: off; it  */
+                          /* takes precedence over pod_nmodes.  Either needs    */
+                          /* corr_col_mode 1 (the POD reads column 1).          */
    const int32_t *n_load;     /* [nnl] 1-based node ids          */
    const int32_t *tc;         /* [ntc] 1-based components        */
    const double  *psi;        /* (nm, nnl, ndegw)                */

@@ -367,6 +367,88 @@ def msh_points(prob, fi, fj) -> np.ndarray:
     return out
 
 
+# ---------------------------------------------------------------------------
+# Mesher per-point BFM through the POD of S(f) -- literal restatement
+# ---------------------------------------------------------------------------
+def pod_threshold_count(vals_desc, rlim: float) -> int:
+    """getNPODModesByThreshold_ (functionsImpl.F90:203-230) on values in
+    decreasing order, as its dgesvd build reads them (__IDX = 1, __OP = +):
+    1 when all values are equal, else the leading run of values >= rlim *
+    largest.  The loop is bounded by the number of values (the reference's is
+    not)."""
+    v = np.asarray(vals_desc, dtype=np.float64)
+    if np.all(v == v[0]):
+        return 1
+    limval = rlim * v[0]
+    k = 2                                   # 1-based, as the reference
+    while k <= v.size and v[k - 1] >= limval:
+        k += 1
+    return k - 1
+
+
+def nodal_psd_matrix(prob, f: float, tc: int) -> np.ndarray:
+    """getFullNodalPSD (WindPSDImpl.F90:76-108) with idir = 1, as the POD BFM
+    calls it (functionsImpl.F90:628-629, 770): PSDmat(j, i) = sqrt(S_j S_i) *
+    nod_corr(id(j, i), 1)**|f|."""
+    from bsalib_amd.synth import corr_index
+    nl = prob.n_load
+    idx = corr_index(nl[:, None], nl[None, :], prob.nn) - 1
+    S = eval_psd(prob, [f], int(tc))[0]
+    return np.sqrt(S[:, None] * S[None, :]) * prob.nod_corr[idx, 0] ** abs(f)
+
+
+def msh_points_pod(prob, fi, fj, nmodes: int = 0, lim: float = 0.0) -> np.ndarray:
+    """getFM_full_tm_scalar_msh_POD_ (functionsImpl.F90:474-1019; dsyev build,
+    no POD data caching) at each point: S(fi), S(fj), S(fi+fj) decomposed
+    (eigenvalues ascending, as dsyev), the kept pairs taken from the largest
+    down (:840-997), then blocks 5-2-2 (:858-899), 2-2-5 (:902-941) and 2-5-2
+    (:945-997).  Kept pairs (:803-823): lim > 0 -> pod_threshold_count per
+    matrix; else nmodes > 0 -> nmodes; else all.  Returns bfm(nm^3, npts)."""
+    modes = kept_modes(prob)
+    P = psi(prob, modes)
+    nm, n = len(modes), prob.nnl
+    fi = np.atleast_1d(np.asarray(fi, dtype=np.float64))
+    fj = np.atleast_1d(np.asarray(fj, dtype=np.float64))
+    out = np.zeros((nm ** 3, fi.size), order="F")
+
+    def decomp(f, t):
+        lam, V = np.linalg.eigh(nodal_psd_matrix(prob, f, t))
+        if lim > 0.0:
+            k = pod_threshold_count(lam[::-1], lim)
+        elif nmodes > 0:
+            k = min(nmodes, n)
+        else:
+            k = n
+        keep = np.arange(n - 1, n - 1 - k, -1)          # NNODESL down to NNODESL-(k-1)
+        return lam[keep], V[:, keep]
+
+    for q in range(fi.size):
+        B = np.zeros((nm, nm, nm))                     # B[m, n, o] = bfm(posm)
+        for t in prob.tc:
+            Pl, Pq = P[:, :, t - 1], P[:, :, t + 2]    # tc, tcP3 = tc + 3 (:561-562)
+            D1, V1 = decomp(fi[q], t)
+            D2, V2 = decomp(fj[q], t)
+            D3, V3 = decomp(fi[q] + fj[q], t)
+            L2, L3 = Pl @ V2, Pl @ V3                  # Z_lin_w2, Z_lin_w1w2
+            for p in range(D1.size):
+                vp = V1[:, p]
+                t1 = Pl @ vp                           # V_lin_w1
+                # 5-2-2: 2 VZ_quad_w1w2(m) V_lin_w1(n) Z_lin_w2(o) D_p T_q
+                Q = Pq @ (vp[:, None] * V2)
+                B += 2.0 * D1[p] * np.einsum("mq,n,oq->mno", Q * D2, t1, L2)
+                # 2-2-5: 2 Z_lin_w1w2(m) V_lin_w1(n) VZ_quad_w1w1w2(o) D_p T_q
+                Q = Pq @ (vp[:, None] * V3)
+                B += 2.0 * D1[p] * np.einsum("mq,n,oq->mno", L3 * D3, t1, Q)
+            for p in range(D3.size):
+                vp = V3[:, p]
+                t1 = Pl @ vp                           # V_lin_w1w2
+                # 2-5-2: 2 V_lin_w1w2(m) VZ_quad_w1w2w2(n) Z_lin_w2(o) D_p T_q
+                Q = Pq @ (vp[:, None] * V2)
+                B += 2.0 * D3[p] * np.einsum("m,nq,oq->mno", t1, Q * D2, L2)
+        out[:, q] = B.reshape(-1, order="F")
+    return out
+
+
 def bsacl_run(prob, fi, fj, psd_id=None, nthreads: int = 0) -> np.ndarray:
     """bsaclRun with bfm_kernel id 2 (BsaCL/src/bsacl.c:1307-1592,
     BsaCL/src/bsacl.cl:405-930) in FP64: response third moment over all
@@ -477,7 +559,7 @@ def reference_time(prob, threads: int) -> float:
     raise RuntimeError("reference timing line missing:\n" + r.stdout[-2000:])
 
 
-def mesher_integrate(prob, geom, levels, modes=None):
+def mesher_integrate(prob, geom, levels, modes=None, points=None):
     """Zone quadrature of the Mesher on given tiles (CPU restatement of
     bsahip_mesher_run): per tile, the force bispectrum at the level-l grid by
     getFM_full_tnm_scalar_msh_ (msh_points), bilinear interpolation onto the
@@ -486,7 +568,8 @@ def mesher_integrate(prob, geom, levels, modes=None):
     interior 1, times dw1 dw2).  geom rows: f1a f1b f2a f2b n1 n2 pol1 pol2
     sub1 sub2 (sub = BRM cells per level-0 cell).  Returns m3mf, m3mr and the
     BFM evaluation counts: level-0 points of all tiles counted once (they lie on
-    one tensor grid), refinement points per tile."""
+    one tensor grid), refinement points per tile.  points: the point BFM
+    (default msh_points; e.g. a msh_points_pod partial for the POD BFM)."""
     modes = kept_modes(prob) if modes is None else np.asarray(modes)
     nm = len(modes)
     nm3 = nm ** 3
@@ -503,7 +586,7 @@ def mesher_integrate(prob, geom, levels, modes=None):
         g2 = a2 + np.arange(nb2) * (b2 - a2) / (nb2 - 1)
         g1[-1], g2[-1] = b1, b2
         F1, F2 = np.meshgrid(g1, g2, indexing="ij")
-        B = msh_points(prob, F1.ravel(), F2.ravel()).reshape(nm3, nb1, nb2)
+        B = (points or msh_points)(prob, F1.ravel(), F2.ravel()).reshape(nm3, nb1, nb2)
         if lv > 1:
             ev1 += nb1 * nb2 - (n1 + 1) * (n2 + 1)
         s1, s2 = max(1, sub1 // lv), max(1, sub2 // lv)

@@ -3,11 +3,14 @@
 !! bsa_Run, bsa_Finalise), reading its inputs from a raw stream file.
 !!
 !! Usage:  bsalib_driver <input.bin> <output.bin> [fp64|bf16x3] [grid] [scalar] [mesher|both]
+!!                        [pod] [podn=<modes>] [podt=<percent>]
 !! The input layout is the one tests write for the reference harness (header
 !! of 14 int32, then integer and real64 arrays in Fortran order).  "grid"
 !! leaves test mode off and passes damping ratios, so bsa_Run applies the
 !! frequency-grid checks; "scalar" selects the scalar Classic version;
-!! "mesher" runs analysis type 2 (Mesher only), "both" type 3.
+!! "mesher" runs analysis type 2 (Mesher only), "both" type 3; "pod" sets
+!! isvd = 1 (POD BFM) in bsa_setupMesher, "podn=" / "podt=" call
+!! bsa_setPODNOfModesKept / bsa_setPODTruncationThreshold.
 !! Output: nme (int32), the used modes, then for each of m2mf, m2mr, m2o2mr,
 !! m3mf_cls, m3mr_cls, m3mf_msh, m3mr_msh: its size (int32, 0 when bsa_Run
 !! left it unallocated) and its values (real64).
@@ -27,12 +30,14 @@ program bsalib_driver
    real(real64), allocatable :: m3mf(:), m3mr(:)
    character(len=512) :: fin, fout, arg
    logical :: grid_checks
-   integer :: uin, uout, isuban
+   integer :: uin, uout, isuban, isvd, npod
+   real(real64) :: podt
 
    call get_command_argument(1, fin)
    call get_command_argument(2, fout)
    grid_checks = .false.
    isuban = 1
+   isvd = 0
 
    open(newunit=uin, file=trim(fin), access='stream', form='unformatted', status='old')
    read(uin) hdr
@@ -61,9 +66,18 @@ program bsalib_driver
       if (trim(arg) == 'scalar') call bsa_setClassicMode(BSA_CLASSIC_MODE_SCALAR)
       if (trim(arg) == 'mesher') isuban = 2
       if (trim(arg) == 'both') isuban = 3
+      if (trim(arg) == 'pod') isvd = 1
+      if (arg(1:5) == 'podn=') then
+         read(arg(6:), *) npod
+         call bsa_setPODNOfModesKept(npod)
+      end if
+      if (arg(1:5) == 'podt=') then
+         read(arg(6:), *) podt
+         call bsa_setPODTruncationThreshold(podt)
+      end if
    end do
    call bsa_setAnalysisType(isuban)
-   if (isuban /= 1) call bsa_setupMesher(0, 20, 2.0_real64, 2.0_real64, 2.0_real64, 1, 0)
+   if (isuban /= 1) call bsa_setupMesher(isvd, 20, 2.0_real64, 2.0_real64, 2.0_real64, 1, 0)
 
    call bsa_setTotalNOfNodes(nn)
    call bsa_setNumOfNodalDOFs(nlibs)

@@ -60,3 +60,46 @@ def test_no_cpu_fallback_without_gpu(libpath):
     from bsalib_amd import abi, synth
     with pytest.raises(abi.BsaHipError):
         abi.classic_run(synth.tower(nn=3, nm=2, nfreqs=5, df=0.2))
+
+
+def _c_offsets(struct: str, fields):
+    """offsetof of each field and sizeof, compiled from include/bsahip.h with gcc."""
+    import tempfile
+    body = "\n".join(f'   printf("%zu\\n", offsetof({struct}, {f}));' for f in fields)
+    src = (f'#include <stdio.h>\n#include <stddef.h>\n#include "bsahip.h"\n'
+           f'int main(void) {{\n{body}\n   printf("%zu\\n", sizeof({struct}));\n   return 0;\n}}\n')
+    with tempfile.TemporaryDirectory() as td:
+        c, exe = Path(td) / "probe.c", Path(td) / "probe"
+        c.write_text(src)
+        subprocess.run(["gcc", "-I", str(ROOT / "include"), str(c), "-o", str(exe)], check=True)
+        return [int(x) for x in subprocess.run([str(exe)], check=True, capture_output=True,
+                                               text=True).stdout.split()]
+
+
+@pytest.mark.parametrize("cname,pyname", [("bsahip_problem", "Problem_t"),
+                                          ("bsahip_mesher_cfg", "MesherCfg")])
+def test_ctypes_mirrors_match_the_c_layout(cname, pyname):
+    """The Python mirrors (and, field for field, the Fortran bind(c) types in
+    bsalib_amd/fortran/bsahip_c.F90) must have the header's offsets."""
+    from bsalib_amd import abi
+    T = getattr(abi, pyname)
+    names = [f[0] for f in T._fields_]
+    want = _c_offsets(cname, names)
+    got = [getattr(T, n).offset for n in names] + [C.sizeof(T)]
+    assert got == want, list(zip(names + ["sizeof"], got, want))
+
+
+def test_fortran_types_list_the_c_fields_in_order():
+    """bind(c) derived types are laid out by field order: the Fortran types must
+    name the same fields in the same order as the C structs."""
+    from bsalib_amd import abi
+    src = (ROOT / "bsalib_amd" / "fortran" / "bsahip_c.F90").read_text()
+    for ftype, T in (("bsahip_problem_t", abi.Problem_t), ("bsahip_mesher_cfg_t", abi.MesherCfg)):
+        block = re.search(rf"type, bind\(c\), public :: {ftype}\n(.*?)end type", src, re.S).group(1)
+        fields = []
+        for line in block.splitlines():
+            if "::" not in line:
+                continue
+            decl = line.split("::", 1)[1]
+            fields += [re.match(r"\s*(\w+)", part).group(1) for part in decl.split(",")]
+        assert [f.lower() for f in fields] == [f[0].lower() for f in T._fields_]

@@ -111,3 +111,33 @@ def test_fortran_front_mesher(gpu_lib, mode):
     else:
         g = load_golden("config1")
         assert rel_err(out["m3mr"], g["m3mr"]) < 1e-12
+
+
+@pytest.mark.parametrize("opts,pod", [
+    ((), dict()),
+    (("podn=2",), dict(pod_nmodes=2)),
+    (("podt=20",), dict(pod_trunc_lim=0.2)),
+    (("podn=5",), dict()),                  # >= loaded nodes: ignored (BsaLibImpl.F90:483)
+    (("podt=100",), dict()),                # 100 %: ignored (BsaLibImpl.F90:480)
+])
+def test_fortran_front_mesher_pod(gpu_lib, opts, pod):
+    """bsa_setupMesher(isvd = 1, ...): the POD BFM reads coherence column 1 for every
+    component (T4) and applies bsa_setPODNOfModesKept / bsa_setPODTruncationThreshold
+    as validateAll_ validates them; equal to the C ABI Mesher with the same
+    problem settings."""
+    import copy
+    from bsalib_amd import abi, host
+    p = copy.deepcopy(CASES["tiny_uvw"]())
+    c1 = p.nod_corr[:, 0].copy()
+    p.nod_corr = np.asfortranarray(np.stack([c1, c1 ** 2, c1 ** 3], axis=1))
+    out, r = fdriver.run(p, "mesher", "pod", *opts)
+    assert "Mesher on the GPU" in r.stdout
+    M = abi.Mesher(p, bkg_base_rfmnt=20, tile_cells=32, bkg_peakw=host.bkg_peak_width(p),
+                   bound=abi.Bound(p, corr_col_mode=1, **pod))
+    want = M.run()
+    M.close()
+    assert rel_err(out["m3mf_msh"], want["m3mf"]) < 1e-13
+    assert rel_err(out["m3mr_msh"], want["m3mr"]) < 1e-13
+    if not opts:
+        direct, _ = fdriver.run(p, "mesher")             # isvd = 0: column tc
+        assert rel_err(direct["m3mr_msh"], out["m3mr_msh"]) > 1e-5
