@@ -55,7 +55,7 @@ EXPORTS = ["bsahip_init", "bsahip_finalise", "bsahip_last_error", "bsahip_versio
            "bsahip_mesher_tiles", "bsahip_mesher_features", "bsahip_mesher_run",
            "bsahip_mesher_run_tiles", "bsahip_batch_create", "bsahip_batch_autotune",
            "bsahip_batch_run", "bsahip_batch_info", "bsahip_batch_destroy",
-           "bsahip_mesher_destroy"]
+           "bsahip_mesher_destroy", "bsahip_modal_psd_points", "bsahip_modal_recombination"]
 
 
 def lib():
@@ -190,6 +190,37 @@ def bfm_points(prob, fi, fj, with_brm: bool = True, bound: Bound = None):
                                    _ptr(fj, C.c_double), C.c_int32(0), _ptr(bfm, C.c_double),
                                    _ptr(brm, C.c_double)))
     return bfm, brm
+
+
+def modal_psd_points(prob, f, bound: Bound = None):
+    """Modal force PSD S_Q(n, p; f) at arbitrary frequencies
+    (bsahip_modal_psd_points): array (nm, nm, npts)."""
+    b = bound or Bound(prob)
+    f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+    out = np.zeros((b.nm * b.nm, f.size), order="F")
+    _check(lib().bsahip_modal_psd_points(C.byref(b.struct), C.c_int32(f.size), _ptr(f, C.c_double),
+                                         _ptr(out, C.c_double)))
+    return out.reshape(b.nm, b.nm, f.size, order="F")
+
+
+def modal_recombination(phi, m2, m3, m2o2=None, ccc: bool = True):
+    """Nodal moments from modal ones (bsahip_modal_recombination): phi (ndofs, nm),
+    m2/m2o2 (nm*nm), m3 (nm^3).  Returns dict m2, m3, m2o2 (or None), sk."""
+    phi = np.asfortranarray(phi, dtype=np.float64)
+    ndofs, nm = phi.shape
+    m2 = np.ascontiguousarray(m2, dtype=np.float64).reshape(-1)
+    m3 = np.ascontiguousarray(m3, dtype=np.float64).reshape(-1)
+    m22 = None if m2o2 is None else np.ascontiguousarray(m2o2, dtype=np.float64).reshape(-1)
+    if m2.size != nm * nm or m3.size != nm ** 3 or (m22 is not None and m22.size != nm * nm):
+        raise ValueError("m2/m2o2 need nm*nm and m3 nm^3 entries")
+    o = {k: np.zeros(ndofs) for k in ("m2", "m3", "sk")}
+    o["m2o2"] = np.zeros(ndofs) if m22 is not None else None
+    _check(lib().bsahip_modal_recombination(
+        C.c_int32(ndofs), C.c_int32(nm), _ptr(phi, C.c_double), _ptr(m2, C.c_double),
+        _ptr(m3, C.c_double), _ptr(m22, C.c_double), C.c_int32(1 if ccc else 0),
+        _ptr(o["m2"], C.c_double), _ptr(o["m3"], C.c_double), _ptr(o["m2o2"], C.c_double),
+        _ptr(o["sk"], C.c_double)))
+    return o
 
 
 def last_kernel_ms():

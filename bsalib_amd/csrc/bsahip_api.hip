@@ -877,5 +877,6 @@ int bsahip_selftest_mfma_bf16_layout(float *out768)
 }  // extern "C"
 
 #include "mesher.hip"
+#include "recomb.hip"
 #include "mlp.hip"
 #include "batch.hip"

@@ -35,7 +35,7 @@ module BsaHip_C
 
    public :: bsahip_init, bsahip_finalise, bsahip_classic_run, bsahip_bfm_points
    public :: bsahip_mesher_create, bsahip_mesher_run, bsahip_mesher_destroy
-   public :: bsahip_last_error_f
+   public :: bsahip_last_error_f, bsahip_modal_psd_points, bsahip_modal_recombination
 
    interface
       integer(c_int) function bsahip_init(device) bind(c, name="bsahip_init")
@@ -77,6 +77,21 @@ module BsaHip_C
             bind(c, name="bsahip_mesher_run")
          import c_int, c_ptr
          type(c_ptr), value :: mesher, levels, m3mf, m3mr, counts
+      end function
+
+      integer(c_int) function bsahip_modal_psd_points(p, npts, f, psd) &
+            bind(c, name="bsahip_modal_psd_points")
+         import c_int, c_int32_t, c_ptr, bsahip_problem_t
+         type(bsahip_problem_t), intent(in) :: p
+         integer(c_int32_t), value :: npts
+         type(c_ptr), value :: f, psd
+      end function
+
+      integer(c_int) function bsahip_modal_recombination(ndofs, nm, phi, m2, m3, m2o2, ccc, &
+            m2_r, m3_r, m2o2_r, sk_r) bind(c, name="bsahip_modal_recombination")
+         import c_int, c_int32_t, c_ptr
+         integer(c_int32_t), value :: ndofs, nm, ccc
+         type(c_ptr), value :: phi, m2, m3, m2o2, m2_r, m3_r, m2o2_r, sk_r
       end function
 
       integer(c_int) function bsahip_mesher_destroy(mesher) bind(c, name="bsahip_mesher_destroy")

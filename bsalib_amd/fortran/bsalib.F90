@@ -72,6 +72,7 @@ module BsaLib
    public :: bsa_getUsedModeShapes, bsa_setComputePrecision
    public :: bsa_setupMesher, bsa_setPremeshScheme, bsa_setMaxBkgPeakRestriction
    public :: bsa_setPODTruncationThreshold, bsa_setPODNOfModesKept
+   public :: bsa_computeBRdecomp, bsa_computePeakFactors, bsa_modalRecombination
    integer(bsa_int_t), parameter, public :: BSA_PREMESH_TYPE_DIAG_CREST_NO = 0
    integer(bsa_int_t), parameter, public :: BSA_PREMESH_TYPE_DIAG_CREST_YES = 1
 
@@ -103,6 +104,13 @@ module BsaLib
    real(bsa_real_t), allocatable, target :: u_node_(:), umref_(:), corr_(:, :), wfc_(:, :, :)
    real(bsa_real_t), allocatable, target :: psi_user_(:, :, :)
    type(c_ptr) :: tscl_addr_ = c_null_ptr, tstd_addr_ = c_null_ptr
+
+   ! ---- problem of the last bsa_Run, kept for the post-processing calls ----
+   type(bsahip_problem_t), save :: lp_
+   logical :: lp_ready_ = .false.
+   real(bsa_real_t), allocatable, target :: lp_psi_(:, :, :), lp_Mk_(:), lp_Kk_(:), lp_Ck_(:)
+   real(bsa_real_t), allocatable, target :: lp_tscl_(:, :, :), lp_tstd_(:, :), lp_umref_(:)
+   integer(bsa_int_t), allocatable, target :: lp_tcs_(:)
 
 contains
 
@@ -150,6 +158,8 @@ contains
       if (allocated(psi_user_)) deallocate(psi_user_)
       tscl_addr_ = c_null_ptr
       tstd_addr_ = c_null_ptr
+      if (allocated(lp_psi_)) deallocate(lp_psi_, lp_Mk_, lp_Kk_, lp_Ck_, lp_tscl_, lp_tstd_, lp_umref_, lp_tcs_)
+      lp_ready_ = .false.
       nn_ = 0; nlibs_ = 0; ndofs_ = 0; nm_ = 0; nz_lim_ = 0; ipsd_ = 0
       if (device_ready_) rc = bsahip_finalise()
       device_ready_ = .false.
@@ -647,6 +657,25 @@ contains
       p%Mm = c_loc(Mk)
       p%Km = c_loc(Kk)
       p%Cm = c_loc(Ck)
+      ! keep a copy for bsa_computeBRdecomp (module arrays outlive this call)
+      lp_psi_ = psi
+      lp_Mk_ = Mk
+      lp_Kk_ = Kk
+      lp_Ck_ = Ck
+      lp_tscl_ = tscl
+      lp_tstd_ = tstd
+      lp_umref_ = umref
+      lp_tcs_ = tcs
+      lp_ = p
+      lp_%psi = c_loc(lp_psi_)
+      lp_%tc = c_loc(lp_tcs_)
+      lp_%turb_scales = c_loc(lp_tscl_)
+      lp_%turb_std = c_loc(lp_tstd_)
+      lp_%u_mean_ref = c_loc(lp_umref_)
+      lp_%Mm = c_loc(lp_Mk_)
+      lp_%Km = c_loc(lp_Kk_)
+      lp_%Cm = c_loc(lp_Ck_)
+      lp_ready_ = .true.
 
       if (.not. device_ready_) then
          rc = bsahip_init(0_c_int)
@@ -776,5 +805,125 @@ contains
          allocate(a(n), source=0.0_bsa_real_t)
       end subroutine
    end subroutine bsa_Run
+
+   ! ======================================================================
+   !  post-processing
+   ! ======================================================================
+
+   !> src/bsa/BsaLibImpl.F90:1292-1421: background m2mf(m,m)/K_m^2 and resonant
+   !> pi * 2 pi * f_m * S_Fm(f_m) / (2 xsi_m K_m^2) parts of each kept mode's
+   !> variance, with S_Fm the modal force PSD at the natural frequency
+   !> (getBR_SFm_val_, functionsImpl.F90:2650-2743), evaluated on the GPU.
+   !> Needs a previous bsa_Run (its problem) and the damping ratios.
+   subroutine bsa_computeBRdecomp(m2mf, bkg, res)
+      real(bsa_real_t), intent(in) :: m2mf(:)
+      real(bsa_real_t), allocatable, intent(out) :: bkg(:), res(:)
+      real(bsa_real_t), allocatable, target :: f(:), psd(:)
+      real(bsa_real_t), parameter :: pi_ = acos(-1.0_bsa_real_t)
+      real(bsa_real_t) :: k2, sfm
+      integer(bsa_int_t) :: nm, im, m
+      integer(c_int) :: rc
+      if (.not. lp_ready_) call bsa_Abort('bsa_computeBRdecomp needs a previous bsa_Run.')
+      if (.not. allocated(xsi_)) call bsa_Abort('BR decomposition needs the modal damping ratios.')
+      nm = size(modes_)
+      if (size(m2mf) < nm * nm) call bsa_Abort('m2mf has fewer than NM*NM entries.')
+      allocate(bkg(nm), res(nm))
+      f = natf_(modes_)
+      allocate(psd(nm * nm * nm))
+      rc = bsahip_modal_psd_points(lp_, int(nm, c_int32_t), c_loc(f), c_loc(psd))
+      if (rc /= 0) call bsa_Abort('bsahip_modal_psd_points failed: '//bsahip_last_error_f())
+      do im = 1, nm
+         m = modes_(im)
+         k2 = Kg_(m) * Kg_(m)
+         bkg(im) = m2mf((im - 1) * nm + im) / k2
+         sfm = psd((im - 1) * nm * nm + (im - 1) * nm + im)
+         res(im) = pi_ * 2 * pi_ * f(im) * sfm / (2 * xsi_(m) * k2)
+      end do
+   end subroutine
+
+   !> src/bsa/BsaLibImpl.F90:1429-1519: Gaussian peak factor (Davenport) and,
+   !> given a skewness, the non-Gaussian ones of the Hermite model (Kwon and
+   !> Kareem), with the reference's early returns and warnings.
+   subroutine bsa_computePeakFactors(m2, m2o2, obs_time, peak_g, sk, peak_ng_pos, peak_ng_neg)
+      real(bsa_real_t), intent(in) :: m2(:), m2o2(:)
+      real(bsa_real_t), intent(in) :: obs_time
+      real(bsa_real_t), allocatable, intent(inout) :: peak_g(:)
+      real(bsa_real_t), intent(in), allocatable :: sk(:)
+      real(bsa_real_t), allocatable, intent(inout) :: peak_ng_pos(:)
+      real(bsa_real_t), allocatable, intent(inout), optional :: peak_ng_neg(:)
+      real(bsa_real_t), parameter :: gamma_ = 0.5772d0
+      real(bsa_real_t), parameter :: pi_ = acos(-1.0_bsa_real_t)
+      real(bsa_real_t), allocatable :: beta(:), g4(:), h40(:), h3(:), h4(:), k(:)
+      real(bsa_real_t), allocatable :: b2(:), b3(:), r(:), neg(:)
+      if (all(m2o2 == 0)) then
+         write(output_unit, '(/ 1x, 2a)') WARNMSG, '"m2_ord2"  is null. Cannot compute extremes. Skipping.'
+         return
+      end if
+      beta = sqrt(m2o2 / m2) / (2 * pi_) * obs_time
+      if (any(beta < 1.d0)) then
+         write(output_unit, '(1x, a, f7.2, a)') WARNMSG, 'Observation time of ', obs_time, ' sec.  is too short.'
+         return
+      end if
+      beta = sqrt(2 * log(beta))
+      peak_g = gamma_ / beta + beta
+      if (.not. allocated(sk)) return
+      g4 = 1.5d0 * sk * sk
+      h40 = sqrt(1.d0 + 1.5d0 * g4)
+      h3 = sk / (4.d0 + 2.d0 * h40)
+      h4 = (h40 - 1.d0) / 18.0d0
+      k = 1.d0 / sqrt(1 + 2.d0 * h3 * h3 + 6.d0 * h4 * h4)
+      b2 = beta * beta
+      b3 = b2 * beta
+      peak_ng_pos = 5.44d0 / b3
+      peak_ng_pos = peak_ng_pos + 3.d0 / beta * (pi_ * pi_ / 6.d0 - gamma_ + gamma_**2)
+      peak_ng_pos = peak_ng_pos + (b3 + 3 * beta * (gamma_ - 1))
+      peak_ng_pos = peak_ng_pos * h4
+      neg = peak_ng_pos
+      r = h3 * (b2 + 2.d0 * gamma_ - 1.d0 + 1.98d0 / b2)
+      peak_ng_pos = (peak_ng_pos + r + peak_g) * k
+      if (present(peak_ng_neg)) peak_ng_neg = (neg - r + peak_g) * k
+   end subroutine
+
+   !> Addition (the reference does this in its application, app/bsa.F90:935-1053):
+   !> nodal second and third moments and skewness from the modal ones, by the
+   !> unilateral combination (diagonal modal terms) and the complete cubic
+   !> combination, on the GPU.  phi(ndofs, nm) holds the kept modes' shapes at
+   !> the DOFs of interest; m2o2mr may be unallocated.
+   subroutine bsa_modalRecombination(phi, m2mr, m3mr, m2o2mr, m2_diag, m3_diag, m2o2_diag, &
+                                     sk_diag, m2_full, m3_full, m2o2_full, sk_full)
+      real(bsa_real_t), intent(in) :: phi(:, :), m2mr(:), m3mr(:)
+      real(bsa_real_t), intent(in), allocatable :: m2o2mr(:)
+      real(bsa_real_t), allocatable, target, intent(out) :: m2_diag(:), m3_diag(:), m2o2_diag(:), &
+         sk_diag(:), m2_full(:), m3_full(:), m2o2_full(:), sk_full(:)
+      real(bsa_real_t), allocatable, target :: ph(:, :), c2(:), c3(:), c22(:)
+      integer(bsa_int_t) :: nd, nm
+      type(c_ptr) :: p22, o22
+      integer(c_int) :: rc
+      nd = size(phi, 1)
+      nm = size(phi, 2)
+      if (size(m2mr) /= nm * nm .or. size(m3mr) /= nm**3) &
+         call bsa_Abort('Modal recombination needs NM*NM second and NM^3 third moments.')
+      ph = phi                   ! contiguous copies for the C ABI
+      c2 = m2mr
+      c3 = m3mr
+      allocate(m2_diag(nd), m3_diag(nd), m2o2_diag(nd), sk_diag(nd))
+      allocate(m2_full(nd), m3_full(nd), m2o2_full(nd), sk_full(nd))
+      m2o2_diag = 0
+      m2o2_full = 0
+      p22 = c_null_ptr
+      if (allocated(m2o2mr)) then
+         c22 = m2o2mr
+         p22 = c_loc(c22)
+      end if
+      o22 = c_null_ptr
+      if (allocated(m2o2mr)) o22 = c_loc(m2o2_diag)
+      rc = bsahip_modal_recombination(int(nd, c_int32_t), int(nm, c_int32_t), c_loc(ph), c_loc(c2), &
+         c_loc(c3), p22, 0_c_int32_t, c_loc(m2_diag), c_loc(m3_diag), o22, c_loc(sk_diag))
+      if (rc /= 0) call bsa_Abort('bsahip_modal_recombination failed: '//bsahip_last_error_f())
+      if (allocated(m2o2mr)) o22 = c_loc(m2o2_full)
+      rc = bsahip_modal_recombination(int(nd, c_int32_t), int(nm, c_int32_t), c_loc(ph), c_loc(c2), &
+         c_loc(c3), p22, 1_c_int32_t, c_loc(m2_full), c_loc(m3_full), o22, c_loc(sk_full))
+      if (rc /= 0) call bsa_Abort('bsahip_modal_recombination failed: '//bsahip_last_error_f())
+   end subroutine
 
 end module BsaLib

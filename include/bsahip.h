@@ -145,6 +145,27 @@ int bsahip_bfm_points(const bsahip_problem *p, int32_t npts, const double *fi,
                       const double *fj, int32_t flags, double *bfm, double *brm);
 
 /*
+ * Modal force PSD S_Q(n, p; f) at arbitrary frequencies: psd (NM*NM, npts),
+ * index n + p*NM per point, under the problem's PSD convention and coherence
+ * columns.  Its diagonal at a natural frequency is getBR_SFm_val_
+ * (functionsImpl.F90:2650-2743), the input of bsa_computeBRdecomp
+ * (src/bsa/BsaLibImpl.F90:1292-1421).
+ */
+int bsahip_modal_psd_points(const bsahip_problem *p, int32_t npts, const double *f, double *psd);
+
+/*
+ * Nodal second/third moments from modal ones (modalRecombination,
+ * app/bsa.F90:935-1053): phi (ndofs, nm) column-major, m2 and m2o2 (nm*nm),
+ * m3 (nm^3) in the layouts of bsahip_classic_run.  ccc = 0: unilateral
+ * combination (diagonal modal terms only); 1: complete cubic combination.
+ * Outputs (ndofs each): m2_r, m3_r, m2o2_r (NULL, or m2o2 given), sk_r =
+ * m3_r / m2_r^(3/2) (may be NULL).
+ */
+int bsahip_modal_recombination(int32_t ndofs, int32_t nm, const double *phi, const double *m2,
+                               const double *m3, const double *m2o2, int32_t ccc, double *m2_r,
+                               double *m3_r, double *m2o2_r, double *sk_r);
+
+/*
  * Health check of the matrix-core lane layout the kernels rely on
  * (v_mfma_f64_16x16x4f64): out512[t*64+l] / out512[256+t*64+l] receive the row /
  * column index that accumulator register t of lane l holds.  No reference

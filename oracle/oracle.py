@@ -622,3 +622,111 @@ def mesher_integrate(prob, geom, levels, modes=None, points=None):
         nodes.update(np.round(t[0] + np.arange(n + 1) * (t[1] - t[0]) / n, 12).tolist())
     ev0 = len(nodes) ** 2
     return {"m3mf": m3f, "m3mr": m3r, "evals_pre": ev0, "evals_post": ev1}
+
+
+# ---------------------------------------------------------------------------
+# post-processing: BR decomposition, modal recombination, peak factors
+# ---------------------------------------------------------------------------
+def br_sfm(prob, fnat: float, m: int) -> float:
+    """getBR_SFm_val_ (functionsImpl.F90:2650-2743): modal force PSD of mode m
+    (1-based, full list) at fnat, summed over components and loaded node pairs
+    with their libs (coherence column tc, PSDs by evalPSD at |fnat|)."""
+    from bsalib_amd.synth import corr_index
+    nl = prob.n_load
+    fi = abs(fnat)
+    idx = corr_index(nl[:, None], nl[None, :], prob.nn) - 1      # [j, k]
+    psd = 0.0
+    for t in prob.tc:
+        S = eval_psd(prob, [fnat], int(t))[0]                   # (nnl,)
+        rho = prob.nod_corr[idx, t - 1]
+        SJK = rho ** fi * np.sqrt(S[:, None] * S[None, :])      # [j, k]
+        rows = (nl - 1)[:, None] * prob.nlibs + prob.libs_load[None, :] - 1   # [node, lib]
+        ph = prob.phi[rows, m - 1]                              # phi_(pos_n + libs, m)
+        a = prob.wfc[:, t - 1, :].T                             # wfc(libs, tc, node) -> [node, lib]
+        # sum(matmul(phij, phik) * matmul(akU, ajU) * S_JK): element (a, b) pairs
+        # phij(a) with akU(a) and phik(b) with ajU(b) (:2717-2727)
+        pj_ak = ph[:, None, :] * a[None, :, :]                  # [j, k, lib]: phij(a) * akU(a)
+        pk_aj = ph[None, :, :] * a[:, None, :]                  # [j, k, lib]: phik(b) * ajU(b)
+        psd += float((pj_ak.sum(-1) * pk_aj.sum(-1) * SJK).sum())
+    return psd
+
+
+def br_decomp(prob, m2mf, modes=None):
+    """bsa_computeBRdecomp (BsaLibImpl.F90:1292-1421): background m2mf(m,m)/Km^2
+    and resonant pi * 2pi * f_n * S_Fm(f_n) / (2 xsi Km^2) per kept mode."""
+    modes = kept_modes(prob) if modes is None else np.asarray(modes)
+    nm = len(modes)
+    xsi = np.asarray(prob.meta["xi"], dtype=np.float64)
+    bkg, res = np.zeros(nm), np.zeros(nm)
+    for im, m in enumerate(modes):
+        k2 = prob.Km[m - 1] ** 2
+        bkg[im] = m2mf[im * nm + im] / k2
+        f = prob.natf[m - 1]
+        res[im] = np.pi * TWO_PI * f * br_sfm(prob, f, int(m)) / (2 * xsi[m - 1] * k2)
+    return bkg, res
+
+
+def modal_recombination(phi, m2mr, m3mr, m2o2mr=None) -> dict:
+    """modalRecombination (app/bsa.F90:935-1053): unilateral (diag) and complete
+    cubic combination (full) nodal m2, m3, m2o2 and sk = m3 / m2^(3/2)."""
+    phi = np.asarray(phi, dtype=np.float64)
+    ndofs, nm = phi.shape
+    m2 = np.asarray(m2mr, dtype=np.float64).reshape(nm, nm, order="F")
+    m3 = np.asarray(m3mr, dtype=np.float64).reshape(nm, nm, nm, order="F")
+    m22 = None if m2o2mr is None else np.asarray(m2o2mr, dtype=np.float64).reshape(nm, nm, order="F")
+    o = {k: np.zeros(ndofs) for k in ("m2_diag", "m3_diag", "m2o2_diag",
+                                      "m2_full", "m3_full", "m2o2_full")}
+    for im in range(nm):                                    # :959-984
+        r = phi[:, im] * phi[:, im]
+        o["m2_diag"] += m2[im, im] * r
+        if m22 is not None:
+            o["m2o2_diag"] += m22[im, im] * r
+        r = r * phi[:, im]
+        o["m3_diag"] += m3[im, im, im] * r
+    for k in range(nm):                                     # :1003-1043
+        p3 = phi[:, k]
+        for j in range(nm):
+            p2 = phi[:, j]
+            o["m2_full"] += m2[j, k] * p2 * p3
+            if m22 is not None:
+                o["m2o2_full"] += m22[j, k] * p2 * p3
+            for i in range(nm):
+                o["m3_full"] += m3[i, j, k] * phi[:, i] * p2 * p3
+    o["sk_diag"] = o["m3_diag"] / o["m2_diag"] ** 1.5
+    o["sk_full"] = o["m3_full"] / o["m2_full"] ** 1.5
+    return o
+
+
+def peak_factors(m2, m2o2, obs_time: float, sk=None):
+    """bsa_computePeakFactors (BsaLibImpl.F90:1429-1519): Gaussian peak factor
+    beta + gamma/beta with beta = sqrt(2 ln(sqrt(m2o2/m2) T / 2pi)), and with a
+    skewness the Hermite-model (Kwon & Kareem) non-Gaussian peak factors.
+    Returns (peak_g, peak_ng_pos, peak_ng_neg), or None where the reference
+    returns early (m2o2 all zero, or some beta < 1)."""
+    gamma = 0.5772
+    m2 = np.asarray(m2, dtype=np.float64)
+    m2o2 = np.asarray(m2o2, dtype=np.float64)
+    if np.all(m2o2 == 0):
+        return None
+    beta = np.sqrt(m2o2 / m2) / TWO_PI * obs_time
+    if np.any(beta < 1.0):
+        return None
+    beta = np.sqrt(2 * np.log(beta))
+    peak_g = gamma / beta + beta
+    if sk is None:
+        return peak_g, None, None
+    sk = np.asarray(sk, dtype=np.float64)
+    g4 = 1.5 * sk * sk
+    h40 = np.sqrt(1.0 + 1.5 * g4)
+    h3 = sk / (4.0 + 2.0 * h40)
+    h4 = (h40 - 1.0) / 18.0
+    k = 1.0 / np.sqrt(1 + 2.0 * h3 * h3 + 6.0 * h4 * h4)
+    b2, b3 = beta * beta, beta * beta * beta
+    pos = 5.44 / b3
+    pos = pos + 3.0 / beta * (np.pi ** 2 / 6.0 - gamma + gamma ** 2)
+    pos = pos + (b3 + 3 * beta * (gamma - 1))
+    pos = pos * h4
+    neg = pos.copy()
+    r = h3 * (b2 + 2.0 * gamma - 1.0 + 1.98 / b2)
+    pos, neg = pos + r, neg - r
+    return peak_g, (pos + peak_g) * k, (neg + peak_g) * k

@@ -37,7 +37,12 @@ def run(prob, *opts, check=True, timeout=300):
     modes = np.frombuffer(raw[4:4 + 4 * nme], dtype=np.int32).copy()
     o = 4 + 4 * nme
     out = {"modes": modes}
-    for k in ("m2mf", "m2mr", "m2o2mr", "m3mf", "m3mr", "m3mf_msh", "m3mr_msh"):
+    keys = ["m2mf", "m2mr", "m2o2mr", "m3mf", "m3mr", "m3mf_msh", "m3mr_msh"]
+    if "post" in opts:
+        keys += ["bkg", "res", "m2_diag", "m3_diag", "m2o2_diag", "sk_diag", "m2_full",
+                 "m3_full", "m2o2_full", "sk_full", "peak_g_diag", "peak_pos_diag",
+                 "peak_neg_diag", "peak_g_full", "peak_pos_full", "peak_neg_full"]
+    for k in keys:
         n = int(np.frombuffer(raw[o:o + 4], dtype=np.int32)[0])
         o += 4
         out[k] = np.frombuffer(raw[o:o + 8 * n], dtype=np.float64).copy() if n else None

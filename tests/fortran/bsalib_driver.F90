@@ -3,14 +3,18 @@
 !! bsa_Run, bsa_Finalise), reading its inputs from a raw stream file.
 !!
 !! Usage:  bsalib_driver <input.bin> <output.bin> [fp64|bf16x3] [grid] [scalar] [mesher|both]
-!!                        [pod] [podn=<modes>] [podt=<percent>]
+!!                        [pod] [podn=<modes>] [podt=<percent>] [post]
 !! The input layout is the one tests write for the reference harness (header
 !! of 14 int32, then integer and real64 arrays in Fortran order).  "grid"
 !! leaves test mode off and passes damping ratios, so bsa_Run applies the
 !! frequency-grid checks; "scalar" selects the scalar Classic version;
 !! "mesher" runs analysis type 2 (Mesher only), "both" type 3; "pod" sets
 !! isvd = 1 (POD BFM) in bsa_setupMesher, "podn=" / "podt=" call
-!! bsa_setPODNOfModesKept / bsa_setPODTruncationThreshold.
+!! bsa_setPODNOfModesKept / bsa_setPODTruncationThreshold; "post" runs the
+!! post-processing after bsa_Run (bsa_computeBRdecomp, bsa_modalRecombination
+!! on the used modes at every DOF, bsa_computePeakFactors over 600 s) and
+!! appends bkg, res, the unilateral and complete m2, m3, m2o2, sk and the
+!! Gaussian / positive / negative peak factors of both.
 !! Output: nme (int32), the used modes, then for each of m2mf, m2mr, m2o2mr,
 !! m3mf_cls, m3mr_cls, m3mf_msh, m3mr_msh: its size (int32, 0 when bsa_Run
 !! left it unallocated) and its values (real64).
@@ -28,6 +32,9 @@ program bsalib_driver
    real(real64), allocatable, target :: wfc(:, :, :), psi(:, :, :), xsi(:)
    real(real64), allocatable :: m2mf(:), m2mr(:), m2o2mr(:), m3mf_msh(:), m3mr_msh(:)
    real(real64), allocatable :: m3mf(:), m3mr(:)
+   real(real64), allocatable :: bkg(:), res(:), m2d(:), m3d(:), m22d(:), skd(:), m2c(:), m3c(:)
+   real(real64), allocatable :: m22c(:), skc(:), pgd(:), ppd(:), pnd(:), pgc(:), ppc(:), pnc(:)
+   logical :: post
    character(len=512) :: fin, fout, arg
    logical :: grid_checks
    integer :: uin, uout, isuban, isvd, npod
@@ -36,6 +43,7 @@ program bsalib_driver
    call get_command_argument(1, fin)
    call get_command_argument(2, fout)
    grid_checks = .false.
+   post = .false.
    isuban = 1
    isvd = 0
 
@@ -67,6 +75,7 @@ program bsalib_driver
       if (trim(arg) == 'mesher') isuban = 2
       if (trim(arg) == 'both') isuban = 3
       if (trim(arg) == 'pod') isvd = 1
+      if (trim(arg) == 'post') post = .true.
       if (arg(1:5) == 'podn=') then
          read(arg(6:), *) npod
          call bsa_setPODNOfModesKept(npod)
@@ -106,6 +115,13 @@ program bsalib_driver
 
    call bsa_Run(m2mf, m2mr, m2o2mr, m3mf_msh, m3mr_msh, m3mf, m3mr)
    used = bsa_getUsedModeShapes()
+   if (post) then
+      call bsa_computeBRdecomp(m2mf, bkg, res)
+      call bsa_modalRecombination(phi(:, used), m2mr, m3mr, m2o2mr, m2d, m3d, m22d, skd, &
+                                  m2c, m3c, m22c, skc)
+      call bsa_computePeakFactors(m2d, m22d, 600.0_real64, pgd, skd, ppd, pnd)
+      call bsa_computePeakFactors(m2c, m22c, 600.0_real64, pgc, skc, ppc, pnc)
+   end if
 
    open(newunit=uout, file=trim(fout), access='stream', form='unformatted', status='replace')
    write(uout) int(size(used), int32), used
@@ -116,6 +132,24 @@ program bsalib_driver
    call put(m3mr)
    call put(m3mf_msh)
    call put(m3mr_msh)
+   if (post) then
+      call put(bkg)
+      call put(res)
+      call put(m2d)
+      call put(m3d)
+      call put(m22d)
+      call put(skd)
+      call put(m2c)
+      call put(m3c)
+      call put(m22c)
+      call put(skc)
+      call put(pgd)
+      call put(ppd)
+      call put(pnd)
+      call put(pgc)
+      call put(ppc)
+      call put(pnc)
+   end if
    close(uout)
    call bsa_Finalise()
    if (.not. bsa_isCleaned()) error stop 'bsa_Finalise did not clean up'
