@@ -40,6 +40,8 @@ module BsaLib
 
    integer, parameter, public :: bsa_int_t = int32
    integer, parameter, public :: bsa_real_t = real64
+   ! re-exported, as the reference module (which is public) re-exports them
+   public :: int32, real64
 
    integer(bsa_int_t), parameter, public :: BSA_SPATIAL_SYM_NONE = 0
    integer(bsa_int_t), parameter, public :: BSA_SPATIAL_SYM_HALF = 2
@@ -54,9 +56,28 @@ module BsaLib
    integer(bsa_int_t), parameter, public :: BSA_PREC_FP64 = 0
    integer(bsa_int_t), parameter, public :: BSA_PREC_BF16X3 = 2
 
-   character(len=*), parameter :: INFOMSG = '  --[info]   '
-   character(len=*), parameter :: WARNMSG = '  --[warn]   '
-   character(len=*), parameter :: ERRMSG = '  --[error]  '
+   integer(bsa_int_t), parameter, public :: BSA_EXPORT_FORMAT_FORMATTED = 0
+   integer(bsa_int_t), parameter, public :: BSA_EXPORT_FORMAT_UNFORMATTED = 1
+   integer(bsa_int_t), parameter, public :: BSA_EXPORT_MODE_REPLACE = 0
+   integer(bsa_int_t), parameter, public :: BSA_EXPORT_MODE_APPEND = 1
+   integer(bsa_int_t), parameter, public :: BSA_EXPORT_BRM_MODE_NONE = 0
+   integer(bsa_int_t), parameter, public :: BSA_EXPORT_BRM_MODE_BASE = 1
+   integer(bsa_int_t), parameter, public :: BSA_EXPORT_BRM_MODE_USR = 9
+   integer(bsa_int_t), parameter, public :: BSA_WIND_VERT_PROFILE_POWER = 1
+   integer(bsa_int_t), parameter, public :: BSA_WIND_VERT_PROFILE_LOG = 2
+   ! input file names and the constants programs written against the reference use
+   character(len=*), parameter, public :: BSA_DATA_FNAME = 'bsa.bsadata'
+   character(len=*), parameter, public :: EXT_DATA_FNAME = 'bsa.extdata'
+   character(len=0), parameter, public :: BSA_FILE_NAME_CL_SUFFIX = ''
+   character(len=*), parameter, public :: IO_ACTION_READ = 'READ'
+   character(len=*), parameter, public :: IO_ACTION_WRITE = 'WRITE'
+   real(bsa_real_t), parameter, public :: CST_3d2 = 3.0_bsa_real_t / 2.0_bsa_real_t
+
+   character(len=*), parameter, public :: INFOMSG = '  --[info]   '
+   character(len=*), parameter, public :: WARNMSG = '  --[warn]   '
+   character(len=*), parameter, public :: ERRMSG = '  --[error]  '
+   character(len=*), parameter, public :: MSGCONT = '             '
+   character(len=*), parameter, public :: DBGMSG = '  --[debug]  '
 
    public :: bsa_Init, bsa_Run, bsa_Finalise, bsa_isCleaned, bsa_Abort
    public :: bsa_enableGPU, bsa_setAnalysisType, bsa_setClassicMode, bsa_setScalingConv
@@ -73,6 +94,25 @@ module BsaLib
    public :: bsa_setupMesher, bsa_setPremeshScheme, bsa_setMaxBkgPeakRestriction
    public :: bsa_setPODTruncationThreshold, bsa_setPODNOfModesKept
    public :: bsa_computeBRdecomp, bsa_computePeakFactors, bsa_modalRecombination
+   ! settings that do not change the device path, kept for the input files
+   public :: bsa_setVersion, bsa_setSpectraSymmetries, bsa_setNodalCoords, bsa_setWindVertProf
+   public :: bsa_setWindAltDir, bsa_setAirDensity, bsa_setGlobalRotMatW2G, bsa_setWZRefAlt
+   public :: bsa_setWindCorrCoeffs, bsa_setWindCorrExpnts, bsa_setWZRotMatW2G
+   public :: bsa_setIncidenceAngles, bsa_setNodalWindAltitudes, bsa_setBfmMLR
+   public :: bsa_doValidateZoneDeltas, bsa_forceBsaClsExecution, bsa_generateBSAInputFiles
+   ! exports and run-time I/O settings
+   public :: bsa_printBSAHeader, bsa_setExportInCurrDir, bsa_setExportDirectory
+   public :: bsa_closeUnitsAtEnd, bsa_setExportFileFormat, bsa_setExportAppendMode
+   public :: bsa_setOutFileName, bsa_setBRMExportDefaultMode, bsa_enableVisualMode
+   public :: bsa_setVisualIndexes, bsa_exportMomentToFile, bsa_exportSkewness
+   public :: bsa_exportBRdecomp, bsa_exportExtremeValuesToFile, bsa_saveCoordinatesToFile
+
+   interface bsa_exportSkewness
+      module procedure exportSkewnessCompute_, exportSkewnessGiven_
+   end interface
+   interface bsa_exportBRdecomp
+      module procedure exportBR_
+   end interface
    integer(bsa_int_t), parameter, public :: BSA_PREMESH_TYPE_DIAG_CREST_NO = 0
    integer(bsa_int_t), parameter, public :: BSA_PREMESH_TYPE_DIAG_CREST_YES = 1
 
@@ -104,6 +144,20 @@ module BsaLib
    real(bsa_real_t), allocatable, target :: u_node_(:), umref_(:), corr_(:, :), wfc_(:, :, :)
    real(bsa_real_t), allocatable, target :: psi_user_(:, :, :)
    type(c_ptr) :: tscl_addr_ = c_null_ptr, tstd_addr_ = c_null_ptr
+
+   ! ---- inputs the device path does not read, kept for bsa_generateBSAInputFiles
+   !      and the coordinates export (defaults as src/settings/Settings.F90) ----
+   integer(bsa_int_t) :: ivers_ = 2, ispctrsym_ = 0, iwprof_ = BSA_WIND_VERT_PROFILE_POWER
+   integer(bsa_int_t) :: ivert_ = 3, ifcov_ = 1, idumpmod_ = 0
+   real(bsa_real_t) :: aird_ = 1.25_bsa_real_t
+   real(bsa_real_t) :: rotW2G_(3, 3) = reshape([1, 0, 0, 0, 1, 0, 0, 0, 1], [3, 3]) * 1.0_bsa_real_t
+   real(bsa_real_t), allocatable :: coords_(:, :), zref_(:), incang_(:), walt_(:), wz_lims_(:)
+   type(c_ptr) :: ccoef_addr_ = c_null_ptr, cexp_addr_ = c_null_ptr, rotlw_addr_ = c_null_ptr
+   logical :: gen_inputs_ = .false., run_after_gen_ = .true.
+
+   ! ---- exports ----
+   character(len=:), allocatable :: exp_dir_
+   logical :: exp_binary_ = .false., exp_append_ = .false., visual_ = .false.
 
    ! ---- problem of the last bsa_Run, kept for the post-processing calls ----
    type(bsahip_problem_t), save :: lp_
@@ -159,6 +213,16 @@ contains
       tscl_addr_ = c_null_ptr
       tstd_addr_ = c_null_ptr
       if (allocated(lp_psi_)) deallocate(lp_psi_, lp_Mk_, lp_Kk_, lp_Ck_, lp_tscl_, lp_tstd_, lp_umref_, lp_tcs_)
+      if (allocated(coords_)) deallocate(coords_)
+      if (allocated(zref_)) deallocate(zref_)
+      if (allocated(incang_)) deallocate(incang_)
+      if (allocated(walt_)) deallocate(walt_)
+      if (allocated(wz_lims_)) deallocate(wz_lims_)
+      ccoef_addr_ = c_null_ptr
+      cexp_addr_ = c_null_ptr
+      rotlw_addr_ = c_null_ptr
+      gen_inputs_ = .false.
+      run_after_gen_ = .true.
       lp_ready_ = .false.
       nn_ = 0; nlibs_ = 0; ndofs_ = 0; nm_ = 0; nz_lim_ = 0; ipsd_ = 0
       if (device_ready_) rc = bsahip_finalise()
@@ -323,6 +387,7 @@ contains
       if (nz_lim_ /= 0 .and. nz_lim_ /= size(lim) - 1) &
          call bsa_Abort('N. of wind zones differs from the one previously set.')
       nz_lim_ = size(lim) - 1
+      wz_lims_ = lim
    end subroutine
 
    subroutine bsa_setWZMeanWindVel(mat)
@@ -567,10 +632,17 @@ contains
       integer(c_int) :: rc
       type(c_ptr) :: o_m2mf, o_m2mr, o_m2o2mr, o_m3mf, o_m3mr
 
+      ! input files of the standalone program (src/bsa/BsaLibImpl.F90:156-161)
+      if (gen_inputs_) then
+         write(output_unit, '(/ 1x, 2a)') INFOMSG, 'Generating BSA-compatible input files.'
+         call generateInputFiles_()
+      end if
+      if (.not. run_after_gen_) return
       if (icomp_psd_ == 0 .and. icomp_bisp_ == 0) then
          write(output_unit, '(1x, 2a)') WARNMSG, 'Both  PSD  and  BISP  computation are disabled.'
          return
       end if
+      if (visual_) call bsa_Abort('Visual mode (bispectra re-read from a dump file) is not available in this library.')
       if (gpu_enabled_) isuban_ = 1
       if (isuban_ < 1 .or. isuban_ > 3) call bsa_Abort('Analysis type must be 1, 2 or 3.')
       do_cls = isuban_ == 1 .or. isuban_ == 3
@@ -924,6 +996,460 @@ contains
       rc = bsahip_modal_recombination(int(nd, c_int32_t), int(nm, c_int32_t), c_loc(ph), c_loc(c2), &
          c_loc(c3), p22, 1_c_int32_t, c_loc(m2_full), c_loc(m3_full), o22, c_loc(sk_full))
       if (rc /= 0) call bsa_Abort('bsahip_modal_recombination failed: '//bsahip_last_error_f())
+   end subroutine
+
+   ! ======================================================================
+   !  settings kept for the input files (the device path does not read them)
+   ! ======================================================================
+
+   subroutine bsa_setVersion(ivers)
+      integer(bsa_int_t), value :: ivers
+      ivers_ = ivers
+   end subroutine
+
+   !> Tensor symmetries: the device path computes the unique G values and
+   !> mirrors them, whatever the setting; the value is kept for the input files.
+   subroutine bsa_setSpectraSymmetries(ispctrsym)
+      integer(bsa_int_t), value :: ispctrsym
+      ispctrsym_ = ispctrsym
+   end subroutine
+
+   subroutine bsa_setNodalCoords(nn, coords)
+      integer(bsa_int_t), value :: nn
+      real(bsa_real_t), target, contiguous :: coords(:, :)
+      if (size(coords, 2) /= nn) call bsa_Abort('Nodal coordinates must be (3, NN).')
+      coords_ = coords
+   end subroutine
+
+   subroutine bsa_setWindVertProf(iwprof)
+      integer(bsa_int_t), value :: iwprof
+      iwprof_ = iwprof
+   end subroutine
+
+   subroutine bsa_setWindAltDir(ivert)
+      integer(bsa_int_t), value :: ivert
+      ivert_ = ivert
+   end subroutine
+
+   subroutine bsa_setAirDensity(aird)
+      real(bsa_real_t), value :: aird
+      aird_ = aird
+   end subroutine
+
+   subroutine bsa_setGlobalRotMatW2G(rotW2G)
+      real(bsa_real_t), intent(in) :: rotW2G(3, 3)
+      rotW2G_ = rotW2G
+   end subroutine
+
+   subroutine bsa_setWZRefAlt(Zref)
+      real(bsa_real_t), target, intent(in) :: Zref(:)
+      zref_ = Zref
+   end subroutine
+
+   !> Assumed-size in the reference interface: kept as addresses and read when
+   !> the number of wind zones is known (as L and sigma).
+   subroutine bsa_setWindCorrCoeffs(ccoeffs)
+      real(bsa_real_t), target, intent(in) :: ccoeffs(3, 3, *)
+      ccoef_addr_ = c_loc(ccoeffs)
+   end subroutine
+
+   subroutine bsa_setWindCorrExpnts(cexpn)
+      real(bsa_real_t), target, intent(in) :: cexpn(3, 3, *)
+      cexp_addr_ = c_loc(cexpn)
+   end subroutine
+
+   subroutine bsa_setWZRotMatW2G(rotW2G_L)
+      real(bsa_real_t), target, intent(in) :: rotW2G_L(3, 3, *)
+      rotlw_addr_ = c_loc(rotW2G_L)
+   end subroutine
+
+   subroutine bsa_setIncidenceAngles(incang)
+      real(bsa_real_t), target, intent(in) :: incang(:)
+      incang_ = incang
+   end subroutine
+
+   subroutine bsa_setNodalWindAltitudes(WnodAlt)
+      real(bsa_real_t), target, intent(in) :: WnodAlt(:)
+      walt_ = WnodAlt
+   end subroutine
+
+   !> Mesher options of the reference's zone refinement (multi-level
+   !> refinement, zone-delta validation) and the switch that forces Classic
+   !> under the GPU flag: accepted; the device Mesher has its own refinement
+   !> and the Classic path always runs on the device.
+   subroutine bsa_setBfmMLR(bool)
+      logical, intent(in) :: bool
+   end subroutine
+
+   subroutine bsa_doValidateZoneDeltas(bool)
+      logical, intent(in) :: bool
+   end subroutine
+
+   subroutine bsa_forceBsaClsExecution(bool)
+      logical, intent(in) :: bool
+   end subroutine
+
+   !> src/bsa/BsaLibImpl.F90:126-131: bsa_Run writes bsa.bsadata and
+   !> bsa.extdata in the current directory, then runs unless run is false.
+   subroutine bsa_generateBSAInputFiles(run)
+      logical, value :: run
+      gen_inputs_ = .true.
+      run_after_gen_ = run
+   end subroutine
+
+   !> The two input files of the standalone program, in the layouts its
+   !> reader expects (bsa.bsadata: labelled sections, one value per line;
+   !> bsa.extdata: stream of int32 / real64 arrays), from the setters' state.
+   !> Reals are written with full precision.  Data never set (coordinates,
+   !> reference altitudes, ...) is written as zeros of the right size.
+   subroutine generateInputFiles_()
+      integer :: iun, ios, i, nwz, nnl, nlibsl, ncorr, nd
+      real(bsa_real_t), pointer :: p3(:, :, :), p2(:, :)
+      real(bsa_real_t), allocatable :: corr3(:, :), z3(:, :, :), xs(:)
+      integer(bsa_int_t), allocatable :: tcs(:), drs(:)
+      character(len=*), parameter :: fi = '(i8)', fr = '(es25.17e3)'
+
+      if (allocated(tc_)) then
+         tcs = tc_
+      else
+         tcs = [1]
+      end if
+      if (allocated(dirs_)) then
+         drs = dirs_
+      else
+         drs = [1]
+      end if
+      open(newunit=iun, file=BSA_DATA_FNAME, iostat=ios, form='formatted', status='replace', &
+           action='write')
+      if (ios /= 0) call bsa_Abort('Error opening file  '//BSA_DATA_FNAME)
+      write(iun, '(a)') '-GENERAL:'
+      write(iun, fi) isuban_
+      write(iun, fi) ivers_
+      write(iun, fi) iconv_
+      write(iun, fi) icomp_psd_
+      write(iun, fi) icomp_bisp_
+      write(iun, fi) ionly_diag_
+      write(iun, fi) isym_
+      write(iun, fi) ispctrsym_
+      write(iun, fi) itest_
+      write(iun, '(a)') '-CLASSIC:'
+      write(iun, fi) iclassic_
+      write(iun, fi) nfreqs_
+      write(iun, fr) df_
+      write(iun, '(a)') '-MESHER:'
+      write(iun, fi) isvd_
+      write(iun, fi) bkgrfmt_
+      write(iun, fr) bkgaext_
+      write(iun, fr) genpaext_
+      write(iun, fr) maxaext_
+      write(iun, fi) ifcov_
+      write(iun, fi) idumpmod_
+      write(iun, '(a)') '-DIRECTIONS:'
+      write(iun, fi) size(drs)
+      do i = 1, size(drs)
+         write(iun, fi) drs(i)
+      end do
+      write(iun, '(a)') '-TURBULENCE:'
+      write(iun, fi) size(tcs)
+      do i = 1, size(tcs)
+         write(iun, fi) tcs(i)
+      end do
+      close(iun)
+
+      if (.not. (allocated(n_load_) .and. allocated(phi_) .and. allocated(wfc_) .and. &
+                 allocated(corr_) .and. allocated(u_node_) .and. allocated(wz_node_))) &
+         call bsa_Abort('Input files need the structure, modal and wind data to be set.')
+      if (.not. (c_associated(tscl_addr_) .and. c_associated(tstd_addr_))) &
+         call bsa_Abort('Input files need the turbulence scales and standard deviations.')
+      if (nz_lim_ > 0) then
+         nwz = nz_lim_
+      else if (allocated(umref_)) then
+         nwz = size(umref_)
+      else
+         nwz = maxval(wz_node_)
+      end if
+      nnl = size(n_load_)
+      nlibsl = 1
+      if (allocated(libs_load_)) nlibsl = size(libs_load_)
+      ncorr = (nn_ * nn_ + nn_) / 2
+      allocate(corr3(ncorr, 3), source=0.0_bsa_real_t)
+      corr3(:, 1:min(3, size(corr_, 2))) = corr_(:, 1:min(3, size(corr_, 2)))
+      nd = size(phi_, 1)
+      open(newunit=iun, file=EXT_DATA_FNAME, iostat=ios, form='unformatted', access='stream', &
+           status='replace', action='write')
+      if (ios /= 0) call bsa_Abort('Error opening file  '//EXT_DATA_FNAME)
+      write(iun) int(nn_, int32), int(nlibs_, int32), int(nnl, int32), int(nlibsl, int32)
+      write(iun) int(n_load_, int32)
+      if (allocated(libs_load_)) then
+         write(iun) int(libs_load_, int32)
+      else
+         write(iun) 1_int32
+      end if
+      if (allocated(coords_)) then
+         write(iun) coords_
+      else
+         write(iun) spread(spread(0.0_bsa_real_t, 1, 3), 2, nn_)
+      end if
+      write(iun) int(iwprof_, int32), int(ipsd_, int32), int(ivert_, int32), &
+                 int(size(wfc_, 2) - 3, int32)
+      write(iun) aird_
+      write(iun) rotW2G_
+      write(iun) int(nwz, int32)
+      call zoneVector_(zref_)
+      call zoneVector_(umref_)
+      call c_f_pointer(tscl_addr_, p3, [3, 3, nwz])
+      write(iun) p3
+      call c_f_pointer(tstd_addr_, p2, [3, nwz])
+      write(iun) p2
+      allocate(z3(3, 3, nwz), source=0.0_bsa_real_t)
+      call zoneTensor_(ccoef_addr_)
+      call zoneTensor_(cexp_addr_)
+      if (allocated(wz_lims_)) then
+         write(iun) wz_lims_
+      else
+         write(iun) spread(0.0_bsa_real_t, 1, nwz + 1)
+      end if
+      call zoneTensor_(rotlw_addr_)
+      call zoneVector_(incang_)
+      write(iun) int(wz_node_, int32)
+      if (allocated(walt_)) then
+         write(iun) walt_
+      else
+         write(iun) spread(0.0_bsa_real_t, 1, nn_)
+      end if
+      write(iun) u_node_
+      write(iun) corr3
+      write(iun) wfc_
+      write(iun) int(nd, int32), int(nm_, int32)
+      write(iun) natf_
+      write(iun) phi_
+      write(iun) Mg_
+      write(iun) Kg_
+      write(iun) Cg_
+      if (allocated(xsi_)) then
+         xs = xsi_
+      else
+         allocate(xs(nm_), source=0.0_bsa_real_t)
+      end if
+      write(iun) xs
+      close(iun)
+   contains
+      subroutine zoneVector_(v)
+         real(bsa_real_t), allocatable, intent(in) :: v(:)
+         if (allocated(v)) then
+            if (size(v) /= nwz) call bsa_Abort('Per-zone array size differs from the number of wind zones.')
+            write(iun) v
+         else
+            write(iun) spread(0.0_bsa_real_t, 1, nwz)
+         end if
+      end subroutine
+      subroutine zoneTensor_(addr)
+         type(c_ptr), intent(in) :: addr
+         real(bsa_real_t), pointer :: t3(:, :, :)
+         if (c_associated(addr)) then
+            call c_f_pointer(addr, t3, [3, 3, nwz])
+            write(iun) t3
+         else
+            write(iun) z3
+         end if
+      end subroutine
+   end subroutine
+
+   ! ======================================================================
+   !  exports and run-time I/O settings (src/IO/BsaIO.F90, BsaLibImpl.F90:1564-1760)
+   ! ======================================================================
+
+   subroutine bsa_printBSAHeader()
+      write(output_unit, '(/ 1x, a)') 'BsaLib -- bispectral stochastic analysis, MI355X device path'
+      write(output_unit, '(1x, a /)') '          (bsalib_amd: modal force and response bispectra on the GPU)'
+   end subroutine
+
+   !> Exports go to the current directory (the default here).
+   subroutine bsa_setExportInCurrDir()
+      if (allocated(exp_dir_)) deallocate(exp_dir_)
+   end subroutine
+
+   subroutine bsa_setExportDirectory(dirname)
+      character(len=*), intent(in) :: dirname
+      exp_dir_ = trim(dirname)
+      if (len(exp_dir_) > 0) then
+         if (exp_dir_(len(exp_dir_):) /= '/') exp_dir_ = exp_dir_//'/'
+      end if
+   end subroutine
+
+   !> Units are closed after every export here, so this only exists for callers.
+   subroutine bsa_closeUnitsAtEnd()
+   end subroutine
+
+   subroutine bsa_setExportFileFormat(iform)
+      integer(bsa_int_t), value :: iform
+      if (iform == BSA_EXPORT_FORMAT_FORMATTED) exp_binary_ = .false.
+      if (iform == BSA_EXPORT_FORMAT_UNFORMATTED) exp_binary_ = .true.
+   end subroutine
+
+   subroutine bsa_setExportAppendMode(imode)
+      integer(bsa_int_t), value :: imode
+      if (imode == BSA_EXPORT_MODE_REPLACE) exp_append_ = .false.
+      if (imode == BSA_EXPORT_MODE_APPEND) exp_append_ = .true.
+   end subroutine
+
+   !> Accepted; the library writes no log file of its own.
+   subroutine bsa_setOutFileName(fname)
+      character(len=*), intent(in) :: fname
+   end subroutine
+
+   !> Only BSA_EXPORT_BRM_MODE_NONE: bispectra are reduced to moments on the
+   !> device and never written out.
+   subroutine bsa_setBRMExportDefaultMode(imode)
+      integer(bsa_int_t), value :: imode
+      if (imode /= BSA_EXPORT_BRM_MODE_NONE) &
+         write(output_unit, '(1x, 2a)') WARNMSG, 'Bispectra export is not available in this library; ignored.'
+   end subroutine
+
+   !> Visual mode re-reads bispectra from a dump file; bsa_Run aborts with a
+   !> message when it is enabled.
+   subroutine bsa_enableVisualMode()
+      visual_ = .true.
+   end subroutine
+
+   subroutine bsa_setVisualIndexes(indexes, modal)
+      integer(bsa_int_t), intent(in) :: indexes(3)
+      logical, value :: modal
+   end subroutine
+
+   function openExport_(fname) result(iun)
+      character(len=*), intent(in) :: fname
+      integer :: iun, ios
+      character(len=:), allocatable :: path, pos, st
+      path = fname
+      if (allocated(exp_dir_)) path = exp_dir_//fname
+      pos = merge('append', 'asis  ', exp_append_)
+      st = merge('unknown', 'replace', exp_append_)
+      if (exp_binary_) then
+         open(newunit=iun, file=path, form='unformatted', access='stream', action='write', &
+              status=trim(st), position=trim(pos), iostat=ios)
+      else
+         open(newunit=iun, file=path, form='formatted', action='write', status=trim(st), &
+              position=trim(pos), iostat=ios)
+      end if
+      if (ios /= 0) call bsa_Abort('Cannot open export file  '//path)
+   end function
+
+   !> src/bsa/BsaLibImpl.F90:1590-1604: size, then one value per line.
+   subroutine bsa_exportMomentToFile(fname, vec)
+      character(len=*), intent(in) :: fname
+      real(bsa_real_t), intent(in) :: vec(:)
+      call writeVector_(fname, vec, .false.)
+   end subroutine
+
+   !> BsaLibImpl.F90:1745-1760.
+   subroutine bsa_exportExtremeValuesToFile(fname, rvar)
+      character(len=*), intent(in) :: fname
+      real(bsa_real_t), intent(in) :: rvar(:)
+      call writeVector_(fname, rvar, .false.)
+   end subroutine
+
+   !> BsaLibImpl.F90:1608-1613, 1715-1736: with the modal header (number of
+   !> used modes and their indices) when the array holds NM^3 values.
+   subroutine exportSkewnessGiven_(fname, sk)
+      character(len=*), intent(in) :: fname
+      real(bsa_real_t), intent(in) :: sk(:)
+      logical :: modal
+      modal = .false.
+      if (allocated(modes_)) modal = size(sk) == size(modes_)**3
+      call writeVector_(fname, sk, modal)
+   end subroutine
+
+   !> BsaLibImpl.F90:1617-1706: sk(i,j,k) = m3(i,j,k) / (sigma_i sigma_j sigma_k),
+   !> sigma_i = sqrt(m2(i,i)) (the reference's indexing of the m2 diagonal).
+   subroutine exportSkewnessCompute_(fname, dim, m2, m3)
+      character(len=*), intent(in) :: fname
+      integer(bsa_int_t), value :: dim
+      real(bsa_real_t), intent(in) :: m2(:), m3(:)
+      real(bsa_real_t), allocatable :: sk(:)
+      integer :: i, j, k, pm3
+      if (ionly_diag_ /= 0) then
+         sk = m3 / m2**CST_3d2
+      else
+         allocate(sk(size(m3)))
+         pm3 = 1
+         do k = 1, dim
+            do j = 1, dim
+               do i = 1, dim
+                  sk(pm3) = m3(pm3) / (sqrt(m2((k - 1) * dim + k)) * sqrt(m2((j - 1) * dim + j)) &
+                                       * sqrt(m2((i - 1) * dim + i)))
+                  pm3 = pm3 + 1
+               end do
+            end do
+         end do
+      end if
+      call exportSkewnessGiven_(fname, sk)
+   end subroutine
+
+   !> BsaLibImpl.F90:1564-1585: used modes, damping ratios, then bkg and res per mode.
+   subroutine exportBR_(fname, bkg, res, xsi)
+      character(len=*), intent(in) :: fname
+      real(bsa_real_t), intent(in) :: bkg(:), res(:), xsi(:)
+      integer :: iun, im
+      iun = openExport_(fname)
+      if (exp_binary_) then
+         write(iun) int(size(modes_), int32), int(modes_, int32), xsi, bkg, res
+      else
+         write(iun, *) size(modes_)
+         write(iun, *) modes_
+         write(iun, *) xsi
+         do im = 1, size(modes_)
+            write(iun, *) bkg(im), res(im)
+         end do
+      end if
+      close(iun)
+   end subroutine
+
+   !> BsaLibImpl.F90:1937-1967: number of nodes, then (x, y, z) per node.
+   subroutine bsa_saveCoordinatesToFile(fname, coords)
+      character(len=*), intent(in) :: fname
+      real(bsa_real_t), intent(in), target, optional :: coords(:, :)
+      integer :: iun, ios, i
+      if (.not. present(coords) .and. .not. allocated(coords_)) then
+         write(output_unit, '(1x, 2a)') WARNMSG, 'Cannot save coordinates to file. Data not provided. Skipping.'
+         return
+      end if
+      open(newunit=iun, file=fname, form='formatted', action='write', status='replace', iostat=ios)
+      if (ios /= 0) return
+      if (present(coords)) then
+         write(iun, *) size(coords, 2)
+         do i = 1, size(coords, 2)
+            write(iun, *) coords(:, i)
+         end do
+      else
+         write(iun, *) size(coords_, 2)
+         do i = 1, size(coords_, 2)
+            write(iun, *) coords_(:, i)
+         end do
+      end if
+      close(iun)
+   end subroutine
+
+   subroutine writeVector_(fname, vec, modal)
+      character(len=*), intent(in) :: fname
+      real(bsa_real_t), intent(in) :: vec(:)
+      logical, intent(in) :: modal
+      integer :: iun, i
+      iun = openExport_(fname)
+      if (exp_binary_) then
+         if (modal) write(iun) int(size(modes_), int32), int(modes_, int32)
+         write(iun) int(size(vec), int32), vec
+      else
+         if (modal) then
+            write(iun, *) size(modes_)
+            write(iun, *) modes_
+         end if
+         write(iun, *) size(vec)
+         do i = 1, size(vec)
+            write(iun, *) vec(i)
+         end do
+      end if
+      close(iun)
    end subroutine
 
 end module BsaLib

@@ -8,6 +8,9 @@
 # bsacl_module_driver.F90, linked against this repository's libbsahip.so
 # (built first by bsalib_amd/build.py) in place of the reference BsaCL runtime.
 #
+# Third product: oracle/_ref/bsa_app, the reference's standalone program
+# app/bsa.F90 (unmodified) compiled against this repository's module BsaLib.
+#
 # Units compiled (dependency order): CONSTANTS, Timer, BsaIO, Settings,
 # StructureData, WindType, utility, BsaLibData(+Impl), IO/Settings/Structure/
 # Wind implementations, BsaLib_Functions(+Impl).  The reference's own build
@@ -57,4 +60,20 @@ if [ -f "$LIBDIR/libbsahip.so" ]; then
   echo "built $OUT/bsacl_module_driver"
 else
   echo "libbsahip.so not built yet: skipping bsacl_module_driver" >&2
+fi
+
+# Third product: oracle/_ref/bsa_app, the reference's standalone program
+# (app/bsa.F90, unmodified) compiled against this repository's module BsaLib
+# (bsalib_amd/_lib/fortran) and linked to libbsalibf.a + libbsahip.so.  It reads
+# bsa.bsadata / bsa.extdata with the reference's own reader.
+FMOD=$(cd "$OUT/../.." && pwd)/bsalib_amd/_lib/fortran
+if [ -f "$FMOD/libbsalibf.a" ] && [ -f "$LIBDIR/libbsahip.so" ]; then
+  mkdir -p "$OUT/obj/app"
+  cd "$OUT/obj/app"
+  $FC -cpp -O2 -I"$FMOD" -c "$REF/app/bsa.F90" -o bsa.o
+  $FC -o "$OUT/bsa_app" bsa.o "$FMOD/libbsalibf.a" -L"$LIBDIR" -lbsahip \
+      -Wl,-rpath,'$ORIGIN/../../bsalib_amd/_lib'
+  echo "built $OUT/bsa_app"
+else
+  echo "Fortran front not built yet: skipping bsa_app" >&2
 fi

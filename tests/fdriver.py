@@ -20,14 +20,15 @@ def ensure_built():
     return build.FDRIVER
 
 
-def run(prob, *opts, check=True, timeout=300):
-    """Returns (outputs dict or None, completed process)."""
+def run(prob, *opts, check=True, timeout=300, workdir=None):
+    """Returns (outputs dict or None, completed process).  The driver runs in
+    workdir (default: a temporary directory), where files it writes stay."""
     drv = ensure_built()
     with tempfile.TemporaryDirectory() as td:
         fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
         O._write_ref_input(prob, fin, 1)
         r = subprocess.run([str(drv), fin, fout, *opts], capture_output=True, text=True,
-                           timeout=timeout)
+                           timeout=timeout, cwd=str(workdir or td))
         if r.returncode != 0:
             if check:
                 raise RuntimeError(f"bsalib_fdriver failed ({r.returncode}):\n{r.stdout}\n{r.stderr}")

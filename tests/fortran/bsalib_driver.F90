@@ -3,7 +3,7 @@
 !! bsa_Run, bsa_Finalise), reading its inputs from a raw stream file.
 !!
 !! Usage:  bsalib_driver <input.bin> <output.bin> [fp64|bf16x3] [grid] [scalar] [mesher|both]
-!!                        [pod] [podn=<modes>] [podt=<percent>] [post]
+!!                        [pod] [podn=<modes>] [podt=<percent>] [post] [gen]
 !! The input layout is the one tests write for the reference harness (header
 !! of 14 int32, then integer and real64 arrays in Fortran order).  "grid"
 !! leaves test mode off and passes damping ratios, so bsa_Run applies the
@@ -14,7 +14,9 @@
 !! post-processing after bsa_Run (bsa_computeBRdecomp, bsa_modalRecombination
 !! on the used modes at every DOF, bsa_computePeakFactors over 600 s) and
 !! appends bkg, res, the unilateral and complete m2, m3, m2o2, sk and the
-!! Gaussian / positive / negative peak factors of both.
+!! Gaussian / positive / negative peak factors of both.  "gen" calls
+!! bsa_generateBSAInputFiles(.false.): bsa_Run writes bsa.bsadata and
+!! bsa.extdata in the current directory and returns (no device needed).
 !! Output: nme (int32), the used modes, then for each of m2mf, m2mr, m2o2mr,
 !! m3mf_cls, m3mr_cls, m3mf_msh, m3mr_msh: its size (int32, 0 when bsa_Run
 !! left it unallocated) and its values (real64).
@@ -76,6 +78,7 @@ program bsalib_driver
       if (trim(arg) == 'both') isuban = 3
       if (trim(arg) == 'pod') isvd = 1
       if (trim(arg) == 'post') post = .true.
+      if (trim(arg) == 'gen') call bsa_generateBSAInputFiles(.false.)
       if (arg(1:5) == 'podn=') then
          read(arg(6:), *) npod
          call bsa_setPODNOfModesKept(npod)
