@@ -8,6 +8,7 @@
 // CPU fallback: without a usable GPU every entry point returns BSAHIP_EDEVICE.
 #include "kernels.hip"
 #include "pod.hip"
+#include "diag.hip"
 #include "../../include/bsahip.h"
 #include "bsahip_internal.h"
 
@@ -276,15 +277,22 @@ int set_pod(const bsahip_problem *p, Prob &D)
          return fail(BSAHIP_EDEVICE, std::string(#x) + ": " + rocblas_status_to_string(st_)); \
    } while (0)
 
-// A_K(y, i; f) = (Psi_a V) diag(w) V^T for the truncated POD of S(f) (see
-// pod.hip), written into the table layout of k_project.  sq: sqrt(S_i) at fl.
-int pod_project(Bufs &B, const Prob &D, const double *sq, const double *fl, int nfl,
-                int rows_per_mode, double *U, long ldU)
+// rocBLAS / rocSOLVER handle on the library stream, created on first use.
+int blas_handle()
 {
    if (!g.blas) {
       BLASC(rocblas_create_handle(&g.blas));
       BLASC(rocblas_set_stream(g.blas, g.s));
    }
+   return 0;
+}
+
+// A_K(y, i; f) = (Psi_a V) diag(w) V^T for the truncated POD of S(f) (see
+// pod.hip), written into the table layout of k_project.  sq: sqrt(S_i) at fl.
+int pod_project(Bufs &B, const Prob &D, const double *sq, const double *fl, int nfl,
+                int rows_per_mode, double *U, long ldU)
+{
+   CHK(blas_handle());
    const int n = D.nnl, nm = D.nm;
    const size_t nn2 = (size_t)n * n, mn = (size_t)nm * n;
    const long total = (long)D.ntc * nfl;
@@ -430,6 +438,8 @@ int plan_setup(bsahip_plan *P, const bsahip_problem *p, int32_t nfh, double df, 
 {
    CHK(check_flags(flags, true, true));
    if (nfh < 2 || !(df > 0.0)) return fail(BSAHIP_EINVAL, "need nfh >= 2 and df > 0");
+   if (flags & BSAHIP_ONLY_DIAG)
+      return fail(BSAHIP_EUNSUPPORTED, "the ONLY_DIAG model runs through bsahip_classic_run");
    CHK(load_problem(p, psd_mode, P->B, P->D));
    P->flags = flags;
    P->prec = flags & BSAHIP_PREC_MASK;
@@ -677,12 +687,86 @@ int bsahip_classic_run(const bsahip_problem *p, int32_t nfh, double df, int32_t 
    return bsahip__classic_moments(p, nfh, df, flags, 0, m2mf, m2mr, m2o2mr, m3mf, m3mr);
 }
 
+}  // extern "C"
+
+namespace {
+
+// The ONLY_DIAG Classic model (diag.hip): outputs have NM entries each.
+int classic_diag(const bsahip_problem *p, int32_t nfh, double df, int32_t flags, double *m2mf,
+                 double *m2mr, double *m2o2mr, double *m3mf, double *m3mr)
+{
+   if ((flags & BSAHIP_PREC_MASK) != BSAHIP_PREC_FP64)
+      return fail(BSAHIP_EUNSUPPORTED, "the ONLY_DIAG model runs in FP64");
+   if (nfh < 2 || !(df > 0.0)) return fail(BSAHIP_EINVAL, "need nfh >= 2 and df > 0");
+   Bufs &B = scratch();
+   Prob D;
+   CHK(load_problem(p, 0, B, D));
+   CHK(blas_handle());
+   const int nh = nfh, nf = 2 * nh - 1, nft = 2 * nh - 1, nm = D.nm, K = D.K;
+   const int rule = (flags & BSAHIP_QUAD_MASK) >> 4;
+   const double d = p->pulsation ? df * 2.0 * M_PI : df;
+   std::vector<double> fl(nh);
+   for (int u = 0; u < nh; ++u) fl[u] = (double)u * df;
+   double *dfl, *sq, *X, *Y, *T, *mom;
+   double2 *H;
+   CHK(upload(B, "fl", fl, &dfl));
+   CHK(B.get("sq", (size_t)D.ntc * nh * D.nnl, &sq));
+   CHK(B.get("dg_X", (size_t)K * nh, &X));
+   CHK(B.get("dg_Y", (size_t)nm * K * nh, &Y));
+   CHK(B.get("dg_T", (size_t)nm * nh * nh, &T));
+   CHK(B.get("dg_H", (size_t)nm * nft, &H));
+   CHK(B.get("dg_mom", 5 * (size_t)nm, &mom));
+   dim3 gp((D.nnl + 63) / 64, nh, D.ntc);
+   k_psd_sqrt<<<gp, 64, 0, g.s>>>(dfl, nh, D.prm, D.formula, D.nnl, D.ntc, D.pulsation, sq);
+   HIPC(hipGetLastError());
+   k_diag_tables<<<dim3((unsigned)((nh + 63) / 64), (unsigned)K), 64, 0, g.s>>>(
+      sq, nh, D.nnl, D.nnlp, D.pa_k, D.pb_k, K, nm, X, Y);
+   HIPC(hipGetLastError());
+   CHK(launch_frf(D, nft, df, H));
+   CHK(fill(mom, 5 * (size_t)nm, 0.0));
+   double *m2f = mom, *m2r = mom + nm, *m2o = mom + 2 * nm, *m3f = mom + 3 * nm, *m3r = mom + 4 * nm;
+   if (!(flags & BSAHIP_NO_BISP)) {
+      const double one = 1.0, zero = 0.0;
+      BLASC(rocblas_dgemm_strided_batched(g.blas, rocblas_operation_none, rocblas_operation_transpose,
+                                          nh, nh, K, &one, X, nh, 0, Y, nh, (rocblas_stride)K * nh,
+                                          &zero, T, nh, (rocblas_stride)nh * nh, nm));
+      k_diag_m3<<<dim3((unsigned)nf, (unsigned)nm), 256, 0, g.s>>>(T, nh, H, nft, rule, d * d, m3f, m3r);
+      HIPC(hipGetLastError());
+   }
+   if (!(flags & BSAHIP_NO_PSD)) {
+      k_diag_m2<<<dim3((unsigned)((nh + 63) / 64), (unsigned)nm), 64, 0, g.s>>>(
+         X, nh, D.pa_k, K, H, nft, d, 2.0 * M_PI * df, m2f, m2r, m2o);
+      HIPC(hipGetLastError());
+   }
+   std::vector<double> h(5 * (size_t)nm);
+   HIPC(hipMemcpyAsync(h.data(), mom, h.size() * sizeof(double), hipMemcpyDeviceToHost, g.s));
+   HIPC(hipStreamSynchronize(g.s));
+   auto put = [&](double *dst, int slot, bool on) {
+      if (dst && on) std::memcpy(dst, h.data() + (size_t)slot * nm, nm * sizeof(double));
+   };
+   const bool psd = !(flags & BSAHIP_NO_PSD), bisp = !(flags & BSAHIP_NO_BISP);
+   put(m2mf, 0, psd);
+   put(m2mr, 1, psd);
+   put(m2o2mr, 2, psd);
+   put(m3mf, 3, bisp);
+   put(m3mr, 4, bisp);
+   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
 int bsahip__classic_moments(const bsahip_problem *p, int32_t nfh, double df, int32_t flags,
                             int32_t psd_mode, double *m2mf, double *m2mr, double *m2o2mr,
                             double *m3mf, double *m3mr)
 {
    std::lock_guard<std::recursive_mutex> lk(g.mu);
    CHK(ensure_init());
+   if (flags & BSAHIP_ONLY_DIAG) {
+      if (psd_mode != 0) return fail(BSAHIP_EUNSUPPORTED, "the ONLY_DIAG model has no BsaCL form");
+      return classic_diag(p, nfh, df, flags, m2mf, m2mr, m2o2mr, m3mf, m3mr);
+   }
    bsahip_plan P;
    CHK(plan_setup(&P, p, nfh, df, flags, psd_mode));
    return plan_exec(&P, m2mf, m2mr, m2o2mr, m3mf, m3mr);

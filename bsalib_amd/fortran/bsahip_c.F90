@@ -15,6 +15,7 @@ module BsaHip_C
    integer(c_int32_t), parameter, public :: BSAHIP_SUM_EXACT = 256
    integer(c_int32_t), parameter, public :: BSAHIP_NO_PSD = 512
    integer(c_int32_t), parameter, public :: BSAHIP_NO_BISP = 1024
+   integer(c_int32_t), parameter, public :: BSAHIP_ONLY_DIAG = 2048
 
    type, bind(c), public :: bsahip_problem_t
       integer(c_int32_t) :: nn = 0, nnl = 0, nm = 0, ntc = 0, ndegw = 0, nwz = 0

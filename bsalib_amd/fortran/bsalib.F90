@@ -626,7 +626,7 @@ contains
       integer(bsa_int_t), allocatable, target :: tcs(:)
       real(bsa_real_t), pointer :: fp3(:, :, :), fp2(:, :)
       real(bsa_real_t) :: df
-      integer(bsa_int_t) :: nfh, nme, nwz, i, iclassic
+      integer(bsa_int_t) :: nfh, nme, nwz, i, iclassic, n2, n3
       logical :: do_cls, do_msh
       integer(c_int32_t) :: flags
       integer(c_int) :: rc
@@ -647,7 +647,8 @@ contains
       if (isuban_ < 1 .or. isuban_ > 3) call bsa_Abort('Analysis type must be 1, 2 or 3.')
       do_cls = isuban_ == 1 .or. isuban_ == 3
       do_msh = isuban_ == 2 .or. isuban_ == 3
-      if (ionly_diag_ /= 0) call bsa_Abort('Diagonal-only spectra are not available in this library.')
+      if (ionly_diag_ /= 0 .and. do_msh) &
+         call bsa_Abort('The ONLY_DIAG model is available for the Classic approach only in this library.')
       if (iconv_ /= BSA_PSD_CONVENTION_PULS) &
          call bsa_Abort('Only the pulsation convention (BSA_PSD_CONVENTION_PULS) is available in this library.')
 
@@ -767,9 +768,16 @@ contains
       ! dw^2, BsaClassicImpl.F90:331,341); the vectorised version uses the
       ! edge/corner-corrected rule of intgSpectraVect_.  The element count is
       ! formed in 64-bit here (the reference forms it in 32-bit, which wraps).
+      ! ONLY_DIAG model: NM diagonal entries per output (dimM_psd_ = dimM_bisp_ = NM)
+      n2 = nme**2
+      n3 = nme**3
+      if (ionly_diag_ /= 0) then
+         n2 = nme
+         n3 = nme
+      end if
       iclassic = iclassic_
       if (itest_ == 0 .and. icomp_bisp_ /= 0 .and. iclassic == BSA_CLASSIC_MODE_VECTOR) then
-         if (int(2 * nfh - 1, int64)**2 * int(nme, int64)**3 > 1000000000_int64) then
+         if (int(2 * nfh - 1, int64)**2 * int(n3, int64) > 1000000000_int64) then
             write(output_unit, '(1x, 2a)') WARNMSG, 'Too high allocation size for VECTORISED BSA version: using SCALAR.'
             iclassic = BSA_CLASSIC_MODE_SCALAR
          end if
@@ -780,12 +788,13 @@ contains
       else
          flags = int(iprec_, c_int32_t) + BSAHIP_QUAD_VECT
       end if
+      if (ionly_diag_ /= 0) flags = ior(flags, BSAHIP_ONLY_DIAG)
       o_m2mf = c_null_ptr; o_m2mr = c_null_ptr; o_m2o2mr = c_null_ptr
       o_m3mf = c_null_ptr; o_m3mr = c_null_ptr
       if (icomp_psd_ /= 0) then
-         call realloc_(m2mf_cls, nme**2)
-         call realloc_(m2mr_cls, nme**2)
-         call realloc_(m2o2mr_cls, nme**2)
+         call realloc_(m2mf_cls, n2)
+         call realloc_(m2mr_cls, n2)
+         call realloc_(m2o2mr_cls, n2)
          o_m2mf = c_loc(m2mf_cls)
          o_m2mr = c_loc(m2mr_cls)
          o_m2o2mr = c_loc(m2o2mr_cls)
@@ -793,8 +802,8 @@ contains
          flags = ior(flags, BSAHIP_NO_PSD)
       end if
       if (icomp_bisp_ /= 0) then
-         call realloc_(m3mf_cls, nme**3)
-         call realloc_(m3mr_cls, nme**3)
+         call realloc_(m3mf_cls, n3)
+         call realloc_(m3mr_cls, n3)
          o_m3mf = c_loc(m3mf_cls)
          o_m3mr = c_loc(m3mr_cls)
       else

@@ -57,6 +57,12 @@ extern "C" {
 #define BSAHIP_NO_PSD        (1 << 9)  /* skip m2 (modal PSD) outputs   */
 #define BSAHIP_NO_BISP       (1 << 10) /* skip m3 (bispectrum) outputs  */
 
+/* the ONLY_DIAG model (flag bit 11; bsahip_classic_run, FP64): loaded nodes
+ * without coherence between them, diagonal modal entries only, every m2* and
+ * m3* output holds NM values (getFM_diag_tnlm_vect_cls_ / getRM_diag_vect_cls_,
+ * functionsImpl.F90:1747-2101) */
+#define BSAHIP_ONLY_DIAG     (1 << 11)
+
 /* Problem description: borrowed host pointers, read during the call only. */
 typedef struct bsahip_problem {
    int32_t nn;            /* total nodes (packed coherence index base)       */

@@ -506,7 +506,8 @@ def _write_ref_input(prob, path, mode: int, fi=None, fj=None):
 def reference_run(prob, mode: int = 1, fi=None, fj=None, workdir=None,
                   malloc_perturb: int = 255) -> dict:
     """Run the reference's own operators through oracle/_ref/ref_harness.
-    mode bit 1: Classic vect BFM + BRM on the full grid; bit 2: Mesher BFM at (fi, fj).
+    mode bit 1: Classic vect BFM + BRM on the full grid; bit 2: Mesher BFM at (fi, fj);
+    bit 4: the ONLY_DIAG Classic model (getFM_diag_tnlm_vect_cls_ / getRM_diag_vect_cls_).
 
     The Classic BFM reads S(f1+f2) beyond +-fmax from padding it never
     initialises (functionsImpl.F90:1409-1413); glibc's MALLOC_PERTURB_=255
@@ -539,6 +540,11 @@ def reference_run(prob, mode: int = 1, fi=None, fj=None, workdir=None,
         out["bisp_f"] = take(nf * nf * nm ** 3, (nf, nf, nm ** 3))
         out["psd_r"] = take(nf * nm * nm, (nf, nm * nm))
         out["bisp_r"] = take(nf * nf * nm ** 3, (nf, nf, nm ** 3))
+    if mode & 4:
+        out["psd_f_diag"] = take(nf * nm, (nf, nm))
+        out["bisp_f_diag"] = take(nf * nf * nm, (nf, nf, nm))
+        out["psd_r_diag"] = take(nf * nm, (nf, nm))
+        out["bisp_r_diag"] = take(nf * nf * nm, (nf, nf, nm))
     if mode & 2 and fi is not None:
         out["bfm_msh"] = take(nm ** 3 * np.size(fi), (nm ** 3, np.size(fi)))
     return out
@@ -730,3 +736,57 @@ def peak_factors(m2, m2o2, obs_time: float, sk=None):
     r = h3 * (b2 + 2.0 * gamma - 1.0 + 1.98 / b2)
     pos, neg = pos + r, neg - r
     return peak_g, (pos + peak_g) * k, (neg + peak_g) * k
+
+
+# ---------------------------------------------------------------------------
+# ONLY_DIAG Classic model -- numpy restatement
+# ---------------------------------------------------------------------------
+def classic_diag(prob) -> dict:
+    """getFM_diag_tnlm_vect_cls_ (functionsImpl.F90:1747-1970) and
+    getRM_diag_vect_cls_ (:1975-2101) on the Classic grid, then
+    intgSpectraVect_: each loaded node on its own (no coherence between nodes),
+    diagonal modal entries only.  Per mode m, with Psi_l / Psi_q the linear /
+    quadratic projections of node n and S_n the nodal PSD:
+      psd(f)      = sum_{tc,n} Psi_l^2 S_n(f)
+      bisp(f1,f2) = 2 sum_{tc,n} Psi_q Psi_l^2 [S_n(f1) S_n(f2)
+                      + S_n(f1+f2) S_n(f2) + S_n(f1+f2) S_n(f1)]
+    (the three lib products of :1899-1903 collapse to Psi_q Psi_l^2 after the
+    lib sums), S(f1+f2) = 0 beyond +-fmax (zero padding, T2); response spectra
+    times |H_m(f)|^2 and Re[H_m(f1) H_m(f2) conj H_m(f1+f2)], H at w1 + w2.
+    Returns psd_f, psd_r (nf, nm), bisp_f, bisp_r (nf, nf, nm) and m2mf,
+    m2mr, m3mf, m3mr (nm)."""
+    modes = kept_modes(prob)
+    P = psi(prob, modes)
+    nm = modes.size
+    f = prob.freqs()
+    nf = f.size
+    c = nf // 2
+    psd = np.zeros((nf, nm))
+    bisp = np.zeros((nf, nf, nm))
+    ii, jj = np.meshgrid(np.arange(nf), np.arange(nf), indexing="ij")
+    s = ii + jj - c                                     # grid index of f1 + f2
+    inside = (s >= 0) & (s < nf)
+    for t in prob.tc:
+        S = eval_psd(prob, f, int(t))                   # (nf, nnl)
+        Pl, Pq = P[:, :, t - 1], P[:, :, t + 2]
+        psd += S @ (Pl ** 2).T
+        for n in range(prob.nnl):
+            Sn = S[:, n]
+            Ss = np.where(inside, Sn[np.clip(s, 0, nf - 1)], 0.0)
+            tmp = Sn[:, None] * Sn[None, :] + Ss * Sn[None, :] + Ss * Sn[:, None]
+            bisp += 2.0 * tmp[:, :, None] * (Pq[:, n] * Pl[:, n] ** 2)[None, None, :]
+    mo = modes - 1
+    w = TWO_PI * f
+    H = 1.0 / (prob.Km[mo][None, :] - (w * w)[:, None] * prob.Mm[mo][None, :]
+               + 1j * w[:, None] * np.diag(prob.Cm)[mo][None, :])            # (nf, nm)
+    w12 = w[:, None] + w[None, :]
+    H12 = 1.0 / (prob.Km[mo][None, None, :] - (w12 * w12)[:, :, None] * prob.Mm[mo][None, None, :]
+                 + 1j * w12[:, :, None] * np.diag(prob.Cm)[mo][None, None, :])
+    psd_r = psd * np.abs(H) ** 2
+    bisp_r = bisp * (H[:, None, :] * H[None, :, :] * np.conj(H12)).real
+    out = dict(psd_f=psd, psd_r=psd_r, bisp_f=bisp, bisp_r=bisp_r)
+    for k, (ps, bs) in dict(f=(psd, bisp), r=(psd_r, bisp_r)).items():
+        m2, m3 = np.zeros(nm), np.zeros(nm)
+        _intg(f, np.asfortranarray(ps), m2, np.asfortranarray(bs), m3)
+        out[f"m2m{k}"], out[f"m3m{k}"] = m2, m3
+    return out

@@ -9,6 +9,8 @@
 !!   getFM_full_tnm_vect_cls_      (src/bsa/functions/functionsImpl.F90:1312-1587)
 !!   getRM_full_vect_cls_          (src/bsa/functions/functionsImpl.F90:1598-1738)
 !!   getFM_full_tnm_scalar_msh_    (src/bsa/functions/functionsImpl.F90:247-465)
+!!   getFM_diag_tnlm_vect_cls_     (src/bsa/functions/functionsImpl.F90:1747-1970)
+!!   getRM_diag_vect_cls_          (src/bsa/functions/functionsImpl.F90:1975-2101)
 !! and writes their outputs to a raw stream file for tests/golden.
 !!
 !! Usage:  ref_harness <input.bin> <output.bin> [time]
@@ -18,7 +20,8 @@ program ref_harness
    use, intrinsic :: iso_fortran_env, only: int32, real64
    use BsaLib_CONSTANTS
    use BsaLib_Data
-   use BsaLib_Functions, only: setBsaFunctionLocalVars, getFM_full_tnm_vect_cls_, getRM_full_vect_cls_, getFM_full_tnm_scalar_msh_
+   use BsaLib_Functions, only: setBsaFunctionLocalVars, getFM_full_tnm_vect_cls_, getRM_full_vect_cls_, getFM_full_tnm_scalar_msh_, &
+                               getFM_diag_tnlm_vect_cls_, getRM_diag_vect_cls_
    use omp_lib
    implicit none
 
@@ -115,6 +118,19 @@ program ref_harness
       write(uout) psd, bisp
       call getRM_full_vect_cls_(f, psd, bisp)
       write(uout) psd, bisp
+   endif
+   if (iand(imode, 4) /= 0) then
+      ! ONLY_DIAG model: NM modal diagonal spectra (dimensions as validateAll_ sets them)
+      if (allocated(psd)) deallocate(psd)
+      if (allocated(bisp)) deallocate(bisp)
+      dimM_psd_ = nme
+      dimM_bisp_ = nme
+      call getFM_diag_tnlm_vect_cls_(f, suvw, psd, bisp)
+      write(uout) psd, bisp
+      call getRM_diag_vect_cls_(f, psd, bisp)
+      write(uout) psd, bisp
+      dimM_psd_ = nme*nme
+      dimM_bisp_ = nme*nme*nme
    endif
    if (iand(imode, 2) /= 0 .and. npts > 0) then
       allocate(bfm(nme*nme*nme, 1))

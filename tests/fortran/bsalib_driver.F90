@@ -3,7 +3,7 @@
 !! bsa_Run, bsa_Finalise), reading its inputs from a raw stream file.
 !!
 !! Usage:  bsalib_driver <input.bin> <output.bin> [fp64|bf16x3] [grid] [scalar] [mesher|both]
-!!                        [pod] [podn=<modes>] [podt=<percent>] [post] [gen]
+!!                        [pod] [podn=<modes>] [podt=<percent>] [post] [gen] [diag]
 !! The input layout is the one tests write for the reference harness (header
 !! of 14 int32, then integer and real64 arrays in Fortran order).  "grid"
 !! leaves test mode off and passes damping ratios, so bsa_Run applies the
@@ -16,7 +16,8 @@
 !! appends bkg, res, the unilateral and complete m2, m3, m2o2, sk and the
 !! Gaussian / positive / negative peak factors of both.  "gen" calls
 !! bsa_generateBSAInputFiles(.false.): bsa_Run writes bsa.bsadata and
-!! bsa.extdata in the current directory and returns (no device needed).
+!! bsa.extdata in the current directory and returns (no device needed);
+!! "diag" selects the ONLY_DIAG model (bsa_setSpectraExtension(1)).
 !! Output: nme (int32), the used modes, then for each of m2mf, m2mr, m2o2mr,
 !! m3mf_cls, m3mr_cls, m3mf_msh, m3mr_msh: its size (int32, 0 when bsa_Run
 !! left it unallocated) and its values (real64).
@@ -79,6 +80,7 @@ program bsalib_driver
       if (trim(arg) == 'pod') isvd = 1
       if (trim(arg) == 'post') post = .true.
       if (trim(arg) == 'gen') call bsa_generateBSAInputFiles(.false.)
+      if (trim(arg) == 'diag') call bsa_setSpectraExtension(1)
       if (arg(1:5) == 'podn=') then
          read(arg(6:), *) npod
          call bsa_setPODNOfModesKept(npod)

@@ -15,6 +15,10 @@ problems and stores inputs checksums and outputs as small fixtures:
                      tests/test_oracle.py)
 * bfm_msh            getFM_full_tnm_scalar_msh_ at listed points (Mesher BFM,
                      exact S(f1+f2), including points whose f1+f2 is off-grid)
+* *_diag             the ONLY_DIAG model (getFM_diag_tnlm_vect_cls_ /
+                     getRM_diag_vect_cls_): psd (nf, nm), bispectra (full for
+                     the tiny cases, the same sample for config 1) and the
+                     quadrature m2mf/m2mr/m3mf/m3mr_diag (nm)
 
 The reference run uses MALLOC_PERTURB_=255 (zero-filled allocations) so that
 the never-initialised padding it reads for |f1+f2| > fmax holds the intended
@@ -51,6 +55,13 @@ def _quad(prob, psd, bisp):
     return m2, m3
 
 
+def _quad_diag(prob, psd, bisp):
+    nm = psd.shape[1]
+    m2, m3 = np.zeros(nm), np.zeros(nm)
+    O._intg(prob.freqs(), np.asfortranarray(psd), m2, np.asfortranarray(bisp), m3)
+    return m2, m3
+
+
 def make(name: str):
     prob = CASES[name]()
     f = prob.freqs()
@@ -61,7 +72,7 @@ def make(name: str):
     pts = rng.uniform(-fmax, fmax, size=(12, 2))
     pts = np.vstack([pts, [[fmax, fmax], [-fmax, 0.9 * fmax], [0.0, 0.0], [f[1], f[-2]]]])
     # zero-filled allocations: the padding of S(f1+f2) holds its intended zeros
-    R = O.reference_run(prob, mode=3, fi=pts[:, 0], fj=pts[:, 1], malloc_perturb=255)
+    R = O.reference_run(prob, mode=7, fi=pts[:, 0], fj=pts[:, 1], malloc_perturb=255)
     # second run with 0xAA-filled allocations: points whose value moves read the padding
     R2 = O.reference_run(prob, mode=1, malloc_perturb=170)
     t2 = np.abs(R2["bisp_f"] - R["bisp_f"]).max(axis=2) > 0
@@ -74,9 +85,15 @@ def make(name: str):
                suvw=R["suvw"], psd_f=R["psd_f"], psd_r=R["psd_r"], m2mf=m2f, m2mr=m2r,
                m3mf=m3f, m3mr=m3r, msh_fi=pts[:, 0], msh_fj=pts[:, 1], bfm_msh=R["bfm_msh"],
                t2_points=np.array(int(t2.sum())))
+    for k in ("f", "r"):
+        m2d, m3d = _quad_diag(prob, R[f"psd_{k}_diag"], R[f"bisp_{k}_diag"])
+        out[f"psd_{k}_diag"] = R[f"psd_{k}_diag"]
+        out[f"m2m{k}_diag"], out[f"m3m{k}_diag"] = m2d, m3d
     if nf <= 20:
         out["bisp_f"] = R["bisp_f"]
         out["bisp_r"] = R["bisp_r"]
+        out["bisp_f_diag"] = R["bisp_f_diag"]
+        out["bisp_r_diag"] = R["bisp_r_diag"]
     else:
         i1 = rng.integers(0, nf, 300)
         i2 = rng.integers(0, nf, 300)
@@ -85,6 +102,8 @@ def make(name: str):
         out["sample_i1"], out["sample_i2"] = i1, i2
         out["bisp_f_sample"] = R["bisp_f"][i1, i2, :]
         out["bisp_r_sample"] = R["bisp_r"][i1, i2, :]
+        out["bisp_f_diag_sample"] = R["bisp_f_diag"][i1, i2, :]
+        out["bisp_r_diag_sample"] = R["bisp_r_diag"][i1, i2, :]
     np.savez_compressed(OUT / f"{name}.npz", **out)
     print(name, {k: np.shape(v) for k, v in out.items()})
 
