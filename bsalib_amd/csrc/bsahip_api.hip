@@ -357,15 +357,14 @@ int build_table(Bufs &B, const Prob &D, const double *fl, int nfl, int rows_per_
    HIPC(hipGetLastError());
    if (D.pod_nmodes > 0 || D.pod_lim > 0.0)
       return pod_project(B, D, sq, fl, nfl, rows_per_mode, U, ldU);
-   const int nq = (D.nm + 3) / 4;
-#define LAUNCH_PROJ(NQ)                                                                       \
-   k_project<NQ><<<gp, 256, 0, g.s>>>(D.pa_t, D.lnr, sq, fl, nfl, D.nnl, D.nm, D.nnlp,        \
+   const int mt = (D.nm + 15) / 16;
+#define LAUNCH_PROJ(MT)                                                                       \
+   k_project<MT><<<gp, 256, 0, g.s>>>(D.pa_t, D.lnr, sq, fl, nfl, D.nnl, D.nm, D.nnlp,        \
                                       rows_per_mode, U, ldU)
-   if (nq <= 1) LAUNCH_PROJ(1);
-   else if (nq <= 2) LAUNCH_PROJ(2);
-   else if (nq <= 4) LAUNCH_PROJ(4);
-   else if (nq <= 8) LAUNCH_PROJ(8);
-   else LAUNCH_PROJ(16);
+   if (mt <= 1) LAUNCH_PROJ(1);
+   else if (mt <= 2) LAUNCH_PROJ(2);
+   else if (mt <= 3) LAUNCH_PROJ(3);
+   else LAUNCH_PROJ(4);
 #undef LAUNCH_PROJ
    HIPC(hipGetLastError());
    return 0;

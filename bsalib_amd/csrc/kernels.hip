@@ -120,10 +120,17 @@ __global__ void k_frf_table(const double *__restrict__ Mm, const double *__restr
 }
 
 // ---------------------------------------------------------------------------
-// projection A(y,i;f) = s_i(f) sum_j Psi_a(y,j) s_j(f) exp(|f| ln rho_ji)
-// grid (ceil(nnl/64), nfl, ntc), block 256 = 64 nodes i x 4 mode groups
+// projection A(y,i;f) = s_i(f) sum_j Psi_a(y,j) s_j(f) exp(|f| ln rho_ji) on
+// FP64 MFMA: per frequency an (MT*16 x nnl) x (nnl x 64) product whose right
+// operand, the coherence tile rho_ji^|f|, is generated in LDS chunk by chunk.
+// grid (ceil(nnl/64), nfl, ntc), block 256 = 4 waves; wave w owns the 16 node
+// columns 16w.. of the block's 64 and all MT row blocks of 16 modes.
+// v_mfma_f64_16x16x4f64: A lane l holds (row l%16, k l/16), B (k l/16, col l%16),
+// D register t of lane l holds (row l/16 + 4t, col l%16).
 // ---------------------------------------------------------------------------
-template <int NQ>
+#define PKC 32   // node (j) chunk staged in LDS
+
+template <int MT>
 __global__ __launch_bounds__(256) void k_project(
    const double *__restrict__ pa,      // [tc][nm][nnl]
    const double *__restrict__ lnr,     // [tc][nnl][nnl]  ln rho_ji (row j, col i)
@@ -132,28 +139,29 @@ __global__ __launch_bounds__(256) void k_project(
    int row_stride,                     // rows per mode in U (nhp)
    double *__restrict__ U, long ldU)
 {
-   __shared__ double ps[64][NQ * 4 + 1];
-   __shared__ double ee[64][65];
+   constexpr int NR = MT * 16;
+   __shared__ double ps[PKC][NR];
+   __shared__ double ee[PKC][64];
    const int tc = blockIdx.z;
    const int q = blockIdx.y;
    const int i0 = blockIdx.x * 64;
-   const int li = threadIdx.x & 63;
-   const int yg = threadIdx.x >> 6;
+   const int tid = threadIdx.x, lane = tid & 63;
+   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
    const double af = fabs_list[q];
    const double *sqq = sq + ((size_t)tc * nfl + q) * nnl;
    const double *lnt = lnr + (size_t)tc * nnl * nnl;
    const double *pat = pa + (size_t)tc * nm * nnl;
-   double acc[NQ];
+   d4 acc[MT];
 #pragma unroll
-   for (int t = 0; t < NQ; ++t) acc[t] = 0.0;
+   for (int r = 0; r < MT; ++r) acc[r] = (d4){0.0, 0.0, 0.0, 0.0};
 
-   for (int j0 = 0; j0 < nnl; j0 += 64) {
-      for (int t = threadIdx.x; t < 64 * NQ * 4; t += 256) {
-         const int jj = t / (NQ * 4), y = t % (NQ * 4);
+   for (int j0 = 0; j0 < nnl; j0 += PKC) {
+      for (int t = tid; t < PKC * NR; t += 256) {
+         const int jj = t / NR, y = t % NR;
          const int j = j0 + jj;
          ps[jj][y] = (j < nnl && y < nm) ? pat[(size_t)y * nnl + j] * sqq[j] : 0.0;
       }
-      for (int t = threadIdx.x; t < 64 * 64; t += 256) {
+      for (int t = tid; t < PKC * 64; t += 256) {
          const int jj = t >> 6, ii = t & 63;
          const int j = j0 + jj, i = i0 + ii;
          double e = 0.0;
@@ -164,22 +172,27 @@ __global__ __launch_bounds__(256) void k_project(
          ee[jj][ii] = e;
       }
       __syncthreads();
-      const int jn = min(64, nnl - j0);
-      for (int jj = 0; jj < jn; ++jj) {
-         const double e = ee[jj][li];
 #pragma unroll
-         for (int t = 0; t < NQ; ++t) acc[t] = fma(ps[jj][yg + 4 * t], e, acc[t]);
+      for (int kk = 0; kk < PKC; kk += 4) {
+         const int kr = kk + (lane >> 4);
+         const double b = ee[kr][16 * w + (lane & 15)];
+#pragma unroll
+         for (int r = 0; r < MT; ++r)
+            acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(ps[kr][16 * r + (lane & 15)], b, acc[r],
+                                                          0, 0, 0);
       }
       __syncthreads();
    }
-   const int i = i0 + li;
+   const int i = i0 + 16 * w + (lane & 15);
    if (i < nnl) {
       const double si = sqq[i];
 #pragma unroll
-      for (int t = 0; t < NQ; ++t) {
-         const int y = yg + 4 * t;
-         if (y < nm) U[((size_t)tc * nnlp + i) * ldU + (size_t)y * row_stride + q] = acc[t] * si;
-      }
+      for (int r = 0; r < MT; ++r)
+#pragma unroll
+         for (int t = 0; t < 4; ++t) {
+            const int y = 16 * r + (lane >> 4) + 4 * t;
+            if (y < nm) U[((size_t)tc * nnlp + i) * ldU + (size_t)y * row_stride + q] = acc[r][t] * si;
+         }
    }
 }
 
