@@ -90,7 +90,8 @@ struct Ctx {
    bool ready = false;
    int dev = 0;
    hipStream_t s = nullptr;
-   hipEvent_t e0 = nullptr, e1 = nullptr;
+   hipStream_t s2 = nullptr;             // side stream: modal PSD moments beside the G kernel
+   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
    Bufs *bufs = nullptr;                 // scratch of the one-shot entry points
    double last_ms = 0.0;
    int last_launches = 0;
@@ -573,12 +574,26 @@ int plan_exec(bsahip_plan *P, double *m2mf, double *m2mr, double *m2o2mr, double
    const bool want_bisp = !(P->flags & BSAHIP_NO_BISP);
    CHK(plan_prepare(P));
    P->last_ms = 0.0;
+   // the modal PSD moments only read the tables: they run on the side stream
+   // while the G kernel runs, and the copy-out waits for both
+   const bool side = want_psd && want_bisp;
+   if (side) {
+      HIPC(hipEventRecord(g.e2, g.s));
+      HIPC(hipStreamWaitEvent(g.s2, g.e2, 0));
+      hipStream_t main_s = g.s;
+      g.s = g.s2;
+      const int rc = plan_m2(P);
+      g.s = main_s;
+      CHK(rc);
+      HIPC(hipEventRecord(g.e3, g.s2));
+   }
    if (want_bisp) {
       HIPC(hipEventRecord(g.e0, g.s));
       CHK(plan_gsym(P));
       HIPC(hipEventRecord(g.e1, g.s));
    }
-   if (want_psd) CHK(plan_m2(P));
+   if (want_psd && !side) CHK(plan_m2(P));
+   if (side) HIPC(hipStreamWaitEvent(g.s, g.e3, 0));
    CHK(plan_copyout(P, m2mf, m2mr, m2o2mr, m3mf, m3mr));
    HIPC(hipStreamSynchronize(g.s));
    g.last_ms = 0.0;
@@ -618,8 +633,11 @@ int bsahip_init(int device)
    if (device < 0 || device >= n) return fail(BSAHIP_EINVAL, "device index out of range");
    HIPC(hipSetDevice(device));
    HIPC(hipStreamCreateWithFlags(&g.s, hipStreamNonBlocking));
+   HIPC(hipStreamCreateWithFlags(&g.s2, hipStreamNonBlocking));
    HIPC(hipEventCreate(&g.e0));
    HIPC(hipEventCreate(&g.e1));
+   HIPC(hipEventCreateWithFlags(&g.e2, hipEventDisableTiming));
+   HIPC(hipEventCreateWithFlags(&g.e3, hipEventDisableTiming));
    g.dev = device;
    g.ready = true;
    return 0;
@@ -634,8 +652,12 @@ int bsahip_finalise(void)
    g.bufs = nullptr;
    if (g.blas) (void)rocblas_destroy_handle(g.blas);
    g.blas = nullptr;
+   (void)hipStreamSynchronize(g.s2);
    (void)hipEventDestroy(g.e0);
    (void)hipEventDestroy(g.e1);
+   (void)hipEventDestroy(g.e2);
+   (void)hipEventDestroy(g.e3);
+   (void)hipStreamDestroy(g.s2);
    (void)hipStreamDestroy(g.s);
    g.ready = false;
    return 0;

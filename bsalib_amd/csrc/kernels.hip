@@ -128,7 +128,7 @@ __global__ void k_frf_table(const double *__restrict__ Mm, const double *__restr
 // v_mfma_f64_16x16x4f64: A lane l holds (row l%16, k l/16), B (k l/16, col l%16),
 // D register t of lane l holds (row l/16 + 4t, col l%16).
 // ---------------------------------------------------------------------------
-#define PKC 32   // node (j) chunk staged in LDS
+#define PKC 8    // node (j) chunk staged in LDS
 
 template <int MT>
 __global__ __launch_bounds__(256) void k_project(
