@@ -20,6 +20,8 @@ QUAD_VECT, QUAD_RIEMANN, QUAD_TRAPZ = 0 << 4, 1 << 4, 2 << 4
 SUM_TRUNCATED, SUM_EXACT = 0, 1 << 8
 NO_PSD, NO_BISP = 1 << 9, 1 << 10
 ONLY_DIAG = 1 << 11
+# return codes (include/bsahip.h)
+EINVAL, EDEVICE, ENOMEM, EUNSUPPORTED, ESTATE = -1, -2, -3, -4, -5
 
 QUAD = {"vect": QUAD_VECT, "riemann": QUAD_RIEMANN, "trapezoid": QUAD_TRAPZ}
 PREC = {"fp64": PREC_FP64, "fp32": PREC_FP32, "bf16x3": PREC_BF16X3}
@@ -50,7 +52,8 @@ class Problem_t(C.Structure):
 _lib = None
 
 EXPORTS = ["bsahip_init", "bsahip_finalise", "bsahip_last_error", "bsahip_version",
-           "bsahip_classic_run", "bsahip_classic_spectra", "bsahip_bfm_points",
+           "bsahip_classic_run", "bsahip_classic_spectra", "bsahip_classic_bispectra_sel",
+           "bsahip_bfm_points",
            "bsahip_last_kernel_time", "bsahip_plan_create", "bsahip_plan_run",
            "bsahip_plan_destroy", "bsahip_mesher_create", "bsahip_mesher_layout", "bsahip_mesher_ntiles",
            "bsahip_mesher_tiles", "bsahip_mesher_features", "bsahip_mesher_run",
@@ -180,6 +183,26 @@ def classic_spectra(prob, exact: bool = False, bound: Bound = None) -> dict:
                                         _ptr(psd_f, C.c_double), _ptr(psd_r, C.c_double),
                                         _ptr(bfm, C.c_double), _ptr(brm, C.c_double)))
     return {"psd_f": psd_f, "psd_r": psd_r, "bisp_f": bfm, "bisp_r": brm}
+
+
+def classic_bispectra_sel(prob, sel, exact: bool = False, only_diag: bool = False,
+                          bound: Bound = None) -> dict:
+    """Bispectra of the selected modal entries on the Classic grid, (NF, NF,
+    len(sel)); sel is 1-based over the kept modes, m + (n-1) NM + (p-1) NM^2
+    (the mode position with only_diag).  bsahip_classic_bispectra_sel."""
+    b = bound or Bound(prob)
+    nf = prob.nf
+    s = np.ascontiguousarray(np.atleast_1d(sel), dtype=np.int32)
+    bfm = np.zeros((nf, nf, s.size), order="F")
+    brm = np.zeros((nf, nf, s.size), order="F")
+    flags = SUM_EXACT if exact else SUM_TRUNCATED
+    if only_diag:
+        flags |= ONLY_DIAG
+    _check(lib().bsahip_classic_bispectra_sel(C.byref(b.struct), C.c_int32(prob.nfreqs),
+                                              C.c_double(prob.df), C.c_int32(flags),
+                                              C.c_int32(s.size), _ptr(s, C.c_int32),
+                                              _ptr(bfm, C.c_double), _ptr(brm, C.c_double)))
+    return {"bisp_f": bfm, "bisp_r": brm}
 
 
 def bfm_points(prob, fi, fj, with_brm: bool = True, bound: Bound = None):

@@ -712,6 +712,40 @@ int bsahip_classic_run(const bsahip_problem *p, int32_t nfh, double df, int32_t 
 
 namespace {
 
+// Tables of the ONLY_DIAG model on the half grid u df, u < nh: X (K, nh) the
+// nodal PSDs, T (nm, nh, nh) the per-mode GEMM T_m = S W_m S^T (when with_T),
+// H (nm, 2 nh - 1) the FRFs.
+int diag_tables(Bufs &B, const Prob &D, int nh, double df, bool with_T, double **X, double **T,
+                double2 **H)
+{
+   CHK(blas_handle());
+   const int nft = 2 * nh - 1, nm = D.nm, K = D.K;
+   std::vector<double> fl(nh);
+   for (int u = 0; u < nh; ++u) fl[u] = (double)u * df;
+   double *dfl, *sq, *Y;
+   CHK(upload(B, "fl", fl, &dfl));
+   CHK(B.get("sq", (size_t)D.ntc * nh * D.nnl, &sq));
+   CHK(B.get("dg_X", (size_t)K * nh, X));
+   CHK(B.get("dg_Y", (size_t)nm * K * nh, &Y));
+   CHK(B.get("dg_H", (size_t)nm * nft, H));
+   dim3 gp((D.nnl + 63) / 64, nh, D.ntc);
+   k_psd_sqrt<<<gp, 64, 0, g.s>>>(dfl, nh, D.prm, D.formula, D.nnl, D.ntc, D.pulsation, sq);
+   HIPC(hipGetLastError());
+   k_diag_tables<<<dim3((unsigned)((nh + 63) / 64), (unsigned)K), 64, 0, g.s>>>(
+      sq, nh, D.nnl, D.nnlp, D.pa_k, D.pb_k, K, nm, *X, Y);
+   HIPC(hipGetLastError());
+   CHK(launch_frf(D, nft, df, *H));
+   *T = nullptr;
+   if (with_T) {
+      CHK(B.get("dg_T", (size_t)nm * nh * nh, T));
+      const double one = 1.0, zero = 0.0;
+      BLASC(rocblas_dgemm_strided_batched(g.blas, rocblas_operation_none, rocblas_operation_transpose,
+                                          nh, nh, K, &one, *X, nh, 0, Y, nh, (rocblas_stride)K * nh,
+                                          &zero, *T, nh, (rocblas_stride)nh * nh, nm));
+   }
+   return 0;
+}
+
 // The ONLY_DIAG Classic model (diag.hip): outputs have NM entries each.
 int classic_diag(const bsahip_problem *p, int32_t nfh, double df, int32_t flags, double *m2mf,
                  double *m2mr, double *m2o2mr, double *m3mf, double *m3mr)
@@ -722,35 +756,16 @@ int classic_diag(const bsahip_problem *p, int32_t nfh, double df, int32_t flags,
    Bufs &B = scratch();
    Prob D;
    CHK(load_problem(p, 0, B, D));
-   CHK(blas_handle());
    const int nh = nfh, nf = 2 * nh - 1, nft = 2 * nh - 1, nm = D.nm, K = D.K;
    const int rule = (flags & BSAHIP_QUAD_MASK) >> 4;
    const double d = p->pulsation ? df * 2.0 * M_PI : df;
-   std::vector<double> fl(nh);
-   for (int u = 0; u < nh; ++u) fl[u] = (double)u * df;
-   double *dfl, *sq, *X, *Y, *T, *mom;
+   double *X, *T, *mom;
    double2 *H;
-   CHK(upload(B, "fl", fl, &dfl));
-   CHK(B.get("sq", (size_t)D.ntc * nh * D.nnl, &sq));
-   CHK(B.get("dg_X", (size_t)K * nh, &X));
-   CHK(B.get("dg_Y", (size_t)nm * K * nh, &Y));
-   CHK(B.get("dg_T", (size_t)nm * nh * nh, &T));
-   CHK(B.get("dg_H", (size_t)nm * nft, &H));
+   CHK(diag_tables(B, D, nh, df, !(flags & BSAHIP_NO_BISP), &X, &T, &H));
    CHK(B.get("dg_mom", 5 * (size_t)nm, &mom));
-   dim3 gp((D.nnl + 63) / 64, nh, D.ntc);
-   k_psd_sqrt<<<gp, 64, 0, g.s>>>(dfl, nh, D.prm, D.formula, D.nnl, D.ntc, D.pulsation, sq);
-   HIPC(hipGetLastError());
-   k_diag_tables<<<dim3((unsigned)((nh + 63) / 64), (unsigned)K), 64, 0, g.s>>>(
-      sq, nh, D.nnl, D.nnlp, D.pa_k, D.pb_k, K, nm, X, Y);
-   HIPC(hipGetLastError());
-   CHK(launch_frf(D, nft, df, H));
    CHK(fill(mom, 5 * (size_t)nm, 0.0));
    double *m2f = mom, *m2r = mom + nm, *m2o = mom + 2 * nm, *m3f = mom + 3 * nm, *m3r = mom + 4 * nm;
    if (!(flags & BSAHIP_NO_BISP)) {
-      const double one = 1.0, zero = 0.0;
-      BLASC(rocblas_dgemm_strided_batched(g.blas, rocblas_operation_none, rocblas_operation_transpose,
-                                          nh, nh, K, &one, X, nh, 0, Y, nh, (rocblas_stride)K * nh,
-                                          &zero, T, nh, (rocblas_stride)nh * nh, nm));
       k_diag_m3<<<dim3((unsigned)nf, (unsigned)nm), 256, 0, g.s>>>(T, nh, H, nft, rule, d * d, m3f, m3r);
       HIPC(hipGetLastError());
    }
@@ -903,6 +918,62 @@ int bsahip_classic_spectra(const bsahip_problem *p, int32_t nfh, double df, int3
       if (bfm) HIPC(hipMemcpyAsync(bfm, ob, nm3 * npts * sizeof(double), hipMemcpyDeviceToHost, g.s));
       if (brm) HIPC(hipMemcpyAsync(brm, orr, nm3 * npts * sizeof(double), hipMemcpyDeviceToHost, g.s));
    }
+   HIPC(hipStreamSynchronize(g.s));
+   return 0;
+}
+
+int bsahip_classic_bispectra_sel(const bsahip_problem *p, int32_t nfh, double df, int32_t flags,
+                                 int32_t nsel, const int32_t *sel, double *bfm, double *brm)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   CHK(ensure_init());
+   CHK(check_flags(flags, true, false));
+   if (nfh < 2 || !(df > 0.0)) return fail(BSAHIP_EINVAL, "need nfh >= 2 and df > 0");
+   if (nsel < 0 || nsel > 65535 || (nsel > 0 && !sel))
+      return fail(BSAHIP_EINVAL, "nsel outside 0..65535 or NULL selection");
+   const bool diag = (flags & BSAHIP_ONLY_DIAG) != 0, exact = (flags & BSAHIP_SUM_EXACT) != 0;
+   if (diag && exact)
+      return fail(BSAHIP_EUNSUPPORTED, "the ONLY_DIAG model pads S(f1+f2) with zeros (no SUM_EXACT)");
+   if (nsel == 0 || (!bfm && !brm)) return 0;
+   if (nfh > 46341) return fail(BSAHIP_EINVAL, "nfh too large for one selected bispectrum");
+   Bufs &B = scratch();
+   Prob D;
+   CHK(load_problem(p, 0, B, D));
+   const long ncol = diag ? D.nm : (long)D.nm * D.nm * D.nm;
+   std::vector<int> s0(nsel);
+   for (int s = 0; s < nsel; ++s) {
+      if (sel[s] < 1 || sel[s] > ncol)
+         return fail(BSAHIP_EINVAL, "selection index out of 1..NM^3 (1..NM with ONLY_DIAG)");
+      s0[s] = sel[s] - 1;
+   }
+   const int nh = nfh, nf = 2 * nh - 1;
+   const size_t npts = (size_t)nf * nf, nout = npts * (size_t)nsel;
+   int *dsel;
+   double *ob = nullptr, *orr = nullptr;
+   CHK(upload(B, "sel", s0, &dsel));
+   if (bfm) CHK(B.get("bisp_sel_f", nout, &ob));
+   if (brm) CHK(B.get("bisp_sel_r", nout, &orr));
+   const dim3 gs((unsigned)((npts + 255) / 256), (unsigned)nsel);
+   if (diag) {
+      double *X, *T;
+      double2 *H;
+      CHK(diag_tables(B, D, nh, df, true, &X, &T, &H));
+      k_diag_bisp_sel<<<gs, 256, 0, g.s>>>(T, nh, H, 2 * nh - 1, dsel, ob, orr);
+   } else {
+      const int nu = exact ? nf : nh;
+      const long ldU = (long)D.nm * nu;
+      std::vector<double> fl(nu);
+      for (int u = 0; u < nu; ++u) fl[u] = (double)u * df;
+      double *dfl, *U;
+      CHK(upload(B, "fl", fl, &dfl));
+      CHK(B.get("U", (size_t)D.K * ldU, &U));
+      CHK(build_table(B, D, dfl, nu, nu, U, ldU));
+      k_bisp_grid_sel<<<gs, 256, 0, g.s>>>(U, ldU, nu, D.pb_k, D.K, D.nm, nf, exact ? 1 : 0, df,
+                                           dsel, D.Mm, D.Km, D.Cm, ob, orr);
+   }
+   HIPC(hipGetLastError());
+   if (bfm) HIPC(hipMemcpyAsync(bfm, ob, nout * sizeof(double), hipMemcpyDeviceToHost, g.s));
+   if (brm) HIPC(hipMemcpyAsync(brm, orr, nout * sizeof(double), hipMemcpyDeviceToHost, g.s));
    HIPC(hipStreamSynchronize(g.s));
    return 0;
 }

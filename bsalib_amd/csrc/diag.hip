@@ -104,3 +104,29 @@ __global__ void k_diag_m2(const double *__restrict__ X, int nh, const double *__
       if (m2o) atomicAdd(m2o + m, so);
    }
 }
+
+// Diagonal bispectra of selected modes on the Classic grid from T, the terms
+// of k_diag_m3: out[q1 + nf*q2 + nf*nf*s] for mode sel[s] (the reference's
+// bisp(i, j, m) of getFM_diag_tnlm_vect_cls_ / getRM_diag_vect_cls_).
+// grid (ceil(nf*nf/256), nsel), block 256
+__global__ __launch_bounds__(256) void k_diag_bisp_sel(const double *__restrict__ T, int nh,
+                                                       const double2 *__restrict__ H, int nft,
+                                                       const int *__restrict__ sel,
+                                                       double *__restrict__ bfm,
+                                                       double *__restrict__ brm)
+{
+   const int n = nh - 1, nf = 2 * nh - 1;
+   const long npts = (long)nf * nf;
+   const long pt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (pt >= npts) return;
+   const int m = sel[blockIdx.y];
+   const double *Tm = T + (size_t)m * nh * nh;
+   const int F1 = (int)(pt % nf) - n, F2 = (int)(pt / nf) - n, s = F1 + F2;
+   const int a1 = abs(F1), a2 = abs(F2), as = abs(s);
+   double b = Tm[a1 + (size_t)a2 * nh];
+   if (as <= n) b += Tm[as + (size_t)a2 * nh] + Tm[as + (size_t)a1 * nh];
+   b *= 2.0;
+   const size_t o = (size_t)blockIdx.y * npts + pt;
+   if (bfm) bfm[o] = b;
+   if (brm) brm[o] = b * re3(hget(H, nft, m, F1), hget(H, nft, m, F2), hget(H, nft, m, s));
+}

@@ -843,6 +843,46 @@ __global__ void k_psd_grid(const double *__restrict__ U, long ldU, const double 
 }
 
 // ---------------------------------------------------------------------------
+// Modal force bispectrum of triple (m, n, p) at one point from the projection
+// table U: rows a1 -> |f1|, a2 -> |f2|, a3 -> |f1+f2| (a3 < 0: S(f1+f2) = 0).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double bfm_point(const double *__restrict__ U, long ldU, int rows,
+                                            const double *__restrict__ pb, int K, int m, int n,
+                                            int p, int a1, int a2, int a3)
+{
+   double s = 0.0;
+   for (int k = 0; k < K; ++k) {
+      const double *row = U + (size_t)k * ldU;
+      const double An1 = row[(size_t)n * rows + a1], Ap2 = row[(size_t)p * rows + a2];
+      double v = pb[(size_t)m * K + k] * An1 * Ap2;
+      if (a3 >= 0) {
+         const double Am3 = row[(size_t)m * rows + a3];
+         v += pb[(size_t)n * K + k] * Am3 * Ap2 + pb[(size_t)p * K + k] * Am3 * An1;
+      }
+      s += v;
+   }
+   return 2.0 * s;
+}
+
+// Response factor Re[H_m(f1) H_n(f2) conj H_p(f1+f2)] from the modal mass,
+// stiffness and damping; H at f1+f2 uses w1 + w2 as the reference does
+// (functionsImpl.F90:1659-1663).
+__device__ __forceinline__ double brm_factor(double f1, double f2, int m, int n, int p,
+                                             const double *__restrict__ Mm,
+                                             const double *__restrict__ Km,
+                                             const double *__restrict__ Cm)
+{
+   auto H = [&](int mo, double w) {
+      const double r = -(w * w * Mm[mo]) + Km[mo];
+      const double im = w * Cm[mo];
+      const double d = r * r + im * im;
+      return make_double2(r / d, -im / d);
+   };
+   const double w1 = 2.0 * M_PI * f1, w2 = 2.0 * M_PI * f2;
+   return re3(H(m, w1), H(n, w2), H(p, w1 + w2));
+}
+
+// ---------------------------------------------------------------------------
 // Per-point bispectrum: B(m,n,p) at points given by row indices into U
 // (i1 -> f1, i2 -> f2, i3 -> f1+f2, i3 < 0 means S(f1+f2) = 0) and the
 // signed frequencies for the FRF.  out index = pt*spt + mnp*smn.
@@ -860,36 +900,36 @@ __global__ __launch_bounds__(256) void k_bfm_points(
    const long pt = gid / nm3;
    const int mnp = (int)(gid % nm3);
    const int m = mnp % nm, n = (mnp / nm) % nm, p = mnp / (nm * nm);
-   const int a1 = i1[pt], a2 = i2[pt], a3 = i3[pt];
-   double s = 0.0;
-   for (int k = 0; k < K; ++k) {
-      const double *row = U + (size_t)k * ldU;
-      const double An1 = row[(size_t)n * rows + a1], Ap2 = row[(size_t)p * rows + a2];
-      double v = pb[(size_t)m * K + k] * An1 * Ap2;
-      if (a3 >= 0) {
-         const double Am3 = row[(size_t)m * rows + a3];
-         v += pb[(size_t)n * K + k] * Am3 * Ap2 + pb[(size_t)p * K + k] * Am3 * An1;
-      }
-      s += v;
-   }
-   s *= 2.0;
+   const double s = bfm_point(U, ldU, rows, pb, K, m, n, p, i1[pt], i2[pt], i3[pt]);
    if (bfm) bfm[pt * spt + mnp * smn] = s;
-   if (brm) {
-      const double f1 = f1s[pt], f2 = f2s[pt];
-      auto H = [&](int mo, double f) {
-         const double w = 2.0 * M_PI * f;
-         const double r = -(w * w * Mm[mo]) + Km[mo];
-         const double im = w * Cm[mo];
-         const double d = r * r + im * im;
-         return make_double2(r / d, -im / d);
-      };
-      const double w1 = 2.0 * M_PI * f1, w2 = 2.0 * M_PI * f2;
-      // H at f1+f2 uses w1 + w2 as the reference does (functionsImpl.F90:1659-1663)
-      const double w3 = w1 + w2;
-      const double r3 = -(w3 * w3 * Mm[p]) + Km[p], i3v = w3 * Cm[p], d3 = r3 * r3 + i3v * i3v;
-      const double2 h3 = make_double2(r3 / d3, -i3v / d3);
-      brm[pt * spt + mnp * smn] = s * re3(H(m, f1), H(n, f2), h3);
-   }
+   if (brm) brm[pt * spt + mnp * smn] = s * brm_factor(f1s[pt], f2s[pt], m, n, p, Mm, Km, Cm);
+}
+
+// ---------------------------------------------------------------------------
+// Bispectra of selected mode triples on the Classic grid f = (q - c) df,
+// q = 0 .. nf-1: out[q1 + nf*q2 + nf*nf*s] for triple sel[s] (linear index
+// m + n*nm + p*nm^2, m fastest), the reference's bisp(i, j, k) layout.
+// Rows of U: |F| for F within the grid; with exact, |f1+f2| up to 2c as well,
+// otherwise S(f1+f2) = 0 beyond the grid.  grid (ceil(nf*nf/256), nsel)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bisp_grid_sel(
+   const double *__restrict__ U, long ldU, int rows, const double *__restrict__ pb, int K,
+   int nm, int nf, int exact, double df, const int *__restrict__ sel,
+   const double *__restrict__ Mm, const double *__restrict__ Km, const double *__restrict__ Cm,
+   double *__restrict__ bfm, double *__restrict__ brm)
+{
+   const long npts = (long)nf * nf;
+   const long pt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (pt >= npts) return;
+   const int c = (nf - 1) / 2;
+   const int mnp = sel[blockIdx.y];
+   const int m = mnp % nm, n = (mnp / nm) % nm, p = mnp / (nm * nm);
+   const int F1 = (int)(pt % nf) - c, F2 = (int)(pt / nf) - c, F3 = F1 + F2;
+   const int a3 = (exact || abs(F3) <= c) ? abs(F3) : -1;
+   const double s = bfm_point(U, ldU, rows, pb, K, m, n, p, abs(F1), abs(F2), a3);
+   const size_t o = (size_t)blockIdx.y * npts + pt;
+   if (bfm) bfm[o] = s;
+   if (brm) brm[o] = s * brm_factor(F1 * df, F2 * df, m, n, p, Mm, Km, Cm);
 }
 
 // Lane-layout self test of v_mfma_f64_16x16x4f64 (one wave, 512 doubles out):

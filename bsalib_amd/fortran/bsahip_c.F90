@@ -35,6 +35,7 @@ module BsaHip_C
    end type bsahip_mesher_cfg_t
 
    public :: bsahip_init, bsahip_finalise, bsahip_classic_run, bsahip_bfm_points
+   public :: bsahip_classic_bispectra_sel
    public :: bsahip_mesher_create, bsahip_mesher_run, bsahip_mesher_destroy
    public :: bsahip_last_error_f, bsahip_modal_psd_points, bsahip_modal_recombination
 
@@ -55,6 +56,15 @@ module BsaHip_C
          integer(c_int32_t), value :: nfh, flags
          real(c_double), value :: df
          type(c_ptr), value :: m2mf, m2mr, m2o2mr, m3mf, m3mr
+      end function
+
+      integer(c_int) function bsahip_classic_bispectra_sel(p, nfh, df, flags, nsel, sel, bfm, brm) &
+            bind(c, name="bsahip_classic_bispectra_sel")
+         import c_int, c_int32_t, c_double, c_ptr, bsahip_problem_t
+         type(bsahip_problem_t), intent(in) :: p
+         integer(c_int32_t), value :: nfh, flags, nsel
+         real(c_double), value :: df
+         type(c_ptr), value :: sel, bfm, brm
       end function
 
       integer(c_int) function bsahip_bfm_points(p, npts, fi, fj, flags, bfm, brm) &

@@ -32,7 +32,7 @@
 !!    split-bf16x3 node contraction (FP32 accumulation on matrix cores,
 !!    FP64 everywhere else); the default is FP64.
 module BsaLib
-   use, intrinsic :: iso_fortran_env, only: int32, int64, real64, output_unit
+   use, intrinsic :: iso_fortran_env, only: int32, int64, real32, real64, output_unit
    use, intrinsic :: iso_c_binding
    use BsaHip_C
    implicit none
@@ -68,6 +68,11 @@ module BsaLib
    ! input file names and the constants programs written against the reference use
    character(len=*), parameter, public :: BSA_DATA_FNAME = 'bsa.bsadata'
    character(len=*), parameter, public :: EXT_DATA_FNAME = 'bsa.extdata'
+   ! bispectrum export files (src/CONSTANTS/_CONSTANTS.F90:165-168)
+   character(len=*), parameter, public :: BRM_EXPORT_FNAME_CLS = 'bsaexport_cls.brm'
+   character(len=*), parameter, public :: BRN_EXPORT_FNAME_CLS = 'bsaexport_cls.brn'
+   character(len=*), parameter, public :: BRM_EXPORT_FNAME_MSH = 'bsaexport_msh.brm'
+   character(len=*), parameter, public :: BRN_EXPORT_FNAME_MSH = 'bsaexport_msh.brn'
    character(len=0), parameter, public :: BSA_FILE_NAME_CL_SUFFIX = ''
    character(len=*), parameter, public :: IO_ACTION_READ = 'READ'
    character(len=*), parameter, public :: IO_ACTION_WRITE = 'WRITE'
@@ -158,6 +163,10 @@ module BsaLib
    ! ---- exports ----
    character(len=:), allocatable :: exp_dir_
    logical :: exp_binary_ = .false., exp_append_ = .false., visual_ = .false.
+   ! visual mode: indexes and modal (BRM) or nodal (BRN) export
+   ! (src/bsa/data/BsaLibData.F90:87-100)
+   integer(bsa_int_t) :: vis_idx_(3) = 0
+   logical :: vis_modal_ = .true.
 
    ! ---- problem of the last bsa_Run, kept for the post-processing calls ----
    type(bsahip_problem_t), save :: lp_
@@ -284,16 +293,20 @@ contains
       itest_ = itest
    end subroutine
 
-   !> Accepted values as src/bsa/BsaLibImpl.F90 (bsa_setSpatialSymmetry); the
-   !> Classic approach does not use it.
+   !> Accepted values and the fallback to HALF for any other value as
+   !> src/bsa/BsaLibImpl.F90:750-767 (bsa_setSpatialSymmetry).  The device
+   !> paths do not use it; it is written to the input files and to the header
+   !> of the bispectrum export file.
    subroutine bsa_setSpatialSymmetry(isym)
       integer(bsa_int_t), value :: isym
-      if (isym == BSA_SPATIAL_SYM_NONE .or. isym == BSA_SPATIAL_SYM_HALF .or. &
-          isym == BSA_SPATIAL_SYM_FOUR) then
+      if (isym == BSA_SPATIAL_SYM_NONE .or. isym == BSA_SPATIAL_SYM_FOUR) then
          isym_ = isym
       else
-         write(output_unit, '(1x, 2a, i0)') WARNMSG, 'Unsupported spatial symmetry value ', isym
-         isym_ = BSA_SPATIAL_SYM_NONE
+         if (isym /= BSA_SPATIAL_SYM_HALF) then
+            write(output_unit, '(1x, 2a, i0, a)') WARNMSG, 'Invalid   ', isym, '  spatial symmetry value.'
+            write(output_unit, '(1x, 2a)') MSGCONT, 'Setting default (HALF).'
+         end if
+         isym_ = BSA_SPATIAL_SYM_HALF
       end if
    end subroutine
 
@@ -642,19 +655,26 @@ contains
          write(output_unit, '(1x, 2a)') WARNMSG, 'Both  PSD  and  BISP  computation are disabled.'
          return
       end if
-      if (visual_) call bsa_Abort('Visual mode (bispectra re-read from a dump file) is not available in this library.')
-      if (gpu_enabled_) isuban_ = 1
+      ! the reference's GPU switch forces the Classic approach outside visual mode
+      ! (src/bsa/BsaLibImpl.F90:406-412)
+      if (gpu_enabled_ .and. .not. visual_) isuban_ = 1
       if (isuban_ < 1 .or. isuban_ > 3) call bsa_Abort('Analysis type must be 1, 2 or 3.')
       do_cls = isuban_ == 1 .or. isuban_ == 3
       do_msh = isuban_ == 2 .or. isuban_ == 3
       if (ionly_diag_ /= 0 .and. do_msh) &
          call bsa_Abort('The ONLY_DIAG model is available for the Classic approach only in this library.')
+      if (visual_ .and. do_msh) call bsa_Abort('Visual mode with the Mesher re-reads the BFM dump file '// &
+         'of a previous run; this library keeps Mesher zones on the device and has no dump file.')
       if (iconv_ /= BSA_PSD_CONVENTION_PULS) &
          call bsa_Abort('Only the pulsation convention (BSA_PSD_CONVENTION_PULS) is available in this library.')
 
       ! ---- structure checks (validateAll_, BsaLibImpl.F90:468-541) ----
       if (.not. allocated(phi_)) call bsa_Abort('Modal matrix was not acquired. Aborting.')
       if (ndofs_ == 0) ndofs_ = nn_ * nlibs_
+      if (visual_ .and. .not. vis_modal_) then         ! BsaLibImpl.F90:202-207
+         if (vis_idx_(1) > nn_) call bsa_Abort('Node index exceeds max n. of nodes')
+         if (vis_idx_(2) > nlibs_) call bsa_Abort('DOF index exceeds max n. of DOFs per node')
+      end if
       if (ndofs_ /= size(phi_, 1)) call bsa_Abort('NDOFs does not match modal matrix size.')
       if (.not. allocated(n_load_)) n_load_ = [(i, i = 1, nn_)]
       if (.not. allocated(libs_load_)) libs_load_ = [(i, i = 1, nlibs_)]
@@ -816,6 +836,20 @@ contains
       write(output_unit, '(1x, 2a, i0, a, i0, a, a)') INFOMSG, 'Classic run on the GPU: ', &
          nme, ' modes, ', 2 * nfh - 1, '^2 frequency points, ', &
          merge('SCALAR    ', 'VECTORISED', iclassic == BSA_CLASSIC_MODE_SCALAR)
+      if (visual_) then
+         ! the reference exports the modal BRM of the vectorised version only
+         ! (BsaClassicImpl.F90:158); its other visual cases re-read an empty
+         ! export file (writeNTotPtsToBispUn_, BsaLibImpl.F90:1832-1845)
+         if (.not. vis_modal_) then
+            write(output_unit, '(1x, 2a)') WARNMSG, &
+               'Nodal (BRN) bispectrum export is not defined for the Classic approach; no file written.'
+         else if (icomp_bisp_ == 0 .or. iclassic /= BSA_CLASSIC_MODE_VECTOR) then
+            write(output_unit, '(1x, 2a)') WARNMSG, &
+               'BRM export needs the bispectrum and the VECTORISED Classic version; no file written.'
+         else
+            call exportVisualCls_(p, nfh, df, nme)
+         end if
+      end if
    contains
       subroutine runMesher_(p, m3mf, m3mr, tscl, umref)
          type(bsahip_problem_t), intent(in) :: p
@@ -1315,15 +1349,72 @@ contains
          write(output_unit, '(1x, 2a)') WARNMSG, 'Bispectra export is not available in this library; ignored.'
    end subroutine
 
-   !> Visual mode re-reads bispectra from a dump file; bsa_Run aborts with a
-   !> message when it is enabled.
+   !> Visual mode (src/bsa/BsaLibImpl.F90:119-122): with the Classic approach,
+   !> bsa_Run also writes the response bispectrum of the modal entry given by
+   !> bsa_setVisualIndexes over the whole frequency plane to
+   !> BRM_EXPORT_FNAME_CLS.  With the Mesher the reference re-reads a BFM dump
+   !> file; bsa_Run aborts with a message for that case.
    subroutine bsa_enableVisualMode()
       visual_ = .true.
    end subroutine
 
+   !> Modal (i, j, k) kept-mode positions (modal = .true.), or node and DOF
+   !> (modal = .false.) as src/bsa/BsaLibImpl.F90:134-141.
    subroutine bsa_setVisualIndexes(indexes, modal)
       integer(bsa_int_t), intent(in) :: indexes(3)
       logical, value :: modal
+      vis_idx_ = indexes
+      vis_modal_ = modal
+   end subroutine
+
+   !> Visual-mode export of the Classic approach (BsaClassicImpl.F90:158-194
+   !> with exportBRM_baseHeaderWriter_internal_ / exportBRM_base_internal_ and
+   !> writeNTotPtsToBispUn_, BsaLibImpl.F90:1832-1913): an unformatted stream of
+   !> int32 header values -- 3, the three visual indexes, 1 (combinations), the
+   !> spatial symmetry setting, 1 (zones), NF^2 (points) -- the zone header
+   !> 1 (rectangle), NF, NF, then for j = 1..NF, i = 1..NF the real32 triple
+   !> f(i), f(j), BRM(f(i), f(j)) of the modal entry (its diagonal-model entry
+   !> with the ONLY_DIAG model).  The values come from the device in FP64.
+   subroutine exportVisualCls_(p, nfh, df, nme)
+      type(bsahip_problem_t), intent(in) :: p
+      integer(bsa_int_t), intent(in) :: nfh, nme
+      real(bsa_real_t), intent(in) :: df
+      integer(c_int32_t), target :: sel(1)
+      real(bsa_real_t), allocatable, target :: brm(:, :)
+      real(real32), allocatable :: rec(:, :)
+      integer(c_int32_t) :: fl
+      integer(c_int) :: rc
+      integer :: iun, ios, i, j, nf
+      nf = 2 * nfh - 1
+      fl = 0
+      if (ionly_diag_ /= 0) then
+         fl = BSAHIP_ONLY_DIAG
+         if (vis_idx_(1) < 1 .or. vis_idx_(1) > nme) call bsa_Abort('Visual mode index out of the kept modes.')
+         sel(1) = int(vis_idx_(1), c_int32_t)
+      else
+         if (any(vis_idx_ < 1) .or. any(vis_idx_ > nme)) call bsa_Abort('Visual mode indexes out of the kept modes.')
+         sel(1) = int((vis_idx_(3) - 1) * nme * nme + (vis_idx_(2) - 1) * nme + vis_idx_(1), c_int32_t)
+      end if
+      allocate(brm(nf, nf))
+      rc = bsahip_classic_bispectra_sel(p, int(nfh, c_int32_t), real(df, c_double), fl, 1_c_int32_t, &
+                                        c_loc(sel), c_null_ptr, c_loc(brm))
+      if (rc /= 0) call bsa_Abort('bsahip_classic_bispectra_sel failed: '//bsahip_last_error_f())
+      write(output_unit, '( /, 1x, 2a, i0, " x ", i0, a )') INFOMSG, 'Exporting  BRM  to file...  (', nf, nf, ')'
+      open(newunit=iun, file=BRM_EXPORT_FNAME_CLS, form='unformatted', access='stream', &
+           status='replace', action='readwrite', iostat=ios)
+      if (ios /= 0) call bsa_Abort('Error opening file  '//BRM_EXPORT_FNAME_CLS)
+      write(iun) 3_int32, int(vis_idx_, int32), 1_int32, int(isym_, int32), 1_int32, int(nf * nf, int32)
+      write(iun) 1_int32, int(nf, int32), int(nf, int32)
+      allocate(rec(3, nf))
+      do i = 1, nf
+         rec(1, i) = real(real(i - nfh, bsa_real_t) * df, real32)
+      end do
+      do j = 1, nf
+         rec(2, :) = real(real(j - nfh, bsa_real_t) * df, real32)
+         rec(3, :) = real(brm(:, j), real32)
+         write(iun) rec
+      end do
+      close(iun)
    end subroutine
 
    function openExport_(fname) result(iun)

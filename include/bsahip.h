@@ -136,11 +136,25 @@ int bsahip_plan_destroy(bsahip_plan *plan);
  * Bispectra on the Classic grid in the reference layout bisp(NF, NF, NM^3):
  * bfm = getFM_full_tnm_vect_cls_ output, brm = after getRM_full_vect_cls_
  * (functionsImpl.F90:1312-1738).  psd_f/psd_r: (NF, NM^2) modal PSDs.
- * Any output may be NULL.  Meant for parity checks and visual export; the
+ * Any output may be NULL.  Meant for parity checks on small grids; the
  * moments path (bsahip_classic_run) never materialises them.
  */
 int bsahip_classic_spectra(const bsahip_problem *p, int32_t nfh, double df, int32_t flags,
                            double *psd_f, double *psd_r, double *bfm, double *brm);
+
+/*
+ * Bispectra of selected modal entries on the Classic grid, the data of the
+ * reference's visual-mode export (src/bsa/classic/BsaClassicImpl.F90:158-194,
+ * exportBRM_base_internal_ in src/bsa/BsaLibImpl.F90:1876-1913):
+ * bfm, brm (NF, NF, nsel), column s = bisp(:, :, sel[s]) of the reference.
+ * sel is 1-based: m + (n-1) NM + (p-1) NM^2 over the kept modes (visual_idx_,
+ * BsaLibImpl.F90:387-393); with BSAHIP_ONLY_DIAG the mode position 1..NM and the
+ * diagonal model (getFM_diag_tnlm_vect_cls_ / getRM_diag_vect_cls_).
+ * BSAHIP_SUM_EXACT as bsahip_classic_spectra (full model only).  FP64;
+ * nsel at most 65535.
+ */
+int bsahip_classic_bispectra_sel(const bsahip_problem *p, int32_t nfh, double df, int32_t flags,
+                                 int32_t nsel, const int32_t *sel, double *bfm, double *brm);
 
 /*
  * Modal bispectra at an arbitrary list of points (fi[q], fj[q]), S(f1+f2)

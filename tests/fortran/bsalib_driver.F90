@@ -4,6 +4,7 @@
 !!
 !! Usage:  bsalib_driver <input.bin> <output.bin> [fp64|bf16x3] [grid] [scalar] [mesher|both]
 !!                        [pod] [podn=<modes>] [podt=<percent>] [post] [gen] [diag]
+!!                        [visual=<i>,<j>,<k>] [visualn=<node>,<dof>] [sym=<n>]
 !! The input layout is the one tests write for the reference harness (header
 !! of 14 int32, then integer and real64 arrays in Fortran order).  "grid"
 !! leaves test mode off and passes damping ratios, so bsa_Run applies the
@@ -17,7 +18,10 @@
 !! Gaussian / positive / negative peak factors of both.  "gen" calls
 !! bsa_generateBSAInputFiles(.false.): bsa_Run writes bsa.bsadata and
 !! bsa.extdata in the current directory and returns (no device needed);
-!! "diag" selects the ONLY_DIAG model (bsa_setSpectraExtension(1)).
+!! "diag" selects the ONLY_DIAG model (bsa_setSpectraExtension(1)); "visual="
+!! enables visual mode with modal indexes (i, j, k), "visualn=" with a node and
+!! DOF (bsa_enableVisualMode, bsa_setVisualIndexes); "sym=" calls
+!! bsa_setSpatialSymmetry.
 !! Output: nme (int32), the used modes, then for each of m2mf, m2mr, m2o2mr,
 !! m3mf_cls, m3mr_cls, m3mf_msh, m3mr_msh: its size (int32, 0 when bsa_Run
 !! left it unallocated) and its values (real64).
@@ -28,6 +32,7 @@ program bsalib_driver
 
    integer(int32) :: hdr(14)
    integer(int32) :: nn, nlibs, nnl, nlibsl, nm, nme, ntc, ndegw, nwz, psdid, nfh, ncorr, npts, i
+   integer(int32) :: vis(3), isym
    integer(int32), allocatable :: n_load(:), libs_load(:), modes(:), tcs(:), wz(:), used(:)
    real(real64) :: df
    real(real64), allocatable, target :: phi(:, :), natf(:), Mm(:), Km(:), Cm(:, :), unod(:)
@@ -81,6 +86,21 @@ program bsalib_driver
       if (trim(arg) == 'post') post = .true.
       if (trim(arg) == 'gen') call bsa_generateBSAInputFiles(.false.)
       if (trim(arg) == 'diag') call bsa_setSpectraExtension(1)
+      if (arg(1:7) == 'visual=') then
+         read(arg(8:), *) vis
+         call bsa_enableVisualMode()
+         call bsa_setVisualIndexes(vis, .true.)
+      end if
+      if (arg(1:8) == 'visualn=') then
+         vis = 0
+         read(arg(9:), *) vis(1:2)
+         call bsa_enableVisualMode()
+         call bsa_setVisualIndexes(vis, .false.)
+      end if
+      if (arg(1:4) == 'sym=') then
+         read(arg(5:), *) isym
+         call bsa_setSpatialSymmetry(isym)
+      end if
       if (arg(1:5) == 'podn=') then
          read(arg(6:), *) npod
          call bsa_setPODNOfModesKept(npod)

@@ -1,0 +1,138 @@
+"""Bispectra of selected modal entries on the Classic grid and the visual-mode
+export built on them.
+
+* bsahip_classic_bispectra_sel equals the reference's own bispectra (golden
+  bisp_f / bisp_r written by the reference harness: the whole grid of the tiny
+  cases, 300 sampled points of the 20-node tower) -- 1e-12; its ONLY_DIAG form
+  equals the reference's diagonal-model bispectra; with SUM_EXACT it equals
+  the columns of bsahip_classic_spectra.
+* The Fortran front in visual mode writes bsaexport_cls.brm in the reference's
+  stream layout: header values, the grid f = i df, and real32 values equal
+  to the reference's BRM rounded to real32 (within one real32 spacing: the
+  device value and the reference differ by ~1e-15 relative before rounding).
+"""
+import numpy as np
+import pytest
+
+from bsalib_amd import abi, exports
+from tests import fdriver
+from tests.cases import CASES
+from tests.conftest import load_golden, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TINY = ["tiny_u", "tiny_uvw", "tiny_dav"]
+
+
+def _nm(g):
+    return int(round(g["m3mf"].size ** (1 / 3)))
+
+
+def _lin(nm, i, j, k):
+    return i + (j - 1) * nm + (k - 1) * nm * nm
+
+
+@pytest.mark.parametrize("name", TINY)
+def test_selected_bispectra_equal_reference(gpu_lib, name):
+    p = CASES[name]()
+    g = load_golden(name)
+    nm = _nm(g)
+    sel = sorted({1, nm ** 3, _lin(nm, nm, 1, min(2, nm)), _lin(nm, 1, nm, 1)})
+    r = abi.classic_bispectra_sel(p, sel)
+    for c, s in enumerate(sel):
+        assert rel_err(r["bisp_f"][:, :, c], g["bisp_f"][:, :, s - 1]) < 1e-12, s
+        assert rel_err(r["bisp_r"][:, :, c], g["bisp_r"][:, :, s - 1]) < 1e-12, s
+    d = abi.classic_bispectra_sel(p, list(range(1, nm + 1)), only_diag=True)
+    assert rel_err(d["bisp_f"], g["bisp_f_diag"]) < 1e-12
+    assert rel_err(d["bisp_r"], g["bisp_r_diag"]) < 1e-12
+
+
+def test_selected_bispectra_tower_sample(gpu_lib):
+    p = CASES["config1"]()
+    g = load_golden("config1")
+    nm = _nm(g)
+    sel = list(range(1, nm ** 3 + 1))
+    r = abi.classic_bispectra_sel(p, sel)
+    i1, i2 = g["sample_i1"], g["sample_i2"]
+    assert rel_err(r["bisp_f"][i1, i2, :], g["bisp_f_sample"]) < 1e-12
+    assert rel_err(r["bisp_r"][i1, i2, :], g["bisp_r_sample"]) < 1e-12
+    d = abi.classic_bispectra_sel(p, list(range(1, nm + 1)), only_diag=True)
+    assert rel_err(d["bisp_f"][i1, i2, :], g["bisp_f_diag_sample"]) < 1e-12
+    assert rel_err(d["bisp_r"][i1, i2, :], g["bisp_r_diag_sample"]) < 1e-12
+
+
+def test_selected_bispectra_exact_equal_full_spectra(gpu_lib):
+    p = CASES["tiny_uvw"]()
+    full = abi.classic_spectra(p, exact=True)
+    nm3 = full["bisp_f"].shape[2]
+    sel = [nm3, 1, 2]
+    r = abi.classic_bispectra_sel(p, sel, exact=True)
+    for c, s in enumerate(sel):
+        assert np.array_equal(r["bisp_f"][:, :, c], full["bisp_f"][:, :, s - 1])
+        assert np.array_equal(r["bisp_r"][:, :, c], full["bisp_r"][:, :, s - 1])
+
+
+def test_selection_errors(gpu_lib):
+    p = CASES["tiny_u"]()
+    nm = len(abi.Bound(p).modes)
+    for bad in ([0], [nm ** 3 + 1]):
+        with pytest.raises(abi.BsaHipError) as e:
+            abi.classic_bispectra_sel(p, bad)
+        assert e.value.code == abi.EINVAL
+    with pytest.raises(abi.BsaHipError) as e:
+        abi.classic_bispectra_sel(p, [nm + 1], only_diag=True)
+    assert e.value.code == abi.EINVAL
+    with pytest.raises(abi.BsaHipError) as e:
+        abi.classic_bispectra_sel(p, [1], exact=True, only_diag=True)
+    assert e.value.code == abi.EUNSUPPORTED
+
+
+def _check_export(path, p, idx, sym, want):
+    e = exports.read_bisp_export(path)
+    nf = p.nf
+    assert e["nm"] == 3 and list(e["indexes"]) == list(idx)
+    assert (e["ncomb"], e["ispsym"], e["nzones"], e["npts"]) == (1, sym, 1, nf * nf)
+    (z,) = e["zones"]
+    assert (z["id"], z["ni"], z["nj"]) == (1, nf, nf)
+    f = (np.arange(nf) - (nf - 1) // 2) * p.df
+    assert np.array_equal(z["fi"], np.tile(f, nf).astype(np.float32))
+    assert np.array_equal(z["fj"], np.repeat(f, nf).astype(np.float32))
+    got = z["values"][:, 0].astype(np.float64)
+    w32 = want.reshape(-1, order="F").astype(np.float32).astype(np.float64)
+    # one real32 spacing, plus 1e-12 of the largest value for entries that
+    # are sums cancelling to near zero
+    tol = np.spacing(np.abs(w32).astype(np.float32)).astype(np.float64) + 1e-12 * np.abs(want).max()
+    bad = np.abs(got - w32) > tol
+    assert not bad.any(), (np.flatnonzero(bad)[:5], got[bad][:5], w32[bad][:5])
+
+
+@pytest.mark.parametrize("name", TINY)
+def test_front_visual_export_modal(gpu_lib, tmp_path, name):
+    p = CASES[name]()
+    g = load_golden(name)
+    nm = _nm(g)
+    idx = (nm, 1, min(2, nm))
+    out, r = fdriver.run(p, "visual=%d,%d,%d" % idx, "sym=2", workdir=tmp_path)
+    assert "Exporting  BRM" in r.stdout
+    # the moments are those of the ordinary run
+    assert rel_err(out["m3mr"], g["m3mr"]) < 1e-12
+    _check_export(tmp_path / "bsaexport_cls.brm", p, idx, 2, g["bisp_r"][:, :, _lin(nm, *idx) - 1])
+
+
+def test_front_visual_export_diag_and_nodal(gpu_lib, tmp_path):
+    p = CASES["tiny_uvw"]()
+    g = load_golden("tiny_uvw")
+    nm = _nm(g)
+    d1 = tmp_path / "diag"
+    d1.mkdir()
+    fdriver.run(p, "diag", "visual=%d,1,1" % nm, workdir=d1)
+    _check_export(d1 / "bsaexport_cls.brm", p, (nm, 1, 1), 0, g["bisp_r_diag"][:, :, nm - 1])
+    # nodal (BRN) visual mode: the Classic approach writes no file
+    d2 = tmp_path / "nodal"
+    d2.mkdir()
+    _, r = fdriver.run(p, "visualn=1,1", workdir=d2)
+    assert "no file written" in r.stdout
+    assert not any(d2.iterdir())
+    # modal indexes outside the kept modes stop the run
+    _, r = fdriver.run(p, "visual=%d,1,1" % (nm + 1), workdir=d2, check=False)
+    assert r.returncode != 0 and "out of the kept modes" in r.stdout + r.stderr

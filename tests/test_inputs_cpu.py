@@ -45,3 +45,22 @@ def test_front_generated_files_match_python_writer(tmp_path, name):
             assert np.allclose(ef[k], ep[k], rtol=1e-14)
         else:
             assert np.array_equal(np.asarray(ef[k]), np.asarray(ep[k])), k
+
+
+def test_bisp_export_reader_round_trip(tmp_path):
+    from bsalib_amd import exports
+    nf = 5
+    f = (np.arange(nf) - 2) * 0.25
+    vals = np.arange(nf * nf, dtype=np.float64).reshape(nf, nf, order="F") * 1e-3
+    head = np.array([3, 2, 1, 3, 1, 0, 1, nf * nf, 1, nf, nf], dtype=np.int32)
+    rec = np.stack([np.tile(f, nf), np.repeat(f, nf), vals.reshape(-1, order="F")], 1)
+    (tmp_path / "x.brm").write_bytes(head.tobytes() + rec.astype(np.float32).tobytes())
+    e = exports.read_bisp_export(tmp_path / "x.brm")
+    assert e["nm"] == 3 and list(e["indexes"]) == [2, 1, 3]
+    assert (e["ncomb"], e["ispsym"], e["nzones"], e["npts"]) == (1, 0, 1, nf * nf)
+    (z,) = e["zones"]
+    assert (z["id"], z["ni"], z["nj"]) == (1, nf, nf)
+    assert np.array_equal(z["values"][:, 0], vals.reshape(-1, order="F").astype(np.float32))
+    (tmp_path / "y.brm").write_bytes(head.tobytes() + rec.astype(np.float32).tobytes() + b"\0")
+    with pytest.raises(ValueError):
+        exports.read_bisp_export(tmp_path / "y.brm")
