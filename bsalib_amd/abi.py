@@ -57,7 +57,8 @@ EXPORTS = ["bsahip_init", "bsahip_finalise", "bsahip_last_error", "bsahip_versio
            "bsahip_last_kernel_time", "bsahip_plan_create", "bsahip_plan_run",
            "bsahip_plan_destroy", "bsahip_mesher_create", "bsahip_mesher_layout", "bsahip_mesher_ntiles",
            "bsahip_mesher_tiles", "bsahip_mesher_features", "bsahip_mesher_run",
-           "bsahip_mesher_run_tiles", "bsahip_batch_create", "bsahip_batch_autotune",
+           "bsahip_mesher_run_tiles", "bsahip_mesher_export", "bsahip_mesher_save",
+           "bsahip_mesher_load", "bsahip_batch_create", "bsahip_batch_autotune",
            "bsahip_batch_run", "bsahip_batch_info", "bsahip_batch_destroy",
            "bsahip_mesher_destroy", "bsahip_modal_psd_points", "bsahip_modal_recombination"]
 
@@ -86,6 +87,13 @@ def lib():
         L.bsahip_mesher_run_tiles.argtypes = [C.c_void_p, C.POINTER(C.c_int32),
                                               C.POINTER(C.c_double), C.POINTER(C.c_double),
                                               C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.bsahip_mesher_export.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.c_int32,
+                                           C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int32),
+                                           C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                           C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.bsahip_mesher_save.argtypes = [C.c_void_p, C.c_char_p]
+        L.bsahip_mesher_load.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_char_p,
+                                         C.POINTER(C.c_void_p)]
         L.bsahip_mesher_destroy.argtypes = [C.c_void_p]
         _lib = L
     return _lib
@@ -342,7 +350,10 @@ class Mesher:
 
     def __init__(self, prob, bkg_base_rfmnt: int = 20, tile_cells: int = 32, bkg_peakw=None,
                  bkg_area_ext: float = 2.0, peak_area_ext: float = 2.0, max_area_ext: float = 2.0,
-                 bound: Bound = None, diag_crest: bool = False, natf=None, xsi=None):
+                 bound: Bound = None, diag_crest: bool = False, natf=None, xsi=None,
+                 load: str = None):
+        """load: a checkpoint written by save() for the same problem and
+        settings, read instead of evaluating the pre-mesh (bsahip_mesher_load)."""
         b = bound or Bound(prob)
         self.nm = b.nm
         self.modes = b.modes
@@ -353,8 +364,12 @@ class Mesher:
         self.bkg_peakw = cfg.bkg_peakw
         h = C.c_void_p()
         self.h = None
-        _check(lib().bsahip_mesher_create(C.addressof(b.struct), C.addressof(cfg), C.c_int32(0),
-                                          C.byref(h)))
+        if load is None:
+            _check(lib().bsahip_mesher_create(C.addressof(b.struct), C.addressof(cfg),
+                                              C.c_int32(0), C.byref(h)))
+        else:
+            _check(lib().bsahip_mesher_load(C.addressof(b.struct), C.addressof(cfg), C.c_int32(0),
+                                            str(load).encode(), C.byref(h)))
         self.h = h
         n = C.c_int32(0)
         _check(lib().bsahip_mesher_ntiles(self.h, C.byref(n)))
@@ -379,8 +394,7 @@ class Mesher:
         counts = np.zeros(3, dtype=np.int64)
         lv = None
         if levels is not None:
-            lv = np.ascontiguousarray(np.broadcast_to(np.asarray(levels, dtype=np.int32),
-                                                      (self.ntiles,)))
+            lv = self._levels(levels)
         tiles = np.zeros((self.ntiles, nm3)) if per_tile else None
         _check(lib().bsahip_mesher_run_tiles(self.h, _ptr(lv, C.c_int32), _ptr(tiles, C.c_double),
                                              _ptr(m3mf, C.c_double), _ptr(m3mr, C.c_double),
@@ -390,6 +404,37 @@ class Mesher:
         if per_tile:
             out["tile_m3mr"] = tiles
         return out
+
+    def _levels(self, levels):
+        if levels is None:
+            return None
+        return np.ascontiguousarray(np.broadcast_to(np.asarray(levels, dtype=np.int32),
+                                                    (self.ntiles,)))
+
+    def export(self, levels=None, coef=None) -> dict:
+        """BRM at every tile's post-mesh points (bsahip_mesher_export): coef
+        (NM^3, ncol) combinations, None for every entry.  Returns dims
+        (ntiles, 2), fi, fj (npts) and val (npts, ncol)."""
+        lv = self._levels(levels)
+        nm3 = self.nm ** 3
+        dims = np.zeros((self.ntiles, 2), dtype=np.int32)
+        n = C.c_int64(0)
+        L = lib()
+        _check(L.bsahip_mesher_export(self.h, _ptr(lv, C.c_int32), 0, None, 0,
+                                      _ptr(dims, C.c_int32), None, None, None, C.byref(n)))
+        if coef is not None:
+            coef = np.asfortranarray(np.asarray(coef, dtype=np.float64).reshape(nm3, -1, order="F"))
+        ncol = nm3 if coef is None else coef.shape[1]
+        fi, fj = np.zeros(n.value), np.zeros(n.value)
+        val = np.zeros((n.value, ncol))
+        _check(L.bsahip_mesher_export(self.h, _ptr(lv, C.c_int32), ncol, _ptr(coef, C.c_double),
+                                      n.value, _ptr(dims, C.c_int32), _ptr(fi, C.c_double),
+                                      _ptr(fj, C.c_double), _ptr(val, C.c_double), C.byref(n)))
+        return {"dims": dims, "fi": fi, "fj": fj, "val": val}
+
+    def save(self, path):
+        """Checkpoint of the resident zones (bsahip_mesher_save)."""
+        _check(lib().bsahip_mesher_save(self.h, str(path).encode()))
 
     def close(self):
         if self.h:

@@ -246,6 +246,62 @@ __global__ __launch_bounds__(256) void k_msh_integrate(
    atomicAdd(m3r + mnp, sr * wscale);
 }
 
+// ---------------------------------------------------------------------------
+// Tile export: the integrand of k_msh_integrate without the weights at every
+// BRM point (a, b) of the tile, pt = a + b*(NA+1).  Column c of val
+// (val[pt*ncol + c]) is BRM entry sel[c] when sel[c] >= 0, else
+// sum_mnp coef[c*nm3 + mnp] BRM_mnp.  grid (ceil(npts/256)), block 256
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_msh_export(
+   const double *__restrict__ bf, long ldq, int nm, int nb1, int nb2, int s1, int s2,
+   const double2 *__restrict__ H1, const double2 *__restrict__ H2,
+   const double2 *__restrict__ H3, double f3a, double h1, double h2,
+   const double *__restrict__ Mm, const double *__restrict__ Km, const double *__restrict__ Cm,
+   int ncol, const int *__restrict__ sel, const double *__restrict__ coef,
+   double *__restrict__ val)
+{
+   const int nm3 = nm * nm * nm;
+   const int nc1 = nb1 - 1, nc2 = nb2 - 1;
+   const long NA1 = (long)nc1 * s1 + 1, NB1 = (long)nc2 * s2 + 1;
+   const long pt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (pt >= NA1 * NB1) return;
+   const int a = (int)(pt % NA1), b = (int)(pt / NA1);
+   const int ci = min(a / s1, nc1 - 1), cj = min(b / s2, nc2 - 1);
+   const int aa = a - ci * s1, bb = b - cj * s2;
+   const size_t q00 = (size_t)ci + (size_t)cj * ldq;
+   const double inv1 = 1.0 / s1, inv2 = 1.0 / s2;
+   const double sv = bb * inv2, t = aa * inv1;
+   auto brm = [&](int mnp) {
+      const int m = mnp % nm, n = (mnp / nm) % nm, p = mnp / (nm * nm);
+      const double b00 = bf[q00 * nm3 + mnp], b10 = bf[(q00 + 1) * nm3 + mnp];
+      const double b01 = bf[(q00 + ldq) * nm3 + mnp], b11 = bf[(q00 + ldq + 1) * nm3 + mnp];
+      const double e0 = (1.0 - sv) * b00 + sv * b01, e1 = (1.0 - sv) * b10 + sv * b11;
+      const double bv = (1.0 - t) * e0 + t * e1;
+      const double2 h1v = H1[(size_t)a * nm + m], h2v = H2[(size_t)b * nm + n];
+      double2 h3v;
+      if (H3) {
+         h3v = H3[(size_t)(a + b) * nm + p];
+      } else {
+         const double w3 = 2.0 * M_PI * (f3a + a * h1 + b * h2);
+         const double r = Km[p] - w3 * w3 * Mm[p], im = w3 * Cm[p], d = r * r + im * im;
+         h3v = make_double2(r / d, -im / d);
+      }
+      return bv * re3(h1v, h2v, h3v);
+   };
+   for (int c = 0; c < ncol; ++c) {
+      double v;
+      if (sel[c] >= 0) {
+         v = brm(sel[c]);
+      } else {
+         v = 0.0;
+         const double *cc = coef + (size_t)c * nm3;
+         for (int mnp = 0; mnp < nm3; ++mnp)
+            if (cc[mnp] != 0.0) v += cc[mnp] * brm(mnp);
+      }
+      val[(size_t)pt * ncol + c] = v;
+   }
+}
+
 // Level-0 points of a tile into its level-l grid:
 // fine[(i*l + j*l*nb1)] = lv0[i + j*ldq].  grid (ceil(nm3/256), (n1+1)*(n2+1))
 __global__ void k_lv0_to_fine(const double *__restrict__ lv0, long ldq, int n1, int l, int nb1,
@@ -592,58 +648,161 @@ void msh_grid(const MshTile &t, int l, std::vector<double> &f1, std::vector<doub
       }
 }
 
-int msh_integrate(bsahip_mesher *M, const MshTile &t, int l, const double *bf, long ldq, int nb1,
-                  int nb2, double *m3f, double *m3r, long &brm_pts)
+// BRM grid of a tile at BFM level l: s1 x s2 BRM cells per BFM cell, NA x NB
+// cells, spacings h1, h2 (independent of l up to the cap s >= 1).
+struct MshBrmGrid {
+   int s1, s2, NA, NB;
+   double h1, h2;
+   double2 *H1 = nullptr, *H2 = nullptr, *H3 = nullptr;
+};
+
+MshBrmGrid msh_brm_grid(const MshTile &t, int l)
+{
+   MshBrmGrid G;
+   G.s1 = std::max(1, t.sub1 / l);
+   G.s2 = std::max(1, t.sub2 / l);
+   G.NA = t.n1 * l * G.s1;
+   G.NB = t.n2 * l * G.s2;
+   G.h1 = (t.b1 - t.a1) / G.NA;
+   G.h2 = (t.b2 - t.a2) / G.NB;
+   return G;
+}
+
+// FRF tables along the two axes and, with equal spacings, along f1 + f2.
+int msh_frf_tables(bsahip_mesher *M, const MshTile &t, MshBrmGrid &G)
 {
    Prob &D = M->D;
-   const int s1 = std::max(1, t.sub1 / l), s2 = std::max(1, t.sub2 / l);
-   const int NA = (nb1 - 1) * s1, NB = (nb2 - 1) * s2;
-   const double h1 = (t.b1 - t.a1) / NA, h2 = (t.b2 - t.a2) / NB;
-   const double ws = (2.0 * M_PI * h1) * (2.0 * M_PI * h2);
-   const int nm3 = D.nm * D.nm * D.nm;
-   // FRF tables along the two axes and, with equal spacings, along f1 + f2
-   const bool diag = std::fabs(h1 - h2) <= 1e-13 * std::max(std::fabs(h1), std::fabs(h2));
+   const int NA = G.NA, NB = G.NB;
+   const bool diag = std::fabs(G.h1 - G.h2) <= 1e-13 * std::max(std::fabs(G.h1), std::fabs(G.h2));
    const size_t n3 = diag ? (size_t)(NA + NB + 1) : 0;
    double2 *H;
    CHK(M->B.get("m_H", ((size_t)(NA + 1) + (NB + 1) + n3) * D.nm, &H));
-   double2 *H1 = H, *H2 = H + (size_t)(NA + 1) * D.nm, *H3 = diag ? H2 + (size_t)(NB + 1) * D.nm : nullptr;
+   G.H1 = H;
+   G.H2 = H + (size_t)(NA + 1) * D.nm;
+   G.H3 = diag ? G.H2 + (size_t)(NB + 1) * D.nm : nullptr;
    auto table = [&](double f0, double h, int n, double2 *out) -> int {
       const int tot = n * D.nm;
       k_frf_axis<<<(tot + 255) / 256, 256, 0, g.s>>>(f0, h, n, D.Mm, D.Km, D.Cm, D.nm, out);
       HIPC(hipGetLastError());
       return 0;
    };
-   CHK(table(t.a1, h1, NA + 1, H1));
-   CHK(table(t.a2, h2, NB + 1, H2));
-   if (diag) CHK(table(t.a1 + t.a2, h1, NA + NB + 1, H3));
+   CHK(table(t.a1, G.h1, NA + 1, G.H1));
+   CHK(table(t.a2, G.h2, NB + 1, G.H2));
+   if (diag) CHK(table(t.a1 + t.a2, G.h1, NA + NB + 1, G.H3));
+   return 0;
+}
+
+int msh_integrate(bsahip_mesher *M, const MshTile &t, int l, const double *bf, long ldq, int nb1,
+                  int nb2, double *m3f, double *m3r, long &brm_pts)
+{
+   Prob &D = M->D;
+   MshBrmGrid G = msh_brm_grid(t, l);
+   CHK(msh_frf_tables(M, t, G));
+   const double ws = (2.0 * M_PI * G.h1) * (2.0 * M_PI * G.h2);
+   const int nm3 = D.nm * D.nm * D.nm;
    const int ncell = (nb1 - 1) * (nb2 - 1);
    const int bx = (nm3 + 255) / 256;
    const int nchunk = std::max(1, std::min(ncell, std::max(1, 4096 / bx)));
    dim3 gs(bx, nchunk);
-   k_msh_integrate<<<gs, 256, 0, g.s>>>(bf, ldq, D.nm, nb1, nb2, s1, s2, H1, H2, H3,
-                                         t.a1 + t.a2, h1, h2, ws, D.Mm, D.Km, D.Cm, m3f, m3r);
+   k_msh_integrate<<<gs, 256, 0, g.s>>>(bf, ldq, D.nm, nb1, nb2, G.s1, G.s2, G.H1, G.H2, G.H3,
+                                         t.a1 + t.a2, G.h1, G.h2, ws, D.Mm, D.Km, D.Cm, m3f, m3r);
    HIPC(hipGetLastError());
-   brm_pts += (long)(NA + 1) * (NB + 1);
+   brm_pts += (long)(G.NA + 1) * (G.NB + 1);
    return 0;
 }
 
-}  // namespace
+// Per-tile visitor of the post-mesh: the tile's index, its level and its BFM
+// grid (nb1 x nb2 points, bf[(i + j*ldq)*nm3 + mnp], on the device).
+using MshVisit = std::function<int(size_t, const MshTile &, int, const double *, long, int, int)>;
 
-extern "C" {
-
-int bsahip_mesher_create(const bsahip_problem *p, const bsahip_mesher_cfg *cfg, int32_t flags,
-                         bsahip_mesher **out)
+// Post-mesh traversal shared by the run and the export: level-1 tiles read the
+// resident level-0 BFM in place; refined tiles are evaluated in groups whose
+// fine grids fit the staging buffer.  Returns the refinement evaluations.
+int msh_post(bsahip_mesher *m, const int32_t *levels, const MshVisit &visit, long &evals_post)
 {
-   std::lock_guard<std::recursive_mutex> lk(g.mu);
-   if (!out || !cfg || !cfg->natf || !cfg->xsi) return fail(BSAHIP_EINVAL, "NULL Mesher argument");
-   *out = nullptr;
+   Prob &D = m->D;
+   const size_t nm3 = (size_t)D.nm * D.nm * D.nm, nt = m->tiles.size();
+   const size_t cap = std::max<size_t>(66049, ((size_t)1 << 29) / nm3);   // points (4 GiB)
+   std::vector<double> f1, f2, e1, e2;
+   std::vector<long> slot;
+   std::vector<size_t> grp, goff;
+   size_t used = 0;
+   evals_post = 0;
+   auto flush = [&]() -> int {
+      if (grp.empty()) return 0;
+      double *fine;
+      CHK(m->B.get("m_fine", used * nm3, &fine));
+      CHK(msh_eval(m, e1, e2, slot, fine));
+      evals_post += (long)e1.size();
+      for (size_t k = 0; k < grp.size(); ++k) {
+         const MshTile &t = m->tiles[grp[k]];
+         const int l = levels ? levels[grp[k]] : t.rule;
+         const int nb1 = t.n1 * l + 1, nb2 = t.n2 * l + 1;
+         double *fg = fine + goff[k] * nm3;
+         dim3 gc((unsigned)((nm3 + 255) / 256), (unsigned)((t.n1 + 1) * (t.n2 + 1)));
+         k_lv0_to_fine<<<gc, 256, 0, g.s>>>(m->lv0 + msh_lv0_at(m, t) * nm3, m->nx, t.n1, l, nb1,
+                                            (int)nm3, fg);
+         HIPC(hipGetLastError());
+         CHK(visit(grp[k], t, l, fg, nb1, nb1, nb2));
+      }
+      HIPC(hipStreamSynchronize(g.s));
+      grp.clear();
+      goff.clear();
+      e1.clear();
+      e2.clear();
+      slot.clear();
+      used = 0;
+      return 0;
+   };
+   for (size_t i = 0; i < nt; ++i) {
+      const MshTile &t = m->tiles[i];
+      const int l = levels ? levels[i] : t.rule;
+      if (l == 1) {
+         CHK(visit(i, t, 1, m->lv0 + msh_lv0_at(m, t) * nm3, m->nx, t.n1 + 1, t.n2 + 1));
+         continue;
+      }
+      const size_t np = (size_t)(t.n1 * l + 1) * (t.n2 * l + 1);
+      if (!grp.empty() && used + np > cap) CHK(flush());
+      int nb1, nb2;
+      msh_grid(t, l, f1, f2, nb1, nb2);
+      for (int j = 0; j < nb2; ++j)
+         for (int ii = 0; ii < nb1; ++ii)
+            if ((ii % l) || (j % l)) {
+               const size_t q = ii + (size_t)j * nb1;
+               e1.push_back(f1[q]);
+               e2.push_back(f2[q]);
+               slot.push_back((long)(used + q));
+            }
+      grp.push_back(i);
+      goff.push_back(used);
+      used += np;
+   }
+   return flush();
+}
+
+int msh_check_levels(const bsahip_mesher *m, const int32_t *levels)
+{
+   for (size_t i = 0; i < m->tiles.size(); ++i) {
+      const int l = levels ? levels[i] : m->tiles[i].rule;
+      if (!(l == 1 || l == 2 || l == 4 || l == 8))
+         return fail(BSAHIP_EINVAL, "tile levels must be 1, 2, 4 or 8");
+   }
+   return 0;
+}
+
+// Everything of bsahip_mesher_create but the pre-mesh evaluation: problem
+// tables, tile layout, the level-0 buffer and its axis nodes x.
+int msh_setup(const bsahip_problem *p, const bsahip_mesher_cfg *cfg, int32_t flags,
+              std::unique_ptr<bsahip_mesher> &M, std::vector<double> &x)
+{
+   if (!cfg || !cfg->natf || !cfg->xsi) return fail(BSAHIP_EINVAL, "NULL Mesher argument");
    CHK(ensure_init());
    if ((flags & BSAHIP_PREC_MASK) != BSAHIP_PREC_FP64)
       return fail(BSAHIP_EUNSUPPORTED, "the Mesher runs in FP64");
    if (!(cfg->bkg_peakw > 0.0) || !(cfg->bkg_area_ext > 0.0) || !(cfg->peak_area_ext > 0.0) ||
        !(cfg->max_area_ext > 0.0) || cfg->tile_cells < 1 || cfg->tile_cells > 128)
       return fail(BSAHIP_EINVAL, "Mesher extensions and bkg_peakw must be > 0, tile_cells in 1..128");
-   std::unique_ptr<bsahip_mesher> M(new bsahip_mesher());
+   M.reset(new bsahip_mesher());
    CHK(load_problem(p, 0, M->B, M->D));
    CHK(set_pod(p, M->D));
    Prob &D = M->D;
@@ -658,13 +817,46 @@ int bsahip_mesher_create(const bsahip_problem *p, const bsahip_mesher_cfg *cfg, 
    CHK(M->B.get("m_PB", (size_t)D.K * M->nmp, &M->PB));
    k_pbt<<<(unsigned)((D.K * M->nmp + 255) / 256), 256, 0, g.s>>>(D.pb_k, D.nm, D.K, M->nmp, M->PB);
    HIPC(hipGetLastError());
+   const size_t nm3 = (size_t)D.nm * D.nm * D.nm;
+   x = msh_axis_nodes(M->tiles);
+   M->nx = (long)x.size();
+   CHK(M->B.get("m_lv0", x.size() * x.size() * nm3, &M->lv0));
+   return 0;
+}
+
+void msh_geom(const bsahip_mesher *m, double *geom, int32_t *rule_level)
+{
+   for (size_t i = 0; i < m->tiles.size(); ++i) {
+      const MshTile &t = m->tiles[i];
+      if (geom) {
+         double *gq = geom + 10 * i;
+         gq[0] = t.a1; gq[1] = t.b1; gq[2] = t.a2; gq[3] = t.b2;
+         gq[4] = t.n1; gq[5] = t.n2; gq[6] = t.pol1; gq[7] = t.pol2;
+         gq[8] = t.sub1; gq[9] = t.sub2;
+      }
+      if (rule_level) rule_level[i] = t.rule;
+   }
+}
+
+// checkpoint file of a Mesher's resident zones
+const char MSH_MAGIC[16] = "BSAHIP-MSH-DUMP";
+
+}  // namespace
+
+extern "C" {
+
+int bsahip_mesher_create(const bsahip_problem *p, const bsahip_mesher_cfg *cfg, int32_t flags,
+                         bsahip_mesher **out)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   if (!out) return fail(BSAHIP_EINVAL, "NULL Mesher argument");
+   *out = nullptr;
+   std::unique_ptr<bsahip_mesher> M;
+   std::vector<double> x;
+   CHK(msh_setup(p, cfg, flags, M, x));
    // pre-mesh: the level-0 BFM on the global grid (every tile's level-0 points
    // once, shared edges and corners included once), kept resident
-   const size_t nm3 = (size_t)D.nm * D.nm * D.nm;
-   const std::vector<double> x = msh_axis_nodes(M->tiles);
-   M->nx = (long)x.size();
    const size_t npts = x.size() * x.size();
-   CHK(M->B.get("m_lv0", npts * nm3, &M->lv0));
    std::vector<double> f1, f2;
    std::vector<long> slot;
    for (size_t j = 0; j < x.size(); ++j)
@@ -719,16 +911,7 @@ int bsahip_mesher_ntiles(const bsahip_mesher *m, int32_t *ntiles)
 int bsahip_mesher_tiles(const bsahip_mesher *m, double *geom, int32_t *rule_level)
 {
    if (!m) return fail(BSAHIP_EINVAL, "NULL Mesher");
-   for (size_t i = 0; i < m->tiles.size(); ++i) {
-      const MshTile &t = m->tiles[i];
-      if (geom) {
-         double *gq = geom + 10 * i;
-         gq[0] = t.a1; gq[1] = t.b1; gq[2] = t.a2; gq[3] = t.b2;
-         gq[4] = t.n1; gq[5] = t.n2; gq[6] = t.pol1; gq[7] = t.pol2;
-         gq[8] = t.sub1; gq[9] = t.sub2;
-      }
-      if (rule_level) rule_level[i] = t.rule;
-   }
+   msh_geom(m, geom, rule_level);
    return 0;
 }
 
@@ -758,76 +941,19 @@ int bsahip_mesher_run_tiles(bsahip_mesher *m, const int32_t *levels, double *til
    if (!m) return fail(BSAHIP_EINVAL, "NULL Mesher");
    Prob &D = m->D;
    const size_t nm3 = (size_t)D.nm * D.nm * D.nm, nt = m->tiles.size();
-   for (size_t i = 0; i < nt; ++i) {
-      const int l = levels ? levels[i] : m->tiles[i].rule;
-      if (!(l == 1 || l == 2 || l == 4 || l == 8))
-         return fail(BSAHIP_EINVAL, "tile levels must be 1, 2, 4 or 8");
-   }
+   CHK(msh_check_levels(m, levels));
    // accumulators: m3f, m3r, then one m3r slot per tile when asked for
    double *acc;
    const size_t nacc = 2 * nm3 + (tile_m3mr ? nt * nm3 : 0);
    CHK(m->B.get("m_acc", nacc, &acc));
    CHK(fill(acc, nacc, 0.0));
    long evals_post = 0, brm = 0;
-   // level-1 tiles integrate straight from the resident level-0 BFM; refined
-   // tiles are evaluated in groups whose fine grids fit the staging buffer
-   const size_t cap = std::max<size_t>(66049, ((size_t)1 << 29) / nm3);   // points (4 GiB)
-   std::vector<double> f1, f2, e1, e2;
-   std::vector<long> slot;
-   std::vector<size_t> grp, goff;
-   size_t used = 0;
    auto slot_r = [&](size_t k) { return tile_m3mr ? acc + 2 * nm3 + k * nm3 : acc + nm3; };
-   auto flush = [&]() -> int {
-      if (grp.empty()) return 0;
-      double *fine;
-      CHK(m->B.get("m_fine", used * nm3, &fine));
-      CHK(msh_eval(m, e1, e2, slot, fine));
-      evals_post += (long)e1.size();
-      for (size_t k = 0; k < grp.size(); ++k) {
-         const MshTile &t = m->tiles[grp[k]];
-         const int l = levels ? levels[grp[k]] : t.rule;
-         const int nb1 = t.n1 * l + 1, nb2 = t.n2 * l + 1;
-         double *fg = fine + goff[k] * nm3;
-         dim3 gc((unsigned)((nm3 + 255) / 256), (unsigned)((t.n1 + 1) * (t.n2 + 1)));
-         k_lv0_to_fine<<<gc, 256, 0, g.s>>>(m->lv0 + msh_lv0_at(m, t) * nm3, m->nx, t.n1, l, nb1,
-                                            (int)nm3, fg);
-         HIPC(hipGetLastError());
-         CHK(msh_integrate(m, t, l, fg, nb1, nb1, nb2, acc, slot_r(grp[k]), brm));
-      }
-      HIPC(hipStreamSynchronize(g.s));
-      grp.clear();
-      goff.clear();
-      e1.clear();
-      e2.clear();
-      slot.clear();
-      used = 0;
-      return 0;
-   };
-   for (size_t i = 0; i < nt; ++i) {
-      const MshTile &t = m->tiles[i];
-      const int l = levels ? levels[i] : t.rule;
-      if (l == 1) {
-         CHK(msh_integrate(m, t, 1, m->lv0 + msh_lv0_at(m, t) * nm3, m->nx, t.n1 + 1, t.n2 + 1, acc,
-                           slot_r(i), brm));
-         continue;
-      }
-      const size_t np = (size_t)(t.n1 * l + 1) * (t.n2 * l + 1);
-      if (!grp.empty() && used + np > cap) CHK(flush());
-      int nb1, nb2;
-      msh_grid(t, l, f1, f2, nb1, nb2);
-      for (int j = 0; j < nb2; ++j)
-         for (int ii = 0; ii < nb1; ++ii)
-            if ((ii % l) || (j % l)) {
-               const size_t q = ii + (size_t)j * nb1;
-               e1.push_back(f1[q]);
-               e2.push_back(f2[q]);
-               slot.push_back((long)(used + q));
-            }
-      grp.push_back(i);
-      goff.push_back(used);
-      used += np;
-   }
-   CHK(flush());
+   CHK(msh_post(m, levels,
+                [&](size_t k, const MshTile &t, int l, const double *bf, long ldq, int nb1, int nb2) {
+                   return msh_integrate(m, t, l, bf, ldq, nb1, nb2, acc, slot_r(k), brm);
+                },
+                evals_post));
    if (tile_m3mr) {
       HIPC(hipMemcpyAsync(tile_m3mr, acc + 2 * nm3, nt * nm3 * sizeof(double),
                           hipMemcpyDeviceToHost, g.s));
@@ -857,6 +983,154 @@ int bsahip_mesher_run(bsahip_mesher *m, const int32_t *levels, double *m3mf, dou
                       int64_t *counts)
 {
    return bsahip_mesher_run_tiles(m, levels, nullptr, m3mf, m3mr, counts);
+}
+
+int bsahip_mesher_export(bsahip_mesher *m, const int32_t *levels, int32_t ncol, const double *coef,
+                         int64_t cap, int32_t *dims, double *fi, double *fj, double *val,
+                         int64_t *npts)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   if (!m || !npts) return fail(BSAHIP_EINVAL, "NULL Mesher or point count");
+   CHK(msh_check_levels(m, levels));
+   Prob &D = m->D;
+   const int nm3 = D.nm * D.nm * D.nm;
+   const size_t nt = m->tiles.size();
+   // sizes: the BRM grid of every tile at its level
+   std::vector<int64_t> off(nt + 1, 0);
+   for (size_t i = 0; i < nt; ++i) {
+      const MshTile &t = m->tiles[i];
+      const MshBrmGrid G = msh_brm_grid(t, levels ? levels[i] : t.rule);
+      if (dims) {
+         dims[2 * i] = G.NA + 1;
+         dims[2 * i + 1] = G.NB + 1;
+      }
+      off[i + 1] = off[i] + (int64_t)(G.NA + 1) * (G.NB + 1);
+   }
+   *npts = off[nt];
+   if (!fi && !fj && !val) return 0;
+   if (ncol < 0 || (ncol > 0 && !val) || (!coef && ncol != nm3))
+      return fail(BSAHIP_EINVAL, "ncol < 0, NULL values, or NULL coef with ncol != NM^3");
+   if (cap < off[nt]) return fail(BSAHIP_EINVAL, "cap below the number of export points");
+   // unit columns take the single-entry path
+   std::vector<int> sel(std::max(ncol, 1), -1);
+   for (int c = 0; c < ncol; ++c) {
+      if (!coef) {
+         sel[c] = c;
+         continue;
+      }
+      int one = -1, nz = 0;
+      for (int q = 0; q < nm3; ++q)
+         if (coef[(size_t)c * nm3 + q] != 0.0) {
+            ++nz;
+            if (coef[(size_t)c * nm3 + q] == 1.0) one = q;
+         }
+      if (nz == 1 && one >= 0) sel[c] = one;
+   }
+   int *dsel;
+   double *dcoef = nullptr;
+   CHK(upload(m->B, "m_xsel", sel, &dsel));
+   if (coef && ncol > 0) {
+      std::vector<double> cc(coef, coef + (size_t)ncol * nm3);
+      CHK(upload(m->B, "m_xcoef", cc, &dcoef));
+   }
+   long evals = 0;
+   CHK(msh_post(m, levels,
+                [&](size_t k, const MshTile &t, int l, const double *bf, long ldq, int nb1, int nb2) {
+                   MshBrmGrid G = msh_brm_grid(t, l);
+                   CHK(msh_frf_tables(m, t, G));
+                   const int64_t np = (int64_t)(G.NA + 1) * (G.NB + 1);
+                   for (int b = 0; b <= G.NB; ++b)
+                      for (int a = 0; a <= G.NA; ++a) {
+                         const int64_t q = off[k] + a + (int64_t)b * (G.NA + 1);
+                         if (fi) fi[q] = t.a1 + a * G.h1;
+                         if (fj) fj[q] = t.a2 + b * G.h2;
+                      }
+                   if (ncol == 0) return 0;
+                   double *dv;
+                   CHK(m->B.get("m_xval", (size_t)np * ncol, &dv));
+                   k_msh_export<<<(unsigned)((np + 255) / 256), 256, 0, g.s>>>(
+                      bf, ldq, D.nm, nb1, nb2, G.s1, G.s2, G.H1, G.H2, G.H3, t.a1 + t.a2, G.h1,
+                      G.h2, D.Mm, D.Km, D.Cm, ncol, dsel, dcoef, dv);
+                   HIPC(hipGetLastError());
+                   HIPC(hipMemcpyAsync(val + (size_t)off[k] * ncol, dv,
+                                       (size_t)np * ncol * sizeof(double), hipMemcpyDeviceToHost,
+                                       g.s));
+                   HIPC(hipStreamSynchronize(g.s));
+                   return 0;
+                },
+                evals));
+   return 0;
+}
+
+int bsahip_mesher_save(const bsahip_mesher *m, const char *path)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   if (!m || !path) return fail(BSAHIP_EINVAL, "NULL Mesher or path");
+   const size_t nt = m->tiles.size(), nm3 = (size_t)m->D.nm * m->D.nm * m->D.nm;
+   std::unique_ptr<FILE, int (*)(FILE *)> f(std::fopen(path, "wb"), std::fclose);
+   if (!f) return fail(BSAHIP_EINVAL, std::string("cannot open ") + path + " for writing");
+   const int32_t head[4] = {1, m->D.nm, (int32_t)nt, 0};
+   const int64_t sizes[2] = {m->nx, m->evals_lv0};
+   std::vector<double> geom(10 * nt);
+   std::vector<int32_t> rule(nt);
+   msh_geom(m, geom.data(), rule.data());
+   bool ok = std::fwrite(MSH_MAGIC, 1, 16, f.get()) == 16 &&
+             std::fwrite(head, sizeof(head), 1, f.get()) == 1 &&
+             std::fwrite(sizes, sizeof(sizes), 1, f.get()) == 1 &&
+             std::fwrite(geom.data(), sizeof(double), geom.size(), f.get()) == geom.size() &&
+             std::fwrite(rule.data(), sizeof(int32_t), nt, f.get()) == nt;
+   const size_t tot = (size_t)m->nx * m->nx * nm3, chunk = (size_t)1 << 23;
+   std::vector<double> h(std::min(tot, chunk));
+   for (size_t o = 0; ok && o < tot; o += chunk) {
+      const size_t n = std::min(chunk, tot - o);
+      HIPC(hipMemcpy(h.data(), m->lv0 + o, n * sizeof(double), hipMemcpyDeviceToHost));
+      ok = std::fwrite(h.data(), sizeof(double), n, f.get()) == n;
+   }
+   if (!ok || std::fflush(f.get()) != 0) return fail(BSAHIP_EINVAL, std::string("write error on ") + path);
+   return 0;
+}
+
+int bsahip_mesher_load(const bsahip_problem *p, const bsahip_mesher_cfg *cfg, int32_t flags,
+                       const char *path, bsahip_mesher **out)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   if (!out || !path) return fail(BSAHIP_EINVAL, "NULL Mesher argument or path");
+   *out = nullptr;
+   std::unique_ptr<bsahip_mesher> M;
+   std::vector<double> x;
+   CHK(msh_setup(p, cfg, flags, M, x));
+   std::unique_ptr<FILE, int (*)(FILE *)> f(std::fopen(path, "rb"), std::fclose);
+   if (!f) return fail(BSAHIP_EINVAL, std::string("cannot open ") + path);
+   const size_t nt = M->tiles.size(), nm3 = (size_t)M->D.nm * M->D.nm * M->D.nm;
+   char magic[16];
+   int32_t head[4];
+   int64_t sizes[2];
+   if (std::fread(magic, 1, 16, f.get()) != 16 || std::memcmp(magic, MSH_MAGIC, 16) != 0 ||
+       std::fread(head, sizeof(head), 1, f.get()) != 1 || head[0] != 1 ||
+       std::fread(sizes, sizeof(sizes), 1, f.get()) != 1)
+      return fail(BSAHIP_EINVAL, std::string(path) + " is not a Mesher checkpoint of this version");
+   if (head[1] != M->D.nm || head[2] != (int32_t)nt || sizes[0] != M->nx)
+      return fail(BSAHIP_EINVAL, "checkpoint modes or zone layout differ from this problem and configuration");
+   std::vector<double> geom(10 * nt), want(10 * nt);
+   std::vector<int32_t> rule(nt), rwant(nt);
+   msh_geom(M.get(), want.data(), rwant.data());
+   if (std::fread(geom.data(), sizeof(double), geom.size(), f.get()) != geom.size() ||
+       std::fread(rule.data(), sizeof(int32_t), nt, f.get()) != nt)
+      return fail(BSAHIP_EINVAL, std::string(path) + " is truncated");
+   if (std::memcmp(geom.data(), want.data(), geom.size() * sizeof(double)) != 0 || rule != rwant)
+      return fail(BSAHIP_EINVAL, "checkpoint zone layout differs from this configuration");
+   const size_t tot = (size_t)M->nx * M->nx * nm3, chunk = (size_t)1 << 23;
+   std::vector<double> h(std::min(tot, chunk));
+   for (size_t o = 0; o < tot; o += chunk) {
+      const size_t n = std::min(chunk, tot - o);
+      if (std::fread(h.data(), sizeof(double), n, f.get()) != n)
+         return fail(BSAHIP_EINVAL, std::string(path) + " is truncated");
+      HIPC(hipMemcpy(M->lv0 + o, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+   }
+   if (std::fgetc(f.get()) != EOF) return fail(BSAHIP_EINVAL, std::string(path) + " has trailing data");
+   M->evals_lv0 = (long)sizes[1];
+   *out = M.release();
+   return 0;
 }
 
 int bsahip_mesher_destroy(bsahip_mesher *m)

@@ -36,7 +36,8 @@ module BsaHip_C
 
    public :: bsahip_init, bsahip_finalise, bsahip_classic_run, bsahip_bfm_points
    public :: bsahip_classic_bispectra_sel
-   public :: bsahip_mesher_create, bsahip_mesher_run, bsahip_mesher_destroy
+   public :: bsahip_mesher_create, bsahip_mesher_run, bsahip_mesher_destroy, bsahip_mesher_export
+   public :: bsahip_mesher_ntiles
    public :: bsahip_last_error_f, bsahip_modal_psd_points, bsahip_modal_recombination
 
    interface
@@ -103,6 +104,21 @@ module BsaHip_C
          import c_int, c_int32_t, c_ptr
          integer(c_int32_t), value :: ndofs, nm, ccc
          type(c_ptr), value :: phi, m2, m3, m2o2, m2_r, m3_r, m2o2_r, sk_r
+      end function
+
+      integer(c_int) function bsahip_mesher_export(mesher, levels, ncol, coef, cap, dims, fi, fj, &
+                                                   val, npts) bind(c, name="bsahip_mesher_export")
+         import c_int, c_int32_t, c_int64_t, c_ptr
+         type(c_ptr), value :: mesher, levels, coef, dims, fi, fj, val
+         integer(c_int32_t), value :: ncol
+         integer(c_int64_t), value :: cap
+         integer(c_int64_t), intent(out) :: npts
+      end function
+
+      integer(c_int) function bsahip_mesher_ntiles(mesher, ntiles) bind(c, name="bsahip_mesher_ntiles")
+         import c_int, c_int32_t, c_ptr
+         type(c_ptr), value :: mesher
+         integer(c_int32_t), intent(out) :: ntiles
       end function
 
       integer(c_int) function bsahip_mesher_destroy(mesher) bind(c, name="bsahip_mesher_destroy")

@@ -167,6 +167,8 @@ module BsaLib
    ! (src/bsa/data/BsaLibData.F90:87-100)
    integer(bsa_int_t) :: vis_idx_(3) = 0
    logical :: vis_modal_ = .true.
+   ! bispectrum export of Mesher runs (bsa_setBRMExportDefaultMode)
+   logical :: brm_base_ = .false.
 
    ! ---- problem of the last bsa_Run, kept for the post-processing calls ----
    type(bsahip_problem_t), save :: lp_
@@ -663,8 +665,6 @@ contains
       do_msh = isuban_ == 2 .or. isuban_ == 3
       if (ionly_diag_ /= 0 .and. do_msh) &
          call bsa_Abort('The ONLY_DIAG model is available for the Classic approach only in this library.')
-      if (visual_ .and. do_msh) call bsa_Abort('Visual mode with the Mesher re-reads the BFM dump file '// &
-         'of a previous run; this library keeps Mesher zones on the device and has no dump file.')
       if (iconv_ /= BSA_PSD_CONVENTION_PULS) &
          call bsa_Abort('Only the pulsation convention (BSA_PSD_CONVENTION_PULS) is available in this library.')
 
@@ -908,6 +908,7 @@ contains
          allocate(m3mf(nmk**3), m3mr(nmk**3))
          rc2 = bsahip_mesher_run(msh, c_null_ptr, c_loc(m3mf), c_loc(m3mr), c_loc(counts))
          if (rc2 /= 0) call bsa_Abort('bsahip_mesher_run failed: '//bsahip_last_error_f())
+         if (visual_ .or. brm_base_) call exportMesher_(msh, nmk)
          rc2 = bsahip_mesher_destroy(msh)
          write(output_unit, '(1x, 2a, i0, a, i0, a, i0, a)') INFOMSG, 'Mesher on the GPU: ', &
             counts(1) + counts(2), ' BFM points (', counts(1), ' level-0), ', counts(3), ' BRM points.'
@@ -1341,19 +1342,26 @@ contains
       character(len=*), intent(in) :: fname
    end subroutine
 
-   !> Only BSA_EXPORT_BRM_MODE_NONE: bispectra are reduced to moments on the
-   !> device and never written out.
+   !> BSA_EXPORT_BRM_MODE_BASE (and any other value, with a warning) makes a
+   !> Mesher run write every bispectrum entry at every post-mesh point to
+   !> BRM_EXPORT_FNAME_MSH; NONE leaves the setting as it is
+   !> (src/bsa/BsaLibImpl.F90:1917-1932).  The Classic approach exports only in
+   !> visual mode, as in the reference.
    subroutine bsa_setBRMExportDefaultMode(imode)
       integer(bsa_int_t), value :: imode
-      if (imode /= BSA_EXPORT_BRM_MODE_NONE) &
-         write(output_unit, '(1x, 2a)') WARNMSG, 'Bispectra export is not available in this library; ignored.'
+      if (imode == BSA_EXPORT_BRM_MODE_NONE) return
+      if (imode /= BSA_EXPORT_BRM_MODE_BASE) &
+         write(output_unit, '(1x, 2a)') WARNMSG, 'Unknown BRM export mode. Setting default  (base).'
+      brm_base_ = .true.
    end subroutine
 
-   !> Visual mode (src/bsa/BsaLibImpl.F90:119-122): with the Classic approach,
-   !> bsa_Run also writes the response bispectrum of the modal entry given by
-   !> bsa_setVisualIndexes over the whole frequency plane to
-   !> BRM_EXPORT_FNAME_CLS.  With the Mesher the reference re-reads a BFM dump
-   !> file; bsa_Run aborts with a message for that case.
+   !> Visual mode (src/bsa/BsaLibImpl.F90:119-122): bsa_Run also writes one
+   !> entry of the response bispectrum, the modal entry or the nodal bispectrum
+   !> given by bsa_setVisualIndexes: over the whole Classic plane to
+   !> BRM_EXPORT_FNAME_CLS (modal entries only, as the reference), at the Mesher's
+   !> post-mesh points to BRM_EXPORT_FNAME_MSH / BRN_EXPORT_FNAME_MSH.  The
+   !> reference's Mesher re-reads the BFM dump of an earlier run for it; the
+   !> device Mesher evaluates its pre-mesh in the same run instead.
    subroutine bsa_enableVisualMode()
       visual_ = .true.
    end subroutine
@@ -1365,6 +1373,98 @@ contains
       logical, value :: modal
       vis_idx_ = indexes
       vis_modal_ = modal
+   end subroutine
+
+   !> Bispectrum export of a Mesher run, in the reference's stream layout
+   !> (exportBRM_baseHeaderWriter_internal_ / exportBRM_base_internal_,
+   !> src/bsa/BsaLibImpl.F90:1849-1913, one zone per device tile):
+   !> * visual mode, modal indexes (i, j, k): BRM_EXPORT_FNAME_MSH with header
+   !>   3, i, j, k, 1 and the BRM entry (i, j, k) per point;
+   !> * visual mode, node and DOF: BRN_EXPORT_FNAME_MSH with header 2, node,
+   !>   DOF, 1 and the nodal bispectrum sum_mnp phi_m phi_n phi_p BRM_mnp of that
+   !>   DOF per point (getBRN_full_scalar_, BsaLibImpl.F90:1781-1803);
+   !> * otherwise (bsa_setBRMExportDefaultMode): BRM_EXPORT_FNAME_MSH with
+   !>   header NM, the kept modes, NM^3 and every entry per point
+   !>   (BsaLibImpl.F90:235-240).
+   !> Then the spatial-symmetry setting, the number of zones and of points; per
+   !> zone 1 (rectangle), nI, nJ and per point real32 f1, f2 and the values.
+   subroutine exportMesher_(msh, nme)
+      type(c_ptr), intent(in) :: msh
+      integer(bsa_int_t), intent(in) :: nme
+      integer(c_int32_t), allocatable, target :: dims(:, :)
+      real(bsa_real_t), allocatable, target :: coef(:, :), fi(:), fj(:), val(:, :)
+      real(real32), allocatable :: rec(:, :)
+      integer(int32), allocatable :: hidx(:)
+      integer(c_int64_t) :: npts
+      integer(c_int32_t) :: ncol, nt
+      integer(c_int) :: rc
+      integer(int64) :: off, q
+      integer :: iun, ios, k, m, n, o, d, ncomb
+      character(len=:), allocatable :: fname
+      type(c_ptr) :: pcoef
+      rc = bsahip_mesher_ntiles(msh, nt)
+      allocate(dims(2, nt))
+      pcoef = c_null_ptr
+      if (visual_ .and. vis_modal_) then
+         if (any(vis_idx_ < 1) .or. any(vis_idx_ > nme)) call bsa_Abort('Visual mode indexes out of the kept modes.')
+         ncol = 1
+         allocate(coef(nme**3, 1), source=0.0_bsa_real_t)
+         coef((vis_idx_(3) - 1) * nme * nme + (vis_idx_(2) - 1) * nme + vis_idx_(1), 1) = 1
+         fname = BRM_EXPORT_FNAME_MSH
+         hidx = int(vis_idx_, int32)
+         ncomb = 1
+      else if (visual_) then
+         ncol = 1
+         d = (vis_idx_(1) - 1) * nlibs_ + vis_idx_(2)
+         if (vis_idx_(1) < 1 .or. vis_idx_(2) < 1 .or. d > size(phi_, 1)) &
+            call bsa_Abort('Visual mode node / DOF out of range.')
+         allocate(coef(nme**3, 1))
+         do o = 1, nme
+            do n = 1, nme
+               do m = 1, nme
+                  coef(m + (n - 1) * nme + (o - 1) * nme * nme, 1) = &
+                     phi_(d, modes_(m)) * phi_(d, modes_(n)) * phi_(d, modes_(o))
+               end do
+            end do
+         end do
+         fname = BRN_EXPORT_FNAME_MSH
+         hidx = int(vis_idx_(1:2), int32)
+         ncomb = 1
+      else
+         ncol = int(nme**3, c_int32_t)
+         fname = BRM_EXPORT_FNAME_MSH
+         hidx = int(modes_, int32)
+         ncomb = nme**3
+      end if
+      if (allocated(coef)) pcoef = c_loc(coef)
+      rc = bsahip_mesher_export(msh, c_null_ptr, 0_c_int32_t, c_null_ptr, 0_c_int64_t, c_loc(dims), &
+                                c_null_ptr, c_null_ptr, c_null_ptr, npts)
+      if (rc /= 0) call bsa_Abort('bsahip_mesher_export failed: '//bsahip_last_error_f())
+      allocate(fi(npts), fj(npts), val(ncol, npts))
+      rc = bsahip_mesher_export(msh, c_null_ptr, ncol, pcoef, npts, c_loc(dims), c_loc(fi), c_loc(fj), &
+                                c_loc(val), npts)
+      if (rc /= 0) call bsa_Abort('bsahip_mesher_export failed: '//bsahip_last_error_f())
+      write(output_unit, '( /, 1x, 4a, i0, a, i0, a )') INFOMSG, 'Exporting  bispectrum  to  ', &
+         fname, ' (', nt, ' zones, ', npts, ' points)'
+      open(newunit=iun, file=fname, form='unformatted', access='stream', status='replace', &
+           action='readwrite', iostat=ios)
+      if (ios /= 0) call bsa_Abort('Error opening file  '//fname)
+      write(iun) int(size(hidx), int32), hidx, int(ncomb, int32), int(isym_, int32), int(nt, int32), &
+         int(npts, int32)
+      off = 0
+      do k = 1, nt
+         write(iun) 1_int32, dims(1, k), dims(2, k)
+         allocate(rec(2 + ncol, int(dims(1, k), int64) * dims(2, k)))
+         do q = 1, size(rec, 2, int64)
+            rec(1, q) = real(fi(off + q), real32)
+            rec(2, q) = real(fj(off + q), real32)
+            rec(3:, q) = real(val(:, off + q), real32)
+         end do
+         write(iun) rec
+         off = off + size(rec, 2, int64)
+         deallocate(rec)
+      end do
+      close(iun)
    end subroutine
 
    !> Visual-mode export of the Classic approach (BsaClassicImpl.F90:158-194

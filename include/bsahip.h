@@ -259,6 +259,39 @@ int bsahip_mesher_run(bsahip_mesher *m, const int32_t *levels, double *m3mf, dou
  * (ntiles x NM^3; m3mr is then their sum). */
 int bsahip_mesher_run_tiles(bsahip_mesher *m, const int32_t *levels, double *tile_m3mr,
                             double *m3mf, double *m3mr, int64_t *counts);
+/*
+ * Response bispectrum at the post-mesh (BRM) points of every tile, the data of
+ * the reference's Mesher bispectrum export (exportBRM_base_internal_ called per
+ * zone from src/bsa/meshing/zones/_shared_poly2d.fi:1040-2142): tile k holds a
+ * dims[2k] x dims[2k+1] grid, points in order f1 fastest, tiles in layout order,
+ * npts in total.  fi, fj (npts) get the point frequencies; val (ncol, npts) gets
+ * val(c, pt) = sum_mnp coef(mnp, c) BRM_mnp(pt) with coef (NM^3, ncol) -- a unit
+ * column selects one entry (visual mode), phi(d,m) phi(d,n) phi(d,p) gives the
+ * nodal bispectrum of DOF d; coef NULL means every entry (ncol = NM^3).  The BRM
+ * is the integrand of bsahip_mesher_run (interpolated BFM times the FRF
+ * product).  With fi, fj and val all NULL only dims and *npts are returned; else
+ * cap must hold npts.  levels as bsahip_mesher_run.  No reference counterpart
+ * for the coefficient form.
+ */
+int bsahip_mesher_export(bsahip_mesher *m, const int32_t *levels, int32_t ncol, const double *coef,
+                         int64_t cap, int32_t *dims, double *fi, double *fj, double *val,
+                         int64_t *npts);
+
+/*
+ * Checkpoint of the resident zones (the reference's BFM dump file,
+ * src/bsa/meshing/zones/MZone.F90:158-255, BsaMesherImpl.F90:1437-1448):
+ * bsahip_mesher_save writes the zone layout and the level-0 BFM to path;
+ * bsahip_mesher_load builds a Mesher for the same problem and configuration
+ * from it instead of evaluating the pre-mesh.  The layout the configuration
+ * gives must match the file's bit for bit (BSAHIP_EINVAL otherwise).  The file
+ * is this library's own format: magic "BSAHIP-MSH-DUMP", int32 version 1, NM,
+ * tiles, 0; int64 level-0 nodes per axis, level-0 evaluations; the tile
+ * geometry as bsahip_mesher_tiles (real64, 10 per tile); the rule levels
+ * (int32); the level-0 BFM (real64, NM^3 per point, nodes^2 points).
+ */
+int bsahip_mesher_save(const bsahip_mesher *m, const char *path);
+int bsahip_mesher_load(const bsahip_problem *p, const bsahip_mesher_cfg *cfg, int32_t flags,
+                       const char *path, bsahip_mesher **out);
 int bsahip_mesher_destroy(bsahip_mesher *m);
 
 /* ---- Learned Mesher proposer: a multilayer perceptron trained and evaluated

@@ -565,7 +565,7 @@ def reference_time(prob, threads: int) -> float:
     raise RuntimeError("reference timing line missing:\n" + r.stdout[-2000:])
 
 
-def mesher_integrate(prob, geom, levels, modes=None, points=None):
+def mesher_integrate(prob, geom, levels, modes=None, points=None, brm_out=None):
     """Zone quadrature of the Mesher on given tiles (CPU restatement of
     bsahip_mesher_run): per tile, the force bispectrum at the level-l grid by
     getFM_full_tnm_scalar_msh_ (msh_points), bilinear interpolation onto the
@@ -575,7 +575,9 @@ def mesher_integrate(prob, geom, levels, modes=None, points=None):
     sub1 sub2 (sub = BRM cells per level-0 cell).  Returns m3mf, m3mr and the
     BFM evaluation counts: level-0 points of all tiles counted once (they lie on
     one tensor grid), refinement points per tile.  points: the point BFM
-    (default msh_points; e.g. a msh_points_pod partial for the POD BFM)."""
+    (default msh_points; e.g. a msh_points_pod partial for the POD BFM).
+    brm_out: a list that receives, per tile, the BRM grid frequencies f1
+    (NA+1), f2 (NB+1) and the integrand without weights, brm (NM^3, NA+1, NB+1)."""
     modes = kept_modes(prob) if modes is None else np.asarray(modes)
     nm = len(modes)
     nm3 = nm ** 3
@@ -621,6 +623,8 @@ def mesher_integrate(prob, geom, levels, modes=None, points=None):
                                                                            order="F")
         m3f += np.einsum("xab,ab->x", Bb, W)
         m3r += np.einsum("xab,xab,ab->x", Bb, R, W)
+        if brm_out is not None:
+            brm_out.append({"f1": f1, "f2": f2, "brm": Bb * R})
     g = np.asarray(geom)
     nodes = set()
     for t in g[g[:, 2] == g[0, 2]]:

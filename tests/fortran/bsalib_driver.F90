@@ -4,7 +4,7 @@
 !!
 !! Usage:  bsalib_driver <input.bin> <output.bin> [fp64|bf16x3] [grid] [scalar] [mesher|both]
 !!                        [pod] [podn=<modes>] [podt=<percent>] [post] [gen] [diag]
-!!                        [visual=<i>,<j>,<k>] [visualn=<node>,<dof>] [sym=<n>]
+!!                        [visual=<i>,<j>,<k>] [visualn=<node>,<dof>] [sym=<n>] [brmbase]
 !! The input layout is the one tests write for the reference harness (header
 !! of 14 int32, then integer and real64 arrays in Fortran order).  "grid"
 !! leaves test mode off and passes damping ratios, so bsa_Run applies the
@@ -21,7 +21,8 @@
 !! "diag" selects the ONLY_DIAG model (bsa_setSpectraExtension(1)); "visual="
 !! enables visual mode with modal indexes (i, j, k), "visualn=" with a node and
 !! DOF (bsa_enableVisualMode, bsa_setVisualIndexes); "sym=" calls
-!! bsa_setSpatialSymmetry.
+!! bsa_setSpatialSymmetry; "brmbase" calls
+!! bsa_setBRMExportDefaultMode(BSA_EXPORT_BRM_MODE_BASE).
 !! Output: nme (int32), the used modes, then for each of m2mf, m2mr, m2o2mr,
 !! m3mf_cls, m3mr_cls, m3mf_msh, m3mr_msh: its size (int32, 0 when bsa_Run
 !! left it unallocated) and its values (real64).
@@ -97,6 +98,7 @@ program bsalib_driver
          call bsa_enableVisualMode()
          call bsa_setVisualIndexes(vis, .false.)
       end if
+      if (trim(arg) == 'brmbase') call bsa_setBRMExportDefaultMode(BSA_EXPORT_BRM_MODE_BASE)
       if (arg(1:4) == 'sym=') then
          read(arg(5:), *) isym
          call bsa_setSpatialSymmetry(isym)
