@@ -351,12 +351,15 @@ class Mesher:
     def __init__(self, prob, bkg_base_rfmnt: int = 20, tile_cells: int = 32, bkg_peakw=None,
                  bkg_area_ext: float = 2.0, peak_area_ext: float = 2.0, max_area_ext: float = 2.0,
                  bound: Bound = None, diag_crest: bool = False, natf=None, xsi=None,
-                 load: str = None):
+                 load: str = None, only_diag: bool = False):
         """load: a checkpoint written by save() for the same problem and
-        settings, read instead of evaluating the pre-mesh (bsahip_mesher_load)."""
+        settings, read instead of evaluating the pre-mesh (bsahip_mesher_load).
+        only_diag: the ONLY_DIAG model (NM entries per point instead of NM^3)."""
         b = bound or Bound(prob)
         self.nm = b.nm
+        self.nc = b.nm if only_diag else b.nm ** 3
         self.modes = b.modes
+        flags = C.c_int32(ONLY_DIAG if only_diag else 0)
         cfg, self._keep = mesher_cfg(prob, bkg_base_rfmnt, tile_cells, bkg_peakw, bkg_area_ext,
                                      peak_area_ext, max_area_ext, modes=b.modes,
                                      diag_crest=diag_crest, natf=natf, xsi=xsi)
@@ -365,10 +368,10 @@ class Mesher:
         h = C.c_void_p()
         self.h = None
         if load is None:
-            _check(lib().bsahip_mesher_create(C.addressof(b.struct), C.addressof(cfg),
-                                              C.c_int32(0), C.byref(h)))
+            _check(lib().bsahip_mesher_create(C.addressof(b.struct), C.addressof(cfg), flags,
+                                              C.byref(h)))
         else:
-            _check(lib().bsahip_mesher_load(C.addressof(b.struct), C.addressof(cfg), C.c_int32(0),
+            _check(lib().bsahip_mesher_load(C.addressof(b.struct), C.addressof(cfg), flags,
                                             str(load).encode(), C.byref(h)))
         self.h = h
         n = C.c_int32(0)
@@ -388,8 +391,8 @@ class Mesher:
 
     def run(self, levels=None, per_tile: bool = False) -> dict:
         """Post-mesh with per-tile levels (None: rule-based); per_tile adds
-        each tile's m3mr contribution as "tile_m3mr" (ntiles, NM^3)."""
-        nm3 = self.nm ** 3
+        each tile's m3mr contribution as "tile_m3mr" (ntiles, entries)."""
+        nm3 = self.nc
         m3mf, m3mr = np.zeros(nm3), np.zeros(nm3)
         counts = np.zeros(3, dtype=np.int64)
         lv = None
@@ -413,10 +416,10 @@ class Mesher:
 
     def export(self, levels=None, coef=None) -> dict:
         """BRM at every tile's post-mesh points (bsahip_mesher_export): coef
-        (NM^3, ncol) combinations, None for every entry.  Returns dims
+        (entries, ncol) combinations, None for every entry.  Returns dims
         (ntiles, 2), fi, fj (npts) and val (npts, ncol)."""
         lv = self._levels(levels)
-        nm3 = self.nm ** 3
+        nm3 = self.nc
         dims = np.zeros((self.ntiles, 2), dtype=np.int32)
         n = C.c_int64(0)
         L = lib()

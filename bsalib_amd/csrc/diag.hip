@@ -130,3 +130,33 @@ __global__ __launch_bounds__(256) void k_diag_bisp_sel(const double *__restrict_
    if (bfm) bfm[o] = b;
    if (brm) brm[o] = b * re3(hget(H, nft, m, F1), hget(H, nft, m, F2), hget(H, nft, m, s));
 }
+
+// Diagonal-model force bispectrum at points, the Mesher's ONLY_DIAG BFM
+// (getFM_diag_tnm_scalar_msh_, functionsImpl.F90:1123-1216, with S(f1+f2) in
+// its second and third products as the Classic model has them):
+//   out[slot[q]*nm + m] = 2 sum_k W_m(k) [S_k(f1) S_k(f2) + S_k(f1+f2) (S_k(f2) + S_k(f1))],
+// W_m(k) = Psi_q Psi_l^2, S_k = sq^2 from the table sq[(c*nu + u)*nnl + n] at rows
+// i1, i2, i3 (|f1|, |f2|, |f1+f2|).  grid (ceil(npts*nm/256)), block 256
+__global__ __launch_bounds__(256) void k_diag_points(
+   const double *__restrict__ sq, int nu, int nnl, int nnlp, const double *__restrict__ pa,
+   const double *__restrict__ pb, int K, int nm, int npts, const int *__restrict__ i1,
+   const int *__restrict__ i2, const int *__restrict__ i3, const long *__restrict__ slot,
+   double *__restrict__ out)
+{
+   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (gid >= (long)npts * nm) return;
+   const int q = (int)(gid / nm), m = (int)(gid % nm);
+   const int a1 = i1[q], a2 = i2[q], a3 = i3[q];
+   double s = 0.0;
+   for (int k = 0; k < K; ++k) {
+      const int c = k / nnlp, n = k % nnlp;
+      if (n >= nnl) continue;
+      const double *col = sq + (size_t)c * nu * nnl + n;
+      const double r1 = col[(size_t)a1 * nnl], r2 = col[(size_t)a2 * nnl];
+      const double r3 = a3 >= 0 ? col[(size_t)a3 * nnl] : 0.0;
+      const double S1 = r1 * r1, S2 = r2 * r2, S3 = r3 * r3;
+      const double a = pa[(size_t)m * K + k], b = pb[(size_t)m * K + k];
+      s += b * a * a * (S1 * S2 + S3 * S2 + S3 * S1);
+   }
+   out[(size_t)slot[q] * nm + m] = 2.0 * s;
+}

@@ -189,8 +189,21 @@ __global__ void k_frf_axis(double f0, double h, int n, const double *__restrict_
    H[t] = make_double2(r / d, -im / d);
 }
 
+// Entry e of the per-point bispectrum vector: the triple (m, n, p) of
+// m + n*nm + p*nm^2 (full model), or (e, e, e) (ONLY_DIAG model, nm entries).
+__device__ __forceinline__ void msh_entry(int e, int nm, int diag, int &m, int &n, int &p)
+{
+   if (diag) {
+      m = n = p = e;
+   } else {
+      m = e % nm;
+      n = (e / nm) % nm;
+      p = e / (nm * nm);
+   }
+}
+
 __global__ __launch_bounds__(256) void k_msh_integrate(
-   const double *__restrict__ bf, long ldq, int nm, int nb1, int nb2, int s1, int s2,
+   const double *__restrict__ bf, long ldq, int nm, int nc, int diag, int nb1, int nb2, int s1, int s2,
    const double2 *__restrict__ H1, const double2 *__restrict__ H2,
    const double2 *__restrict__ H3, double f3a, double h1, double h2, double wscale,
    const double *__restrict__ Mm, const double *__restrict__ Km, const double *__restrict__ Cm,
@@ -198,10 +211,11 @@ __global__ __launch_bounds__(256) void k_msh_integrate(
 {
    // H1[a*nm + m] at f1 = f1a + a h1, H2[b*nm + n] at f2 = f2a + b h2; H3[(a+b)*nm + p]
    // at f1 + f2 when h1 == h2 (H3 != NULL), else H_p(f1+f2) is evaluated here
-   const int nm3 = nm * nm * nm;
+   const int nm3 = nc;
    const int mnp = blockIdx.x * blockDim.x + threadIdx.x;
    if (mnp >= nm3) return;
-   const int m = mnp % nm, n = (mnp / nm) % nm, p = mnp / (nm * nm);
+   int m, n, p;
+   msh_entry(mnp, nm, diag, m, n, p);
    const int nc1 = nb1 - 1, nc2 = nb2 - 1, ncell = nc1 * nc2;
    const int per = (ncell + gridDim.y - 1) / gridDim.y;
    const int c0 = blockIdx.y * per, c1 = min(ncell, c0 + per);
@@ -250,17 +264,17 @@ __global__ __launch_bounds__(256) void k_msh_integrate(
 // Tile export: the integrand of k_msh_integrate without the weights at every
 // BRM point (a, b) of the tile, pt = a + b*(NA+1).  Column c of val
 // (val[pt*ncol + c]) is BRM entry sel[c] when sel[c] >= 0, else
-// sum_mnp coef[c*nm3 + mnp] BRM_mnp.  grid (ceil(npts/256)), block 256
+// sum_e coef[c*nc + e] BRM_e over the nc entries.  grid (ceil(npts/256)), block 256
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_msh_export(
-   const double *__restrict__ bf, long ldq, int nm, int nb1, int nb2, int s1, int s2,
-   const double2 *__restrict__ H1, const double2 *__restrict__ H2,
+   const double *__restrict__ bf, long ldq, int nm, int nc, int diag, int nb1, int nb2, int s1,
+   int s2, const double2 *__restrict__ H1, const double2 *__restrict__ H2,
    const double2 *__restrict__ H3, double f3a, double h1, double h2,
    const double *__restrict__ Mm, const double *__restrict__ Km, const double *__restrict__ Cm,
    int ncol, const int *__restrict__ sel, const double *__restrict__ coef,
    double *__restrict__ val)
 {
-   const int nm3 = nm * nm * nm;
+   const int nm3 = nc;
    const int nc1 = nb1 - 1, nc2 = nb2 - 1;
    const long NA1 = (long)nc1 * s1 + 1, NB1 = (long)nc2 * s2 + 1;
    const long pt = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -272,7 +286,8 @@ __global__ __launch_bounds__(256) void k_msh_export(
    const double inv1 = 1.0 / s1, inv2 = 1.0 / s2;
    const double sv = bb * inv2, t = aa * inv1;
    auto brm = [&](int mnp) {
-      const int m = mnp % nm, n = (mnp / nm) % nm, p = mnp / (nm * nm);
+      int m, n, p;
+      msh_entry(mnp, nm, diag, m, n, p);
       const double b00 = bf[q00 * nm3 + mnp], b10 = bf[(q00 + 1) * nm3 + mnp];
       const double b01 = bf[(q00 + ldq) * nm3 + mnp], b11 = bf[(q00 + ldq + 1) * nm3 + mnp];
       const double e0 = (1.0 - sv) * b00 + sv * b01, e1 = (1.0 - sv) * b10 + sv * b11;
@@ -557,7 +572,9 @@ struct bsahip_mesher {
    int nmp = 0;
    std::vector<MshTile> tiles;
    double dfref = 0.0;
-   double *lv0 = nullptr;     // resident level-0 BFM on the global grid: [i + j*nx][nm3]
+   int nc = 0;                // bispectrum entries per point: NM^3, or NM (ONLY_DIAG)
+   bool diag = false;         // the ONLY_DIAG model
+   double *lv0 = nullptr;     // resident level-0 BFM on the global grid: [i + j*nx][nc]
    long nx = 0;               // level-0 nodes per axis
    double *PB = nullptr;      // [K][NMP]
    long evals_lv0 = 0;
@@ -599,6 +616,25 @@ int msh_eval_chunk(bsahip_mesher *M, int npts, const double *f1p, const double *
    const std::vector<long> slot(slot_p, slot_p + npts);
    point_tables(npts, f1p, f2p, uq, i1, i2, i3);
    const int nu = (int)uq.size();
+   if (M->diag) {
+      double *fl, *sq;
+      int *d1, *d2, *d3;
+      long *ds;
+      CHK(upload(M->B, "m_fl", uq, &fl));
+      CHK(M->B.get("m_sq", (size_t)D.ntc * nu * D.nnl, &sq));
+      dim3 gp((D.nnl + 63) / 64, nu, D.ntc);
+      k_psd_sqrt<<<gp, 64, 0, g.s>>>(fl, nu, D.prm, D.formula, D.nnl, D.ntc, D.pulsation, sq);
+      HIPC(hipGetLastError());
+      CHK(upload(M->B, "m_i1", i1, &d1));
+      CHK(upload(M->B, "m_i2", i2, &d2));
+      CHK(upload(M->B, "m_i3", i3, &d3));
+      CHK(upload(M->B, "m_slot", slot, &ds));
+      const long tot = (long)npts * D.nm;
+      k_diag_points<<<(unsigned)((tot + 255) / 256), 256, 0, g.s>>>(
+         sq, nu, D.nnl, D.nnlp, D.pa_k, D.pb_k, D.K, D.nm, npts, d1, d2, d3, ds, out);
+      HIPC(hipGetLastError());
+      return 0;
+   }
    const long ldU = (long)D.nm * nu;
    double *fl, *U, *V;
    CHK(upload(M->B, "m_fl", uq, &fl));
@@ -699,12 +735,13 @@ int msh_integrate(bsahip_mesher *M, const MshTile &t, int l, const double *bf, l
    MshBrmGrid G = msh_brm_grid(t, l);
    CHK(msh_frf_tables(M, t, G));
    const double ws = (2.0 * M_PI * G.h1) * (2.0 * M_PI * G.h2);
-   const int nm3 = D.nm * D.nm * D.nm;
+   const int nm3 = M->nc;
    const int ncell = (nb1 - 1) * (nb2 - 1);
    const int bx = (nm3 + 255) / 256;
    const int nchunk = std::max(1, std::min(ncell, std::max(1, 4096 / bx)));
    dim3 gs(bx, nchunk);
-   k_msh_integrate<<<gs, 256, 0, g.s>>>(bf, ldq, D.nm, nb1, nb2, G.s1, G.s2, G.H1, G.H2, G.H3,
+   k_msh_integrate<<<gs, 256, 0, g.s>>>(bf, ldq, D.nm, M->nc, M->diag ? 1 : 0, nb1, nb2, G.s1,
+                                         G.s2, G.H1, G.H2, G.H3,
                                          t.a1 + t.a2, G.h1, G.h2, ws, D.Mm, D.Km, D.Cm, m3f, m3r);
    HIPC(hipGetLastError());
    brm_pts += (long)(G.NA + 1) * (G.NB + 1);
@@ -720,8 +757,7 @@ using MshVisit = std::function<int(size_t, const MshTile &, int, const double *,
 // fine grids fit the staging buffer.  Returns the refinement evaluations.
 int msh_post(bsahip_mesher *m, const int32_t *levels, const MshVisit &visit, long &evals_post)
 {
-   Prob &D = m->D;
-   const size_t nm3 = (size_t)D.nm * D.nm * D.nm, nt = m->tiles.size();
+   const size_t nm3 = (size_t)m->nc, nt = m->tiles.size();
    const size_t cap = std::max<size_t>(66049, ((size_t)1 << 29) / nm3);   // points (4 GiB)
    std::vector<double> f1, f2, e1, e2;
    std::vector<long> slot;
@@ -804,7 +840,9 @@ int msh_setup(const bsahip_problem *p, const bsahip_mesher_cfg *cfg, int32_t fla
       return fail(BSAHIP_EINVAL, "Mesher extensions and bkg_peakw must be > 0, tile_cells in 1..128");
    M.reset(new bsahip_mesher());
    CHK(load_problem(p, 0, M->B, M->D));
-   CHK(set_pod(p, M->D));
+   // the ONLY_DIAG model has no coherence between nodes and no POD
+   // (BsaMesherImpl.F90:68: the POD only with i_only_diag_ == 0)
+   if (!(flags & BSAHIP_ONLY_DIAG)) CHK(set_pod(p, M->D));
    Prob &D = M->D;
    for (int i = 0; i < D.nm; ++i)
       if (!(cfg->natf[i] > 0.0) || !(cfg->xsi[i] > 0.0))
@@ -817,7 +855,9 @@ int msh_setup(const bsahip_problem *p, const bsahip_mesher_cfg *cfg, int32_t fla
    CHK(M->B.get("m_PB", (size_t)D.K * M->nmp, &M->PB));
    k_pbt<<<(unsigned)((D.K * M->nmp + 255) / 256), 256, 0, g.s>>>(D.pb_k, D.nm, D.K, M->nmp, M->PB);
    HIPC(hipGetLastError());
-   const size_t nm3 = (size_t)D.nm * D.nm * D.nm;
+   M->diag = (flags & BSAHIP_ONLY_DIAG) != 0;
+   M->nc = M->diag ? D.nm : D.nm * D.nm * D.nm;
+   const size_t nm3 = (size_t)M->nc;
    x = msh_axis_nodes(M->tiles);
    M->nx = (long)x.size();
    CHK(M->B.get("m_lv0", x.size() * x.size() * nm3, &M->lv0));
@@ -919,7 +959,7 @@ int bsahip_mesher_features(bsahip_mesher *m, int32_t ns, double *stencil)
 {
    std::lock_guard<std::recursive_mutex> lk(g.mu);
    if (!m || !stencil || ns < 2) return fail(BSAHIP_EINVAL, "need a Mesher, ns >= 2 and an output");
-   const int nm3 = m->D.nm * m->D.nm * m->D.nm;
+   const int nm3 = m->nc;
    const size_t nt = m->tiles.size(), per = (size_t)ns * ns;
    double *d;
    CHK(m->B.get("m_stencil", nt * per, &d));
@@ -939,8 +979,7 @@ int bsahip_mesher_run_tiles(bsahip_mesher *m, const int32_t *levels, double *til
 {
    std::lock_guard<std::recursive_mutex> lk(g.mu);
    if (!m) return fail(BSAHIP_EINVAL, "NULL Mesher");
-   Prob &D = m->D;
-   const size_t nm3 = (size_t)D.nm * D.nm * D.nm, nt = m->tiles.size();
+   const size_t nm3 = (size_t)m->nc, nt = m->tiles.size();
    CHK(msh_check_levels(m, levels));
    // accumulators: m3f, m3r, then one m3r slot per tile when asked for
    double *acc;
@@ -993,7 +1032,7 @@ int bsahip_mesher_export(bsahip_mesher *m, const int32_t *levels, int32_t ncol, 
    if (!m || !npts) return fail(BSAHIP_EINVAL, "NULL Mesher or point count");
    CHK(msh_check_levels(m, levels));
    Prob &D = m->D;
-   const int nm3 = D.nm * D.nm * D.nm;
+   const int nm3 = m->nc;
    const size_t nt = m->tiles.size();
    // sizes: the BRM grid of every tile at its level
    std::vector<int64_t> off(nt + 1, 0);
@@ -1009,7 +1048,7 @@ int bsahip_mesher_export(bsahip_mesher *m, const int32_t *levels, int32_t ncol, 
    *npts = off[nt];
    if (!fi && !fj && !val) return 0;
    if (ncol < 0 || (ncol > 0 && !val) || (!coef && ncol != nm3))
-      return fail(BSAHIP_EINVAL, "ncol < 0, NULL values, or NULL coef with ncol != NM^3");
+      return fail(BSAHIP_EINVAL, "ncol < 0, NULL values, or NULL coef with ncol != entries");
    if (cap < off[nt]) return fail(BSAHIP_EINVAL, "cap below the number of export points");
    // unit columns take the single-entry path
    std::vector<int> sel(std::max(ncol, 1), -1);
@@ -1049,7 +1088,7 @@ int bsahip_mesher_export(bsahip_mesher *m, const int32_t *levels, int32_t ncol, 
                    double *dv;
                    CHK(m->B.get("m_xval", (size_t)np * ncol, &dv));
                    k_msh_export<<<(unsigned)((np + 255) / 256), 256, 0, g.s>>>(
-                      bf, ldq, D.nm, nb1, nb2, G.s1, G.s2, G.H1, G.H2, G.H3, t.a1 + t.a2, G.h1,
+                      bf, ldq, D.nm, m->nc, m->diag ? 1 : 0, nb1, nb2, G.s1, G.s2, G.H1, G.H2, G.H3, t.a1 + t.a2, G.h1,
                       G.h2, D.Mm, D.Km, D.Cm, ncol, dsel, dcoef, dv);
                    HIPC(hipGetLastError());
                    HIPC(hipMemcpyAsync(val + (size_t)off[k] * ncol, dv,
@@ -1066,10 +1105,10 @@ int bsahip_mesher_save(const bsahip_mesher *m, const char *path)
 {
    std::lock_guard<std::recursive_mutex> lk(g.mu);
    if (!m || !path) return fail(BSAHIP_EINVAL, "NULL Mesher or path");
-   const size_t nt = m->tiles.size(), nm3 = (size_t)m->D.nm * m->D.nm * m->D.nm;
+   const size_t nt = m->tiles.size(), nm3 = (size_t)m->nc;
    std::unique_ptr<FILE, int (*)(FILE *)> f(std::fopen(path, "wb"), std::fclose);
    if (!f) return fail(BSAHIP_EINVAL, std::string("cannot open ") + path + " for writing");
-   const int32_t head[4] = {1, m->D.nm, (int32_t)nt, 0};
+   const int32_t head[4] = {1, m->D.nm, (int32_t)nt, m->diag ? 1 : 0};
    const int64_t sizes[2] = {m->nx, m->evals_lv0};
    std::vector<double> geom(10 * nt);
    std::vector<int32_t> rule(nt);
@@ -1101,7 +1140,7 @@ int bsahip_mesher_load(const bsahip_problem *p, const bsahip_mesher_cfg *cfg, in
    CHK(msh_setup(p, cfg, flags, M, x));
    std::unique_ptr<FILE, int (*)(FILE *)> f(std::fopen(path, "rb"), std::fclose);
    if (!f) return fail(BSAHIP_EINVAL, std::string("cannot open ") + path);
-   const size_t nt = M->tiles.size(), nm3 = (size_t)M->D.nm * M->D.nm * M->D.nm;
+   const size_t nt = M->tiles.size(), nm3 = (size_t)M->nc;
    char magic[16];
    int32_t head[4];
    int64_t sizes[2];
@@ -1109,8 +1148,10 @@ int bsahip_mesher_load(const bsahip_problem *p, const bsahip_mesher_cfg *cfg, in
        std::fread(head, sizeof(head), 1, f.get()) != 1 || head[0] != 1 ||
        std::fread(sizes, sizeof(sizes), 1, f.get()) != 1)
       return fail(BSAHIP_EINVAL, std::string(path) + " is not a Mesher checkpoint of this version");
-   if (head[1] != M->D.nm || head[2] != (int32_t)nt || sizes[0] != M->nx)
-      return fail(BSAHIP_EINVAL, "checkpoint modes or zone layout differ from this problem and configuration");
+   if (head[1] != M->D.nm || head[2] != (int32_t)nt || head[3] != (M->diag ? 1 : 0) ||
+       sizes[0] != M->nx)
+      return fail(BSAHIP_EINVAL,
+                  "checkpoint modes, model or zone layout differ from this problem and configuration");
    std::vector<double> geom(10 * nt), want(10 * nt);
    std::vector<int32_t> rule(nt), rwant(nt);
    msh_geom(M.get(), want.data(), rwant.data());

@@ -663,8 +663,6 @@ contains
       if (isuban_ < 1 .or. isuban_ > 3) call bsa_Abort('Analysis type must be 1, 2 or 3.')
       do_cls = isuban_ == 1 .or. isuban_ == 3
       do_msh = isuban_ == 2 .or. isuban_ == 3
-      if (ionly_diag_ /= 0 .and. do_msh) &
-         call bsa_Abort('The ONLY_DIAG model is available for the Classic approach only in this library.')
       if (iconv_ /= BSA_PSD_CONVENTION_PULS) &
          call bsa_Abort('Only the pulsation convention (BSA_PSD_CONVENTION_PULS) is available in this library.')
 
@@ -861,7 +859,8 @@ contains
          real(bsa_real_t), allocatable, target :: fk(:), xk(:)
          real(bsa_real_t) :: v(3, 3), bw
          integer(c_int64_t), target :: counts(3)
-         integer(bsa_int_t) :: z, nmk
+         integer(bsa_int_t) :: z, nmk, nc
+         integer(c_int32_t) :: mflags
          integer(c_int) :: rc2
          if (.not. allocated(xsi_)) &
             call bsa_Abort('The Mesher needs the modal damping ratios (bsa_setTotDamping).')
@@ -903,9 +902,16 @@ contains
             if (do_trunc_pod_) pm%pod_trunc_lim = pod_trunc_lim_
             if (npod_set_) pm%pod_nmodes = npod_
          end if
-         rc2 = bsahip_mesher_create(pm, cfg, 0_c_int32_t, msh)
+         ! ONLY_DIAG: NM entries (dimM_bisp_ = NM), no POD (BsaMesherImpl.F90:68)
+         nc = nmk**3
+         mflags = 0
+         if (ionly_diag_ /= 0) then
+            nc = nmk
+            mflags = BSAHIP_ONLY_DIAG
+         end if
+         rc2 = bsahip_mesher_create(pm, cfg, mflags, msh)
          if (rc2 /= 0) call bsa_Abort('bsahip_mesher_create failed: '//bsahip_last_error_f())
-         allocate(m3mf(nmk**3), m3mr(nmk**3))
+         allocate(m3mf(nc), m3mr(nc))
          rc2 = bsahip_mesher_run(msh, c_null_ptr, c_loc(m3mf), c_loc(m3mr), c_loc(counts))
          if (rc2 /= 0) call bsa_Abort('bsahip_mesher_run failed: '//bsahip_last_error_f())
          if (visual_ .or. brm_base_) call exportMesher_(msh, nmk)
@@ -1399,42 +1405,59 @@ contains
       integer(c_int32_t) :: ncol, nt
       integer(c_int) :: rc
       integer(int64) :: off, q
-      integer :: iun, ios, k, m, n, o, d, ncomb
+      integer :: iun, ios, k, m, n, o, d, ncomb, nc
       character(len=:), allocatable :: fname
       type(c_ptr) :: pcoef
       rc = bsahip_mesher_ntiles(msh, nt)
       allocate(dims(2, nt))
       pcoef = c_null_ptr
+      ! entries per point: NM^3, or NM with the ONLY_DIAG model, whose visual
+      ! index is the mode (BsaLibImpl.F90:387-396)
+      nc = nme**3
+      if (ionly_diag_ /= 0) nc = nme
       if (visual_ .and. vis_modal_) then
-         if (any(vis_idx_ < 1) .or. any(vis_idx_ > nme)) call bsa_Abort('Visual mode indexes out of the kept modes.')
          ncol = 1
-         allocate(coef(nme**3, 1), source=0.0_bsa_real_t)
-         coef((vis_idx_(3) - 1) * nme * nme + (vis_idx_(2) - 1) * nme + vis_idx_(1), 1) = 1
+         allocate(coef(nc, 1), source=0.0_bsa_real_t)
+         if (ionly_diag_ /= 0) then
+            if (vis_idx_(1) < 1 .or. vis_idx_(1) > nme) call bsa_Abort('Visual mode index out of the kept modes.')
+            coef(vis_idx_(1), 1) = 1
+         else
+            if (any(vis_idx_ < 1) .or. any(vis_idx_ > nme)) call bsa_Abort('Visual mode indexes out of the kept modes.')
+            coef((vis_idx_(3) - 1) * nme * nme + (vis_idx_(2) - 1) * nme + vis_idx_(1), 1) = 1
+         end if
          fname = BRM_EXPORT_FNAME_MSH
          hidx = int(vis_idx_, int32)
          ncomb = 1
       else if (visual_) then
+         ! nodal bispectrum of one DOF: getBRN_full_scalar_ / getBRN_diag_scalar_
+         ! (BsaLibImpl.F90:1760-1803)
          ncol = 1
          d = (vis_idx_(1) - 1) * nlibs_ + vis_idx_(2)
          if (vis_idx_(1) < 1 .or. vis_idx_(2) < 1 .or. d > size(phi_, 1)) &
             call bsa_Abort('Visual mode node / DOF out of range.')
-         allocate(coef(nme**3, 1))
-         do o = 1, nme
-            do n = 1, nme
-               do m = 1, nme
-                  coef(m + (n - 1) * nme + (o - 1) * nme * nme, 1) = &
-                     phi_(d, modes_(m)) * phi_(d, modes_(n)) * phi_(d, modes_(o))
+         allocate(coef(nc, 1))
+         if (ionly_diag_ /= 0) then
+            do m = 1, nme
+               coef(m, 1) = phi_(d, modes_(m))**3
+            end do
+         else
+            do o = 1, nme
+               do n = 1, nme
+                  do m = 1, nme
+                     coef(m + (n - 1) * nme + (o - 1) * nme * nme, 1) = &
+                        phi_(d, modes_(m)) * phi_(d, modes_(n)) * phi_(d, modes_(o))
+                  end do
                end do
             end do
-         end do
+         end if
          fname = BRN_EXPORT_FNAME_MSH
          hidx = int(vis_idx_(1:2), int32)
          ncomb = 1
       else
-         ncol = int(nme**3, c_int32_t)
+         ncol = int(nc, c_int32_t)
          fname = BRM_EXPORT_FNAME_MSH
          hidx = int(modes_, int32)
-         ncomb = nme**3
+         ncomb = nc
       end if
       if (allocated(coef)) pcoef = c_loc(coef)
       rc = bsahip_mesher_export(msh, c_null_ptr, 0_c_int32_t, c_null_ptr, 0_c_int64_t, c_loc(dims), &

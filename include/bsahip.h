@@ -57,7 +57,8 @@ extern "C" {
 #define BSAHIP_NO_PSD        (1 << 9)  /* skip m2 (modal PSD) outputs   */
 #define BSAHIP_NO_BISP       (1 << 10) /* skip m3 (bispectrum) outputs  */
 
-/* the ONLY_DIAG model (flag bit 11; bsahip_classic_run, FP64): loaded nodes
+/* the ONLY_DIAG model (flag bit 11; bsahip_classic_run, bsahip_classic_bispectra_sel,
+ * bsahip_mesher_create; FP64): loaded nodes
  * without coherence between them, diagonal modal entries only, every m2* and
  * m3* output holds NM values (getFM_diag_tnlm_vect_cls_ / getRM_diag_vect_cls_,
  * functionsImpl.F90:1747-2101) */
@@ -234,7 +235,11 @@ typedef struct bsahip_mesher_cfg {
 typedef struct bsahip_mesher bsahip_mesher;
 
 /* Builds the layout and evaluates the level-0 BFM of every tile (pre-mesh),
- * kept resident until bsahip_mesher_destroy.  FP64 only; at most 32 modes. */
+ * kept resident until bsahip_mesher_destroy.  FP64 only; at most 32 modes.
+ * flags may hold BSAHIP_ONLY_DIAG: the diagonal model
+ * (getFM_diag_tnm_scalar_msh_ / getRM_diag_scalar_msh_, functionsImpl.F90:1123-1286,
+ * with S(f1+f2) in its products as the Classic diagonal model has them), NM
+ * entries per point in every Mesher output instead of NM^3; no POD. */
 int bsahip_mesher_create(const bsahip_problem *p, const bsahip_mesher_cfg *cfg, int32_t flags,
                          bsahip_mesher **out);
 /* The tile layout alone (host computation, no device): *ntiles always; geom

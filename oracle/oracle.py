@@ -565,7 +565,7 @@ def reference_time(prob, threads: int) -> float:
     raise RuntimeError("reference timing line missing:\n" + r.stdout[-2000:])
 
 
-def mesher_integrate(prob, geom, levels, modes=None, points=None, brm_out=None):
+def mesher_integrate(prob, geom, levels, modes=None, points=None, brm_out=None, diag=False):
     """Zone quadrature of the Mesher on given tiles (CPU restatement of
     bsahip_mesher_run): per tile, the force bispectrum at the level-l grid by
     getFM_full_tnm_scalar_msh_ (msh_points), bilinear interpolation onto the
@@ -577,10 +577,14 @@ def mesher_integrate(prob, geom, levels, modes=None, points=None, brm_out=None):
     one tensor grid), refinement points per tile.  points: the point BFM
     (default msh_points; e.g. a msh_points_pod partial for the POD BFM).
     brm_out: a list that receives, per tile, the BRM grid frequencies f1
-    (NA+1), f2 (NB+1) and the integrand without weights, brm (NM^3, NA+1, NB+1)."""
+    (NA+1), f2 (NB+1) and the integrand without weights, brm (NM^3, NA+1, NB+1).
+    diag: the ONLY_DIAG model -- NM entries per point (default points
+    msh_points_diag) and the FRF product Re[H_m(f1) H_m(f2) conj(H_m(f1+f2))]."""
     modes = kept_modes(prob) if modes is None else np.asarray(modes)
     nm = len(modes)
-    nm3 = nm ** 3
+    nm3 = nm if diag else nm ** 3
+    if points is None:
+        points = msh_points_diag if diag else msh_points
     m3f = np.zeros(nm3)
     m3r = np.zeros(nm3)
     ev1 = 0
@@ -594,7 +598,7 @@ def mesher_integrate(prob, geom, levels, modes=None, points=None, brm_out=None):
         g2 = a2 + np.arange(nb2) * (b2 - a2) / (nb2 - 1)
         g1[-1], g2[-1] = b1, b2
         F1, F2 = np.meshgrid(g1, g2, indexing="ij")
-        B = (points or msh_points)(prob, F1.ravel(), F2.ravel()).reshape(nm3, nb1, nb2)
+        B = points(prob, F1.ravel(), F2.ravel()).reshape(nm3, nb1, nb2)
         if lv > 1:
             ev1 += nb1 * nb2 - (n1 + 1) * (n2 + 1)
         s1, s2 = max(1, sub1 // lv), max(1, sub2 // lv)
@@ -619,8 +623,11 @@ def mesher_integrate(prob, geom, levels, modes=None, points=None, brm_out=None):
         H1 = frf(prob, f1, modes)                                       # (NA+1, nm)
         H2 = frf(prob, f2, modes)                                       # (NB+1, nm)
         H3 = frf(prob, f1[:, None] + f2[None, :], modes)               # (NA+1, NB+1, nm)
-        R = np.einsum("am,bn,abp->mnpab", H1, H2, np.conj(H3)).real.reshape(nm3, NA + 1, NB + 1,
-                                                                           order="F")
+        if diag:
+            R = np.einsum("am,bm,abm->mab", H1, H2, np.conj(H3)).real
+        else:
+            R = np.einsum("am,bn,abp->mnpab", H1, H2, np.conj(H3)).real.reshape(
+                nm3, NA + 1, NB + 1, order="F")
         m3f += np.einsum("xab,ab->x", Bb, W)
         m3r += np.einsum("xab,xab,ab->x", Bb, R, W)
         if brm_out is not None:
@@ -745,6 +752,29 @@ def peak_factors(m2, m2o2, obs_time: float, sk=None):
 # ---------------------------------------------------------------------------
 # ONLY_DIAG Classic model -- numpy restatement
 # ---------------------------------------------------------------------------
+def msh_points_diag(prob, fi, fj) -> np.ndarray:
+    """getFM_diag_tnm_scalar_msh_ (functionsImpl.F90:1123-1216) at each point,
+    returns bfm(nm, npts): per loaded node n and component,
+      2 Psi_q Psi_l^2 [S_n(f1) S_n(f2) + S_n(f1+f2) S_n(f2) + S_n(f1+f2) S_n(f1)]
+    (after the lib sums, as classic_diag), S(f1+f2) evaluated at the point.
+    The reference forms the second and third products as S(f1) S(f2)^2 and
+    S(f1)^2 S(f2) (Suvw_IJJ, Suvw_IJI at :1161-1163) and leaves S(f1+f2),
+    which it evaluates, unused; this follows the Classic model instead
+    (classic_diag), so that both approaches integrate one spectrum."""
+    modes = kept_modes(prob)
+    P = psi(prob, modes)
+    fi = np.atleast_1d(np.asarray(fi, dtype=np.float64))
+    fj = np.atleast_1d(np.asarray(fj, dtype=np.float64))
+    out = np.zeros((len(modes), fi.size))
+    for t in prob.tc:
+        S1 = eval_psd(prob, fi, int(t))                 # (npts, nnl)
+        S2 = eval_psd(prob, fj, int(t))
+        S3 = eval_psd(prob, fi + fj, int(t))
+        W = P[:, :, t + 2] * P[:, :, t - 1] ** 2        # (nm, nnl)
+        out += 2.0 * W @ (S1 * S2 + S3 * S2 + S3 * S1).T
+    return out
+
+
 def classic_diag(prob) -> dict:
     """getFM_diag_tnlm_vect_cls_ (functionsImpl.F90:1747-1970) and
     getRM_diag_vect_cls_ (:1975-2101) on the Classic grid, then

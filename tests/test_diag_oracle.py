@@ -36,3 +36,19 @@ def test_diag_model_is_not_the_diagonal_of_the_full_model():
     nm = g["m2mf_diag"].size
     full_diag = g["m2mf"].reshape(nm, nm, order="F").diagonal()
     assert np.all(full_diag > g["m2mf_diag"])
+
+
+@pytest.mark.parametrize("name", ["tiny_u", "tiny_uvw", "tiny_dav"])
+def test_msh_points_diag_equals_reference_classic_diag(name):
+    """The point form of the diagonal model (the Mesher's BFM) equals the
+    reference's own Classic diagonal bispectrum at every grid point where
+    f1 + f2 stays on the grid (there the Classic zero padding does not act)."""
+    p = CASES[name]()
+    g = load_golden(name)
+    f = p.freqs()
+    nf = f.size
+    c = nf // 2
+    ii, jj = np.meshgrid(np.arange(nf), np.arange(nf), indexing="ij")
+    inside = np.abs(ii + jj - 2 * c) <= c
+    b = O.msh_points_diag(p, f[ii[inside]], f[jj[inside]])          # (nm, npts)
+    assert rel_err(b.T, g["bisp_f_diag"][inside]) < 1e-13

@@ -2,7 +2,13 @@
 BSAHIP_ONLY_DIAG): moments equal the quadrature of the reference's own
 getFM_diag_tnlm_vect_cls_ / getRM_diag_vect_cls_ spectra (golden *_diag) --
 1e-12; the Riemann rule equals the oracle's spectra with plain weights; the
-Fortran front's bsa_setSpectraExtension(1) returns the same NM-entry moments."""
+Fortran front's bsa_setSpectraExtension(1) returns the same NM-entry moments.
+
+The Mesher's ONLY_DIAG model (bsahip_mesher_create with BSAHIP_ONLY_DIAG):
+moments on mixed-level tiles and the exported integrand equal the CPU
+restatement (oracle.msh_points_diag, pinned to the reference's diagonal
+bispectra) at 1e-12; its checkpoint reloads only as a diagonal Mesher; the
+front's analysis types 2 and 3 with bsa_setSpectraExtension(1) equal the C ABI."""
 import numpy as np
 import pytest
 
@@ -48,3 +54,50 @@ def test_fortran_front_diag(gpu_lib, name):
     for k in ("m2mf", "m2mr", "m3mf", "m3mr"):
         assert out[k].size == g[f"{k}_diag"].size
         assert rel_err(out[k], g[f"{k}_diag"]) < 1e-12, k
+
+
+def test_mesher_diag_matches_oracle(gpu_lib, tmp_path):
+    p = CASES["tiny_uvw"]()
+    kw = dict(bkg_base_rfmnt=5, tile_cells=2, bkg_peakw=0.25)
+    M = abi.Mesher(p, only_diag=True, **kw)
+    geom, _ = M.tiles()
+    lv = np.random.default_rng(4).choice([1, 2, 4, 8], size=M.ntiles).astype(np.int32)
+    r = M.run(levels=lv)
+    tiles = []
+    o = O.mesher_integrate(p, geom, lv, diag=True, brm_out=tiles)
+    assert r["m3mf"].size == M.nm
+    assert rel_err(r["m3mf"], o["m3mf"]) < 1e-12
+    assert rel_err(r["m3mr"], o["m3mr"]) < 1e-12
+    assert (r["evals_pre"], r["evals_post"]) == (o["evals_pre"], o["evals_post"])
+    e = M.export(levels=lv)
+    want = np.concatenate([t["brm"].reshape(M.nm, -1, order="F").T for t in tiles])
+    assert rel_err(e["val"], want) < 1e-12
+    # the checkpoint carries the model
+    path = tmp_path / "diag.dump"
+    M.save(path)
+    M.close()
+    L = abi.Mesher(p, only_diag=True, load=path, **kw)
+    assert rel_err(L.run(levels=lv)["m3mr"], r["m3mr"]) < 1e-13
+    L.close()
+    with pytest.raises(abi.BsaHipError) as ei:
+        abi.Mesher(p, load=path, **kw)
+    assert ei.value.code == abi.EINVAL
+
+
+@pytest.mark.parametrize("mode", ["mesher", "both"])
+def test_fortran_front_mesher_diag(gpu_lib, mode):
+    from bsalib_amd import host
+    p = CASES["tiny_uvw"]()
+    out, r = fdriver.run(p, mode, "diag")
+    assert "Mesher on the GPU" in r.stdout
+    M = abi.Mesher(p, bkg_base_rfmnt=20, tile_cells=32, bkg_peakw=host.bkg_peak_width(p),
+                   only_diag=True)
+    want = M.run()
+    M.close()
+    assert out["m3mf_msh"].size == M.nm
+    assert rel_err(out["m3mf_msh"], want["m3mf"]) < 1e-13
+    assert rel_err(out["m3mr_msh"], want["m3mr"]) < 1e-13
+    if mode == "both":
+        g = load_golden("tiny_uvw")
+        assert rel_err(out["m3mr"], g["m3mr_diag"]) < 1e-12
+
