@@ -20,6 +20,7 @@ QUAD_VECT, QUAD_RIEMANN, QUAD_TRAPZ = 0 << 4, 1 << 4, 2 << 4
 SUM_TRUNCATED, SUM_EXACT = 0, 1 << 8
 NO_PSD, NO_BISP = 1 << 9, 1 << 10
 ONLY_DIAG = 1 << 11
+EPI_GENERIC = 1 << 12          # testing: the generic quadrature epilogue only
 # return codes (include/bsahip.h)
 EINVAL, EDEVICE, ENOMEM, EUNSUPPORTED, ESTATE = -1, -2, -3, -4, -5
 
@@ -157,17 +158,19 @@ def init(device: int = 0):
 
 def classic_run(prob, quad: str = "vect", prec: str = "fp64", bound: Bound = None,
                 psd: bool = True, bisp: bool = True, exact: bool = False,
-                only_diag: bool = False) -> dict:
+                only_diag: bool = False, epi_generic: bool = False) -> dict:
     """Classic full-grid moments (bsahip_classic_run); exact=True evaluates
     S(f1+f2) beyond the grid instead of taking it as zero; only_diag=True runs
-    the ONLY_DIAG model (every output then holds NM values)."""
+    the ONLY_DIAG model (every output then holds NM values); epi_generic=True
+    sends every block of the plane quadrature through the generic epilogue."""
     b = bound or Bound(prob)
     nm = b.nm
     n2, n3 = (nm, nm) if only_diag else (nm * nm, nm ** 3)
     out = {k: np.zeros(n2) for k in ("m2mf", "m2mr", "m2o2mr")}
     out.update({k: np.zeros(n3) for k in ("m3mf", "m3mr")})
     flags = PREC[prec] | QUAD[quad] | (0 if psd else NO_PSD) | (0 if bisp else NO_BISP) | \
-        (SUM_EXACT if exact else SUM_TRUNCATED) | (ONLY_DIAG if only_diag else 0)
+        (SUM_EXACT if exact else SUM_TRUNCATED) | (ONLY_DIAG if only_diag else 0) | \
+        (EPI_GENERIC if epi_generic else 0)
     D = C.POINTER(C.c_double)
     _check(lib().bsahip_classic_run(C.byref(b.struct), C.c_int32(prob.nfreqs), C.c_double(prob.df),
                                     C.c_int32(flags), _ptr(out["m2mf"], C.c_double),

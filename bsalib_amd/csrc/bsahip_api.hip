@@ -512,6 +512,7 @@ void plan_args(const bsahip_plan *P, GsymArgs &a, GsymArgs16 &b16)
    a.nft = P->nft;
    a.d2 = P->d * P->d;
    a.rule = (P->flags & BSAHIP_QUAD_MASK) >> 4;
+   a.epi_plain = (P->flags & BSAHIP_EPI_GENERIC) ? 1 : 0;
    a.m3f = P->m3;
    a.m3r = P->m3 + nm3;
    b16.UR = P->UR;

@@ -251,6 +251,7 @@ struct GsymArgs {
    int nft;               // >= 2*nu - 1
    double d2;             // (2 pi df)^2 or df^2
    int rule;
+   int epi_plain;         // 1: every 16x16 block through the generic epilogue
    double *m3f, *m3r;     // [nm^3] accumulated (atomic)
 };
 
@@ -280,9 +281,11 @@ __device__ __forceinline__ double qw(bool ep, bool op, bool eq, bool oq, const Q
 //   s0 force (x,y,z)=(x,z,y)  s1 resp (x,y,z)  s2 force (y,x,z)=(y,z,x)
 //   s3 resp (y,x,z)  s4 resp (y,z,x)  s5 resp (x,z,y)
 //   s6 force (z,x,y)=(z,y,x)  s7 resp (z,x,y)  s8 resp (z,y,x)
-// (transposed entries s5..s8 only when !diag).
-__device__ __forceinline__ void epi_accum(const GsymArgs &a, int x, int y, int z, bool diag,
-                                          int ur0, int vq0, int lane, const double (&gval)[4],
+// (transposed entries s5..s8 only when !DIAG; a compile-time flag keeps the
+// four rows of a lane in one basic block, so their loads can overlap).
+template <bool DIAG>
+__device__ __forceinline__ void epi_accum(const GsymArgs &a, int x, int y, int z, int ur0,
+                                          int vq0, int lane, const double (&gval)[4],
                                           bool interleaved, double (&s)[9])
 {
    const int n = a.nh - 1;
@@ -336,7 +339,7 @@ __device__ __forceinline__ void epi_accum(const GsymArgs &a, int x, int y, int z
       s[2] += gbs + gdo;
       s[3] += gbs * F2s + gdo * F2o;
       s[4] += gbs * F3s + gdo * F3o;
-      if (!diag) {
+      if constexpr (!DIAG) {
          const double We = qw(ed, od, eu, ou, qr);
          const double Wf = qw(es, os, eu, ou, qr);
          const double2 hymn = make_double2(hym.x, -hym.y);   // H_y(v-u)
@@ -349,6 +352,106 @@ __device__ __forceinline__ void epi_accum(const GsymArgs &a, int x, int y, int z
          s[7] += ges * F3s + gfo * F1s;
          s[8] += ges * G3s + gfo * G3o;
       }
+   }
+}
+
+// The same nine sums for a 16x16 block whose every point has the weight of
+// the plane's interior in all five quadrature weights (CLS 1: 1 <= u, v and
+// u + v <= n - 1) or lies beyond the plane's f1+f2 edge with the other four
+// weights interior (CLS 2: u, v <= n - 1 and u + v >= n + 1, so Wd = Wf = 0).
+// With Wa = Wb = We = 1, ms = mo = 2 and h = 2 g = 4 G the contributions of
+// epi_accum reduce to the sums below; no lane is out of range.
+template <int CLS, bool DIAG>
+__device__ __forceinline__ void epi_accum_inner(const GsymArgs &a, int x, int y, int z, int ur0,
+                                                int vq0, int lane, const double (&gval)[4],
+                                                bool interleaved, double (&s)[9])
+{
+   const double2 *Hx = a.H + (size_t)x * a.nft;
+   const double2 *Hy = a.H + (size_t)y * a.nft;
+   const double2 *Hz = a.H + (size_t)z * a.nft;
+   const int v = vq0 + (lane & 15);
+   const double2 hxv = Hx[v], hyv = Hy[v];
+   const double2 cyv = make_double2(hyv.x, -hyv.y);
+   const double2 cxv = make_double2(hxv.x, -hxv.y);
+#pragma unroll
+   for (int t = 0; t < 4; ++t) {
+      const int u = ur0 + (interleaved ? ((lane >> 4) + 4 * t) : (4 * (lane >> 4) + t));
+      const double h = 4.0 * gval[t];
+      const int sp = u + v;
+      const int dm = u - v;
+      const int ad = dm < 0 ? -dm : dm;
+      const double2 hxu = Hx[u], hzu = Hz[u], hyp = Hy[sp], hzp = Hz[sp];
+      double2 hym = Hy[ad], hzm = Hz[ad];
+      if (dm < 0) {
+         hym.y = -hym.y;
+         hzm.y = -hzm.y;
+      }
+      const double F1s = re3(hxu, hyv, hzp);
+      const double F1o = re3(hxu, cyv, hzm);
+      const double F2s = re3(hym, hxv, hzu);
+      const double F2o = re3(hyp, cxv, hzu);
+      const double F3s = re3(hyv, hzm, hxu);
+      s[0] += 2.0 * h;
+      s[1] += h * (F1s + F1o);
+      if constexpr (CLS == 1) {
+         const double F3o = re3(cyv, hzp, hxu);
+         s[2] += 2.0 * h;
+         s[3] += h * (F2s + F2o);
+         s[4] += h * (F3s + F3o);
+      } else {
+         s[2] += h;
+         s[3] += h * F2s;
+         s[4] += h * F3s;
+      }
+      if constexpr (!DIAG) {
+         const double2 hymn = make_double2(hym.x, -hym.y);   // H_y(v-u)
+         const double G3s = re3(hzu, hymn, hxv);
+         s[5] += h * (F2o + F2s);
+         if constexpr (CLS == 1) {
+            const double2 czu = make_double2(hzu.x, -hzu.y);
+            const double G3o = re3(czu, hyp, hxv);
+            s[6] += 2.0 * h;
+            s[7] += h * (F3s + F1s);
+            s[8] += h * (G3s + G3o);
+         } else {
+            s[6] += h;
+            s[7] += h * F3s;
+            s[8] += h * G3s;
+         }
+      }
+   }
+}
+
+// Class of a 16x16 block with half-grid origin (u0, v0): 1 or 2 as above,
+// 0 for every other block (edges, the f1+f2 = fmax band, rows beyond the plane).
+__device__ __forceinline__ int epi_class(const GsymArgs &a, int u0, int v0)
+{
+   const int n = a.nh - 1;
+   if (a.epi_plain || u0 < 1 || v0 < 1 || u0 + 15 > n - 1 || v0 + 15 > n - 1) return 0;
+   if (u0 + v0 + 30 <= n - 1) return 1;
+   if (u0 + v0 >= n + 1) return 2;
+   return 0;
+}
+
+__device__ __forceinline__ void epi_accum_any(const GsymArgs &a, int x, int y, int z, bool diag,
+                                              int ur0, int vq0, int lane, const double (&g)[4],
+                                              bool interleaved, double (&s)[9])
+{
+   const int cls = epi_class(a, ur0, vq0);           // uniform per block
+   if (diag) {
+      if (cls == 1)
+         epi_accum_inner<1, true>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
+      else if (cls == 2)
+         epi_accum_inner<2, true>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
+      else
+         epi_accum<true>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
+   } else {
+      if (cls == 1)
+         epi_accum_inner<1, false>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
+      else if (cls == 2)
+         epi_accum_inner<2, false>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
+      else
+         epi_accum<false>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
    }
 }
 
@@ -405,8 +508,8 @@ __device__ __forceinline__ void epi_wave(const GsymArgs &a, int x, bool diag, in
          const int i = b / BC, j = b % BC;
          double g[4];
          getv(b, g);
-         epi_accum(a, x, y0, z0, diag, rw0 + 16 * i - y0 * a.nhp, cw0 + 16 * j - z0 * a.nhp,
-                   lane, g, interleaved, s);
+         epi_accum_any(a, x, y0, z0, diag, rw0 + 16 * i - y0 * a.nhp, cw0 + 16 * j - z0 * a.nhp,
+                       lane, g, interleaved, s);
       }
       epi_flush(a, x, y0, z0, diag, lane, s);
    } else {
@@ -418,7 +521,7 @@ __device__ __forceinline__ void epi_wave(const GsymArgs &a, int x, bool diag, in
          if (y >= nm || z >= nm) continue;             // uniform per block
          double g[4];
          getv(b, g);
-         epi_accum(a, x, y, z, diag, r0 - y * a.nhp, q0 - z * a.nhp, lane, g, interleaved, s);
+         epi_accum_any(a, x, y, z, diag, r0 - y * a.nhp, q0 - z * a.nhp, lane, g, interleaved, s);
          epi_flush(a, x, y, z, diag, lane, s);
       }
    }

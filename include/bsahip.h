@@ -64,6 +64,11 @@ extern "C" {
  * functionsImpl.F90:1747-2101) */
 #define BSAHIP_ONLY_DIAG     (1 << 11)
 
+/* testing (flag bit 12; Classic runs and plans): every block of the plane
+ * quadrature through the generic epilogue instead of the specialised ones for
+ * interior blocks; the results agree to rounding.  No reference counterpart. */
+#define BSAHIP_EPI_GENERIC   (1 << 12)
+
 /* Problem description: borrowed host pointers, read during the call only. */
 typedef struct bsahip_problem {
    int32_t nn;            /* total nodes (packed coherence index base)       */

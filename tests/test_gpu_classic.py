@@ -122,3 +122,17 @@ def test_plan_runs_repeat_and_match_one_shot(gpu_lib, prec):
     for k in ("m2mf", "m2mr", "m2o2mr", "m3mf", "m3mr"):
         assert rel_err(a[k], one[k]) < 1e-14, k
         assert rel_err(b[k], a[k]) < 1e-14, k
+
+
+@pytest.mark.parametrize("cfg,prec,exact", [("config2", "fp64", False), ("config2", "bf16x3", True),
+                                            ("config3", "fp64", False), ("config3", "bf16x3", False)])
+def test_specialised_epilogue_equals_generic(gpu_lib, cfg, prec, exact):
+    """Full-size grids (513^2, 1025^2): the epilogue's specialised paths for
+    interior blocks and for blocks beyond the f1+f2 edge give the generic
+    epilogue's moments to rounding (the same G values, other summation)."""
+    p = getattr(synth, cfg)()
+    b = gpu_lib.Bound(p)
+    fast = gpu_lib.classic_run(p, prec=prec, exact=exact, bound=b, psd=False)
+    slow = gpu_lib.classic_run(p, prec=prec, exact=exact, bound=b, psd=False, epi_generic=True)
+    for k in ("m3mf", "m3mr"):
+        assert rel_err(fast[k], slow[k]) < 1e-13, k
