@@ -63,7 +63,7 @@ __device__ __forceinline__ void batch_drain(const long *__restrict__ offs, int n
    }
 }
 
-__global__ __launch_bounds__(256) void k_gsym_f64_batch(const GsymArgs *__restrict__ args,
+__global__ __launch_bounds__(256, 4) void k_gsym_f64_batch(const GsymArgs *__restrict__ args,
                                                         const long *__restrict__ offs, int ns,
                                                         const long *__restrict__ qbeg,
                                                         unsigned long long *counters, int chunk)
