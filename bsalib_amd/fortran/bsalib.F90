@@ -31,6 +31,14 @@
 !!  * bsa_setComputePrecision is an addition: BSA_PREC_BF16X3 selects the
 !!    split-bf16x3 node contraction (FP32 accumulation on matrix cores,
 !!    FP64 everywhere else); the default is FP64.
+!!  * Bispectrum export files: visual mode writes one entry (or
+!!    the nodal bispectrum of one DOF) at the Mesher's device tiles without an
+!!    earlier run's BFM dump file, which the reference's visual mode re-reads.
+!!  * Reference defects that are not reproduced: the frequency convention
+!!    (its vectorised Classic BFM indexes S(f1+f2) as if the grid were
+!!    centred, functionsImpl.F90:1307-1309) stops bsa_Run with a message; the
+!!    Mesher's diagonal model uses S(f1+f2) in its second and third products
+!!    (functionsImpl.F90:1161-1163 uses S(f1) S(f2)^2 and S(f1)^2 S(f2)).
 module BsaLib
    use, intrinsic :: iso_fortran_env, only: int32, int64, real32, real64, output_unit
    use, intrinsic :: iso_c_binding
