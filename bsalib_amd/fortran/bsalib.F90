@@ -48,6 +48,52 @@ module BsaLib
 
    integer, parameter, public :: bsa_int_t = int32
    integer, parameter, public :: bsa_real_t = real64
+   ! kind names and constants of the reference's module (src/CONSTANTS/_CONSTANTS.F90)
+   integer, parameter, public :: IK = int32, RK = real64
+   integer(bsa_int_t), parameter, public :: BSA_PREMESH_MODE_BASE = 0
+   integer(bsa_int_t), parameter, public :: BSA_PREMESH_MODE_ZONE_REFINED = 1
+   character(len=*), parameter, public :: BSA_OUT_DIRNAME_DEFAULT = '.\bsaresults\'
+   character(len=*), parameter, public :: BSA_OUT_FILENAME_PREFIX_DEFAULT_ = 'bsaout_def'
+   character(len=*), parameter, public :: BSA_EXPORT_M2MF_CLS_FNAME = 'm2mf_cls'
+   character(len=*), parameter, public :: BSA_EXPORT_M2MR_CLS_FNAME = 'm2mr_cls'
+   character(len=*), parameter, public :: BSA_EXPORT_M2O2MR_CLS_FNAME = 'm2o2mr_cls'
+   character(len=*), parameter, public :: BSA_EXPORT_M3MF_CLS_FNAME = 'm3mf_cls'
+   character(len=*), parameter, public :: BSA_EXPORT_M3MR_CLS_FNAME = 'm3mr_cls'
+   character(len=*), parameter, public :: BSA_EXPORT_M2MF_MSH_FNAME = 'm2mf_msh'
+   character(len=*), parameter, public :: BSA_EXPORT_M2MR_MSH_FNAME = 'm2mr_msh'
+   character(len=*), parameter, public :: BSA_EXPORT_M2O2MR_MSH_FNAME = 'm2o2mr_msh'
+   character(len=*), parameter, public :: BSA_EXPORT_M3MF_MSH_FNAME = 'm3mf_msh'
+   character(len=*), parameter, public :: BSA_EXPORT_M3MR_MSH_FNAME = 'm3mr_msh'
+   character(len=*), parameter, public :: IO_ACCESS_DIRECT = 'DIRECT', IO_ACCESS_SEQUEN = 'SEQUENTIAL'
+   character(len=*), parameter, public :: IO_ACCESS_STREAM = 'STREAM', IO_ACCESS_APPEND = 'APPEND'
+   character(len=*), parameter, public :: IO_ACTION_READWRITE = 'READWRITE'
+   character(len=*), parameter, public :: IO_ASYNC_YES = 'YES', IO_ASYNC_NO = 'NO'
+   character(len=*), parameter, public :: IO_FORM_FORMATTED = 'FORMATTED'
+   character(len=*), parameter, public :: IO_FORM_UNFORMATTED = 'UNFORMATTED', IO_FORM_BINARY = 'BINARY'
+   character(len=*), parameter, public :: IO_POSITION_ASIS = 'ASIS', IO_POSITION_REWIND = 'REWIND'
+   character(len=*), parameter, public :: IO_POSITION_APPEND = 'APPEND'
+   character(len=*), parameter, public :: IO_STATUS_OLD = 'OLD', IO_STATUS_NEW = 'NEW'
+   character(len=*), parameter, public :: IO_STATUS_SCRATCH = 'SCRATCH', IO_STATUS_REPLACE = 'REPLACE'
+   character(len=*), parameter, public :: IO_STATUS_UNKNOWN = 'UNKNOWN'
+   character(len=*), parameter, public :: NOTEMSG = '  --[note]   '
+   real(RK), parameter, public :: MACHINE_PRECISION = 1e-12_RK
+   real(RK), parameter, public :: CST_PIGREC = 4.0_RK * atan(1.0_RK)
+   real(RK), parameter, public :: CST_PIt2 = CST_PIGREC * 2.0_RK, CST_PIt4 = CST_PIGREC * 4.0_RK
+   real(RK), parameter, public :: CST_PId2 = CST_PIGREC / 2.0_RK, CST_PId4 = CST_PIGREC / 4.0_RK
+   real(RK), parameter, public :: CST_2d3 = 2.0_RK / 3.0_RK
+   real(RK), parameter, public :: CST_PIt3d2 = CST_PIGREC * 1.5_RK
+
+   ! user callback for bispectrum exports (src/CONSTANTS/_CONSTANTS.F90:171-185)
+   abstract interface
+      subroutine exportInterf_vect_(f1, f2, brm, pdata)
+         import :: bsa_real_t
+         real(bsa_real_t), intent(in) :: f1(:)
+         real(bsa_real_t), intent(in) :: f2(:)
+         real(bsa_real_t), intent(in) :: brm(:, :)
+         class(*), pointer, intent(in) :: pdata
+      end subroutine
+   end interface
+   public :: exportInterf_vect_
    ! re-exported, as the reference module (which is public) re-exports them
    public :: int32, real64
 
@@ -119,6 +165,9 @@ module BsaLib
    public :: bsa_setOutFileName, bsa_setBRMExportDefaultMode, bsa_enableVisualMode
    public :: bsa_setVisualIndexes, bsa_exportMomentToFile, bsa_exportSkewness
    public :: bsa_exportBRdecomp, bsa_exportExtremeValuesToFile, bsa_saveCoordinatesToFile
+   public :: bsa_setPremeshMode, bsa_enableOnlyPremesh, bsa_setPolicyIDValidationValues
+   public :: bsa_isFullComp, bsa_setOutputDirectory, bsa_setOutUnit, bsa_exportPSDToFile
+   public :: bsa_exportBispToFile, bsa_exportModalData, bsa_setBispExportCallback
 
    interface bsa_exportSkewness
       module procedure exportSkewnessCompute_, exportSkewnessGiven_
@@ -175,8 +224,14 @@ module BsaLib
    ! (src/bsa/data/BsaLibData.F90:87-100)
    integer(bsa_int_t) :: vis_idx_(3) = 0
    logical :: vis_modal_ = .true.
-   ! bispectrum export of Mesher runs (bsa_setBRMExportDefaultMode)
+   ! bispectrum export of Mesher runs (bsa_setBRMExportDefaultMode), a user
+   ! callback in place of the export files (bsa_setBispExportCallback)
    logical :: brm_base_ = .false.
+   procedure(exportInterf_vect_), pointer :: brm_cb_ => null()
+   class(*), pointer :: null_pdata_ => null()
+   ! Mesher pre-mesh only, pre-mesh mode, modal data file at bsa_Run
+   logical :: only_premesh_ = .false., export_modal_ = .false.
+   integer(bsa_int_t) :: premesh_mode_ = BSA_PREMESH_MODE_ZONE_REFINED
 
    ! ---- problem of the last bsa_Run, kept for the post-processing calls ----
    type(bsahip_problem_t), save :: lp_
@@ -688,6 +743,7 @@ contains
          call bsa_Abort('Modal matrices were not acquired (bsa_setModalMatrices).')
       if (allocated(modes_)) deallocate(modes_)
       call selectModes_()
+      if (export_modal_) call exportModalData_()
       nme = size(modes_)
       if (nme == 0) call bsa_Abort('Effective number of modes to be kept is 0.')
       if (any(modes_ < 1) .or. any(modes_ > nm_)) call bsa_Abort('Kept mode index out of range.')
@@ -919,6 +975,11 @@ contains
          end if
          rc2 = bsahip_mesher_create(pm, cfg, mflags, msh)
          if (rc2 /= 0) call bsa_Abort('bsahip_mesher_create failed: '//bsahip_last_error_f())
+         if (only_premesh_) then
+            write(output_unit, '(1x, 2a)') NOTEMSG, 'Skipping Post-meshing phase!'
+            rc2 = bsahip_mesher_destroy(msh)
+            return
+         end if
          allocate(m3mf(nc), m3mr(nc))
          rc2 = bsahip_mesher_run(msh, c_null_ptr, c_loc(m3mf), c_loc(m3mr), c_loc(counts))
          if (rc2 /= 0) call bsa_Abort('bsahip_mesher_run failed: '//bsahip_last_error_f())
@@ -1380,6 +1441,151 @@ contains
       visual_ = .true.
    end subroutine
 
+   !> Accepted values as src/bsa/BsaLibImpl.F90:813-827.  The device Mesher lays
+   !> out its zones the same way in either mode.
+   subroutine bsa_setPremeshMode(imode)
+      integer(bsa_int_t), value :: imode
+      if (imode == BSA_PREMESH_MODE_BASE) then
+         premesh_mode_ = imode
+      else
+         if (imode /= BSA_PREMESH_MODE_ZONE_REFINED) then
+            write(output_unit, '(1x, 2a, i0, a)') WARNMSG, 'Invalid   ', imode, '  pre-meshing mode value.'
+            write(output_unit, '(1x, 2a)') MSGCONT, 'Setting default (ZONE REFINED).'
+         end if
+         premesh_mode_ = BSA_PREMESH_MODE_ZONE_REFINED
+      end if
+   end subroutine
+
+   !> The Mesher stops after its pre-mesh (src/bsa/BsaLibImpl.F90:146-148,
+   !> BsaMesherImpl.F90:113-119): the level-0 BFM of every zone is evaluated on
+   !> the device and no post-mesh runs, so m3mf_msh and m3mr_msh stay
+   !> unallocated.
+   subroutine bsa_enableOnlyPremesh()
+      only_premesh_ = .true.
+   end subroutine
+
+   !> Accepted for interface compatibility (src/bsa/BsaLibImpl.F90:847-865):
+   !> the device Mesher's rule-based levels use the built-in policy factors.
+   subroutine bsa_setPolicyIDValidationValues(id, i_bfm, j_bfm, i_brm, j_brm)
+      integer(int32), value :: id, i_bfm, j_bfm, i_brm, j_brm
+      if (id < 0 .or. id > 9) write(output_unit, '(1x, 2a, i0)') WARNMSG, 'Invalid policy ID  ', id
+   end subroutine
+
+   !> .true. unless the ONLY_DIAG model is selected (src/bsa/BsaLibImpl.F90:902-906).
+   logical function bsa_isFullComp()
+      bsa_isFullComp = ionly_diag_ == 0
+   end function
+
+   !> Output directory (src/bsa/BsaLibImpl.F90:706-710): the exports go there
+   !> unless bsa_setExportDirectory or bsa_setExportInCurrDir chose otherwise.
+   subroutine bsa_setOutputDirectory(dirname)
+      character(len=*), intent(in) :: dirname
+      if (.not. allocated(exp_dir_)) call bsa_setExportDirectory(dirname)
+   end subroutine
+
+   !> The reference's debug-log unit (src/bsa/BsaLibImpl.F90:726-730); this
+   !> library writes no debug log, so the unit is not used.
+   subroutine bsa_setOutUnit(iunit)
+      integer(bsa_int_t), value :: iunit
+   end subroutine
+
+   !> PSD series to a text file as src/bsa/BsaLibImpl.F90:1980-2012: the two
+   !> sizes, then one row per frequency (f first when given).
+   subroutine bsa_exportPSDToFile(fname, psd, f)
+      character(len=*), intent(in) :: fname
+      real(bsa_real_t), intent(in) :: psd(:, :)
+      real(bsa_real_t), intent(in), optional :: f(:)
+      integer :: iun, j
+      iun = openText_(fname)
+      write(iun, *) size(psd, 1)
+      write(iun, *) size(psd, 2)
+      do j = 1, size(psd, 1)
+         if (present(f)) then
+            write(iun, '(*(g0, 1x))') f(j), psd(j, :)
+         else
+            write(iun, '(*(g0, 1x))') psd(j, :)
+         end if
+      end do
+      close(iun)
+   end subroutine
+
+   !> Bispectrum series to a text file as src/bsa/BsaLibImpl.F90:2017-2040: the
+   !> three sizes, then bisp(:, i, j) as one row per (i, j).
+   subroutine bsa_exportBispToFile(fname, bisp)
+      character(len=*), intent(in) :: fname
+      real(bsa_real_t), intent(in) :: bisp(:, :, :)
+      integer :: iun, i, j
+      if (size(bisp, 2) /= size(bisp, 1)) call bsa_Abort('First two dimensions of bisp do not match.')
+      iun = openText_(fname)
+      write(iun, *) size(bisp, 1)
+      write(iun, *) size(bisp, 2)
+      write(iun, *) size(bisp, 3)
+      do j = 1, size(bisp, 3)
+         do i = 1, size(bisp, 2)
+            write(iun, '(*(g0, 1x))') bisp(:, i, j)
+         end do
+      end do
+      close(iun)
+   end subroutine
+
+   !> bsa_Run writes modal.txt (nodes, DOFs per node, modes, then per mode its
+   !> natural frequency and shape) as src/bsa/BsaLibImpl.F90:1972-1974, 2045-2057.
+   subroutine bsa_exportModalData()
+      export_modal_ = .true.
+   end subroutine
+
+   !> A user routine receives the bispectra of visual mode and of
+   !> bsa_setBRMExportDefaultMode in place of the export files: per zone (the
+   !> Classic plane, or a Mesher tile) the point frequencies f1, f2 and the
+   !> values brm(:, point), with a null pdata.  The reference resets its
+   !> callback at every bsa_Run (BsaLibImpl.F90:182-188), so there the user
+   !> routine never runs.
+   subroutine bsa_setBispExportCallback(fptr)
+      procedure(exportInterf_vect_), pointer, intent(in) :: fptr
+      brm_cb_ => fptr
+   end subroutine
+
+   !> The user callback on one rectangular zone: points (f1axis(i), f2axis(j)),
+   !> i fastest, values vals(:, point).
+   subroutine callbackZone_(ni, nj, f1axis, f2axis, vals)
+      integer, intent(in) :: ni, nj
+      real(bsa_real_t), intent(in) :: f1axis(ni), f2axis(nj), vals(:, :)
+      real(bsa_real_t), allocatable :: f1(:), f2(:)
+      integer :: i, j
+      allocate(f1(ni * nj), f2(ni * nj))
+      do j = 1, nj
+         do i = 1, ni
+            f1(i + (j - 1) * ni) = f1axis(i)
+            f2(i + (j - 1) * ni) = f2axis(j)
+         end do
+      end do
+      call brm_cb_(f1, f2, vals, null_pdata_)
+   end subroutine
+
+   !> modal.txt as src/bsa/BsaLibImpl.F90:2045-2057.
+   subroutine exportModalData_()
+      integer :: iun, j
+      open(newunit=iun, file='modal.txt', form='formatted', action='write', status='replace')
+      write(iun, *) nn_
+      write(iun, *) nlibs_
+      write(iun, *) nm_
+      do j = 1, nm_
+         write(iun, '(1x, g20.10)') natf_(j)
+         write(iun, '(1x, g20.10)') phi_(:, j)
+      end do
+      close(iun)
+   end subroutine
+
+   integer function openText_(fname) result(iun)
+      character(len=*), intent(in) :: fname
+      character(len=:), allocatable :: path
+      integer :: ios
+      path = fname
+      if (allocated(exp_dir_)) path = exp_dir_//fname
+      open(newunit=iun, file=path, form='formatted', status='replace', action='write', iostat=ios)
+      if (ios /= 0) call bsa_Abort('Error opening file  '//path)
+   end function
+
    !> Modal (i, j, k) kept-mode positions (modal = .true.), or node and DOF
    !> (modal = .false.) as src/bsa/BsaLibImpl.F90:134-141.
    subroutine bsa_setVisualIndexes(indexes, modal)
@@ -1475,6 +1681,16 @@ contains
       rc = bsahip_mesher_export(msh, c_null_ptr, ncol, pcoef, npts, c_loc(dims), c_loc(fi), c_loc(fj), &
                                 c_loc(val), npts)
       if (rc /= 0) call bsa_Abort('bsahip_mesher_export failed: '//bsahip_last_error_f())
+      if (associated(brm_cb_)) then
+         off = 0
+         do k = 1, nt
+            call brm_cb_(fi(off + 1:off + int(dims(1, k), int64) * dims(2, k)), &
+                         fj(off + 1:off + int(dims(1, k), int64) * dims(2, k)), &
+                         val(:, off + 1:off + int(dims(1, k), int64) * dims(2, k)), null_pdata_)
+            off = off + int(dims(1, k), int64) * dims(2, k)
+         end do
+         return
+      end if
       write(output_unit, '( /, 1x, 4a, i0, a, i0, a )') INFOMSG, 'Exporting  bispectrum  to  ', &
          fname, ' (', nt, ' zones, ', npts, ' points)'
       open(newunit=iun, file=fname, form='unformatted', access='stream', status='replace', &
@@ -1530,6 +1746,11 @@ contains
       rc = bsahip_classic_bispectra_sel(p, int(nfh, c_int32_t), real(df, c_double), fl, 1_c_int32_t, &
                                         c_loc(sel), c_null_ptr, c_loc(brm))
       if (rc /= 0) call bsa_Abort('bsahip_classic_bispectra_sel failed: '//bsahip_last_error_f())
+      if (associated(brm_cb_)) then
+         call callbackZone_(nf, nf, [(real(i - nfh, bsa_real_t) * df, i = 1, nf)], &
+                            [(real(i - nfh, bsa_real_t) * df, i = 1, nf)], reshape(brm, [1, nf * nf]))
+         return
+      end if
       write(output_unit, '( /, 1x, 2a, i0, " x ", i0, a )') INFOMSG, 'Exporting  BRM  to file...  (', nf, nf, ')'
       open(newunit=iun, file=BRM_EXPORT_FNAME_CLS, form='unformatted', access='stream', &
            status='replace', action='readwrite', iostat=ios)

@@ -5,6 +5,7 @@
 !! Usage:  bsalib_driver <input.bin> <output.bin> [fp64|bf16x3] [grid] [scalar] [mesher|both]
 !!                        [pod] [podn=<modes>] [podt=<percent>] [post] [gen] [diag]
 !!                        [visual=<i>,<j>,<k>] [visualn=<node>,<dof>] [sym=<n>] [brmbase]
+!!                        [cb] [premesh] [modal]
 !! The input layout is the one tests write for the reference harness (header
 !! of 14 int32, then integer and real64 arrays in Fortran order).  "grid"
 !! leaves test mode off and passes damping ratios, so bsa_Run applies the
@@ -22,13 +23,37 @@
 !! enables visual mode with modal indexes (i, j, k), "visualn=" with a node and
 !! DOF (bsa_enableVisualMode, bsa_setVisualIndexes); "sym=" calls
 !! bsa_setSpatialSymmetry; "brmbase" calls
-!! bsa_setBRMExportDefaultMode(BSA_EXPORT_BRM_MODE_BASE).
+!! bsa_setBRMExportDefaultMode(BSA_EXPORT_BRM_MODE_BASE); "cb" registers a
+!! bispectrum export callback (bsa_setBispExportCallback) that prints, at the
+!! end, "CALLBACK <zones> <points> <sum of values>"; "premesh" calls
+!! bsa_enableOnlyPremesh, "modal" bsa_exportModalData.
 !! Output: nme (int32), the used modes, then for each of m2mf, m2mr, m2o2mr,
 !! m3mf_cls, m3mr_cls, m3mf_msh, m3mr_msh: its size (int32, 0 when bsa_Run
 !! left it unallocated) and its values (real64).
+! the export callback lives in a module: a procedure pointer to an internal
+! procedure would need an executable stack
+module driver_cb
+   use, intrinsic :: iso_fortran_env, only: int64, real64
+   implicit none
+   integer :: cb_zones = 0
+   integer(int64) :: cb_pts = 0
+   real(real64) :: cb_sum = 0
+contains
+   subroutine cb_(f1, f2, brm, pdata)
+      real(real64), intent(in) :: f1(:), f2(:), brm(:, :)
+      class(*), pointer, intent(in) :: pdata
+      if (size(f1) /= size(f2) .or. size(brm, 2) /= size(f1) .or. associated(pdata)) &
+         error stop 'callback arguments'
+      cb_zones = cb_zones + 1
+      cb_pts = cb_pts + size(f1)
+      cb_sum = cb_sum + sum(brm)
+   end subroutine
+end module driver_cb
+
 program bsalib_driver
-   use, intrinsic :: iso_fortran_env, only: int32, real64
+   use, intrinsic :: iso_fortran_env, only: int32, int64, real64
    use BsaLib
+   use driver_cb
    implicit none
 
    integer(int32) :: hdr(14)
@@ -48,6 +73,7 @@ program bsalib_driver
    logical :: grid_checks
    integer :: uin, uout, isuban, isvd, npod
    real(real64) :: podt
+   procedure(exportInterf_vect_), pointer :: pcb
 
    call get_command_argument(1, fin)
    call get_command_argument(2, fout)
@@ -99,6 +125,12 @@ program bsalib_driver
          call bsa_setVisualIndexes(vis, .false.)
       end if
       if (trim(arg) == 'brmbase') call bsa_setBRMExportDefaultMode(BSA_EXPORT_BRM_MODE_BASE)
+      if (trim(arg) == 'cb') then
+         pcb => cb_
+         call bsa_setBispExportCallback(pcb)
+      end if
+      if (trim(arg) == 'premesh') call bsa_enableOnlyPremesh()
+      if (trim(arg) == 'modal') call bsa_exportModalData()
       if (arg(1:4) == 'sym=') then
          read(arg(5:), *) isym
          call bsa_setSpatialSymmetry(isym)
@@ -178,6 +210,7 @@ program bsalib_driver
       call put(pnc)
    end if
    close(uout)
+   if (cb_zones > 0) print '(a, 1x, i0, 1x, i0, 1x, es25.17)', 'CALLBACK', cb_zones, cb_pts, cb_sum
    call bsa_Finalise()
    if (.not. bsa_isCleaned()) error stop 'bsa_Finalise did not clean up'
 contains

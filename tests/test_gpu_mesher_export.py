@@ -203,3 +203,53 @@ def test_front_mesher_brm_export_base(gpu_lib, tmp_path):
     assert f["nm"] == M.nm and list(f["indexes"]) == list(M.modes)
     assert (f["ncomb"], f["nzones"], f["npts"]) == (nm3, M.ntiles, e["fi"].size)
     _same_values(f["zones"], e, nm3)
+
+
+def _callback(stdout):
+    line = [x for x in stdout.splitlines() if x.startswith("CALLBACK")]
+    assert line, stdout[-2000:]
+    _, zones, pts, total = line[0].split()
+    return int(zones), int(pts), float(total)
+
+
+def test_front_export_callback_replaces_files(gpu_lib, tmp_path):
+    """bsa_setBispExportCallback: the user routine receives, per zone, the
+    values the export file would hold (and no file is written)."""
+    from tests.conftest import load_golden
+    p = CASES["tiny_u"]()
+    g = load_golden("tiny_u")
+    nm = 3
+    d1 = tmp_path / "cls"
+    d1.mkdir()
+    _, r = fdriver.run(p, "visual=3,1,2", "cb", workdir=d1)
+    zones, pts, total = _callback(r.stdout)
+    want = g["bisp_r"][:, :, 3 - 1 + (1 - 1) * nm + (2 - 1) * nm * nm]
+    assert (zones, pts) == (1, p.nf ** 2)
+    assert abs(total - want.sum()) <= 1e-12 * np.abs(want).sum()
+    assert not (d1 / "bsaexport_cls.brm").exists()
+    q = _damped("tiny_uvw", 20)
+    M = _front_mesher(q)
+    e = M.export()
+    M.close()
+    d2 = tmp_path / "msh"
+    d2.mkdir()
+    _, r = fdriver.run(q, "mesher", "brmbase", "cb", workdir=d2)
+    zones, pts, total = _callback(r.stdout)
+    assert (zones, pts) == (M.ntiles, e["fi"].size)
+    assert abs(total - e["val"].sum()) <= 1e-12 * np.abs(e["val"]).sum()
+    assert not (d2 / "bsaexport_msh.brm").exists()
+
+
+def test_front_only_premesh_and_modal_data(gpu_lib, tmp_path):
+    """bsa_enableOnlyPremesh stops the Mesher after its pre-mesh (no Mesher
+    outputs); bsa_exportModalData writes modal.txt at bsa_Run."""
+    p = _damped("tiny_uvw", 20)
+    out, r = fdriver.run(p, "mesher", "premesh", "modal", workdir=tmp_path)
+    assert "Skipping Post-meshing phase!" in r.stdout
+    assert out["m3mf_msh"] is None and out["m3mr_msh"] is None
+    v = (tmp_path / "modal.txt").read_text().split()
+    nn, nlibs, nm = int(v[0]), int(v[1]), int(v[2])
+    assert (nn, nlibs, nm) == (p.nn, p.nlibs, p.phi.shape[1])
+    vals = np.array(v[3:], dtype=float).reshape(nm, 1 + nn * nlibs)
+    assert np.allclose(vals[:, 0], p.natf, rtol=1e-9)
+    assert np.allclose(vals[:, 1:].T, p.phi, rtol=1e-9, atol=1e-12)

@@ -64,3 +64,40 @@ def test_bisp_export_reader_round_trip(tmp_path):
     (tmp_path / "y.brm").write_bytes(head.tobytes() + rec.astype(np.float32).tobytes() + b"\0")
     with pytest.raises(ValueError):
         exports.read_bisp_export(tmp_path / "y.brm")
+
+
+def test_reference_module_extras_without_device(tmp_path):
+    """Procedures and constants of the reference's module BsaLib that need no
+    device (src/BsaLib.F90: bsa_exportPSDToFile, bsa_exportBispToFile,
+    bsa_isFullComp, bsa_setPremeshMode, bsa_setPolicyIDValidationValues,
+    bsa_setOutUnit, the CST_* and export-name constants): a program using them
+    compiles against module BsaLib and writes the reference's text layouts."""
+    import subprocess
+    from bsalib_amd import build
+    fdriver.ensure_built()
+    src = build.ROOT / "tests" / "fortran" / "api_extras.F90"
+    exe = tmp_path / "api_extras"
+    subprocess.run([build.FC, "-cpp", "-O2", "-I", str(build.FOUT), str(src), str(build.FLIB),
+                    "-L", str(build.OUT_DIR), "-lbsahip", f"-Wl,-rpath,{build.OUT_DIR}",
+                    "-o", str(exe)], check=True, cwd=tmp_path, capture_output=True)
+    r = subprocess.run([str(exe)], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.splitlines()
+    assert [x.split()[1] for x in lines if x.startswith("FULL")] == ["T", "F"]
+    cst = [x for x in lines if x.startswith("CST")][0].split()
+    assert float(cst[1]) == 2 * np.pi and abs(float(cst[2]) - 1.5 * np.pi) < 1e-15
+    assert [x for x in lines if x.startswith("NAMES")][0].split()[1:] == ["m3mr_cls", "bsaexport_msh.brm"]
+    assert "pre-meshing mode value" in r.stdout
+    psd = np.arange(1, 7).reshape(3, 2, order="F") * 0.5
+    f = np.array([0.0, 0.1, 0.2])
+    v = (tmp_path / "psd.txt").read_text().split()
+    assert (int(v[0]), int(v[1])) == (3, 2)
+    assert np.allclose(np.array(v[2:], float).reshape(3, 3), np.column_stack([f, psd]), rtol=1e-15)
+    v = (tmp_path / "psd_nof.txt").read_text().split()
+    assert np.allclose(np.array(v[2:], float).reshape(3, 2), psd, rtol=1e-15)
+    bisp = np.arange(1, 13).reshape(2, 2, 3, order="F") * 0.25
+    v = (tmp_path / "bisp.txt").read_text().split()
+    assert [int(x) for x in v[:3]] == [2, 2, 3]
+    rows = np.array(v[3:], float).reshape(6, 2)
+    want = np.array([bisp[:, i, j] for j in range(3) for i in range(2)])
+    assert np.allclose(rows, want, rtol=1e-15)
