@@ -75,8 +75,8 @@ def build_fortran(verbose: bool = True) -> Path:
     subprocess.check_call(["ar", "rcs", str(FLIB), *objs])
     if FDRIVER_SRC.exists():
         obj = FOUT / "bsalib_driver.o"
-        subprocess.check_call([FC, "-cpp", "-O2", "-I", str(FOUT), "-c", str(FDRIVER_SRC),
-                               "-o", str(obj)])
+        subprocess.check_call([FC, "-cpp", "-O2", "-I", str(FOUT), "-J", str(FOUT), "-c",
+                               str(FDRIVER_SRC), "-o", str(obj)])
         cmd = [FC, "-o", str(FDRIVER), str(obj), str(FLIB), "-L", str(OUT_DIR), "-lbsahip",
                "-Wl,-rpath,$ORIGIN"]
         if verbose:
