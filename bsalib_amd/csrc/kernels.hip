@@ -283,6 +283,12 @@ __device__ __forceinline__ double qw(bool ep, bool op, bool eq, bool oq, const Q
 //   s6 force (z,x,y)=(z,y,x)  s7 resp (z,x,y)  s8 resp (z,y,x)
 // (transposed entries s5..s8 only when !DIAG; a compile-time flag keeps the
 // four rows of a lane in one basic block, so their loads can overlap).
+// Re[a b conj c] is unchanged when all three factors are conjugated, so of the
+// eight FRF products these sums weight only four differ:
+//   (y,z,x) at (v,u-v):  Re[H_y(v) H_z(u-v) conj H_x(u)]       = F1o
+//   (y,z,x) at (-v,u+v): Re[conj H_y(v) H_z(u+v) conj H_x(u)]  = F1s
+//   (z,y,x) at (u,v-u):  Re[H_z(u) H_y(v-u) conj H_x(v)]       = F2s
+//   (z,y,x) at (-u,u+v): Re[conj H_z(u) H_y(u+v) conj H_x(v)]  = F2o
 template <bool DIAG>
 __device__ __forceinline__ void epi_accum(const GsymArgs &a, int x, int y, int z, int ur0,
                                           int vq0, int lane, const double (&gval)[4],
@@ -326,31 +332,25 @@ __device__ __forceinline__ void epi_accum(const GsymArgs &a, int x, int y, int z
       const double Wb = qw(ed, od, ev, ov, qr);
       const double Wd = qw(es, os, ev, ov, qr);
       // FRF products Re[H_m(f1) H_n(f2) conj H_p(f1+f2)]; Re[conj w] = Re[w]
-      // makes four of the twelve contributions reuse these six:
+      // makes the other eight of the twelve contributions reuse these four:
       const double F1s = re3(hxu, hyv, hzp);      // (x,y,z) at (u,v);    = (z,x,y) at (v+u,-u)
       const double F1o = re3(hxu, cyv, hzm);      // (x,y,z) at (u,-v)
       const double F2s = re3(hym, hxv, hzu);      // (y,x,z) at (u-v,v);  = (x,z,y) at (v,-u)
       const double F2o = re3(hyp, cxv, hzu);      // (y,x,z) at (u+v,-v); = (x,z,y) at (v,u)
-      const double F3s = re3(hyv, hzm, hxu);      // (y,z,x) at (v,u-v);  = (z,x,y) at (v-u,u)
-      const double F3o = re3(cyv, hzp, hxu);      // (y,z,x) at (-v,u+v)
       const double ga = g * Wa, gbs = ms * g * Wb, gdo = mo * g * Wd;
       s[0] += ga * (ms + mo);
       s[1] += ga * (ms * F1s + mo * F1o);
       s[2] += gbs + gdo;
       s[3] += gbs * F2s + gdo * F2o;
-      s[4] += gbs * F3s + gdo * F3o;
+      s[4] += gbs * F1o + gdo * F1s;
       if constexpr (!DIAG) {
          const double We = qw(ed, od, eu, ou, qr);
          const double Wf = qw(es, os, eu, ou, qr);
-         const double2 hymn = make_double2(hym.x, -hym.y);   // H_y(v-u)
-         const double2 czu = make_double2(hzu.x, -hzu.y);
-         const double G3s = re3(hzu, hymn, hxv);  // (z,y,x) at (u,v-u)
-         const double G3o = re3(czu, hyp, hxv);   // (z,y,x) at (-u,u+v)
          const double ges = ms * g * We, gfo = mo * g * Wf;
          s[5] += ga * (ms * F2o + mo * F2s);
          s[6] += ges + gfo;
-         s[7] += ges * F3s + gfo * F1s;
-         s[8] += ges * G3s + gfo * G3o;
+         s[7] += ges * F1o + gfo * F1s;
+         s[8] += ges * F2s + gfo * F2o;
       }
    }
 }
@@ -360,7 +360,8 @@ __device__ __forceinline__ void epi_accum(const GsymArgs &a, int x, int y, int z
 // u + v <= n - 1) or lies beyond the plane's f1+f2 edge with the other four
 // weights interior (CLS 2: u, v <= n - 1 and u + v >= n + 1, so Wd = Wf = 0).
 // With Wa = Wb = We = 1, ms = mo = 2 and h = 2 g = 4 G the contributions of
-// epi_accum reduce to the sums below; no lane is out of range.
+// epi_accum reduce to three (CLS 1) or five (CLS 2) distinct sums, kept per
+// block and spread over the nine at its end; no lane is out of range.
 template <int CLS, bool DIAG>
 __device__ __forceinline__ void epi_accum_inner(const GsymArgs &a, int x, int y, int z, int ur0,
                                                 int vq0, int lane, const double (&gval)[4],
@@ -373,6 +374,9 @@ __device__ __forceinline__ void epi_accum_inner(const GsymArgs &a, int x, int y,
    const double2 hxv = Hx[v], hyv = Hy[v];
    const double2 cyv = make_double2(hyv.x, -hyv.y);
    const double2 cxv = make_double2(hxv.x, -hxv.y);
+   // lh = sum h, lA = sum h (F1s + F1o), lB = sum h F2s (CLS 2) or h (F2s + F2o)
+   // (CLS 1), lC = sum h F1o, lD = sum h (F2s + F2o) (CLS 2)
+   double lh = 0.0, lA = 0.0, lB = 0.0, lC = 0.0, lD = 0.0;
 #pragma unroll
    for (int t = 0; t < 4; ++t) {
       const int u = ur0 + (interleaved ? ((lane >> 4) + 4 * t) : (4 * (lane >> 4) + t));
@@ -380,7 +384,7 @@ __device__ __forceinline__ void epi_accum_inner(const GsymArgs &a, int x, int y,
       const int sp = u + v;
       const int dm = u - v;
       const int ad = dm < 0 ? -dm : dm;
-      const double2 hxu = Hx[u], hzu = Hz[u], hyp = Hy[sp], hzp = Hz[sp];
+      const double2 hxu = Hx[u], hzu = Hz[u], hzp = Hz[sp];
       double2 hym = Hy[ad], hzm = Hz[ad];
       if (dm < 0) {
          hym.y = -hym.y;
@@ -389,35 +393,45 @@ __device__ __forceinline__ void epi_accum_inner(const GsymArgs &a, int x, int y,
       const double F1s = re3(hxu, hyv, hzp);
       const double F1o = re3(hxu, cyv, hzm);
       const double F2s = re3(hym, hxv, hzu);
-      const double F2o = re3(hyp, cxv, hzu);
-      const double F3s = re3(hyv, hzm, hxu);
-      s[0] += 2.0 * h;
-      s[1] += h * (F1s + F1o);
+      lh += h;
+      lA += h * (F1s + F1o);
       if constexpr (CLS == 1) {
-         const double F3o = re3(cyv, hzp, hxu);
-         s[2] += 2.0 * h;
-         s[3] += h * (F2s + F2o);
-         s[4] += h * (F3s + F3o);
+         const double F2o = re3(Hy[sp], cxv, hzu);
+         lB += h * (F2s + F2o);
       } else {
-         s[2] += h;
-         s[3] += h * F2s;
-         s[4] += h * F3s;
-      }
-      if constexpr (!DIAG) {
-         const double2 hymn = make_double2(hym.x, -hym.y);   // H_y(v-u)
-         const double G3s = re3(hzu, hymn, hxv);
-         s[5] += h * (F2o + F2s);
-         if constexpr (CLS == 1) {
-            const double2 czu = make_double2(hzu.x, -hzu.y);
-            const double G3o = re3(czu, hyp, hxv);
-            s[6] += 2.0 * h;
-            s[7] += h * (F3s + F1s);
-            s[8] += h * (G3s + G3o);
-         } else {
-            s[6] += h;
-            s[7] += h * F3s;
-            s[8] += h * G3s;
+         lB += h * F2s;
+         lC += h * F1o;
+         if constexpr (!DIAG) {
+            const double F2o = re3(Hy[sp], cxv, hzu);
+            lD += h * (F2s + F2o);
          }
+      }
+   }
+   if constexpr (CLS == 1) {
+      // s0 = s2 = s6 = 2h, s1 = s4 = s7 = h(F1s+F1o), s3 = s5 = s8 = h(F2s+F2o)
+      s[0] += 2.0 * lh;
+      s[1] += lA;
+      s[2] += 2.0 * lh;
+      s[3] += lB;
+      s[4] += lA;
+      if constexpr (!DIAG) {
+         s[5] += lB;
+         s[6] += 2.0 * lh;
+         s[7] += lA;
+         s[8] += lB;
+      }
+   } else {
+      // beyond f1+f2 = fmax: Wd = Wf = 0
+      s[0] += 2.0 * lh;
+      s[1] += lA;
+      s[2] += lh;
+      s[3] += lB;
+      s[4] += lC;
+      if constexpr (!DIAG) {
+         s[5] += lD;
+         s[6] += lh;
+         s[7] += lC;
+         s[8] += lB;
       }
    }
 }
