@@ -526,7 +526,28 @@ __device__ __forceinline__ void epi_wave(const GsymArgs &a, int x, bool diag, in
                        lane, g, interleaved, s);
       }
       epi_flush(a, x, y0, z0, diag, lane, s);
+   } else if constexpr (BR * BC > 4) {
+      // the wave's rows (cols) span modes y0..y1 (z0..z1), each 16-block in one
+      // of them (nhp is a multiple of 16): one reduction per (y, z) pair
+#pragma unroll 1
+      for (int y = y0; y <= y1; ++y)
+#pragma unroll 1
+         for (int z = z0; z <= z1; ++z) {
+            if (y >= nm || z >= nm) continue;          // uniform per wave
+#pragma unroll 1
+            for (int b = 0; b < BR * BC; ++b) {
+               const int i = b / BC, j = b % BC;
+               const int r0 = rw0 + 16 * i, q0 = cw0 + 16 * j;
+               if (r0 / a.nhp != y || q0 / a.nhp != z) continue;
+               double g[4];
+               getv(b, g);
+               epi_accum_any(a, x, y, z, diag, r0 - y * a.nhp, q0 - z * a.nhp, lane, g, interleaved,
+                             s);
+            }
+            epi_flush(a, x, y, z, diag, lane, s);
+         }
    } else {
+      // few blocks per wave (and fewer registers to spare): one reduction per block
 #pragma unroll 1
       for (int b = 0; b < BR * BC; ++b) {
          const int i = b / BC, j = b % BC;
