@@ -37,7 +37,7 @@ extern "C" {
 
 /* precision of the node contraction (flags bits 0-3) */
 #define BSAHIP_PREC_FP64       0  /* FP64 MFMA, the plain form                */
-#define BSAHIP_PREC_FP32       1  /* FP32 operands, FP32 MFMA accumulation    */
+#define BSAHIP_PREC_FP32       1  /* reserved: every entry returns EUNSUPPORTED */
 #define BSAHIP_PREC_BF16X3     2  /* split-bf16 x3 products, FP32 accumulate  */
 #define BSAHIP_PREC_MASK     0xF
 
