@@ -311,6 +311,9 @@ __device__ __forceinline__ void epi_accum(const GsymArgs &a, int x, int y, int z
    const bool ev = (v == n), ov = (v > n);
 #pragma unroll
    for (int t = 0; t < 4; ++t) {
+      // rows past the plane (padding up to a multiple of 16) add nothing: stop
+      // once the lowest row of this register is one of them (uniform)
+      if (ur0 + (interleaved ? 4 * t : t) >= a.nu) break;
       const int ur = ur0 + (interleaved ? ((lane >> 4) + 4 * t) : (4 * (lane >> 4) + t));
       const bool ok = vok && ur < a.nu;
       const int u = ok ? ur : 0;
@@ -359,8 +362,10 @@ __device__ __forceinline__ void epi_accum(const GsymArgs &a, int x, int y, int z
 // the plane's interior in all five quadrature weights (CLS 1: 1 <= u, v and
 // u + v <= n - 1) or lies beyond the plane's f1+f2 edge with the other four
 // weights interior (CLS 2: u, v <= n - 1 and u + v >= n + 1, so Wd = Wf = 0).
+// CLS 3: the band between them (u, v, |u-v| interior, u + v on both sides of
+// n), where Wd = Wf = w(u+v): 1 below n, the edge weight at n, 0 beyond.
 // With Wa = Wb = We = 1, ms = mo = 2 and h = 2 g = 4 G the contributions of
-// epi_accum reduce to three (CLS 1) or five (CLS 2) distinct sums, kept per
+// epi_accum reduce to three (CLS 1) or five (CLS 2, 3) distinct sums, kept per
 // block and spread over the nine at its end; no lane is out of range.
 template <int CLS, bool DIAG>
 __device__ __forceinline__ void epi_accum_inner(const GsymArgs &a, int x, int y, int z, int ur0,
@@ -374,9 +379,12 @@ __device__ __forceinline__ void epi_accum_inner(const GsymArgs &a, int x, int y,
    const double2 hxv = Hx[v], hyv = Hy[v];
    const double2 cyv = make_double2(hyv.x, -hyv.y);
    const double2 cxv = make_double2(hxv.x, -hxv.y);
-   // lh = sum h, lA = sum h (F1s + F1o), lB = sum h F2s (CLS 2) or h (F2s + F2o)
-   // (CLS 1), lC = sum h F1o, lD = sum h (F2s + F2o) (CLS 2)
-   double lh = 0.0, lA = 0.0, lB = 0.0, lC = 0.0, lD = 0.0;
+   // lh = sum h, lA = sum h (F1s + F1o), lB = sum h F2s (CLS 2), h (F2s + F2o)
+   // (CLS 1) or h (F2s + w F2o) (CLS 3), lC = sum h F1o (CLS 2) or h (F1o + w F1s)
+   // (CLS 3), lD = sum h (F2s + F2o) (CLS 2, 3), lW = sum h w (CLS 3)
+   const int n = a.nh - 1;
+   const double ce = qrule(a.rule).c_edge;
+   double lh = 0.0, lA = 0.0, lB = 0.0, lC = 0.0, lD = 0.0, lW = 0.0;
 #pragma unroll
    for (int t = 0; t < 4; ++t) {
       const int u = ur0 + (interleaved ? ((lane >> 4) + 4 * t) : (4 * (lane >> 4) + t));
@@ -398,13 +406,21 @@ __device__ __forceinline__ void epi_accum_inner(const GsymArgs &a, int x, int y,
       if constexpr (CLS == 1) {
          const double F2o = re3(Hy[sp], cxv, hzu);
          lB += h * (F2s + F2o);
-      } else {
+      } else if constexpr (CLS == 2) {
          lB += h * F2s;
          lC += h * F1o;
          if constexpr (!DIAG) {
             const double F2o = re3(Hy[sp], cxv, hzu);
             lD += h * (F2s + F2o);
          }
+      } else {
+         const double w = sp < n ? 1.0 : (sp == n ? ce : 0.0);
+         const double F2o = re3(Hy[sp], cxv, hzu);
+         const double hw = h * w;
+         lW += hw;
+         lB += h * F2s + hw * F2o;
+         lC += h * F1o + hw * F1s;
+         if constexpr (!DIAG) lD += h * (F2s + F2o);
       }
    }
    if constexpr (CLS == 1) {
@@ -418,6 +434,18 @@ __device__ __forceinline__ void epi_accum_inner(const GsymArgs &a, int x, int y,
          s[5] += lB;
          s[6] += 2.0 * lh;
          s[7] += lA;
+         s[8] += lB;
+      }
+   } else if constexpr (CLS == 3) {
+      s[0] += 2.0 * lh;
+      s[1] += lA;
+      s[2] += lh + lW;
+      s[3] += lB;
+      s[4] += lC;
+      if constexpr (!DIAG) {
+         s[5] += lD;
+         s[6] += lh + lW;
+         s[7] += lC;
          s[8] += lB;
       }
    } else {
@@ -436,15 +464,15 @@ __device__ __forceinline__ void epi_accum_inner(const GsymArgs &a, int x, int y,
    }
 }
 
-// Class of a 16x16 block with half-grid origin (u0, v0): 1 or 2 as above,
-// 0 for every other block (edges, the f1+f2 = fmax band, rows beyond the plane).
+// Class of a 16x16 block with half-grid origin (u0, v0): 1, 2 or 3 as above,
+// 0 for every other block (edges of the plane, rows beyond it).
 __device__ __forceinline__ int epi_class(const GsymArgs &a, int u0, int v0)
 {
    const int n = a.nh - 1;
    if (a.epi_plain || u0 < 1 || v0 < 1 || u0 + 15 > n - 1 || v0 + 15 > n - 1) return 0;
    if (u0 + v0 + 30 <= n - 1) return 1;
    if (u0 + v0 >= n + 1) return 2;
-   return 0;
+   return 3;
 }
 
 __device__ __forceinline__ void epi_accum_any(const GsymArgs &a, int x, int y, int z, bool diag,
@@ -457,6 +485,8 @@ __device__ __forceinline__ void epi_accum_any(const GsymArgs &a, int x, int y, i
          epi_accum_inner<1, true>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
       else if (cls == 2)
          epi_accum_inner<2, true>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
+      else if (cls == 3)
+         epi_accum_inner<3, true>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
       else
          epi_accum<true>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
    } else {
@@ -464,6 +494,8 @@ __device__ __forceinline__ void epi_accum_any(const GsymArgs &a, int x, int y, i
          epi_accum_inner<1, false>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
       else if (cls == 2)
          epi_accum_inner<2, false>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
+      else if (cls == 3)
+         epi_accum_inner<3, false>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
       else
          epi_accum<false>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
    }
