@@ -124,15 +124,20 @@ def test_plan_runs_repeat_and_match_one_shot(gpu_lib, prec):
         assert rel_err(b[k], a[k]) < 1e-14, k
 
 
-@pytest.mark.parametrize("cfg,prec,exact", [("config2", "fp64", False), ("config2", "bf16x3", True),
-                                            ("config3", "fp64", False), ("config3", "bf16x3", False)])
-def test_specialised_epilogue_equals_generic(gpu_lib, cfg, prec, exact):
+@pytest.mark.parametrize("cfg,prec,exact,quad", [
+    ("config2", "fp64", False, "vect"), ("config2", "bf16x3", True, "vect"),
+    ("config3", "fp64", False, "vect"), ("config3", "bf16x3", False, "vect"),
+    ("config2", "fp64", False, "riemann"), ("config2", "bf16x3", False, "trapezoid")])
+def test_specialised_epilogue_equals_generic(gpu_lib, cfg, prec, exact, quad):
     """Full-size grids (513^2, 1025^2): the epilogue's specialised paths for
-    interior blocks and for blocks beyond the f1+f2 edge give the generic
-    epilogue's moments to rounding (the same G values, other summation)."""
+    interior blocks, for blocks beyond the f1+f2 edge and for the band that
+    straddles it (whose f1+f2 weight is the rule's edge weight at n) give the
+    generic epilogue's moments to rounding (the same G values, other summation),
+    under each quadrature rule."""
     p = getattr(synth, cfg)()
     b = gpu_lib.Bound(p)
-    fast = gpu_lib.classic_run(p, prec=prec, exact=exact, bound=b, psd=False)
-    slow = gpu_lib.classic_run(p, prec=prec, exact=exact, bound=b, psd=False, epi_generic=True)
+    fast = gpu_lib.classic_run(p, quad=quad, prec=prec, exact=exact, bound=b, psd=False)
+    slow = gpu_lib.classic_run(p, quad=quad, prec=prec, exact=exact, bound=b, psd=False,
+                               epi_generic=True)
     for k in ("m3mf", "m3mr"):
         assert rel_err(fast[k], slow[k]) < 1e-13, k
