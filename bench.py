@@ -28,6 +28,7 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "modal-bispectrum evals/s (NN=1000,NM=30) at skew err≤1e-4: mixed vs FP64 GPU"
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
 # HBM bytes per k_gsym_bf16x3 launch on the deck workload (1025^2, NM=30):
 # rocprofv3 --pmc FETCH_SIZE x 2 + WRITE_SIZE x 1 (KiB), the scales read off a
@@ -49,26 +50,70 @@ def gsym_flops(prob, nm):
     return 2.0 * prob.ntc * prob.nnl * nm * n * (n + 1) / 2.0
 
 
-def cpu_reference_baseline(threads: int):
+def host_description() -> dict:
+    """CPU of the host the baselines run on: logical CPUs, the ones this process
+    may use, model and socket count (from /proc/cpuinfo)."""
+    model, sockets = None, set()
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k == "model name" and model is None:
+                model = v.strip()
+            elif k == "physical id":
+                sockets.add(v.strip())
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"nproc": os.cpu_count(), "usable": usable, "model": model,
+            "sockets": len(sockets) or None}
+
+
+REF_THREADS = 8   # getFM_full_tnm_vect_cls_ opens num_threads(8) (functionsImpl.F90:1490)
+
+
+def cpu_reference_baseline(budget_s: float = 25.0):
     """Reference's own getFM_full_tnm_vect_cls_ (oracle/_ref, amdflang -O2,
-    OpenMP) timed on a bounded sample of the deck: NNL=10 nodes spread over
-    the span, NM=30, NTC=2, 33-point grid.  Its element-update rate divided by
-    the updates one 1025^2-grid eval needs at NNL=1000 (NTC*NNL^3*NM^3) gives
-    the reference's evals/s at the benchmark size (extrapolated in NNL^3, the
-    reference's exact loop count)."""
+    OpenMP with its hard-coded 8 threads) on bounded samples of the deck:
+    NNL = 6, 8, 10, ... nodes spread over the span, NM=30, NTC=2, 17-point grid,
+    until the samples have taken budget_s.  Its loop count is NTC*NNL^3*NM^3 NF-long
+    updates per grid row; the NNL exponent is fitted on the samples (log t vs
+    log NNL) and the largest sample is extrapolated with it to NNL=1000, then
+    divided by its NF^2 points: the reference's evals/s at the benchmark size."""
     from bsalib_amd import synth
     from oracle import oracle as O
     if not O.reference_available():
         return None
-    p = synth.deck(nn=10, nm=30, nfreqs=17, df=0.25, span=1000.0)
-    t = O.reference_time(p, threads)
-    updates = p.ntc * p.nnl ** 3 * 30 ** 3 * p.nf ** 2
-    rate = updates / t
-    per_eval = 2 * 1000 ** 3 * 30 ** 3
-    return {"value": rate / per_eval, "unit": "evals/s", "cores": threads, "kind": "reference",
-            "sample": f"getFM_full_tnm_vect_cls_ NNL=10 NM=30 NTC=2 NF=33 ({t:.2f} s, "
-                      f"{rate:.3e} element updates/s), extrapolated by NTC*NNL^3*NM^3 to NNL=1000",
-            "seconds": t}
+    samples = []
+    spent = 0.0
+    nnl = 6
+    while True:
+        p = synth.deck(nn=nnl, nm=30, nfreqs=9, df=0.5, span=1000.0)
+        t = O.reference_time(p, REF_THREADS)
+        spent += t
+        samples.append({"nnl": nnl, "nf": p.nf, "seconds": t,
+                        "updates_per_s": p.ntc * nnl ** 3 * 30 ** 3 * p.nf ** 2 / t})
+        if len(samples) >= 3 and spent + t * ((nnl + 2) / nnl) ** 3 > budget_s:
+            break
+        nnl += 2
+    x = np.log([s["nnl"] for s in samples])
+    y = np.log([s["seconds"] for s in samples])
+    expo = float(np.polyfit(x, y, 1)[0])
+    last = samples[-1]
+    per_eval_fit = last["seconds"] * (1000.0 / last["nnl"]) ** expo / last["nf"] ** 2
+    per_eval_cube = 2 * 1000 ** 3 * 30 ** 3 / last["updates_per_s"]
+    return {"value": 1.0 / per_eval_fit, "unit": "evals/s", "cores": REF_THREADS,
+            "kind": "reference", "threads": REF_THREADS, "host": host_description(),
+            "sample": f"getFM_full_tnm_vect_cls_ on the deck at NNL="
+                      f"{','.join(str(s['nnl']) for s in samples)}, NM=30, NTC=2, NF=17 "
+                      f"({spent:.1f} s); fitted NNL exponent {expo:.2f}, largest sample "
+                      f"extrapolated to NNL=1000 (extrapolated)",
+            "samples": samples, "nnl_exponent": expo,
+            "value_exact_count": 1.0 / per_eval_cube,
+            "seconds": spent}
 
 
 def cpu_factorised_baseline(prob, threads: int, npts: int = 64):
@@ -83,8 +128,45 @@ def cpu_factorised_baseline(prob, threads: int, npts: int = 64):
     O.classic_factorised(prob, rule="vect", truncate=True, nthreads=threads, point_sample=(ii, jj))
     t = time.perf_counter() - t0
     return {"value": npts / t, "unit": "evals/s", "cores": threads, "kind": "port",
+            "threads": threads, "host": host_description(),
             "sample": f"{npts} random points of the 1025^2 grid incl. the projection table "
                       f"({t:.2f} s)"}
+
+
+def precision_errors(prob, r64: dict, r: dict, prec: str, b64=None, bound=None) -> dict:
+    """Errors of a mixed-precision run against the FP64 run of the same plan,
+    per result: third moments and skewness normwise (max|d| / max|ref|) and per
+    entry (max |d|/|ref| over entries with |ref| >= 1e-3 max|ref|), and the
+    bispectrum L2 error ||B - B64|| / ||B64|| over 27 modal entries (modes 1,
+    NM/2, NM in every position) at every point of the grid, force and response
+    (bsahip_classic_bispectra_sel: the mixed G values with the G kernel's
+    operand rounding and FP32 accumulation)."""
+    from bsalib_amd import abi
+    from bsalib_amd.host import skewness
+    nm = prob.nm
+
+    def norm_entry(a, ref):
+        a, ref = np.asarray(a), np.asarray(ref)
+        big = np.abs(ref) >= 1e-3 * np.abs(ref).max()
+        return (float(np.abs(a - ref).max() / np.abs(ref).max()),
+                float((np.abs(a - ref)[big] / np.abs(ref[big])).max()))
+    out = {}
+    for k in ("m3mf", "m3mr", "m2mr"):
+        out[k + "_normwise"], out[k + "_per_entry"] = norm_entry(r[k], r64[k])
+    sk64 = skewness(r64["m2mr"], r64["m3mr"], nm)
+    sk = skewness(r["m2mr"], r["m3mr"], nm)
+    out["skewness_normwise"], out["skewness_per_entry"] = norm_entry(sk, sk64)
+    modes = np.array([0, nm // 2, nm - 1])
+    sel = (modes[:, None, None] + nm * modes[None, :, None] + nm * nm * modes[None, None, :])
+    sel = np.unique(sel.ravel()) + 1
+    if b64 is None:
+        b64 = abi.classic_bispectra_sel(prob, sel, bound=bound)
+    bm = abi.classic_bispectra_sel(prob, sel, bound=bound, prec=prec)
+    for k in ("bisp_f", "bisp_r"):
+        out[k + "_l2"] = float(np.linalg.norm(bm[k] - b64[k]) / np.linalg.norm(b64[k]))
+    out["bisp_entries"] = int(sel.size)
+    out["bisp_points"] = int(prob.nf ** 2)
+    return out, b64
 
 
 def batch_leg(log):
@@ -144,8 +226,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the structure-batch leg")
     ap.add_argument("--no-mesher", action="store_true", help="skip the learned-Mesher leg")
+    ap.add_argument("--no-errors", action="store_true",
+                    help="skip the per-result precision report (bispectrum L2 etc.)")
     ap.add_argument("--config", default="config3", choices=["config1", "config2", "config3"])
-    ap.add_argument("--prec", default="both", choices=["both", "fp64", "bf16x3"],
+    ap.add_argument("--prec", default="both", choices=["both", "fp64", "fp32", "bf16x3"],
                     help="profiling aid: run only one precision (the JSON line needs both)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="profiling aid: one 1 GiB device-to-device copy (HBM counter scale)")
@@ -197,10 +281,14 @@ def main():
     r64, t64, k64 = timed("fp64")
     log(f"fp64: step {1e3 * t64 / args.steps:.2f} ms, k_gsym_f64 {k64:.2f} ms, "
         f"{flops / k64 / 1e9:.2f} TFLOP/s")
+    r32, t32, k32 = timed("fp32")
+    log(f"fp32: step {1e3 * t32 / args.steps:.2f} ms, k_gsym_f32 {k32:.2f} ms, "
+        f"{flops / k32 / 1e9:.2f} TFLOP/s")
     r, t, k_ms = timed("bf16x3")
     ms_step = 1e3 * t / args.steps
     value = evals * args.steps / t
     value64 = evals * args.steps / t64
+    value32 = evals * args.steps / t32
     achieved = flops / (k_ms * 1e-3) / 1e12
     sk64 = skewness(r64["m2mr"], r64["m3mr"], nm)
     sk16 = skewness(r["m2mr"], r["m3mr"], nm)
@@ -208,15 +296,22 @@ def main():
     m3_err = float(np.abs(r["m3mr"] - r64["m3mr"]).max() / np.abs(r64["m3mr"]).max())
     log(f"bf16x3: step {ms_step:.2f} ms, k_gsym_bf16x3 {k_ms:.2f} ms, {achieved:.2f} TFLOP/s, "
         f"speedup vs fp64 {value / value64:.2f}x, skewness err {sk_err:.2e}")
+    errors = None
+    if not args.no_errors:
+        e16, b64 = precision_errors(prob, r64, r, "bf16x3", bound=bound)
+        e32, _ = precision_errors(prob, r64, r32, "fp32", b64=b64, bound=bound)
+        errors = {"bf16x3": e16, "fp32": e32}
+        log(f"errors vs fp64: {errors}")
 
     batch = None if args.no_batch else batch_leg(log)
     mesher = None if args.no_mesher else mesher_leg(log)
     cpu = None
     cpu2 = None
     if not args.no_cpu:
-        threads = int(os.environ.get("BENCH_CPU_THREADS", str(min(16, os.cpu_count() or 1))))
+        threads = int(os.environ.get("BENCH_CPU_THREADS",
+                                     str(min(16, host_description()["usable"] or 1))))
         try:
-            cpu = cpu_reference_baseline(threads)
+            cpu = cpu_reference_baseline()
             log(f"cpu reference baseline {cpu}")
         except Exception as e:  # noqa: BLE001
             log(f"cpu reference baseline failed: {e}")
@@ -253,9 +348,16 @@ def main():
                  "kernel": "k_gsym_f64", "kernel_ms": k64,
                  "achieved_tflops": flops / (k64 * 1e-3) / 1e12, "peak_tflops": FP64_PEAK_TFLOPS,
                  "frac": flops / (k64 * 1e-3) / 1e12 / FP64_PEAK_TFLOPS},
+        "fp32": {"value": value32, "unit": "evals/s", "ms_per_step": 1e3 * t32 / args.steps,
+                 "kernel": "k_gsym_f32", "kernel_ms": k32,
+                 "achieved_tflops": flops / (k32 * 1e-3) / 1e12, "peak_tflops": FP32_PEAK_TFLOPS,
+                 "frac": flops / (k32 * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+                 "vs_fp64": value32 / value64},
         "mixed_vs_fp64": value / value64,
         "skewness_rel_err_vs_fp64": sk_err,
+        "meets_target": bool(sk_err <= 1e-4 and value / value64 >= 4.0),
         "m3mr_rel_err_vs_fp64": m3_err,
+        "errors": errors,
         "cpu_baseline": cpu,
         "cpu_baseline_factorised": cpu2,
         "batch": batch,

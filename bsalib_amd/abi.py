@@ -21,6 +21,7 @@ SUM_TRUNCATED, SUM_EXACT = 0, 1 << 8
 NO_PSD, NO_BISP = 1 << 9, 1 << 10
 ONLY_DIAG = 1 << 11
 EPI_GENERIC = 1 << 12          # testing: the generic quadrature epilogue only
+DETERMINISTIC = 1 << 13        # fixed-order (bitwise reproducible) moment sums
 # return codes (include/bsahip.h)
 EINVAL, EDEVICE, ENOMEM, EUNSUPPORTED, ESTATE = -1, -2, -3, -4, -5
 
@@ -158,7 +159,8 @@ def init(device: int = 0):
 
 def classic_run(prob, quad: str = "vect", prec: str = "fp64", bound: Bound = None,
                 psd: bool = True, bisp: bool = True, exact: bool = False,
-                only_diag: bool = False, epi_generic: bool = False) -> dict:
+                only_diag: bool = False, epi_generic: bool = False,
+                deterministic: bool = False) -> dict:
     """Classic full-grid moments (bsahip_classic_run); exact=True evaluates
     S(f1+f2) beyond the grid instead of taking it as zero; only_diag=True runs
     the ONLY_DIAG model (every output then holds NM values); epi_generic=True
@@ -170,7 +172,7 @@ def classic_run(prob, quad: str = "vect", prec: str = "fp64", bound: Bound = Non
     out.update({k: np.zeros(n3) for k in ("m3mf", "m3mr")})
     flags = PREC[prec] | QUAD[quad] | (0 if psd else NO_PSD) | (0 if bisp else NO_BISP) | \
         (SUM_EXACT if exact else SUM_TRUNCATED) | (ONLY_DIAG if only_diag else 0) | \
-        (EPI_GENERIC if epi_generic else 0)
+        (EPI_GENERIC if epi_generic else 0) | (DETERMINISTIC if deterministic else 0)
     D = C.POINTER(C.c_double)
     _check(lib().bsahip_classic_run(C.byref(b.struct), C.c_int32(prob.nfreqs), C.c_double(prob.df),
                                     C.c_int32(flags), _ptr(out["m2mf"], C.c_double),
@@ -197,16 +199,17 @@ def classic_spectra(prob, exact: bool = False, bound: Bound = None) -> dict:
 
 
 def classic_bispectra_sel(prob, sel, exact: bool = False, only_diag: bool = False,
-                          bound: Bound = None) -> dict:
+                          bound: Bound = None, prec: str = "fp64") -> dict:
     """Bispectra of the selected modal entries on the Classic grid, (NF, NF,
     len(sel)); sel is 1-based over the kept modes, m + (n-1) NM + (p-1) NM^2
-    (the mode position with only_diag).  bsahip_classic_bispectra_sel."""
+    (the mode position with only_diag).  prec "fp32" / "bf16x3": the G values
+    with that G kernel's operand rounding.  bsahip_classic_bispectra_sel."""
     b = bound or Bound(prob)
     nf = prob.nf
     s = np.ascontiguousarray(np.atleast_1d(sel), dtype=np.int32)
     bfm = np.zeros((nf, nf, s.size), order="F")
     brm = np.zeros((nf, nf, s.size), order="F")
-    flags = SUM_EXACT if exact else SUM_TRUNCATED
+    flags = (SUM_EXACT if exact else SUM_TRUNCATED) | PREC[prec]
     if only_diag:
         flags |= ONLY_DIAG
     _check(lib().bsahip_classic_bispectra_sel(C.byref(b.struct), C.c_int32(prob.nfreqs),
@@ -273,11 +276,12 @@ class Plan:
     many times.  The problem's host arrays are only read by the constructor."""
 
     def __init__(self, prob, quad: str = "vect", prec: str = "fp64", bound: Bound = None,
-                 psd: bool = True, bisp: bool = True):
+                 psd: bool = True, bisp: bool = True, deterministic: bool = False):
         b = bound or Bound(prob)
         self.nm = b.nm
         self.modes = b.modes
-        flags = PREC[prec] | QUAD[quad] | (0 if psd else NO_PSD) | (0 if bisp else NO_BISP)
+        flags = PREC[prec] | QUAD[quad] | (0 if psd else NO_PSD) | (0 if bisp else NO_BISP) | \
+            (DETERMINISTIC if deterministic else 0)
         h = C.c_void_p()
         _check(lib().bsahip_plan_create(C.addressof(b.struct), C.c_int32(prob.nfreqs),
                                         C.c_double(prob.df), C.c_int32(flags), C.byref(h)))
@@ -307,7 +311,7 @@ class Plan:
 class MesherCfg(C.Structure):
     """Mirror of ``bsahip_mesher_cfg`` (include/bsahip.h)."""
     _fields_ = [("bkg_base_rfmnt", C.c_int32), ("tile_cells", C.c_int32),
-                ("diag_crest", C.c_int32), ("reserved", C.c_int32), ("bkg_peakw", C.c_double),
+                ("diag_crest", C.c_int32), ("brm_cells", C.c_int32), ("bkg_peakw", C.c_double),
                 ("bkg_area_ext", C.c_double), ("peak_area_ext", C.c_double),
                 ("max_area_ext", C.c_double),
                 ("natf", C.POINTER(C.c_double)), ("xsi", C.POINTER(C.c_double))]
@@ -315,9 +319,10 @@ class MesherCfg(C.Structure):
 
 def mesher_cfg(prob, bkg_base_rfmnt: int = 20, tile_cells: int = 32, bkg_peakw=None,
                bkg_area_ext: float = 2.0, peak_area_ext: float = 2.0, max_area_ext: float = 2.0,
-               modes=None, diag_crest: bool = False, natf=None, xsi=None):
+               modes=None, diag_crest: bool = False, natf=None, xsi=None, brm_cells: int = 0):
     """bsahip_mesher_cfg for a problem; returns (cfg, arrays kept alive).  natf
-    and xsi (kept modes) default to the problem's; they only shape the layout."""
+    and xsi (kept modes) default to the problem's; they only shape the layout.
+    brm_cells > 0: uniform BRM spacing df_ref / brm_cells on every tile."""
     modes = host.kept_modes(prob) if modes is None else np.asarray(modes, np.int32)
     mo = modes - 1
     natf = np.ascontiguousarray(np.asarray(prob.natf, dtype=np.float64)[mo] if natf is None
@@ -326,7 +331,7 @@ def mesher_cfg(prob, bkg_base_rfmnt: int = 20, tile_cells: int = 32, bkg_peakw=N
                                else np.asarray(xsi, dtype=np.float64))
     cfg = MesherCfg()
     cfg.bkg_base_rfmnt, cfg.tile_cells = bkg_base_rfmnt, tile_cells
-    cfg.diag_crest, cfg.reserved = int(diag_crest), 0
+    cfg.diag_crest, cfg.brm_cells = int(diag_crest), int(brm_cells)
     cfg.bkg_peakw = host.bkg_peak_width(prob) if bkg_peakw is None else float(bkg_peakw)
     cfg.bkg_area_ext, cfg.peak_area_ext, cfg.max_area_ext = bkg_area_ext, peak_area_ext, max_area_ext
     cfg.natf = _ptr(natf, C.c_double)
@@ -354,7 +359,7 @@ class Mesher:
     def __init__(self, prob, bkg_base_rfmnt: int = 20, tile_cells: int = 32, bkg_peakw=None,
                  bkg_area_ext: float = 2.0, peak_area_ext: float = 2.0, max_area_ext: float = 2.0,
                  bound: Bound = None, diag_crest: bool = False, natf=None, xsi=None,
-                 load: str = None, only_diag: bool = False):
+                 load: str = None, only_diag: bool = False, brm_cells: int = 0):
         """load: a checkpoint written by save() for the same problem and
         settings, read instead of evaluating the pre-mesh (bsahip_mesher_load).
         only_diag: the ONLY_DIAG model (NM entries per point instead of NM^3)."""
@@ -365,7 +370,8 @@ class Mesher:
         flags = C.c_int32(ONLY_DIAG if only_diag else 0)
         cfg, self._keep = mesher_cfg(prob, bkg_base_rfmnt, tile_cells, bkg_peakw, bkg_area_ext,
                                      peak_area_ext, max_area_ext, modes=b.modes,
-                                     diag_crest=diag_crest, natf=natf, xsi=xsi)
+                                     diag_crest=diag_crest, natf=natf, xsi=xsi,
+                                     brm_cells=brm_cells)
         self.cfg = cfg
         self.bkg_peakw = cfg.bkg_peakw
         h = C.c_void_p()

@@ -86,6 +86,18 @@ __global__ __launch_bounds__(256, 2) void k_gsym_bf16x3_batch(
    });
 }
 
+__global__ __launch_bounds__(256, 2) void k_gsym_f32_batch(
+   const GsymArgs *__restrict__ args, const GsymArgs16 *__restrict__ args16,
+   const long *__restrict__ offs, int ns, const long *__restrict__ qbeg,
+   unsigned long long *counters, int chunk)
+{
+   batch_drain(offs, ns, qbeg, counters, chunk, [&](int s, long li) {
+      const GsymArgs a = args[s];
+      const GsymArgs16 b = args16[s];
+      gsym_f32_item(a, b, (int)(li % a.nm), li / a.nm);
+   });
+}
+
 // results of every structure into one contiguous buffer: array j = srcs[j]
 // (lens[j] doubles) to dst + offs[j]; one block per array
 __global__ void k_pack(const double *const *__restrict__ srcs, const long *__restrict__ lens,
@@ -159,6 +171,9 @@ int batch_launch(bsahip_batch *b)
    if (b->prec == BSAHIP_PREC_BF16X3)
       k_gsym_bf16x3_batch<<<grid, 256, 0, g.s>>>(b->args, b->args16, b->offs, ns, b->qbeg,
                                                  b->counter, b->chunk);
+   else if (b->prec == BSAHIP_PREC_FP32)
+      k_gsym_f32_batch<<<grid, 256, 0, g.s>>>(b->args, b->args16, b->offs, ns, b->qbeg,
+                                              b->counter, b->chunk);
    else
       k_gsym_f64_batch<<<grid, 256, 0, g.s>>>(b->args, b->offs, ns, b->qbeg, b->counter, b->chunk);
    HIPC(hipGetLastError());
@@ -254,6 +269,8 @@ int bsahip_batch_create(const bsahip_problem *const *probs, const int32_t *nfh, 
    CHK(ensure_init());
    CHK(check_flags(flags, true, true));
    if (flags & BSAHIP_SUM_EXACT) return fail(BSAHIP_EUNSUPPORTED, "batches use the Classic truncated sum");
+   if (flags & BSAHIP_DETERMINISTIC)
+      return fail(BSAHIP_EUNSUPPORTED, "batches sum with atomics (no fixed-order mode)");
    std::unique_ptr<bsahip_batch> B(new bsahip_batch());
    B->prec = flags & BSAHIP_PREC_MASK;
    B->flags = flags;
@@ -297,6 +314,8 @@ int bsahip_batch_create(const bsahip_problem *const *probs, const int32_t *nfh, 
    int occ = 1;
    if (B->prec == BSAHIP_PREC_BF16X3)
       HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_gsym_bf16x3_batch, 256, 0));
+   else if (B->prec == BSAHIP_PREC_FP32)
+      HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_gsym_f32_batch, 256, 0));
    else
       HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_gsym_f64_batch, 256, 0));
    B->occ = std::max(1, occ);

@@ -134,7 +134,7 @@ int validate(const bsahip_problem *p)
    if (p->nn <= 0 || p->nnl <= 0 || p->nm <= 0 || p->ntc <= 0 || p->ntc > 3 || p->nwz <= 0)
       return fail(BSAHIP_EINVAL, "invalid sizes (nn, nnl, nm, ntc in 1..3, nwz must be > 0)");
    if (p->nnl > p->nn) return fail(BSAHIP_EINVAL, "nnl > nn");
-   if (p->nm > 64) return fail(BSAHIP_EUNSUPPORTED, "more than 64 modes");
+   if (p->nm > 256) return fail(BSAHIP_EUNSUPPORTED, "more than 256 modes");
    if (p->ndegw < 6) return fail(BSAHIP_EINVAL, "ndegw must be >= 6 (3 linear + 3 quadratic)");
    if ((long)p->ncorr_rows != ((long)p->nn * p->nn + p->nn) / 2)
       return fail(BSAHIP_EINVAL, "ncorr_rows must equal (nn*nn+nn)/2");
@@ -359,10 +359,13 @@ int build_table(Bufs &B, const Prob &D, const double *fl, int nfl, int rows_per_
    HIPC(hipGetLastError());
    if (D.pod_nmodes > 0 || D.pod_lim > 0.0)
       return pod_project(B, D, sq, fl, nfl, rows_per_mode, U, ldU);
+   // modes in chunks of at most 64 (grid z = component + NTC * chunk)
    const int mt = (D.nm + 15) / 16;
+   const int nchunk = (D.nm + 63) / 64;
+   dim3 gpm(gp.x, gp.y, (unsigned)(D.ntc * nchunk));
 #define LAUNCH_PROJ(MT)                                                                       \
-   k_project<MT><<<gp, 256, 0, g.s>>>(D.pa_t, D.lnr, sq, fl, nfl, D.nnl, D.nm, D.nnlp,        \
-                                      rows_per_mode, U, ldU)
+   k_project<MT><<<gpm, 256, 0, g.s>>>(D.pa_t, D.lnr, sq, fl, nfl, D.nnl, D.nm, D.nnlp,       \
+                                       rows_per_mode, U, ldU, D.ntc)
    if (mt <= 1) LAUNCH_PROJ(1);
    else if (mt <= 2) LAUNCH_PROJ(2);
    else if (mt <= 3) LAUNCH_PROJ(3);
@@ -384,7 +387,8 @@ int check_flags(int32_t flags, bool grid, bool allow_mixed)
 {
    const int prec = flags & BSAHIP_PREC_MASK;
    const int quad = (flags & BSAHIP_QUAD_MASK) >> 4;
-   if (prec != BSAHIP_PREC_FP64 && !(allow_mixed && prec == BSAHIP_PREC_BF16X3))
+   if (prec != BSAHIP_PREC_FP64 &&
+       !(allow_mixed && (prec == BSAHIP_PREC_BF16X3 || prec == BSAHIP_PREC_FP32)))
       return fail(BSAHIP_EUNSUPPORTED, "precision not available in this entry point");
    if (grid && quad > 2) return fail(BSAHIP_EINVAL, "unknown quadrature rule");
    return 0;
@@ -430,9 +434,94 @@ struct bsahip_plan {
    float *UR = nullptr;
    uint16_t *Ch = nullptr, *Cl = nullptr;
    double last_ms = 0.0;
+   // fixed-order summation (BSAHIP_DETERMINISTIC): per-flush slots of the G
+   // epilogue, the slot list of every m3 entry, per-block m2 partials
+   double *slots = nullptr, *m2part = nullptr;
+   long *det_off = nullptr, *det_idx = nullptr;
+   int mflush = 0;
 };
 
 namespace {
+
+// Host pair index of a tile pair as the kernels' pair_from_index: t ->
+// (R, Q), R <= Q, t = Q (Q + 1) / 2 + R.
+void host_pair(long t, int &R, int &Q)
+{
+   long q = (long)((std::sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+   while ((q + 1) * (q + 2) / 2 <= t) ++q;
+   while (q * (q + 1) / 2 > t) --q;
+   Q = (int)q;
+   R = (int)(t - q * (q + 1) / 2);
+}
+
+// Slot lists of the fixed-order mode: the flushes of every wave of every G
+// item in the order epi_wave makes them (kernels.hip), each with the twelve m3
+// entries of epi_flush (m3f entries 0 .. NM^3-1, m3r entries NM^3 .. 2 NM^3-1).
+// off (2 NM^3 + 1) and idx list, per entry, its slots in ascending order;
+// mflush is the largest flush count of a wave.
+void det_slot_lists(const bsahip_plan *P, std::vector<long> &off, std::vector<long> &idx,
+                    int &mflush)
+{
+   const int nm = P->D.nm, nhp = P->nhp;
+   const bool f64 = (P->prec == BSAHIP_PREC_FP64);
+   const int T = f64 ? TILE : T2, WT = f64 ? 32 : 64, BR = WT / 16, BC = BR;
+   const long nm3 = (long)nm * nm * nm, nent = 2 * nm3;
+   const long npairs = (long)P->ntile * (P->ntile + 1) / 2, nitems = npairs * nm;
+   auto walk = [&](auto emit) {
+      for (long lid = 0; lid < nitems; ++lid) {
+         const int x = (int)(lid % nm);
+         int R, Q;
+         host_pair(lid / nm, R, Q);
+         const bool diag = (R == Q);
+         for (int w = 0; w < 4; ++w) {
+            const int rw0 = R * T + (w >> 1) * WT, cw0 = Q * T + (w & 1) * WT;
+            const int y0 = rw0 / nhp, y1 = (rw0 + 16 * BR - 1) / nhp;
+            const int z0 = cw0 / nhp, z1 = (cw0 + 16 * BC - 1) / nhp;
+            int f = 0;
+            if (y0 == y1 && z0 == z1) {
+               if (y0 < nm && z0 < nm) emit(lid, w, f++, x, y0, z0, diag);
+            } else if (BR * BC > 4) {
+               for (int y = y0; y <= y1; ++y)
+                  for (int z = z0; z <= z1; ++z)
+                     if (y < nm && z < nm) emit(lid, w, f++, x, y, z, diag);
+            } else {
+               for (int b = 0; b < BR * BC; ++b) {
+                  const int y = (rw0 + 16 * (b / BC)) / nhp, z = (cw0 + 16 * (b % BC)) / nhp;
+                  if (y < nm && z < nm) emit(lid, w, f++, x, y, z, diag);
+               }
+            }
+         }
+      }
+   };
+   auto entries = [&](int x, int y, int z, bool diag, long (&e)[12]) {
+      const long xyz = x + (long)nm * y + (long)nm * nm * z, yxz = y + (long)nm * x + (long)nm * nm * z,
+                 yzx = y + (long)nm * z + (long)nm * nm * x, xzy = x + (long)nm * z + (long)nm * nm * y,
+                 zxy = z + (long)nm * x + (long)nm * nm * y, zyx = z + (long)nm * y + (long)nm * nm * x;
+      const long v[12] = {xyz, nm3 + xyz, yxz, nm3 + yxz, yzx, nm3 + yzx,
+                          xzy, nm3 + xzy, zxy, nm3 + zxy, zyx, nm3 + zyx};
+      for (int t = 0; t < 12; ++t) e[t] = (diag && t >= 6) ? -1 : v[t];
+   };
+   std::vector<long> cnt(nent + 1, 0);
+   mflush = 1;
+   walk([&](long, int, int f, int x, int y, int z, bool diag) {
+      mflush = std::max(mflush, f + 1);
+      long e[12];
+      entries(x, y, z, diag, e);
+      for (int t = 0; t < 12; ++t)
+         if (e[t] >= 0) ++cnt[e[t] + 1];
+   });
+   off.assign(nent + 1, 0);
+   for (long e = 0; e < nent; ++e) off[e + 1] = off[e] + cnt[e + 1];
+   idx.assign(off[nent], 0);
+   std::vector<long> pos(off.begin(), off.end() - 1);
+   walk([&](long lid, int w, int f, int x, int y, int z, bool diag) {
+      long e[12];
+      entries(x, y, z, diag, e);
+      const long base = ((lid * 4 + w) * (long)mflush + f) * 12;
+      for (int t = 0; t < 12; ++t)
+         if (e[t] >= 0) idx[pos[e[t]]++] = base + t;
+   });
+}
 
 int plan_setup(bsahip_plan *P, const bsahip_problem *p, int32_t nfh, double df, int32_t flags,
                int psd_mode)
@@ -450,13 +539,17 @@ int plan_setup(bsahip_plan *P, const bsahip_problem *p, int32_t nfh, double df, 
    P->nu = (flags & BSAHIP_SUM_EXACT) ? P->nf : P->nh;          // rows of A per mode
    P->nhp = round_up(P->nu, 16);
    P->rows = P->D.nm * P->nhp;
-   const int tile = (P->prec == BSAHIP_PREC_BF16X3) ? T2 : TILE;
+   const int tile = (P->prec == BSAHIP_PREC_FP64) ? TILE : T2;
    P->ntile = (P->rows + tile - 1) / tile;
    P->ldU = (long)round_up(P->rows, T2);
    P->nft = 2 * P->nu - 1;
    P->d = p->pulsation ? df * 2.0 * M_PI : df;
    const long npairs = (long)P->ntile * (P->ntile + 1) / 2;
-   if (npairs * P->D.nm > 0x7fffffffL) return fail(BSAHIP_EUNSUPPORTED, "grid too large for one launch");
+   // one workgroup of 256 threads per (tile pair, mode): the launch's global
+   // size must fit 32 bits
+   if (npairs * P->D.nm * 256L > 0xffffffffL)
+      return fail(BSAHIP_EUNSUPPORTED, "grid too large for one launch: (tile pairs x modes) x 256 "
+                                       "threads exceeds 2^32; use a coarser grid or fewer modes");
    std::vector<double> fl(P->nu);
    for (int u = 0; u < P->nu; ++u) fl[u] = (double)u * df;
    CHK(upload(P->B, "fl", fl, &P->fl));
@@ -465,11 +558,22 @@ int plan_setup(bsahip_plan *P, const bsahip_problem *p, int32_t nfh, double df, 
    const size_t nm3 = (size_t)P->D.nm * P->D.nm * P->D.nm, nm2 = (size_t)P->D.nm * P->D.nm;
    CHK(P->B.get("m3", 2 * nm3, &P->m3));
    CHK(P->B.get("m2", 3 * nm2, &P->m2));
-   if (P->prec == BSAHIP_PREC_BF16X3) {
+   if ((flags & BSAHIP_DETERMINISTIC) && !(flags & BSAHIP_NO_BISP)) {
+      std::vector<long> off, idx;
+      det_slot_lists(P, off, idx, P->mflush);
+      CHK(upload(P->B, "det_off", off, &P->det_off));
+      CHK(upload(P->B, "det_idx", idx, &P->det_idx));
+      CHK(P->B.get("det_slots", (size_t)npairs * P->D.nm * 4 * P->mflush * 12, &P->slots));
+   }
+   if ((flags & BSAHIP_DETERMINISTIC) && !(flags & BSAHIP_NO_PSD))
+      CHK(P->B.get("m2part", (size_t)((P->nh + 63) / 64) * 3 * nm2, &P->m2part));
+   if (P->prec != BSAHIP_PREC_FP64) {
       const size_t n = (size_t)P->ldU * P->D.K;
       CHK(P->B.get("UR32", n, &P->UR));
-      CHK(P->B.get("UCh", n, &P->Ch));
-      CHK(P->B.get("UCl", n, &P->Cl));
+      if (P->prec == BSAHIP_PREC_BF16X3) {
+         CHK(P->B.get("UCh", n, &P->Ch));
+         CHK(P->B.get("UCl", n, &P->Cl));
+      }
    }
    HIPC(hipStreamSynchronize(g.s));
    return 0;
@@ -485,7 +589,7 @@ int plan_prepare(bsahip_plan *P)
    CHK(launch_frf(D, P->nft, P->df, P->H));
    if (!(P->flags & BSAHIP_NO_BISP)) {
       CHK(fill(P->m3, 2 * nm3, 0.0));
-      if (P->prec == BSAHIP_PREC_BF16X3) {
+      if (P->prec != BSAHIP_PREC_FP64) {
          dim3 gs((unsigned)(P->ldU / 32), (unsigned)((D.K + 31) / 32));
          k_split_table<<<gs, 256, 0, g.s>>>(P->U, P->ldU, D.K, (int)P->ldU, D.K, P->UR, P->Ch, P->Cl);
          HIPC(hipGetLastError());
@@ -515,6 +619,8 @@ void plan_args(const bsahip_plan *P, GsymArgs &a, GsymArgs16 &b16)
    a.epi_plain = (P->flags & BSAHIP_EPI_GENERIC) ? 1 : 0;
    a.m3f = P->m3;
    a.m3r = P->m3 + nm3;
+   a.slots = P->slots;
+   a.mflush = P->mflush;
    b16.UR = P->UR;
    b16.UCh = reinterpret_cast<const uint4 *>(P->Ch);
    b16.UCl = reinterpret_cast<const uint4 *>(P->Cl);
@@ -535,9 +641,18 @@ int plan_gsym(bsahip_plan *P)
    const long nwg = plan_items(P);
    if (P->prec == BSAHIP_PREC_BF16X3)
       k_gsym_bf16x3<<<dim3((unsigned)nwg), 256, 0, g.s>>>(a, b16);
+   else if (P->prec == BSAHIP_PREC_FP32)
+      k_gsym_f32<<<dim3((unsigned)nwg), 256, 0, g.s>>>(a, b16);
    else
       k_gsym_f64<<<dim3((unsigned)nwg), 256, 0, g.s>>>(a);
    HIPC(hipGetLastError());
+   if (P->slots) {
+      // fixed order: every m3 entry sums its slots in ascending order
+      const long n = 2 * (long)P->D.nm * P->D.nm * P->D.nm;
+      k_sum_lists<<<(unsigned)((n + 255) / 256), 256, 0, g.s>>>(P->slots, P->det_off, P->det_idx, n,
+                                                                P->m3);
+      HIPC(hipGetLastError());
+   }
    return 0;
 }
 
@@ -546,10 +661,16 @@ int plan_m2(bsahip_plan *P)
    Prob &D = P->D;
    const size_t nm2 = (size_t)D.nm * D.nm;
    CHK(fill(P->m2, 3 * nm2, 0.0));
-   dim3 gm((unsigned)(D.nm * D.nm), (unsigned)((P->nh + 63) / 64));
+   const int nby = (P->nh + 63) / 64;
+   dim3 gm((unsigned)(D.nm * D.nm), (unsigned)nby);
    k_m2<<<gm, 256, 0, g.s>>>(P->U, P->ldU, D.pa_k, D.K, D.nm, P->nhp, P->nh, P->H, P->nft, P->d,
-                             2.0 * M_PI * P->df, P->m2, P->m2 + nm2, P->m2 + 2 * nm2);
+                             2.0 * M_PI * P->df, P->m2, P->m2 + nm2, P->m2 + 2 * nm2, P->m2part);
    HIPC(hipGetLastError());
+   if (P->m2part) {
+      const long n = 3 * (long)nm2;
+      k_sum_fixed<<<(unsigned)((n + 255) / 256), 256, 0, g.s>>>(P->m2part, nby, n, n, P->m2);
+      HIPC(hipGetLastError());
+   }
    return 0;
 }
 
@@ -610,7 +731,7 @@ int plan_exec(bsahip_plan *P, double *m2mf, double *m2mr, double *m2o2mr, double
    return 0;
 }
 
-Bufs &scratch()
+Bufs &oneshot_bufs()
 {
    if (!g.bufs) g.bufs = new Bufs();
    return *g.bufs;
@@ -755,7 +876,7 @@ int classic_diag(const bsahip_problem *p, int32_t nfh, double df, int32_t flags,
    if ((flags & BSAHIP_PREC_MASK) != BSAHIP_PREC_FP64)
       return fail(BSAHIP_EUNSUPPORTED, "the ONLY_DIAG model runs in FP64");
    if (nfh < 2 || !(df > 0.0)) return fail(BSAHIP_EINVAL, "need nfh >= 2 and df > 0");
-   Bufs &B = scratch();
+   Bufs &B = oneshot_bufs();
    Prob D;
    CHK(load_problem(p, 0, B, D));
    const int nh = nfh, nf = 2 * nh - 1, nft = 2 * nh - 1, nm = D.nm, K = D.K;
@@ -803,6 +924,8 @@ int bsahip__classic_moments(const bsahip_problem *p, int32_t nfh, double df, int
    CHK(ensure_init());
    if (flags & BSAHIP_ONLY_DIAG) {
       if (psd_mode != 0) return fail(BSAHIP_EUNSUPPORTED, "the ONLY_DIAG model has no BsaCL form");
+      if (flags & BSAHIP_DETERMINISTIC)
+         return fail(BSAHIP_EUNSUPPORTED, "the ONLY_DIAG model has no fixed-order mode");
       return classic_diag(p, nfh, df, flags, m2mf, m2mr, m2o2mr, m3mf, m3mr);
    }
    bsahip_plan P;
@@ -817,7 +940,7 @@ int bsahip__points_sum(const bsahip_problem *p, int32_t npts, const double *fi, 
    CHK(ensure_init());
    if (npts < 0 || (npts > 0 && (!fi || !fj || !w || !m3)))
       return fail(BSAHIP_EINVAL, "npts < 0 or NULL point/weight/output arrays");
-   Bufs &B = scratch();
+   Bufs &B = oneshot_bufs();
    Prob D;
    CHK(load_problem(p, psd_mode, B, D));
    const size_t nm3 = (size_t)D.nm * D.nm * D.nm;
@@ -862,7 +985,7 @@ int bsahip_classic_spectra(const bsahip_problem *p, int32_t nfh, double df, int3
    CHK(ensure_init());
    CHK(check_flags(flags, true, false));
    if (nfh < 2 || !(df > 0.0)) return fail(BSAHIP_EINVAL, "need nfh >= 2 and df > 0");
-   Bufs &B = scratch();
+   Bufs &B = oneshot_bufs();
    Prob D;
    CHK(load_problem(p, 0, B, D));
    const int nh = nfh, nf = 2 * nh - 1, c = nh - 1;
@@ -913,6 +1036,8 @@ int bsahip_classic_spectra(const bsahip_problem *p, int32_t nfh, double df, int3
       CHK(B.get("bfm_out", nm3 * npts, &ob));
       CHK(B.get("brm_out", nm3 * npts, &orr));
       const long tot = (long)nm3 * npts;
+      if (tot > 0xffffff00L)
+         return fail(BSAHIP_EUNSUPPORTED, "NF^2 * NM^3 exceeds 2^32 bispectrum values in one call");
       k_bfm_points<<<(unsigned)((tot + 255) / 256), 256, 0, g.s>>>(
          U, ldU, nu, D.pb_k, D.K, D.nm, (int)npts, di1, di2, di3, df1, df2, D.Mm, D.Km, D.Cm,
          ob, orr, 1, npts);
@@ -929,8 +1054,11 @@ int bsahip_classic_bispectra_sel(const bsahip_problem *p, int32_t nfh, double df
 {
    std::lock_guard<std::recursive_mutex> lk(g.mu);
    CHK(ensure_init());
-   CHK(check_flags(flags, true, false));
+   CHK(check_flags(flags, true, true));
+   const int prec = flags & BSAHIP_PREC_MASK;
    if (nfh < 2 || !(df > 0.0)) return fail(BSAHIP_EINVAL, "need nfh >= 2 and df > 0");
+   if (prec != BSAHIP_PREC_FP64 && (flags & BSAHIP_ONLY_DIAG))
+      return fail(BSAHIP_EUNSUPPORTED, "the ONLY_DIAG model runs in FP64");
    if (nsel < 0 || nsel > 65535 || (nsel > 0 && !sel))
       return fail(BSAHIP_EINVAL, "nsel outside 0..65535 or NULL selection");
    const bool diag = (flags & BSAHIP_ONLY_DIAG) != 0, exact = (flags & BSAHIP_SUM_EXACT) != 0;
@@ -938,7 +1066,7 @@ int bsahip_classic_bispectra_sel(const bsahip_problem *p, int32_t nfh, double df
       return fail(BSAHIP_EUNSUPPORTED, "the ONLY_DIAG model pads S(f1+f2) with zeros (no SUM_EXACT)");
    if (nsel == 0 || (!bfm && !brm)) return 0;
    if (nfh > 46341) return fail(BSAHIP_EINVAL, "nfh too large for one selected bispectrum");
-   Bufs &B = scratch();
+   Bufs &B = oneshot_bufs();
    Prob D;
    CHK(load_problem(p, 0, B, D));
    const long ncol = diag ? D.nm : (long)D.nm * D.nm * D.nm;
@@ -970,8 +1098,13 @@ int bsahip_classic_bispectra_sel(const bsahip_problem *p, int32_t nfh, double df
       CHK(upload(B, "fl", fl, &dfl));
       CHK(B.get("U", (size_t)D.K * ldU, &U));
       CHK(build_table(B, D, dfl, nu, nu, U, ldU));
-      k_bisp_grid_sel<<<gs, 256, 0, g.s>>>(U, ldU, nu, D.pb_k, D.K, D.nm, nf, exact ? 1 : 0, df,
-                                           dsel, D.Mm, D.Km, D.Cm, ob, orr);
+#define LAUNCH_SEL(PR)                                                                        \
+   k_bisp_grid_sel<PR><<<gs, 256, 0, g.s>>>(U, ldU, nu, D.pb_k, D.K, D.nm, nf, exact ? 1 : 0, df, \
+                                            dsel, D.Mm, D.Km, D.Cm, ob, orr)
+      if (prec == BSAHIP_PREC_FP32) LAUNCH_SEL(1);
+      else if (prec == BSAHIP_PREC_BF16X3) LAUNCH_SEL(2);
+      else LAUNCH_SEL(0);
+#undef LAUNCH_SEL
    }
    HIPC(hipGetLastError());
    if (bfm) HIPC(hipMemcpyAsync(bfm, ob, nout * sizeof(double), hipMemcpyDeviceToHost, g.s));
@@ -989,7 +1122,7 @@ int bsahip_bfm_points(const bsahip_problem *p, int32_t npts, const double *fi, c
    if (npts < 0 || (npts > 0 && (!fi || !fj || !bfm)))
       return fail(BSAHIP_EINVAL, "npts < 0 or NULL point/output arrays");
    if (npts == 0) return 0;
-   Bufs &B = scratch();
+   Bufs &B = oneshot_bufs();
    Prob D;
    CHK(load_problem(p, 0, B, D));
    CHK(set_pod(p, D));
@@ -1014,6 +1147,8 @@ int bsahip_bfm_points(const bsahip_problem *p, int32_t npts, const double *fi, c
    CHK(B.get("bfm_out", nm3 * npts, &ob));
    if (brm) CHK(B.get("brm_out", nm3 * npts, &orr));
    const long tot = (long)nm3 * npts;
+   if (tot > 0xffffff00L)
+      return fail(BSAHIP_EUNSUPPORTED, "npts * NM^3 exceeds 2^32 bispectrum values in one call");
    k_bfm_points<<<(unsigned)((tot + 255) / 256), 256, 0, g.s>>>(
       U, ldU, nu, D.pb_k, D.K, D.nm, npts, di1, di2, di3, df1, df2, D.Mm, D.Km, D.Cm, ob, orr,
       (long)nm3, 1);
@@ -1030,10 +1165,24 @@ int bsahip_selftest_mfma_layout(double *out512)
    CHK(ensure_init());
    if (!out512) return fail(BSAHIP_EINVAL, "NULL output");
    double *d;
-   CHK(scratch().get("selftest", 512, &d));
+   CHK(oneshot_bufs().get("selftest", 512, &d));
    k_mfma_f64_layout<<<1, 64, 0, g.s>>>(d);
    HIPC(hipGetLastError());
    HIPC(hipMemcpyAsync(out512, d, 512 * sizeof(double), hipMemcpyDeviceToHost, g.s));
+   HIPC(hipStreamSynchronize(g.s));
+   return 0;
+}
+
+int bsahip_selftest_mfma_f32_layout(float *out512)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   CHK(ensure_init());
+   if (!out512) return fail(BSAHIP_EINVAL, "NULL output");
+   float *d;
+   CHK(oneshot_bufs().get("selftest32", 512, &d));
+   k_mfma_f32_layout<<<1, 64, 0, g.s>>>(d);
+   HIPC(hipGetLastError());
+   HIPC(hipMemcpyAsync(out512, d, 512 * sizeof(float), hipMemcpyDeviceToHost, g.s));
    HIPC(hipStreamSynchronize(g.s));
    return 0;
 }
@@ -1044,7 +1193,7 @@ int bsahip_selftest_mfma_bf16_layout(float *out768)
    CHK(ensure_init());
    if (!out768) return fail(BSAHIP_EINVAL, "NULL output");
    float *d;
-   CHK(scratch().get("selftest16", 768, &d));
+   CHK(oneshot_bufs().get("selftest16", 768, &d));
    k_mfma_bf16_layout<<<1, 64, 0, g.s>>>(d);
    HIPC(hipGetLastError());
    HIPC(hipMemcpyAsync(out768, d, 768 * sizeof(float), hipMemcpyDeviceToHost, g.s));

@@ -123,8 +123,10 @@ __global__ void k_frf_table(const double *__restrict__ Mm, const double *__restr
 // projection A(y,i;f) = s_i(f) sum_j Psi_a(y,j) s_j(f) exp(|f| ln rho_ji) on
 // FP64 MFMA: per frequency an (MT*16 x nnl) x (nnl x 64) product whose right
 // operand, the coherence tile rho_ji^|f|, is generated in LDS chunk by chunk.
-// grid (ceil(nnl/64), nfl, ntc), block 256 = 4 waves; wave w owns the 16 node
-// columns 16w.. of the block's 64 and all MT row blocks of 16 modes.
+// grid (ceil(nnl/64), nfl, ntc * mode chunks), block 256 = 4 waves; a block
+// takes the modes y0 .. y0 + 16 MT - 1 of its chunk (y0 = chunk * 16 MT); wave
+// w owns the 16 node columns 16w.. of the block's 64 and all MT row blocks of
+// 16 modes.
 // v_mfma_f64_16x16x4f64: A lane l holds (row l%16, k l/16), B (k l/16, col l%16),
 // D register t of lane l holds (row l/16 + 4t, col l%16).
 // ---------------------------------------------------------------------------
@@ -137,12 +139,13 @@ __global__ __launch_bounds__(256) void k_project(
    const double *__restrict__ sq,      // [tc][nfl][nnl]  sqrt(S)
    const double *__restrict__ fabs_list, int nfl, int nnl, int nm, int nnlp,
    int row_stride,                     // rows per mode in U (nhp)
-   double *__restrict__ U, long ldU)
+   double *__restrict__ U, long ldU, int ntc)
 {
    constexpr int NR = MT * 16;
    __shared__ double ps[PKC][NR];
    __shared__ double ee[PKC][64];
-   const int tc = blockIdx.z;
+   const int tc = blockIdx.z % ntc;
+   const int y0 = (blockIdx.z / ntc) * NR;
    const int q = blockIdx.y;
    const int i0 = blockIdx.x * 64;
    const int tid = threadIdx.x, lane = tid & 63;
@@ -159,7 +162,7 @@ __global__ __launch_bounds__(256) void k_project(
       for (int t = tid; t < PKC * NR; t += 256) {
          const int jj = t / NR, y = t % NR;
          const int j = j0 + jj;
-         ps[jj][y] = (j < nnl && y < nm) ? pat[(size_t)y * nnl + j] * sqq[j] : 0.0;
+         ps[jj][y] = (j < nnl && y0 + y < nm) ? pat[(size_t)(y0 + y) * nnl + j] * sqq[j] : 0.0;
       }
       for (int t = tid; t < PKC * 64; t += 256) {
          const int jj = t >> 6, ii = t & 63;
@@ -190,7 +193,7 @@ __global__ __launch_bounds__(256) void k_project(
       for (int r = 0; r < MT; ++r)
 #pragma unroll
          for (int t = 0; t < 4; ++t) {
-            const int y = 16 * r + (lane >> 4) + 4 * t;
+            const int y = y0 + 16 * r + (lane >> 4) + 4 * t;
             if (y < nm) U[((size_t)tc * nnlp + i) * ldU + (size_t)y * row_stride + q] = acc[r][t] * si;
          }
    }
@@ -253,6 +256,10 @@ struct GsymArgs {
    int rule;
    int epi_plain;         // 1: every 16x16 block through the generic epilogue
    double *m3f, *m3r;     // [nm^3] accumulated (atomic)
+   double *slots;         // NULL, or fixed-order mode: each flush of a wave writes
+                          // its 12 m3 contributions to slots + ((item*4 + wave)
+                          // * mflush + flush) * 12 instead of the atomics
+   int mflush;            // flushes per wave at most (fixed-order mode)
 };
 
 // Quadrature weight / d^2 from edge flags of |F1| (p) and |F2| (q) on the
@@ -501,14 +508,39 @@ __device__ __forceinline__ void epi_accum_any(const GsymArgs &a, int x, int y, i
    }
 }
 
-// Wave-reduce the nine sums and add them (times d^2) to the m3 entries.
+// Wave-reduce the nine sums and add them (times d^2) to the m3 entries, or
+// in the fixed-order mode store them in the wave's slot `f` (the order of the
+// twelve values is the one of the atomics below; bsahip_api.hip builds the
+// matching slot lists, det_slot_lists).
 __device__ __forceinline__ void epi_flush(const GsymArgs &a, int x, int y, int z, bool diag,
-                                          int lane, double (&s)[9])
+                                          int lane, double (&s)[9], long sbase, int f)
 {
    const int nm = a.nm;
    const size_t nm2 = (size_t)nm * nm;
 #pragma unroll
    for (int t = 0; t < 9; ++t) s[t] = wave_sum(s[t]) * a.d2;
+   if (a.slots) {
+      if (lane == 0) {
+         double *d = a.slots + sbase + (long)f * 12;
+         d[0] = s[0];
+         d[1] = s[1];
+         d[2] = s[2];
+         d[3] = s[3];
+         d[4] = s[2];
+         d[5] = s[4];
+         if (!diag) {
+            d[6] = s[0];
+            d[7] = s[5];
+            d[8] = s[6];
+            d[9] = s[7];
+            d[10] = s[6];
+            d[11] = s[8];
+         }
+      }
+#pragma unroll
+      for (int t = 0; t < 9; ++t) s[t] = 0.0;
+      return;
+   }
    if (lane == 0) {
       const size_t i_xyz = x + (size_t)nm * y + nm2 * z;
       const size_t i_yxz = y + (size_t)nm * x + nm2 * z;
@@ -539,9 +571,10 @@ __device__ __forceinline__ void epi_flush(const GsymArgs &a, int x, int y, int z
 // accumulation / reduction when all blocks share (y,z), else per block.
 template <int BR, int BC, class GetVal>
 __device__ __forceinline__ void epi_wave(const GsymArgs &a, int x, bool diag, int rw0, int cw0,
-                                         int lane, bool interleaved, GetVal getv)
+                                         int lane, bool interleaved, long sbase, GetVal getv)
 {
    const int nm = a.nm;
+   int f = 0;                                         // flushes of this wave so far
    const int y0 = rw0 / a.nhp, y1 = (rw0 + 16 * BR - 1) / a.nhp;
    const int z0 = cw0 / a.nhp, z1 = (cw0 + 16 * BC - 1) / a.nhp;
    double s[9];
@@ -557,7 +590,7 @@ __device__ __forceinline__ void epi_wave(const GsymArgs &a, int x, bool diag, in
          epi_accum_any(a, x, y0, z0, diag, rw0 + 16 * i - y0 * a.nhp, cw0 + 16 * j - z0 * a.nhp,
                        lane, g, interleaved, s);
       }
-      epi_flush(a, x, y0, z0, diag, lane, s);
+      epi_flush(a, x, y0, z0, diag, lane, s, sbase, f++);
    } else if constexpr (BR * BC > 4) {
       // the wave's rows (cols) span modes y0..y1 (z0..z1), each 16-block in one
       // of them (nhp is a multiple of 16): one reduction per (y, z) pair
@@ -576,7 +609,7 @@ __device__ __forceinline__ void epi_wave(const GsymArgs &a, int x, bool diag, in
                epi_accum_any(a, x, y, z, diag, r0 - y * a.nhp, q0 - z * a.nhp, lane, g, interleaved,
                              s);
             }
-            epi_flush(a, x, y, z, diag, lane, s);
+            epi_flush(a, x, y, z, diag, lane, s, sbase, f++);
          }
    } else {
       // few blocks per wave (and fewer registers to spare): one reduction per block
@@ -589,7 +622,7 @@ __device__ __forceinline__ void epi_wave(const GsymArgs &a, int x, bool diag, in
          double g[4];
          getv(b, g);
          epi_accum_any(a, x, y, z, diag, r0 - y * a.nhp, q0 - z * a.nhp, lane, g, interleaved, s);
-         epi_flush(a, x, y, z, diag, lane, s);
+         epi_flush(a, x, y, z, diag, lane, s, sbase, f++);
       }
    }
 }
@@ -679,7 +712,8 @@ __device__ __forceinline__ void gsym_f64_item(const GsymArgs &a, int x, long tp)
 
    // ---- epilogue: for v_mfma_f64_16x16x4f64, register t of lane l holds
    // D[(l/16) + 4t][l%16] of its 16x16 block (rows interleaved by 4).
-   epi_wave<2, 2>(a, x, diag, R * TILE + wr * 32, Q * TILE + wc * 32, lane, true,
+   const long sbase = ((tp * a.nm + x) * 4 + w) * (long)a.mflush * 12;
+   epi_wave<2, 2>(a, x, diag, R * TILE + wr * 32, Q * TILE + wc * 32, lane, true, sbase,
                   [&](int b, double (&g)[4]) {
                      d4 v;
                      switch (b) {       // uniform branch: constant register indices
@@ -855,7 +889,8 @@ __device__ __forceinline__ void gsym_bf16x3_item(const GsymArgs &a, const GsymAr
    }
 
    // epilogue: f32-output 16x16 MFMAs hold D[4(l/16)+t][l%16]
-   epi_wave<4, 4>(a, x, diag, R * T2 + wr * 64, Q * T2 + wc * 64, lane, false,
+   const long sbase = ((tp * a.nm + x) * 4 + w) * (long)a.mflush * 12;
+   epi_wave<4, 4>(a, x, diag, R * T2 + wr * 64, Q * T2 + wc * 64, lane, false, sbase,
                   [&](int b, double (&g)[4]) {
                      f4v v;
                      switch (b) {       // uniform branch: constant register indices
@@ -876,8 +911,140 @@ __global__ __launch_bounds__(256, 2) void k_gsym_bf16x3(GsymArgs a, GsymArgs16 b
    gsym_bf16x3_item(a, b, (int)(lid % a.nm), lid / a.nm);
 }
 
+// ---------------------------------------------------------------------------
+// G contraction in FP32 on v_mfma_f32_16x16x4f32 (FP32 operands, FP32
+// accumulate); same plane epilogue (FP64) as k_gsym_f64.  Workgroup tile
+// 128x128, 4 waves of 64x64 (4x4 blocks of 16x16), K chunk 32 double-buffered
+// in LDS (two FP32 planes, 64 KB).  Within a chunk, MFMA step s (0..7) takes
+// k = 8 (l/16) + s for lane l in both operands (any assignment of the chunk's
+// 32 k to (step, lane group) gives the same sum), so a lane reads its 8 k of
+// one row as two 16-byte LDS loads.  LDS rows hold eight 16-byte k-groups
+// stored at slot q ^ ((row >> 1) & 7): the 16 rows of a fragment read hit 16
+// distinct (row parity, slot) pairs.
+// ---------------------------------------------------------------------------
+#define KC4 32
+
+__device__ __forceinline__ int swz8(int row, int q) { return row * 8 + (q ^ ((row >> 1) & 7)); }
+
+__device__ __forceinline__ void gsym_f32_item(const GsymArgs &a, const GsymArgs16 &b, int x, long tp)
+{
+   __shared__ float4 As[2][T2 * 8], Bs[2][T2 * 8];
+   int R, Q;
+   pair_from_index(tp, R, Q);
+   const bool diag = (R == Q);
+
+   const int tid = threadIdx.x;
+   const int lane = tid & 63;
+   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform
+   const int wr = w >> 1, wc = w & 1;
+
+   // staging: thread -> (row = tid/2, k-groups 4h..4h+3 of 4 floats), h = tid%2
+   const int srow = tid >> 1, sh = tid & 1;
+   const long ldk = b.ldk;
+   const float *gA = b.UR + ((size_t)R * T2 + srow) * ldk + sh * 16;
+   const float *gB = b.UR + ((size_t)Q * T2 + srow) * ldk + sh * 16;
+   const float *pbx = b.pb32 + (size_t)x * ldk + sh * 16;
+
+   f4v acc[4][4];
+#pragma unroll
+   for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = (f4v){0.f, 0.f, 0.f, 0.f};
+
+   float4 ra[4], rb[4], sc[4];
+   auto gload = [&](int k0) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+         ra[j] = *(const float4 *)(gA + k0 + 4 * j);
+         rb[j] = *(const float4 *)(gB + k0 + 4 * j);
+         sc[j] = *(const float4 *)(pbx + k0 + 4 * j);
+      }
+   };
+   auto lstore = [&](int buf) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+         const float4 v = ra[j], s = sc[j];
+         As[buf][swz8(srow, 4 * sh + j)] = make_float4(v.x * s.x, v.y * s.y, v.z * s.z, v.w * s.w);
+         Bs[buf][swz8(srow, 4 * sh + j)] = rb[j];
+      }
+   };
+
+   const int nchunk = a.K / KC4;
+   gload(0);
+   lstore(0);
+   __syncthreads();
+   const int g = lane >> 4;            // lane group: k = 8g .. 8g+7 of the chunk
+   for (int c = 0; c < nchunk; ++c) {
+      const int buf = c & 1;
+      if (c + 1 < nchunk) gload((c + 1) * KC4);
+#pragma unroll
+      for (int hsel = 0; hsel < 2; ++hsel) {       // steps 4 hsel .. 4 hsel + 3
+         float4 fa[4], fb[4];
+#pragma unroll
+         for (int i = 0; i < 4; ++i) {
+            const int row = wr * 64 + i * 16 + (lane & 15);
+            const int col = wc * 64 + i * 16 + (lane & 15);
+            fa[i] = As[buf][swz8(row, 2 * g + hsel)];
+            fb[i] = Bs[buf][swz8(col, 2 * g + hsel)];
+         }
+#pragma unroll
+         for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+               for (int j = 0; j < 4; ++j)
+                  acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                     ((const float *)&fa[i])[s], ((const float *)&fb[j])[s], acc[i][j], 0, 0, 0);
+      }
+      if (c + 1 < nchunk) lstore(buf ^ 1);
+      __syncthreads();
+   }
+
+   // epilogue: f32-output 16x16 MFMAs hold D[4(l/16)+t][l%16]
+   const long sbase = ((tp * a.nm + x) * 4 + w) * (long)a.mflush * 12;
+   epi_wave<4, 4>(a, x, diag, R * T2 + wr * 64, Q * T2 + wc * 64, lane, false, sbase,
+                  [&](int bb, double (&gv)[4]) {
+                     f4v v;
+                     switch (bb) {      // uniform branch: constant register indices
+#define CASE_(k) case k: v = acc[k / 4][k % 4]; break;
+                     CASE_(0) CASE_(1) CASE_(2) CASE_(3) CASE_(4) CASE_(5) CASE_(6) CASE_(7)
+                     CASE_(8) CASE_(9) CASE_(10) CASE_(11) CASE_(12) CASE_(13) CASE_(14)
+#undef CASE_
+                     default: v = acc[3][3]; break;
+                     }
+#pragma unroll
+                     for (int t = 0; t < 4; ++t) gv[t] = (double)v[t];
+                  });
+}
+
+__global__ __launch_bounds__(256, 2) void k_gsym_f32(GsymArgs a, GsymArgs16 b)
+{
+   const long lid = xcd_lid();
+   gsym_f32_item(a, b, (int)(lid % a.nm), lid / a.nm);
+}
+
+// Lane-layout self test of v_mfma_f32_16x16x4f32 (one wave, 512 floats):
+// out[t*64 + l] = D[4(l/16)+t][l%16] of A[i][k] = i + 16 k, B = identity
+// columns k == j % 4 (so D[i][j] = i + 16 (j % 4)); out[256 + t*64 + l] of
+// A[i][k] = 1, B[k][j] = k + 4 j (D[i][j] = 6 + 16 j).
+__global__ void k_mfma_f32_layout(float *__restrict__ out)
+{
+   const int l = threadIdx.x;
+   const int i = l & 15, k = l >> 4;
+   const f4v z = (f4v){0.f, 0.f, 0.f, 0.f};
+   const f4v d1 = __builtin_amdgcn_mfma_f32_16x16x4f32((float)(i + 16 * k),
+                                                       (k == (i & 3)) ? 1.f : 0.f, z, 0, 0, 0);
+   const f4v d2 = __builtin_amdgcn_mfma_f32_16x16x4f32(1.f, (float)(k + 4 * i), z, 0, 0, 0);
+#pragma unroll
+   for (int t = 0; t < 4; ++t) {
+      out[t * 64 + l] = d1[t];
+      out[256 + t * 64 + l] = d2[t];
+   }
+}
+
 // A table [K][ldU] (FP64) -> row operand [rows][ldk] FP32 and column operand
-// [rows][ldk] split into bf16 hi/lo.  32x32 tiles transposed through LDS.
+// [rows][ldk] split into bf16 hi/lo (Ch, Cl NULL: the FP32 copy only).  32x32
+// tiles transposed through LDS.
 __global__ void k_split_table(const double *__restrict__ U, long ldU, int K, int rows, long ldk,
                               float *__restrict__ UR, uint16_t *__restrict__ Ch,
                               uint16_t *__restrict__ Cl)
@@ -894,12 +1061,14 @@ __global__ void k_split_table(const double *__restrict__ U, long ldU, int K, int
       const int r = r0 + j, k = k0 + tx;
       if (r < rows && k < ldk) {
          const float v = tile[tx][j];
-         const __bf16 h = (__bf16)v;
-         const __bf16 l = (__bf16)(v - (float)h);
          const size_t o = (size_t)r * ldk + k;
          UR[o] = v;
-         Ch[o] = __builtin_bit_cast(uint16_t, h);
-         Cl[o] = __builtin_bit_cast(uint16_t, l);
+         if (Ch) {                 // bf16 planes (split-bf16x3 only)
+            const __bf16 h = (__bf16)v;
+            const __bf16 l = (__bf16)(v - (float)h);
+            Ch[o] = __builtin_bit_cast(uint16_t, h);
+            Cl[o] = __builtin_bit_cast(uint16_t, l);
+         }
       }
    }
 }
@@ -946,19 +1115,20 @@ __global__ void k_mfma_bf16_layout(float *__restrict__ out)
 
 // ---------------------------------------------------------------------------
 // modal PSD and second moments (functionsImpl.F90:1546-1558, 1686-1689;
-// BsaClassicImpl.F90:197-210, 567-589).  One block per (n,p); deterministic.
+// BsaClassicImpl.F90:197-210, 567-589).
 // psd(n,p;u) = sum_k U[k][n*nhp+u] Psi_a(p,k); the grid holds +-u.
 // ---------------------------------------------------------------------------
 // Modal PSD moments: PSD(n,p;u) = sum_k A(n,k;u) Psi_a(p,k) on the half grid,
 // weighted with the vect rule (+u and -u), times Re[H_p conj(H_n)] and w^2.
 // grid (nm*nm, ceil(nh/64)), block 256 = 64 frequencies x 4 slices of K;
-// partial sums are added atomically (the outputs start at zero).
+// partial sums are added atomically (the outputs start at zero), or with
+// m2part stored per block for a fixed-order sum (k_sum_fixed).
 __global__ __launch_bounds__(256) void k_m2(const double *__restrict__ U, long ldU,
                                             const double *__restrict__ pa, int K, int nm,
                                             int nhp, int nh, const double2 *__restrict__ H,
                                             int nft, double d, double om_step,
                                             double *__restrict__ m2f, double *__restrict__ m2r,
-                                            double *__restrict__ m2o)
+                                            double *__restrict__ m2o, double *__restrict__ m2part)
 {
    __shared__ double part[4][64];
    __shared__ double red[3][64];
@@ -991,11 +1161,49 @@ __global__ __launch_bounds__(256) void k_m2(const double *__restrict__ U, long l
       red[2][tu] = wave_sum(so);
       if (tu == 0) {
          const int id = n + nm * p;
-         if (m2f) atomicAdd(m2f + id, red[0][0]);
-         if (m2r) atomicAdd(m2r + id, red[1][0]);
-         if (m2o) atomicAdd(m2o + id, red[2][0]);
+         if (m2part) {                 // fixed order: per-block partials [by][3][nm^2]
+            const size_t nm2 = (size_t)nm * nm, b0 = (size_t)blockIdx.y * 3 * nm2;
+            m2part[b0 + id] = red[0][0];
+            m2part[b0 + nm2 + id] = red[1][0];
+            m2part[b0 + 2 * nm2 + id] = red[2][0];
+         } else {
+            if (m2f) atomicAdd(m2f + id, red[0][0]);
+            if (m2r) atomicAdd(m2r + id, red[1][0]);
+            if (m2o) atomicAdd(m2o + id, red[2][0]);
+         }
       }
    }
+}
+
+// Fixed-order compensated sums.  k_sum_lists: out[e] = sum over the slot list
+// idx[off[e] .. off[e+1]) of vals (ascending slot order), Kahan-compensated;
+// k_sum_fixed: out[i] = sum_b part[b * stride + i] for b = 0 .. nb-1, likewise.
+__device__ __forceinline__ void kahan_add(double &s, double &c, double v)
+{
+   const double y = v - c;
+   const double t = s + y;
+   c = (t - s) - y;
+   s = t;
+}
+
+__global__ void k_sum_lists(const double *__restrict__ vals, const long *__restrict__ off,
+                            const long *__restrict__ idx, long n, double *__restrict__ out)
+{
+   const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (e >= n) return;
+   double s = 0.0, c = 0.0;
+   for (long q = off[e]; q < off[e + 1]; ++q) kahan_add(s, c, vals[idx[q]]);
+   out[e] = s;
+}
+
+__global__ void k_sum_fixed(const double *__restrict__ part, int nb, long stride, long n,
+                            double *__restrict__ out)
+{
+   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (i >= n) return;
+   double s = 0.0, c = 0.0;
+   for (int b = 0; b < nb; ++b) kahan_add(s, c, part[(size_t)b * stride + i]);
+   out[i] = s;
 }
 
 // Modal PSD on grid indices (for the spectra output): psd(q, n+p*nm).
@@ -1033,6 +1241,57 @@ __device__ __forceinline__ double bfm_point(const double *__restrict__ U, long l
          v += pb[(size_t)n * K + k] * Am3 * Ap2 + pb[(size_t)p * K + k] * Am3 * An1;
       }
       s += v;
+   }
+   return 2.0 * s;
+}
+
+// One G value G[x](y,u; z,v) = sum_k Psi_b(x,k) A(y,k;u) A(z,k;v) with the
+// operand rounding of the mixed-precision G kernels: the operand of the lower
+// table row (the kernels' row operand) is fl32(A) fl32(Psi_b) in FP32, the
+// other fl32(A); PREC 1 (FP32) accumulates their FP32 product, PREC 2
+// (split-bf16x3) the three bf16 products lo*hi + hi*lo + hi*hi, all in FP32
+// and in K order (the MFMA path sums in another order; its error per term is
+// the same).
+template <int PREC>
+__device__ __forceinline__ float g_value_mixed(const double *__restrict__ U, long ldU, int rows,
+                                               const double *__restrict__ pb, int K, int x, int y,
+                                               int u, int z, int v)
+{
+   if ((long)z * rows + v < (long)y * rows + u) {
+      const int ty = y, tu = u;
+      y = z;
+      u = v;
+      z = ty;
+      v = tu;
+   }
+   float acc = 0.f;
+   for (int k = 0; k < K; ++k) {
+      const double *row = U + (size_t)k * ldU;
+      const float a = (float)row[(size_t)y * rows + u] * (float)pb[(size_t)x * K + k];
+      const float b = (float)row[(size_t)z * rows + v];
+      if constexpr (PREC == 1) {
+         acc = fmaf(a, b, acc);
+      } else {
+         const float ah = (float)(__bf16)a, bh = (float)(__bf16)b;
+         const float al = (float)(__bf16)(a - ah), bl = (float)(__bf16)(b - bh);
+         acc = fmaf(al, bh, acc);
+         acc = fmaf(ah, bl, acc);
+         acc = fmaf(ah, bh, acc);
+      }
+   }
+   return acc;
+}
+
+// bfm_point with the G values of the mixed-precision kernels (g_value_mixed).
+template <int PREC>
+__device__ __forceinline__ double bfm_point_mixed(const double *__restrict__ U, long ldU, int rows,
+                                                  const double *__restrict__ pb, int K, int m,
+                                                  int n, int p, int a1, int a2, int a3)
+{
+   double s = (double)g_value_mixed<PREC>(U, ldU, rows, pb, K, m, n, a1, p, a2);
+   if (a3 >= 0) {
+      s += (double)g_value_mixed<PREC>(U, ldU, rows, pb, K, n, m, a3, p, a2);
+      s += (double)g_value_mixed<PREC>(U, ldU, rows, pb, K, p, m, a3, n, a1);
    }
    return 2.0 * s;
 }
@@ -1085,6 +1344,9 @@ __global__ __launch_bounds__(256) void k_bfm_points(
 // Rows of U: |F| for F within the grid; with exact, |f1+f2| up to 2c as well,
 // otherwise S(f1+f2) = 0 beyond the grid.  grid (ceil(nf*nf/256), nsel)
 // ---------------------------------------------------------------------------
+// PREC 0: FP64 (bfm_point); 1, 2: the G values of the FP32 / split-bf16x3
+// kernels (bfm_point_mixed).
+template <int PREC>
 __global__ __launch_bounds__(256) void k_bisp_grid_sel(
    const double *__restrict__ U, long ldU, int rows, const double *__restrict__ pb, int K,
    int nm, int nf, int exact, double df, const int *__restrict__ sel,
@@ -1099,7 +1361,9 @@ __global__ __launch_bounds__(256) void k_bisp_grid_sel(
    const int m = mnp % nm, n = (mnp / nm) % nm, p = mnp / (nm * nm);
    const int F1 = (int)(pt % nf) - c, F2 = (int)(pt / nf) - c, F3 = F1 + F2;
    const int a3 = (exact || abs(F3) <= c) ? abs(F3) : -1;
-   const double s = bfm_point(U, ldU, rows, pb, K, m, n, p, abs(F1), abs(F2), a3);
+   const double s = PREC == 0 ? bfm_point(U, ldU, rows, pb, K, m, n, p, abs(F1), abs(F2), a3)
+                              : bfm_point_mixed<PREC>(U, ldU, rows, pb, K, m, n, p, abs(F1),
+                                                      abs(F2), a3);
    const size_t o = (size_t)blockIdx.y * npts + pt;
    if (bfm) bfm[o] = s;
    if (brm) brm[o] = s * brm_factor(F1 * df, F2 * df, m, n, p, Mm, Km, Cm);

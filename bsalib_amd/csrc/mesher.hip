@@ -467,6 +467,17 @@ int msh_full_axis(const bsahip_mesher_cfg *c, int nm, std::vector<MshAxisCell> &
 
 constexpr int MSH_MAX_SUB = 256;   // BRM cells per level-0 cell, at most
 
+int msh_check_cfg(const bsahip_mesher_cfg *cfg)
+{
+   if (!(cfg->bkg_peakw > 0.0) || !(cfg->bkg_area_ext > 0.0) || !(cfg->peak_area_ext > 0.0) ||
+       !(cfg->max_area_ext > 0.0) || cfg->tile_cells < 1 || cfg->tile_cells > 128)
+      return fail(BSAHIP_EINVAL, "Mesher extensions and bkg_peakw must be > 0, tile_cells in 1..128");
+   const int c = cfg->brm_cells;
+   if (c < 0 || c > MSH_MAX_SUB || (c & (c - 1)) != 0)
+      return fail(BSAHIP_EINVAL, "brm_cells must be 0 or a power of two up to 256");
+   return 0;
+}
+
 struct MshTile {
    double a1, b1, a2, b2;   // f1 in [a1,b1], f2 in [a2,b2]
    int n1, n2;              // level-0 cells
@@ -492,7 +503,7 @@ int msh_rule_level(double h0, double w, int pol)
 // the reference's diagonal crest, BSA_PREMESH_TYPE_DIAG_CREST_YES).  w_n is the
 // peak width 2 xsi_n f_n peak_area_ext: 16 BRM cells per peak width, fewer as
 // the FRF tail flattens with distance.
-void msh_brm_sub(MshTile &t, const bsahip_mesher_cfg *c, int nm)
+void msh_brm_sub(MshTile &t, const bsahip_mesher_cfg *c, int nm, double dfref)
 {
    auto dist = [](double a, double b, double f) { return f < a ? a - f : (f > b ? f - b : 0.0); };
    double r1 = 1e300, r2 = 1e300;
@@ -510,6 +521,15 @@ void msh_brm_sub(MshTile &t, const bsahip_mesher_cfg *c, int nm)
       }
    }
    const double h1 = (t.b1 - t.a1) / t.n1, h2 = (t.b2 - t.a2) / t.n2;
+   if (c->brm_cells > 0) {
+      // uniform BRM lattice: spacing df_ref / brm_cells on every tile
+      const double hb = dfref / c->brm_cells * (1.0 + 1e-9);
+      t.sub1 = 1;
+      while (h1 / t.sub1 > hb && t.sub1 < MSH_MAX_SUB) t.sub1 *= 2;
+      t.sub2 = 1;
+      while (h2 / t.sub2 > hb && t.sub2 < MSH_MAX_SUB) t.sub2 *= 2;
+      return;
+   }
    t.sub1 = 8;
    while (h1 / t.sub1 > r1 * (1.0 + 1e-9) && t.sub1 < MSH_MAX_SUB) t.sub1 *= 2;
    t.sub2 = 8;
@@ -517,7 +537,7 @@ void msh_brm_sub(MshTile &t, const bsahip_mesher_cfg *c, int nm)
 }
 
 void msh_tiles(const std::vector<MshAxisCell> &ax, const bsahip_mesher_cfg *cfg, int nm,
-               std::vector<MshTile> &tiles)
+               double dfref, std::vector<MshTile> &tiles)
 {
    const int tile_cells = cfg->tile_cells;
    struct Seg { double a, b; int n, pol, sub; double w; int g; };
@@ -545,7 +565,7 @@ void msh_tiles(const std::vector<MshAxisCell> &ax, const bsahip_mesher_cfg *cfg,
          t.pol1 = s1.pol; t.pol2 = s2.pol; t.sub1 = s1.sub; t.sub2 = s2.sub;
          t.rule = std::max(msh_rule_level((s1.b - s1.a) / s1.n, s1.w, s1.pol),
                            msh_rule_level((s2.b - s2.a) / s2.n, s2.w, s2.pol));
-         msh_brm_sub(t, cfg, nm);
+         msh_brm_sub(t, cfg, nm, dfref);
          tiles.push_back(t);
       }
 }
@@ -835,9 +855,7 @@ int msh_setup(const bsahip_problem *p, const bsahip_mesher_cfg *cfg, int32_t fla
    CHK(ensure_init());
    if ((flags & BSAHIP_PREC_MASK) != BSAHIP_PREC_FP64)
       return fail(BSAHIP_EUNSUPPORTED, "the Mesher runs in FP64");
-   if (!(cfg->bkg_peakw > 0.0) || !(cfg->bkg_area_ext > 0.0) || !(cfg->peak_area_ext > 0.0) ||
-       !(cfg->max_area_ext > 0.0) || cfg->tile_cells < 1 || cfg->tile_cells > 128)
-      return fail(BSAHIP_EINVAL, "Mesher extensions and bkg_peakw must be > 0, tile_cells in 1..128");
+   CHK(msh_check_cfg(cfg));
    M.reset(new bsahip_mesher());
    CHK(load_problem(p, 0, M->B, M->D));
    // the ONLY_DIAG model has no coherence between nodes and no POD
@@ -851,7 +869,7 @@ int msh_setup(const bsahip_problem *p, const bsahip_mesher_cfg *cfg, int32_t fla
    M->nmp = D.nm <= 16 ? 16 : 32;
    std::vector<MshAxisCell> ax;
    CHK(msh_full_axis(cfg, D.nm, ax, M->dfref));
-   msh_tiles(ax, cfg, D.nm, M->tiles);
+   msh_tiles(ax, cfg, D.nm, M->dfref, M->tiles);
    CHK(M->B.get("m_PB", (size_t)D.K * M->nmp, &M->PB));
    k_pbt<<<(unsigned)((D.K * M->nmp + 255) / 256), 256, 0, g.s>>>(D.pb_k, D.nm, D.K, M->nmp, M->PB);
    HIPC(hipGetLastError());
@@ -917,14 +935,12 @@ int bsahip_mesher_layout(const bsahip_mesher_cfg *cfg, int32_t nm, int32_t max_t
 {
    if (!cfg || !cfg->natf || !cfg->xsi || !ntiles || nm < 1)
       return fail(BSAHIP_EINVAL, "NULL Mesher argument or nm < 1");
-   if (!(cfg->bkg_peakw > 0.0) || !(cfg->bkg_area_ext > 0.0) || !(cfg->peak_area_ext > 0.0) ||
-       !(cfg->max_area_ext > 0.0) || cfg->tile_cells < 1 || cfg->tile_cells > 128)
-      return fail(BSAHIP_EINVAL, "Mesher extensions and bkg_peakw must be > 0, tile_cells in 1..128");
+   CHK(msh_check_cfg(cfg));
    std::vector<MshAxisCell> ax;
    double dfref;
    CHK(msh_full_axis(cfg, nm, ax, dfref));
    std::vector<MshTile> tiles;
-   msh_tiles(ax, cfg, nm, tiles);
+   msh_tiles(ax, cfg, nm, dfref, tiles);
    *ntiles = (int32_t)tiles.size();
    if (!geom && !rule_level) return 0;
    if ((int32_t)tiles.size() > max_tiles) return fail(BSAHIP_EINVAL, "max_tiles too small");

@@ -86,7 +86,7 @@ int bsahip_modal_psd_points(const bsahip_problem *p, int32_t npts, const double 
    if (npts < 0 || (npts > 0 && (!f || !psd))) return fail(BSAHIP_EINVAL, "npts < 0 or NULL arrays");
    if (npts == 0) return 0;
    if (npts > 65535) return fail(BSAHIP_EUNSUPPORTED, "more than 65535 frequencies in one call");
-   Bufs &B = scratch();
+   Bufs &B = oneshot_bufs();
    Prob D;
    CHK(load_problem(p, 0, B, D));
    std::vector<double> fl(npts);
@@ -117,7 +117,7 @@ int bsahip_modal_recombination(int32_t ndofs, int32_t nm, const double *phi, con
    if (ndofs == 0) return 0;
    if (!phi || !m2 || !m3 || !m2_r || !m3_r || (m2o2_r && !m2o2))
       return fail(BSAHIP_EINVAL, "NULL recombination array");
-   Bufs &B = scratch();
+   Bufs &B = oneshot_bufs();
    const size_t nm2 = (size_t)nm * nm, nm3 = nm2 * nm;
    double *dphi, *dm2, *dm3, *dm22 = nullptr, *out;
    CHK(B.get("rc_phi", (size_t)ndofs * nm, &dphi));

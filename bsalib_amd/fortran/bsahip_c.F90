@@ -8,6 +8,7 @@ module BsaHip_C
    private
 
    integer(c_int32_t), parameter, public :: BSAHIP_PREC_FP64 = 0
+   integer(c_int32_t), parameter, public :: BSAHIP_PREC_FP32 = 1
    integer(c_int32_t), parameter, public :: BSAHIP_PREC_BF16X3 = 2
    integer(c_int32_t), parameter, public :: BSAHIP_QUAD_VECT = 0
    integer(c_int32_t), parameter, public :: BSAHIP_QUAD_RIEMANN = 16
@@ -29,7 +30,7 @@ module BsaHip_C
    end type bsahip_problem_t
 
    type, bind(c), public :: bsahip_mesher_cfg_t
-      integer(c_int32_t) :: bkg_base_rfmnt = 20, tile_cells = 32, diag_crest = 0, reserved = 0
+      integer(c_int32_t) :: bkg_base_rfmnt = 20, tile_cells = 32, diag_crest = 0, brm_cells = 0
       real(c_double) :: bkg_peakw = 0, bkg_area_ext = 2, peak_area_ext = 2, max_area_ext = 2
       type(c_ptr) :: natf = c_null_ptr, xsi = c_null_ptr
    end type bsahip_mesher_cfg_t

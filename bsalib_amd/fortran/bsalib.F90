@@ -29,8 +29,8 @@
 !!    scans its ascending eigenvalues from the small end and so keeps a single
 !!    mode unless the second-smallest eigenvalue passes the threshold.
 !!  * bsa_setComputePrecision is an addition: BSA_PREC_BF16X3 selects the
-!!    split-bf16x3 node contraction (FP32 accumulation on matrix cores,
-!!    FP64 everywhere else); the default is FP64.
+!!    split-bf16x3 node contraction and BSA_PREC_FP32 the FP32 one (FP32
+!!    accumulation on matrix cores, FP64 everywhere else); the default is FP64.
 !!  * Bispectrum export files: visual mode writes one entry (or
 !!    the nodal bispectrum of one DOF) at the Mesher's device tiles without an
 !!    earlier run's BFM dump file, which the reference's visual mode re-reads.
@@ -108,6 +108,7 @@ module BsaLib
    integer(bsa_int_t), parameter, public :: BSA_WIND_PSD_KAIMAL = 2
    integer(bsa_int_t), parameter, public :: BSA_WIND_PSD_DAVENPORT = 5
    integer(bsa_int_t), parameter, public :: BSA_PREC_FP64 = 0
+   integer(bsa_int_t), parameter, public :: BSA_PREC_FP32 = 1
    integer(bsa_int_t), parameter, public :: BSA_PREC_BF16X3 = 2
 
    integer(bsa_int_t), parameter, public :: BSA_EXPORT_FORMAT_FORMATTED = 0
@@ -315,11 +316,11 @@ contains
    end subroutine
 
    !> Extension (no counterpart in the reference): node-contraction precision,
-   !> BSA_PREC_FP64 (default) or BSA_PREC_BF16X3.
+   !> BSA_PREC_FP64 (default), BSA_PREC_FP32 or BSA_PREC_BF16X3.
    subroutine bsa_setComputePrecision(iprec)
       integer(bsa_int_t), value :: iprec
-      if (iprec /= BSA_PREC_FP64 .and. iprec /= BSA_PREC_BF16X3) &
-         call bsa_Abort('Unknown compute precision (use BSA_PREC_FP64 or BSA_PREC_BF16X3).')
+      if (iprec /= BSA_PREC_FP64 .and. iprec /= BSA_PREC_FP32 .and. iprec /= BSA_PREC_BF16X3) &
+         call bsa_Abort('Unknown compute precision (use BSA_PREC_FP64, BSA_PREC_FP32 or BSA_PREC_BF16X3).')
       iprec_ = iprec
    end subroutine
 

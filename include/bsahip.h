@@ -37,7 +37,7 @@ extern "C" {
 
 /* precision of the node contraction (flags bits 0-3) */
 #define BSAHIP_PREC_FP64       0  /* FP64 MFMA, the plain form                */
-#define BSAHIP_PREC_FP32       1  /* reserved: every entry returns EUNSUPPORTED */
+#define BSAHIP_PREC_FP32       1  /* FP32 operands, FP32 accumulate on MFMA    */
 #define BSAHIP_PREC_BF16X3     2  /* split-bf16 x3 products, FP32 accumulate  */
 #define BSAHIP_PREC_MASK     0xF
 
@@ -68,6 +68,17 @@ extern "C" {
  * quadrature through the generic epilogue instead of the specialised ones for
  * interior blocks; the results agree to rounding.  No reference counterpart. */
 #define BSAHIP_EPI_GENERIC   (1 << 12)
+
+/* fixed-order summation (flag bit 13; Classic runs and plans, not batches):
+ * every partial sum of an m3 or m2 entry is stored instead of added
+ * atomically, then each entry sums its partials in one fixed order with
+ * Kahan compensation (the wave reductions already run in a fixed tree), so
+ * repeated runs give bitwise-equal moments.  Costs an extra buffer of
+ * 12 doubles per wave flush of the G epilogue (0.36 GB at NN=1000, NM=30,
+ * 1025^2 in split-bf16x3) and a reduction pass.  Without it the moments are
+ * FP64 atomic sums whose order, and last bits, vary between runs.  No
+ * reference counterpart (the reference's sums are sequential). */
+#define BSAHIP_DETERMINISTIC (1 << 13)
 
 /* Problem description: borrowed host pointers, read during the call only. */
 typedef struct bsahip_problem {
@@ -156,8 +167,12 @@ int bsahip_classic_spectra(const bsahip_problem *p, int32_t nfh, double df, int3
  * sel is 1-based: m + (n-1) NM + (p-1) NM^2 over the kept modes (visual_idx_,
  * BsaLibImpl.F90:387-393); with BSAHIP_ONLY_DIAG the mode position 1..NM and the
  * diagonal model (getFM_diag_tnlm_vect_cls_ / getRM_diag_vect_cls_).
- * BSAHIP_SUM_EXACT as bsahip_classic_spectra (full model only).  FP64;
- * nsel at most 65535.
+ * BSAHIP_SUM_EXACT as bsahip_classic_spectra (full model only).  nsel at most
+ * 65535.  Precision FP64 (default), or BSAHIP_PREC_FP32 / BSAHIP_PREC_BF16X3
+ * (full model): each of the three G values of a point with the operand
+ * rounding and FP32 accumulation of that precision's G kernel (summed in K
+ * order; the kernel's MFMA sums in another order), for error reports of the
+ * mixed-precision paths.
  */
 int bsahip_classic_bispectra_sel(const bsahip_problem *p, int32_t nfh, double df, int32_t flags,
                                  int32_t nsel, const int32_t *sel, double *bfm, double *brm);
@@ -207,6 +222,12 @@ int bsahip_selftest_mfma_layout(double *out512);
 int bsahip_selftest_mfma_bf16_layout(float *out768);
 
 /*
+ * Same for v_mfma_f32_16x16x4f32 (FP32 path): out512 receives two 4x64
+ * register images of products with known answers.  No reference counterpart.
+ */
+int bsahip_selftest_mfma_f32_layout(float *out512);
+
+/*
  * Last measured device time (milliseconds, HIP events on the library stream)
  * of the dominant kernel of the previous call, and its launch count.
  */
@@ -228,7 +249,13 @@ typedef struct bsahip_mesher_cfg {
    int32_t tile_cells;        /* max level-0 cells per tile side (e.g. 32) */
    int32_t diag_crest;        /* 1: BRM grids also resolve the f1+f2 = +-f_n crests
                                  (BSA_PREMESH_TYPE_DIAG_CREST_YES); reference default 0 */
-   int32_t reserved;          /* 0 */
+   int32_t brm_cells;         /* 0: BRM spacing from the FRF (above).  > 0, a power of two
+                                 <= 256: every tile's BRM spacing is at most
+                                 df_ref / brm_cells, BRM cells per level-0 cell the
+                                 smallest power of two that gives it (exactly brm_cells
+                                 on tiles whose level-0 cell is df_ref): one uniform
+                                 lattice when every level-0 cell is df_ref.  No
+                                 reference counterpart. */
    double bkg_peakw;          /* background peak width [Hz] (max over L/U ratios: U/L) */
    double bkg_area_ext;       /* reference default 2 */
    double peak_area_ext;      /* reference default 2 */

@@ -44,3 +44,29 @@ def test_batch_graph_equals_per_structure_and_plans(gpu_lib, prec):
     info = B.info()
     assert info["items"] > 0 and info["ms_step"] > 0
     B.close()
+
+
+def test_config4_mix_graph_and_streams_equal_plans(gpu_lib):
+    """Configuration 4 at its workload: 256 seeded structures (NN 50-800, NM
+    4-24, 1-3 components, grids 257-1025, towers and decks) in FP64.  The
+    persistent step replayed from the graph (mode 0) and launched on worker
+    streams (mode 2) must give each structure's own plan result (1e-13); the
+    three cheapest structures must match the factorised oracle (1e-12)."""
+    from oracle import oracle as O
+    ps = synth.batch(256, seed=0)
+    assert len(ps) == 256 and min(p.nn for p in ps) >= 50 and max(p.nn for p in ps) <= 800
+    B = abi.Batch(ps, prec="fp64")
+    g0 = B.run(mode=0)
+    s2 = B.run(mode=2)
+    B.close()
+    for i, p in enumerate(ps):
+        one = gpu_lib.classic_run(p, prec="fp64")
+        for k in ("m3mf", "m3mr"):
+            assert rel_err(g0[k][i], one[k]) < 1e-13, (i, k)
+            assert rel_err(s2[k][i], one[k]) < 1e-13, (i, k)
+        assert rel_err(g0["m2mr"][i], one["m2mr"]) < 1e-13, i
+    cost = [p.ntc * p.nnl * len(O.kept_modes(p)) ** 3 * p.nf ** 2 for p in ps]
+    for i in np.argsort(cost)[:3]:
+        o = O.classic_factorised(ps[i], rule="vect", truncate=True, nthreads=16)
+        assert rel_err(g0["m3mf"][i], o["m3mf"]) < 1e-12, i
+        assert rel_err(g0["m3mr"][i], o["m3mr"]) < 1e-12, i

@@ -89,6 +89,31 @@ def test_config2_reduced_grid_matches_oracle(gpu_lib):
         assert rel_err(r[k], m[k]) < TOL, k
 
 
+def test_config2_full_grid_matches_oracle_fixture(gpu_lib):
+    """Configuration 2 at its full size (NN=200, u/v/w, NM=10, 513^2 grid):
+    moments and the bispectra of 27 entries at 300 sampled points (the four
+    corners, |f1+f2| > fmax where the Classic S(f1+f2) is zero, the f1+f2 edge)
+    against tests/golden/config2.npz (the factorised oracle, pinned to the
+    reference harness on the small cases)."""
+    g = load_golden("config2")
+    p = synth.config2()
+    assert p.nf == 513
+    r = gpu_lib.classic_run(p, quad="vect")
+    for k in ("m2mf", "m2mr", "m2o2mr", "m3mf", "m3mr"):
+        assert rel_err(r[k], g[k]) < TOL, (k, rel_err(r[k], g[k]))
+    s = gpu_lib.classic_bispectra_sel(p, g["sel"])
+    i1, i2 = g["sample_i1"], g["sample_i2"]
+    assert rel_err(s["bisp_f"][i1, i2, :], g["bisp_f_sample"]) < TOL
+    assert rel_err(s["bisp_r"][i1, i2, :], g["bisp_r_sample"]) < TOL
+    # entries with |f1+f2| > fmax are present and differ from the exact-sum values
+    c = (p.nf - 1) // 2
+    beyond = np.abs(i1 + i2 - 2 * c) > c
+    assert beyond.sum() >= 70
+    e = gpu_lib.classic_bispectra_sel(p, g["sel"][:3], exact=True)
+    assert np.abs(e["bisp_f"][i1[beyond], i2[beyond], :] -
+                  s["bisp_f"][i1[beyond], i2[beyond], :3]).max() > 0.0
+
+
 def test_mode_subset_consistency_full_size(gpu_lib):
     """Deck of configuration 3 at full size (NN=1000, NM=30, NTC=2, 1025^2):
     a mode triple's third moment does not depend on the other modes, so a run
@@ -141,3 +166,17 @@ def test_specialised_epilogue_equals_generic(gpu_lib, cfg, prec, exact, quad):
                                epi_generic=True)
     for k in ("m3mf", "m3mr"):
         assert rel_err(fast[k], slow[k]) < 1e-13, k
+
+
+def test_more_than_64_modes_matches_oracle(gpu_lib):
+    """NM = 72 kept modes (the projection runs in chunks of 64 modes): moments
+    against the factorised oracle on a small tower."""
+    p = synth.tower(nn=8, nm=72, comps=(1, 3), natf=np.geomspace(0.1, 3.0, 72),
+                    xi=np.full(72, 0.01), nfreqs=9, df=0.25, seed=11)
+    r = gpu_lib.classic_run(p, quad="vect")
+    assert r["m3mr"].size == 72 ** 3
+    o = O.classic_factorised(p, rule="vect", truncate=True, nthreads=16)
+    assert rel_err(r["m3mf"], o["m3mf"]) < TOL
+    assert rel_err(r["m3mr"], o["m3mr"]) < TOL
+    m = O.modal_psd(p)
+    assert rel_err(r["m2mr"], m["m2mr"]) < TOL

@@ -5,13 +5,18 @@
 * on a multi-tile layout with mixed refinement levels it must equal the CPU
   restatement (getFM_full_tnm_scalar_msh_ per BFM point, bilinear interpolation,
   FRF, trapezoid weights) -- 1e-12, with the same BFM evaluation counts;
+* on 256 tiles of five policies whose BRM grids share one lattice, with the BFM
+  at every BRM point, the tiles' trapezoid rules sum to the tensor trapezoid of
+  the union grid: it must equal the Classic G path (trapezoid, S(f1+f2) exact)
+  on that grid, and so must the CPU restatement -- 1e-12 (seams and edge weights
+  pinned by mathematics, not by a restatement of the same design);
 * fully refined, it converges to the Classic integral on a fine uniform grid."""
 import numpy as np
 import pytest
 
 from bsalib_amd import abi
 from oracle import oracle as O
-from tests.cases import CASES
+from tests.cases import CASES, LATTICE_CLASSIC, LATTICE_MESHER
 from tests.conftest import rel_err
 
 pytestmark = pytest.mark.gpu
@@ -33,6 +38,25 @@ def test_single_zone_equals_classic_trapezoid(gpu_lib):
     assert rel_err(r["m3mr"], c["m3mr"]) < 1e-12
     assert r["evals_pre"] == 81 and r["evals_post"] == 65 * 65 - 81
     assert r["brm_points"] == 65 * 65
+
+
+def test_lattice_layout_equals_classic_trapezoid_and_oracle(gpu_lib):
+    p = CASES["tiny_uvw"]()
+    M = abi.Mesher(p, **LATTICE_MESHER)
+    geom, _ = M.tiles()
+    assert M.ntiles == 256 and set(geom[:, 6].astype(int)) == {3, 4, 5, 9, 100}
+    r = M.run(levels=2)
+    M.close()
+    q = CASES["tiny_uvw"]()
+    q.nfreqs, q.df = LATTICE_CLASSIC["nfreqs"], LATTICE_CLASSIC["df"]
+    c = abi.classic_run(q, quad="trapezoid", exact=True)
+    assert rel_err(r["m3mf"], c["m3mf"]) < 1e-12
+    assert rel_err(r["m3mr"], c["m3mr"]) < 1e-12
+    o = O.mesher_integrate(p, geom, 2)
+    assert rel_err(r["m3mf"], o["m3mf"]) < 1e-12
+    assert rel_err(r["m3mr"], o["m3mr"]) < 1e-12
+    assert (r["evals_pre"], r["evals_post"]) == (o["evals_pre"], o["evals_post"])
+    assert r["evals_pre"] == 43 ** 2
 
 
 def test_multi_tile_mixed_levels_match_oracle(gpu_lib):

@@ -94,3 +94,63 @@ def test_diagonal_crest_refines_tiles_crossing_f1_plus_f2_resonance():
         g, _ = abi.mesher_layout(c, 2)
         bkg = g[(g[:, 6] == 100) & (g[:, 7] == 100)]
         assert len(bkg) == 1 and (int(bkg[0, 8]), int(bkg[0, 9])) == (sub, sub)
+
+
+def _lattice():
+    from tests.cases import CASES, LATTICE_MESHER
+    p = CASES["tiny_uvw"]()
+    kw = dict(LATTICE_MESHER)
+    cfg, keep = abi.mesher_cfg(p, **kw)
+    geom, _ = abi.mesher_layout(cfg, 2)
+    return p, geom
+
+
+def test_lattice_layout_tiles_share_one_brm_lattice():
+    p, g = _lattice()
+    assert len(g) == 256 and set(g[:, 6].astype(int)) == {3, 4, 5, 9, 100}
+    # every tile: BRM spacing 1/16 on both axes, corners on the lattice k/16
+    np.testing.assert_array_equal((g[:, 1] - g[:, 0]) / (g[:, 4] * g[:, 8]), 1.0 / 16.0)
+    np.testing.assert_array_equal((g[:, 3] - g[:, 2]) / (g[:, 5] * g[:, 9]), 1.0 / 16.0)
+    np.testing.assert_array_equal(np.round(g[:, :4] * 16.0), g[:, :4] * 16.0)
+    np.testing.assert_array_equal(g[:, 8:10], 2)
+
+
+def test_oracle_mesher_on_lattice_equals_classic_trapezoid():
+    """Tile seams and edge weights against mathematics: BFM at every BRM point
+    (level 2 = BRM cells per level-0 cell) on 256 tiles of five policies sharing one
+    lattice, so the sum of the tiles' trapezoid rules (edges 1/2, vertices 1/4)
+    is the tensor trapezoid on the union grid with S(f1+f2) evaluated exactly."""
+    from oracle import oracle as O
+    from tests.cases import CASES, LATTICE_CLASSIC
+    from tests.conftest import rel_err
+    p, g = _lattice()
+    o = O.mesher_integrate(p, g, 2)
+    q = CASES["tiny_uvw"]()
+    q.nfreqs, q.df = LATTICE_CLASSIC["nfreqs"], LATTICE_CLASSIC["df"]
+    c = O.classic_factorised(q, rule="trapezoid", truncate=False)
+    assert rel_err(o["m3mf"], c["m3mf"]) < 1e-12
+    assert rel_err(o["m3mr"], c["m3mr"]) < 1e-12
+    assert o["evals_pre"] == 43 ** 2
+
+
+def test_oracle_bilinear_interpolation_is_exact_on_bilinear_data():
+    """Level 1 interpolates the BFM bilinearly onto 2 BRM cells per BFM cell;
+    level 2 samples it at every BRM point.  On bilinear data both integrate the
+    same values; a quadratic term makes them differ."""
+    from oracle import oracle as O
+    p, g = _lattice()
+    rng = np.random.default_rng(7)
+    c = rng.standard_normal((4, 8))
+
+    def bilinear(prob, fi, fj, quad=0.0):
+        fi, fj = np.asarray(fi), np.asarray(fj)
+        return (c[0][:, None] + c[1][:, None] * fi + c[2][:, None] * fj
+                + c[3][:, None] * fi * fj + quad * fi * fi)
+
+    a = O.mesher_integrate(p, g, 1, points=bilinear)
+    b = O.mesher_integrate(p, g, 2, points=bilinear)
+    for k in ("m3mf", "m3mr"):
+        np.testing.assert_allclose(a[k], b[k], rtol=1e-13, atol=1e-13 * np.abs(b[k]).max())
+    q1 = O.mesher_integrate(p, g, 1, points=lambda *x: bilinear(*x, quad=1.0))
+    q2 = O.mesher_integrate(p, g, 2, points=lambda *x: bilinear(*x, quad=1.0))
+    assert np.abs(q1["m3mf"] - q2["m3mf"]).max() > 1e-6 * np.abs(q2["m3mf"]).max()
