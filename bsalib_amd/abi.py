@@ -54,7 +54,8 @@ class Problem_t(C.Structure):
 _lib = None
 
 EXPORTS = ["bsahip_init", "bsahip_finalise", "bsahip_last_error", "bsahip_version",
-           "bsahip_classic_run", "bsahip_classic_spectra", "bsahip_classic_bispectra_sel",
+           "bsahip_classic_run", "bsahip_classic_run_noalloc", "bsahip_classic_spectra",
+           "bsahip_classic_bispectra_sel",
            "bsahip_bfm_points",
            "bsahip_last_kernel_time", "bsahip_plan_create", "bsahip_plan_run",
            "bsahip_plan_destroy", "bsahip_mesher_create", "bsahip_mesher_layout", "bsahip_mesher_ntiles",

@@ -49,25 +49,52 @@ int fail(int code, const std::string &msg)
       if (r_ != 0) return r_;                                                                \
    } while (0)
 
-// Named, grow-only device buffers.
+// Named, grow-only device buffers.  With an arena (set_arena), get() carves
+// the buffers out of that fixed device region instead and never allocates.
 struct Bufs {
    struct Buf {
       void *p = nullptr;
       size_t n = 0;
    };
    std::map<std::string, Buf> m;
+   char *arena = nullptr;
+   size_t arena_cap = 0, arena_used = 0;
    ~Bufs() { release(); }
    void release()
    {
-      for (auto &kv : m)
-         if (kv.second.p) (void)hipFree(kv.second.p);
+      if (!arena)
+         for (auto &kv : m)
+            if (kv.second.p) (void)hipFree(kv.second.p);
       m.clear();
+      arena_used = 0;
+   }
+   void set_arena(void *base, size_t cap)
+   {
+      release();
+      arena = static_cast<char *>(base);
+      arena_cap = cap;
    }
    template <class T>
    int get(const char *name, size_t count, T **out)
    {
       Buf &b = m[name];
       const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+      if (arena) {
+         if (b.n < bytes) {
+            const size_t need = (bytes + 255) / 256 * 256;
+            if (arena_used + need > arena_cap)
+               return fail(BSAHIP_EINVAL,
+                           std::string("the problem needs more than the ") +
+                              std::to_string(arena_cap >> 20) +
+                              " MiB of static device scratch of this entry point (at buffer " + name +
+                              "); use the allocating entry point");
+            b.p = arena + arena_used;
+            b.n = need;
+            arena_used += need;
+         }
+         *out = static_cast<T *>(b.p);
+         return 0;
+      }
       if (b.n < bytes) {
          // work queued earlier may still read the old buffer
          if (b.p) {
@@ -87,9 +114,20 @@ struct Bufs {
    }
 };
 
+// Static device scratch of bsahip_classic_run_noalloc: every buffer of its
+// run (problem tables, A table, split planes, FRF table, moment accumulators)
+// is carved out of this array, so that entry point never allocates.  Each
+// device holds its own copy, at the address hipGetSymbolAddress returns with
+// that device current (kept in Ctx::scratch_dev); the name itself is a host
+// stand-in and is never passed to a kernel or a copy.
+#define BSAHIP_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+#define BSAHIP_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[BSAHIP_NOALLOC_SCRATCH_BYTES];
+
 struct Ctx {
    bool ready = false;
    int dev = 0;
+   void *scratch_dev[BSAHIP_MAX_DEVICES] = {};   // device address of `scratch` per device
    hipStream_t s = nullptr;
    hipStream_t s2 = nullptr;             // side stream: modal PSD moments beside the G kernel
    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
@@ -754,7 +792,14 @@ int bsahip_init(int device)
    if (e != hipSuccess || n <= 0)
       return fail(BSAHIP_EDEVICE, std::string("no HIP device available: ") + hipGetErrorString(e));
    if (device < 0 || device >= n) return fail(BSAHIP_EINVAL, "device index out of range");
+   if (device >= BSAHIP_MAX_DEVICES) return fail(BSAHIP_EUNSUPPORTED, "device index above 63");
    HIPC(hipSetDevice(device));
+   if (!g.scratch_dev[device]) {
+      void *sp = nullptr;
+      HIPC(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+      if (!sp) return fail(BSAHIP_EDEVICE, "hipGetSymbolAddress(scratch) returned NULL");
+      g.scratch_dev[device] = sp;
+   }
    HIPC(hipStreamCreateWithFlags(&g.s, hipStreamNonBlocking));
    HIPC(hipStreamCreateWithFlags(&g.s2, hipStreamNonBlocking));
    HIPC(hipEventCreate(&g.e0));
@@ -829,6 +874,24 @@ int bsahip_classic_run(const bsahip_problem *p, int32_t nfh, double df, int32_t 
                        double *m2mf, double *m2mr, double *m2o2mr, double *m3mf, double *m3mr)
 {
    return bsahip__classic_moments(p, nfh, df, flags, 0, m2mf, m2mr, m2o2mr, m3mf, m3mr);
+}
+
+int bsahip_classic_run_noalloc(const bsahip_problem *p, int32_t nfh, double df, int32_t flags,
+                               double *m2mf, double *m2mr, double *m2o2mr, double *m3mf,
+                               double *m3mr)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   if (!g.ready) return fail(BSAHIP_ESTATE, "call bsahip_init first (it sets up the static scratch)");
+   if (flags & BSAHIP_ONLY_DIAG)
+      return fail(BSAHIP_EUNSUPPORTED, "the ONLY_DIAG model runs through bsahip_classic_run");
+   void *base = g.scratch_dev[g.dev];
+   if (!base) return fail(BSAHIP_ESTATE, "no static scratch on the current device");
+   // the plan's buffers come from the static array; the plan is never
+   // destroyed with buffers of its own (Bufs in arena mode frees nothing)
+   bsahip_plan P;
+   P.B.set_arena(base, BSAHIP_NOALLOC_SCRATCH_BYTES);
+   CHK(plan_setup(&P, p, nfh, df, flags, 0));
+   return plan_exec(&P, m2mf, m2mr, m2o2mr, m3mf, m3mr);
 }
 
 }  // extern "C"

@@ -134,6 +134,26 @@ int bsahip_classic_run(const bsahip_problem *p, int32_t nfh, double df, int32_t 
                        double *m3mf, double *m3mr);
 
 /*
+ * bsahip_classic_run without any device allocation: same arguments, flags
+ * (any precision; not BSAHIP_ONLY_DIAG) and results, with every
+ * device buffer of the run carved out of a static 64 MiB array of the library
+ * (one per GPU, held until the process ends, its address taken by bsahip_init,
+ * which must have been called).  No reference counterpart: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  A problem whose buffers exceed
+ * the array (roughly NTC*NNL * NM*NF*8 bytes for the A table plus the bf16x3
+ * planes and the coherence tables) is rejected with BSAHIP_EINVAL and a
+ * message before the contraction runs.  Every call on one GPU uses that GPU's
+ * array, which keeps the last call's contents (each call clears what must
+ * start at zero); calls that overlapped on one GPU would race on it, which the
+ * library prevents by serialising its entry points (a process-wide mutex) and
+ * returning only after the device work.
+ */
+int bsahip_classic_run_noalloc(const bsahip_problem *p, int32_t nfh, double df, int32_t flags,
+                               double *m2mf, double *m2mr, double *m2o2mr, double *m3mf,
+                               double *m3mr);
+
+/*
  * Plans: the same Classic run with the structure resident on the device.
  * bsahip_plan_create validates and uploads the problem (the host arrays may
  * be released afterwards), computes ln rho on the device and allocates every
