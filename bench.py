@@ -210,7 +210,9 @@ def batch_leg(log):
 def mesher_leg(log):
     """Learned refinement proposer on the deck of configuration 5 (NN=500,
     NM=20): BFM evaluations and third-moment error against the converged Mesher
-    (every tile at level 8), next to the rule-based Mesher."""
+    (every tile at level 8), next to every plain setting (uniform levels 1, 2,
+    4, 8, the rule-based levels, the a-posteriori adaptive levels for 1e-3) and
+    the reduction against the cheapest plain setting that meets 1e-3."""
     from bsalib_amd import proposer, synth
     P = proposer.Proposer()
     r = proposer.evaluate(synth.config5(), P, log=log)

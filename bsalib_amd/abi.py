@@ -63,7 +63,8 @@ EXPORTS = ["bsahip_init", "bsahip_finalise", "bsahip_last_error", "bsahip_versio
            "bsahip_mesher_run_tiles", "bsahip_mesher_export", "bsahip_mesher_save",
            "bsahip_mesher_load", "bsahip_batch_create", "bsahip_batch_autotune",
            "bsahip_batch_run", "bsahip_batch_info", "bsahip_batch_destroy",
-           "bsahip_mesher_destroy", "bsahip_modal_psd_points", "bsahip_modal_recombination"]
+           "bsahip_mesher_destroy", "bsahip_modal_psd_points", "bsahip_modal_recombination",
+           "bsahip_mesher_adapt"]
 
 
 def lib():
@@ -98,6 +99,8 @@ def lib():
         L.bsahip_mesher_load.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_char_p,
                                          C.POINTER(C.c_void_p)]
         L.bsahip_mesher_destroy.argtypes = [C.c_void_p]
+        L.bsahip_mesher_adapt.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_int32),
+                                          C.POINTER(C.c_int64)]
         _lib = L
     return _lib
 
@@ -417,6 +420,14 @@ class Mesher:
         if per_tile:
             out["tile_m3mr"] = tiles
         return out
+
+    def adapt(self, tol: float = 1e-3):
+        """A-posteriori refinement levels for a target m3mr error tol relative to
+        max|m3mr| (bsahip_mesher_adapt): (levels, BFM evaluations of those levels)."""
+        lv = np.zeros(self.ntiles, dtype=np.int32)
+        ev = C.c_int64(0)
+        _check(lib().bsahip_mesher_adapt(self.h, C.c_double(tol), _ptr(lv, C.c_int32), C.byref(ev)))
+        return lv, int(ev.value)
 
     def _levels(self, levels):
         if levels is None:

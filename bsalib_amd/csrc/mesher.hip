@@ -1040,6 +1040,79 @@ int bsahip_mesher_run(bsahip_mesher *m, const int32_t *levels, double *m3mf, dou
    return bsahip_mesher_run_tiles(m, levels, nullptr, m3mf, m3mr, counts);
 }
 
+int bsahip_mesher_adapt(bsahip_mesher *m, double tol, int32_t *levels, int64_t *evals)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   if (!m || !levels || !(tol > 0.0)) return fail(BSAHIP_EINVAL, "need a Mesher, levels and tol > 0");
+   const size_t nt = m->tiles.size(), nm3 = (size_t)m->nc;
+   std::vector<double> prev(nt * nm3), cur(nt * nm3), m3f(nm3), m3r(nm3);
+   std::vector<int32_t> lev(nt, 1), lr(nt);
+   // level 1 (the resident level-0 samples), then level 2 on every tile
+   CHK(bsahip_mesher_run_tiles(m, lev.data(), prev.data(), m3f.data(), m3r.data(), nullptr));
+   std::fill(lev.begin(), lev.end(), 2);
+   CHK(bsahip_mesher_run_tiles(m, lev.data(), cur.data(), m3f.data(), m3r.data(), nullptr));
+   double M = 0.0;
+   for (size_t q = 0; q < nm3; ++q) M = std::max(M, std::fabs(m3r[q]));
+   const double budget = 0.5 * tol * M;
+   // error estimate of a tile's contribution at level L from the change since
+   // L/2 (bilinear interpolation is second order: err(L) ~ (c_L - c_L/2) / 3),
+   // kept signed per entry: the error of m3mr is their sum over the tiles
+   std::vector<double> delta(nt * nm3), dnorm(nt);
+   auto set_delta = [&](size_t t) {
+      double e = 0.0;
+      for (size_t q = 0; q < nm3; ++q) {
+         const double d = (cur[t * nm3 + q] - prev[t * nm3 + q]) / 3.0;
+         delta[t * nm3 + q] = d;
+         e = std::max(e, std::fabs(d));
+      }
+      dnorm[t] = e;
+   };
+   for (size_t t = 0; t < nt; ++t) set_delta(t);
+   std::swap(prev, cur);
+   std::vector<double> E(nm3);
+   auto emax = [&]() {
+      double e = 0.0;
+      for (size_t q = 0; q < nm3; ++q) e = std::max(e, std::fabs(E[q]));
+      return e;
+   };
+   for (int round = 0; round < 2; ++round) {
+      std::fill(E.begin(), E.end(), 0.0);
+      for (size_t t = 0; t < nt; ++t)
+         for (size_t q = 0; q < nm3; ++q) E[q] += delta[t * nm3 + q];
+      if (emax() <= budget) break;
+      // refine the tiles of largest estimate, in that order, until the
+      // estimated error with theirs cut by one level (about 1/4) fits
+      std::vector<size_t> ord;
+      for (size_t t = 0; t < nt; ++t)
+         if (lev[t] < 8 && dnorm[t] > 0.0) ord.push_back(t);
+      std::sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return dnorm[a] > dnorm[b]; });
+      std::fill(lr.begin(), lr.end(), 1);
+      bool any = false;
+      for (size_t t : ord) {
+         if (emax() <= budget) break;
+         lr[t] = 2 * lev[t];
+         any = true;
+         for (size_t q = 0; q < nm3; ++q) E[q] -= 0.75 * delta[t * nm3 + q];
+      }
+      if (!any) break;
+      CHK(bsahip_mesher_run_tiles(m, lr.data(), cur.data(), m3f.data(), m3r.data(), nullptr));
+      for (size_t t = 0; t < nt; ++t)
+         if (lr[t] > 1) {
+            set_delta(t);
+            std::copy(cur.begin() + t * nm3, cur.begin() + (t + 1) * nm3, prev.begin() + t * nm3);
+            lev[t] = lr[t];
+         }
+   }
+   int64_t n = m->evals_lv0;
+   for (size_t t = 0; t < nt; ++t) {
+      const MshTile &T = m->tiles[t];
+      levels[t] = lev[t];
+      n += (int64_t)(T.n1 * lev[t] + 1) * (T.n2 * lev[t] + 1) - (int64_t)(T.n1 + 1) * (T.n2 + 1);
+   }
+   if (evals) *evals = n;
+   return 0;
+}
+
 int bsahip_mesher_export(bsahip_mesher *m, const int32_t *levels, int32_t ncol, const double *coef,
                          int64_t cap, int32_t *dims, double *fi, double *fj, double *val,
                          int64_t *npts)

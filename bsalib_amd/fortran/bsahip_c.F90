@@ -38,6 +38,7 @@ module BsaHip_C
    public :: bsahip_init, bsahip_finalise, bsahip_classic_run, bsahip_bfm_points
    public :: bsahip_classic_bispectra_sel
    public :: bsahip_mesher_create, bsahip_mesher_run, bsahip_mesher_destroy, bsahip_mesher_export
+   public :: bsahip_mesher_adapt
    public :: bsahip_mesher_ntiles
    public :: bsahip_last_error_f, bsahip_modal_psd_points, bsahip_modal_recombination
 
@@ -90,6 +91,13 @@ module BsaHip_C
             bind(c, name="bsahip_mesher_run")
          import c_int, c_ptr
          type(c_ptr), value :: mesher, levels, m3mf, m3mr, counts
+      end function
+
+      integer(c_int) function bsahip_mesher_adapt(mesher, tol, levels, evals) &
+            bind(c, name="bsahip_mesher_adapt")
+         import c_int, c_double, c_ptr
+         type(c_ptr), value :: mesher, levels, evals
+         real(c_double), value :: tol
       end function
 
       integer(c_int) function bsahip_modal_psd_points(p, npts, f, psd) &

@@ -20,8 +20,10 @@
 !!    reference's GPU path does (BsaLibImpl.F90:347-353); unlike that path,
 !!    PSDs are still computed when requested.  The Mesher is the device zone
 !!    Mesher of libbsahip (zone limits and policies of BsaMesherImpl.F90,
-!!    tiles of at most 32 x 32 level-0 cells, rule-based refinement levels);
-!!    its results are m3mf_msh, m3mr_msh.  With isvd = 1 its BFM is the POD
+!!    tiles of at most 32 x 32 level-0 cells); its refinement levels are
+!!    chosen a posteriori for a target error of m3mr (bsa_setMesherTolerance,
+!!    an addition; default 1e-3 of max|m3mr|; 0 selects the rule-based levels
+!!    of the policy factors); its results are m3mf_msh, m3mr_msh.  With isvd = 1 its BFM is the POD
 !!    one; without truncation that is evaluated as the direct projection
 !!    with coherence column 1 (equal up to rounding), with truncation through
 !!    eigendecompositions on the GPU.  A threshold counts the modes as the
@@ -167,6 +169,7 @@ module BsaLib
    public :: bsa_setVisualIndexes, bsa_exportMomentToFile, bsa_exportSkewness
    public :: bsa_exportBRdecomp, bsa_exportExtremeValuesToFile, bsa_saveCoordinatesToFile
    public :: bsa_setPremeshMode, bsa_enableOnlyPremesh, bsa_setPolicyIDValidationValues
+   public :: bsa_setMesherTolerance
    public :: bsa_isFullComp, bsa_setOutputDirectory, bsa_setOutUnit, bsa_exportPSDToFile
    public :: bsa_exportBispToFile, bsa_exportModalData, bsa_setBispExportCallback
 
@@ -232,6 +235,10 @@ module BsaLib
    class(*), pointer :: null_pdata_ => null()
    ! Mesher pre-mesh only, pre-mesh mode, modal data file at bsa_Run
    logical :: only_premesh_ = .false., export_modal_ = .false.
+   ! Mesher refinement: target error of m3mr relative to its maximum (0: the
+   ! rule-based levels); levels of the last run, used by its exports
+   real(bsa_real_t) :: mesher_tol_ = 1.0e-3_bsa_real_t
+   integer(c_int32_t), allocatable, target :: msh_levels_(:)
    integer(bsa_int_t) :: premesh_mode_ = BSA_PREMESH_MODE_ZONE_REFINED
 
    ! ---- problem of the last bsa_Run, kept for the post-processing calls ----
@@ -923,10 +930,11 @@ contains
          type(c_ptr) :: msh
          real(bsa_real_t), allocatable, target :: fk(:), xk(:)
          real(bsa_real_t) :: v(3, 3), bw
-         integer(c_int64_t), target :: counts(3)
+         integer(c_int64_t), target :: counts(3), adapt_evals
          integer(bsa_int_t) :: z, nmk, nc
-         integer(c_int32_t) :: mflags
+         integer(c_int32_t) :: mflags, ntl
          integer(c_int) :: rc2
+         type(c_ptr) :: plev
          if (.not. allocated(xsi_)) &
             call bsa_Abort('The Mesher needs the modal damping ratios (bsa_setTotDamping).')
          ! background peak width: 1 / max_z(L/U), maximised over (component, direction)
@@ -982,7 +990,17 @@ contains
             return
          end if
          allocate(m3mf(nc), m3mr(nc))
-         rc2 = bsahip_mesher_run(msh, c_null_ptr, c_loc(m3mf), c_loc(m3mr), c_loc(counts))
+         if (allocated(msh_levels_)) deallocate(msh_levels_)
+         plev = c_null_ptr
+         if (mesher_tol_ > 0.0_bsa_real_t) then
+            rc2 = bsahip_mesher_ntiles(msh, ntl)
+            allocate(msh_levels_(ntl))
+            rc2 = bsahip_mesher_adapt(msh, real(mesher_tol_, c_double), c_loc(msh_levels_), &
+                                      c_loc(adapt_evals))
+            if (rc2 /= 0) call bsa_Abort('bsahip_mesher_adapt failed: '//bsahip_last_error_f())
+            plev = c_loc(msh_levels_)
+         end if
+         rc2 = bsahip_mesher_run(msh, plev, c_loc(m3mf), c_loc(m3mr), c_loc(counts))
          if (rc2 /= 0) call bsa_Abort('bsahip_mesher_run failed: '//bsahip_last_error_f())
          if (visual_ .or. brm_base_) call exportMesher_(msh, nmk)
          rc2 = bsahip_mesher_destroy(msh)
@@ -1465,6 +1483,16 @@ contains
       only_premesh_ = .true.
    end subroutine
 
+   !> Extension (no counterpart in the reference): target error of the
+   !> Mesher's m3mr relative to its largest entry; the refinement level of
+   !> every tile is chosen a posteriori for it (bsahip_mesher_adapt).  0 keeps
+   !> the rule-based levels of the policy factors.  Default 1e-3.
+   subroutine bsa_setMesherTolerance(tol)
+      real(bsa_real_t), value :: tol
+      if (tol < 0.0_bsa_real_t) call bsa_Abort('The Mesher tolerance must be >= 0.')
+      mesher_tol_ = tol
+   end subroutine
+
    !> Accepted for interface compatibility (src/bsa/BsaLibImpl.F90:847-865):
    !> the device Mesher's rule-based levels use the built-in policy factors.
    subroutine bsa_setPolicyIDValidationValues(id, i_bfm, j_bfm, i_brm, j_brm)
@@ -1622,7 +1650,7 @@ contains
       integer(int64) :: off, q
       integer :: iun, ios, k, m, n, o, d, ncomb, nc
       character(len=:), allocatable :: fname
-      type(c_ptr) :: pcoef
+      type(c_ptr) :: pcoef, plev
       rc = bsahip_mesher_ntiles(msh, nt)
       allocate(dims(2, nt))
       pcoef = c_null_ptr
@@ -1675,11 +1703,13 @@ contains
          ncomb = nc
       end if
       if (allocated(coef)) pcoef = c_loc(coef)
-      rc = bsahip_mesher_export(msh, c_null_ptr, 0_c_int32_t, c_null_ptr, 0_c_int64_t, c_loc(dims), &
+      plev = c_null_ptr
+      if (allocated(msh_levels_)) plev = c_loc(msh_levels_)
+      rc = bsahip_mesher_export(msh, plev, 0_c_int32_t, c_null_ptr, 0_c_int64_t, c_loc(dims), &
                                 c_null_ptr, c_null_ptr, c_null_ptr, npts)
       if (rc /= 0) call bsa_Abort('bsahip_mesher_export failed: '//bsahip_last_error_f())
       allocate(fi(npts), fj(npts), val(ncol, npts))
-      rc = bsahip_mesher_export(msh, c_null_ptr, ncol, pcoef, npts, c_loc(dims), c_loc(fi), c_loc(fj), &
+      rc = bsahip_mesher_export(msh, plev, ncol, pcoef, npts, c_loc(dims), c_loc(fi), c_loc(fj), &
                                 c_loc(val), npts)
       if (rc /= 0) call bsa_Abort('bsahip_mesher_export failed: '//bsahip_last_error_f())
       if (associated(brm_cb_)) then

@@ -271,17 +271,18 @@ class Proposer:
         return levels_from_logits(self.logits(prob, mesher), th)
 
 
-def sweep(prob, net: MLP, thresholds, log=print) -> dict:
-    """Plain (rule-based), converged (every tile at level 8) and MLP-guided
-    Mesher runs of one structure for several decision thresholds: BFM
-    evaluations and third-moment errors against the converged m3mr."""
+def sweep(prob, net: MLP, thresholds, log=print, target: float = 1e-3, plain: bool = True) -> dict:
+    """Converged (every tile at level 8) and MLP-guided Mesher runs of one
+    structure for several decision thresholds, and (plain=True) every plain
+    setting: uniform levels 1, 2, 4, the rule-based levels and the a-posteriori
+    adaptive levels for `target` (bsahip_mesher_adapt).  BFM evaluations and
+    third-moment errors against the converged m3mr."""
     M = abi.Mesher(prob, **mesher_settings(prob))
     try:
         t0 = time.perf_counter()
         z = net.forward(tile_features(prob, M))
         t_inf = time.perf_counter() - t0
         conv = M.run(levels=8)
-        rule = M.run()
         ref = np.abs(conv["m3mr"]).max()
 
         def err(r):
@@ -290,8 +291,23 @@ def sweep(prob, net: MLP, thresholds, log=print) -> dict:
         def evals(r):
             return r["evals_pre"] + r["evals_post"]
         out = dict(name=f"{prob.name} NN={prob.nn} NM={prob.nm}", ntiles=M.ntiles,
-                   evals_conv=evals(conv), evals_rule=evals(rule), err_rule=err(rule),
-                   features_and_inference_s=t_inf, mlp={})
+                   evals_conv=evals(conv), features_and_inference_s=t_inf, mlp={}, plain={})
+        if plain:
+            for lv in (1, 2, 4):
+                r = M.run(levels=lv)
+                out["plain"][f"uniform_{lv}"] = dict(evals=evals(r), err=err(r))
+            r = M.run()
+            out["plain"]["rule"] = dict(evals=evals(r), err=err(r))
+            t1 = time.perf_counter()
+            lv, ev = M.adapt(target)
+            r = M.run(levels=lv)
+            out["plain"]["adaptive"] = dict(evals=ev, err=err(r), tol=target,
+                                            run_s=time.perf_counter() - t1)
+            out["plain"]["uniform_8"] = dict(evals=evals(conv), err=0.0)
+            ok = {k: v for k, v in out["plain"].items() if v["err"] <= target}
+            best = min(ok, key=lambda k: ok[k]["evals"])
+            out["cheapest_plain"] = dict(name=best, **ok[best])
+            out["evals_rule"], out["err_rule"] = out["plain"]["rule"]["evals"], out["plain"]["rule"]["err"]
         for th in thresholds:
             t1 = time.perf_counter()
             r = M.run(levels=levels_from_logits(z, th))
@@ -301,13 +317,16 @@ def sweep(prob, net: MLP, thresholds, log=print) -> dict:
     return out
 
 
-def evaluate(prob, proposer: Proposer, log=print) -> dict:
-    """The stored threshold only (see sweep)."""
-    r = sweep(prob, proposer.net, [proposer.threshold])
+def evaluate(prob, proposer: Proposer, log=print, target: float = 1e-3) -> dict:
+    """The stored threshold only (see sweep): the MLP's evaluations and error,
+    every plain setting, and the reduction against the cheapest plain setting
+    that meets the target."""
+    r = sweep(prob, proposer.net, [proposer.threshold], target=target)
     m = r.pop("mlp")[proposer.threshold]
     r.update(evals_mlp=m["evals"], err_mlp=m["err"])
     r["reduction_vs_conv"] = r["evals_conv"] / r["evals_mlp"]
     r["reduction_vs_rule"] = r["evals_rule"] / r["evals_mlp"]
+    r["reduction_vs_cheapest_plain"] = r["cheapest_plain"]["evals"] / r["evals_mlp"]
     log(f"[proposer] {r}")
     return r
 
@@ -330,9 +349,8 @@ def main():
     net, mean, inv, losses = train(X, y, cost, epochs=args.epochs)
     res = []
     for s in range(args.val):
-        r = sweep(validation_structure(s), net, THRESHOLDS)
-        print(f"[proposer] validation {r['name']}: rule {r['evals_rule']} evals err "
-              f"{r['err_rule']:.2e}, converged {r['evals_conv']}; MLP " +
+        r = sweep(validation_structure(s), net, THRESHOLDS, plain=False)
+        print(f"[proposer] validation {r['name']}: converged {r['evals_conv']}; MLP " +
               ", ".join(f"{th:g}: {m['evals']} / {m['err']:.1e}" for th, m in r["mlp"].items()),
               flush=True)
         res.append(r)

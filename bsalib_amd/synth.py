@@ -131,9 +131,13 @@ def tower(nn: int = 20, nm: int = 3, comps=(1,), natf=None, xi=None, nfreqs: int
           df: float = 1.0 / 32.0, seed: int = 0, height: float = 100.0, u10: float = 30.0,
           alpha: float = 0.16, psd_id: int = 1, decay: float = 10.0,
           intensities=(0.15, 0.10, 0.08), scales=(100.0, 40.0, 20.0),
-          cd: float = 1.2, area: float = 2.0, zone_per_node: bool = True) -> Problem:
+          cd: float = 1.2, area: float = 2.0, zone_per_node: bool = True, nlibs: int = 1,
+          libs_load=(1,)) -> Problem:
     """Vertical cantilever tower (configuration 1 by default: NN=20, NM=3, u only,
-    f_n = 0.2/0.6/1.2 Hz, 1 % damping, 129-point grid over [-2, 2] Hz)."""
+    f_n = 0.2/0.6/1.2 Hz, 1 % damping, 129-point grid over [-2, 2] Hz).
+    nlibs > 1: DOFs per node (DOF 1 along-wind, the others rotations about the
+    tower axis whose mode shapes are seeded sinusoids); libs_load lists the
+    loaded DOFs, a rotation carrying the drag times a lever arm of 0.4 m."""
     rng = np.random.default_rng(seed)
     z = np.arange(1, nn + 1, dtype=np.float64) * height / nn
     U = u10 * (z / 10.0) ** alpha
@@ -153,25 +157,31 @@ def tower(nn: int = 20, nm: int = 3, comps=(1,), natf=None, xi=None, nfreqs: int
     packed = pack_corr(rho)
     nod_corr = np.asfortranarray(np.stack([packed] * 3, axis=1))
     # quasi-steady drag F = 1/2 rho Cd A (U+u)^2 -> linear rho Cd A U, quadratic 1/2 rho Cd A
-    wfc = np.zeros((1, 6, nn), order="F")
+    libs_load = np.asarray(libs_load, dtype=np.int32)
+    wfc = np.zeros((libs_load.size, 6, nn), order="F")
     comp_gain = {1: 1.0, 2: 0.5, 3: 0.25}
-    for c in comps:
-        g = comp_gain[c]
-        wfc[0, c - 1, :] = g * RHO_AIR * cd * area * U
-        wfc[0, c + 2, :] = g * 0.5 * RHO_AIR * cd * area
+    for li, lib in enumerate(libs_load):
+        arm = 1.0 if lib == 1 else 0.4
+        for c in comps:
+            g = comp_gain[c] * arm
+            wfc[li, c - 1, :] = g * RHO_AIR * cd * area * U
+            wfc[li, c + 2, :] = g * 0.5 * RHO_AIR * cd * area
     if natf is None:
         natf = np.array([0.2, 0.6, 1.2])[:nm] if nm <= 3 else np.geomspace(0.15, 3.0, nm)
     natf = np.asarray(natf, dtype=np.float64)
     if xi is None:
         xi = np.full(nm, 0.01)
     xi = np.asarray(xi, dtype=np.float64)
-    phi = np.empty((nn, nm))
+    phi = np.empty((nn * nlibs, nm))
     for m in range(nm):
-        phi[:, m] = (z / height) ** (1.5 * (m + 1)) + rng.normal(0.0, 0.02, nn)
+        phi[0::nlibs, m] = (z / height) ** (1.5 * (m + 1)) + rng.normal(0.0, 0.02, nn)
+        for lib in range(1, nlibs):
+            phi[lib::nlibs, m] = 0.3 * np.sin((m + lib) * np.pi * z / (2 * height)) + \
+                rng.normal(0.0, 0.02, nn)
     phi = _normalise_modes(phi)
     Mm, Km, Cm = _modal_matrices(natf, xi)
-    return Problem(nn=nn, nlibs=1, n_load=np.arange(1, nn + 1, dtype=np.int32),
-                   libs_load=np.array([1], dtype=np.int32), phi=phi, natf=natf, Mm=Mm, Km=Km,
+    return Problem(nn=nn, nlibs=nlibs, n_load=np.arange(1, nn + 1, dtype=np.int32),
+                   libs_load=libs_load, phi=phi, natf=natf, Mm=Mm, Km=Km,
                    Cm=Cm, psd_id=psd_id, tc=np.array(comps, dtype=np.int32), wz_node=wz_node,
                    u_node=U, turb_scales=L, turb_std=sig, u_mean_ref=u_ref,
                    nod_corr=nod_corr, wfc=wfc, nfreqs=nfreqs, df=df, name="tower", coords=z,

@@ -312,6 +312,18 @@ int bsahip_mesher_features(bsahip_mesher *m, int32_t ns, double *stencil);
  * evaluations of this run, BRM points of this run}. */
 int bsahip_mesher_run(bsahip_mesher *m, const int32_t *levels, double *m3mf, double *m3mr,
                       int64_t *counts);
+/* A-posteriori refinement levels for a target error tol of m3mr relative to
+ * max|m3mr| (a plain adaptive Mesher: no counterpart in the reference, whose
+ * levels come from the policy table alone): every tile is integrated at
+ * levels 1 and 2; the error of a tile's contribution at level L is estimated
+ * per entry as (c_L - c_L/2) / 3 (bilinear interpolation is second order) and
+ * the error of m3mr as their sum over the tiles; while its largest entry
+ * exceeds tol max|m3mr| / 2, the tiles of largest estimate go up one level
+ * (two rounds at most: levels 2, 4, 8).  levels (ntiles) receives 2, 4 or 8
+ * per tile; *evals the BFM evaluations of those levels (level-0 points
+ * included, each point once).  Run bsahip_mesher_run with the levels for the
+ * moments. */
+int bsahip_mesher_adapt(bsahip_mesher *m, double tol, int32_t *levels, int64_t *evals);
 /* As bsahip_mesher_run, plus each tile's own m3mr contribution in tile_m3mr
  * (ntiles x NM^3; m3mr is then their sum). */
 int bsahip_mesher_run_tiles(bsahip_mesher *m, const int32_t *levels, double *tile_m3mr,

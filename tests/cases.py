@@ -7,6 +7,10 @@ CASES = {
     "tiny_dav": lambda: synth.tower(nn=5, nm=3, comps=(1,), nfreqs=8, df=0.12, seed=5, psd_id=5,
                                     zone_per_node=False),
     "config1": lambda: synth.config1(),
+    # three DOFs per node, DOFs 1 and 2 loaded (NLIBSL = 2; the reference's
+    # Mesher BFM reads wfc by DOF number, so loaded DOFs must lead: trap T21)
+    "tiny_2dof": lambda: synth.tower(nn=5, nm=3, comps=(1, 3), nfreqs=8, df=0.125, seed=7,
+                                     nlibs=3, libs_load=(1, 2)),
 }
 
 

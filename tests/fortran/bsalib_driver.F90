@@ -5,6 +5,7 @@
 !! Usage:  bsalib_driver <input.bin> <output.bin> [fp64|bf16x3] [grid] [scalar] [mesher|both]
 !!                        [pod] [podn=<modes>] [podt=<percent>] [post] [gen] [diag]
 !!                        [visual=<i>,<j>,<k>] [visualn=<node>,<dof>] [sym=<n>] [brmbase]
+!!                        [mtol=<tolerance>]
 !!                        [cb] [premesh] [modal]
 !! The input layout is the one tests write for the reference harness (header
 !! of 14 int32, then integer and real64 arrays in Fortran order).  "grid"
@@ -26,7 +27,8 @@
 !! bsa_setBRMExportDefaultMode(BSA_EXPORT_BRM_MODE_BASE); "cb" registers a
 !! bispectrum export callback (bsa_setBispExportCallback) that prints, at the
 !! end, "CALLBACK <zones> <points> <sum of values>"; "premesh" calls
-!! bsa_enableOnlyPremesh, "modal" bsa_exportModalData.
+!! bsa_enableOnlyPremesh, "modal" bsa_exportModalData; "mtol=" sets the
+!! Mesher's target error (bsa_setMesherTolerance; 0: rule-based levels).
 !! Output: nme (int32), the used modes, then for each of m2mf, m2mr, m2o2mr,
 !! m3mf_cls, m3mr_cls, m3mf_msh, m3mr_msh: its size (int32, 0 when bsa_Run
 !! left it unallocated) and its values (real64).
@@ -142,6 +144,10 @@ program bsalib_driver
       if (arg(1:5) == 'podt=') then
          read(arg(6:), *) podt
          call bsa_setPODTruncationThreshold(podt)
+      end if
+      if (arg(1:5) == 'mtol=') then
+         read(arg(6:), *) podt
+         call bsa_setMesherTolerance(podt)
       end if
    end do
    call bsa_setAnalysisType(isuban)

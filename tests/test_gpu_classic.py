@@ -11,7 +11,7 @@ from tests.conftest import load_golden, rel_err
 
 pytestmark = pytest.mark.gpu
 
-GOLDEN_CASES = ["tiny_u", "tiny_uvw", "tiny_dav", "config1"]
+GOLDEN_CASES = ["tiny_u", "tiny_uvw", "tiny_dav", "config1", "tiny_2dof"]
 TOL = 1e-12
 
 

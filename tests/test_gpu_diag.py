@@ -92,7 +92,7 @@ def test_fortran_front_mesher_diag(gpu_lib, mode):
     assert "Mesher on the GPU" in r.stdout
     M = abi.Mesher(p, bkg_base_rfmnt=20, tile_cells=32, bkg_peakw=host.bkg_peak_width(p),
                    only_diag=True)
-    want = M.run()
+    want = M.run(levels=M.adapt(1e-3)[0])
     M.close()
     assert out["m3mf_msh"].size == M.nm
     assert rel_err(out["m3mf_msh"], want["m3mf"]) < 1e-13

@@ -17,7 +17,7 @@ from tests.conftest import load_golden, rel_err
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", ["tiny_u", "tiny_uvw", "tiny_dav", "config1"])
+@pytest.mark.parametrize("name", ["tiny_u", "tiny_uvw", "tiny_dav", "config1", "tiny_2dof"])
 def test_fortran_front_matches_reference(name):
     g = load_golden(name)
     p = CASES[name]()
@@ -96,16 +96,23 @@ def test_fortran_front_grid_checks(gpu_lib, nfreqs, df, branch):
 def test_fortran_front_mesher(gpu_lib, mode):
     """Analysis type 2 (Mesher) and 3 (both) through bsa_Run: the Mesher moments
     equal the C ABI Mesher with the same settings (bsa_setupMesher(0, 20, 2, 2, 2,
-    ...), background width 1/max(L/U), rule-based levels)."""
+    ...), background width 1/max(L/U)) at the levels bsahip_mesher_adapt gives for
+    the default target error 1e-3; bsa_setMesherTolerance(0) keeps the rule-based
+    levels."""
     from bsalib_amd import abi, host
     p = CASES["config1"]()
     out, r = fdriver.run(p, mode)
     assert "Mesher on the GPU" in r.stdout
     M = abi.Mesher(p, bkg_base_rfmnt=20, tile_cells=32, bkg_peakw=host.bkg_peak_width(p))
-    want = M.run()
+    lv, _ = M.adapt(1e-3)
+    want = M.run(levels=lv)
+    rule = M.run()
     M.close()
     assert rel_err(out["m3mf_msh"], want["m3mf"]) < 1e-13
     assert rel_err(out["m3mr_msh"], want["m3mr"]) < 1e-13
+    if mode == "mesher":
+        out0, _ = fdriver.run(p, mode, "mtol=0")
+        assert rel_err(out0["m3mr_msh"], rule["m3mr"]) < 1e-13
     if mode == "mesher":
         assert out["m3mr"] is None and out["m2mr"] is None
     else:
@@ -134,7 +141,7 @@ def test_fortran_front_mesher_pod(gpu_lib, opts, pod):
     assert "Mesher on the GPU" in r.stdout
     M = abi.Mesher(p, bkg_base_rfmnt=20, tile_cells=32, bkg_peakw=host.bkg_peak_width(p),
                    bound=abi.Bound(p, corr_col_mode=1, **pod))
-    want = M.run()
+    want = M.run(levels=M.adapt(1e-3)[0])
     M.close()
     assert rel_err(out["m3mf_msh"], want["m3mf"]) < 1e-13
     assert rel_err(out["m3mr_msh"], want["m3mr"]) < 1e-13

@@ -114,3 +114,22 @@ def test_refined_mesher_converges_to_classic(gpu_lib):
     assert e8 < 1e-3 and e0 < 1e-3
     assert e1 > 3 * e8                           # BFM refinement matters
     assert rel_err(r["m3mf"], c["m3mf"]) < 1e-3
+
+
+@pytest.mark.parametrize("name,tol", [("config1", 1e-3), ("config1", 1e-4), ("tiny_uvw", 1e-3)])
+def test_adaptive_levels_meet_their_target(gpu_lib, name, tol):
+    """bsahip_mesher_adapt: the a-posteriori levels keep m3mr within tol of the
+    every-tile-level-8 result (relative to its largest entry), evaluating the
+    BFM at no more points than level 8 everywhere (the lightly damped tower needs
+    level 8 on nearly every tile); their count is the distinct points of those
+    levels."""
+    p = CASES[name]()
+    M = abi.Mesher(p, bkg_base_rfmnt=20, tile_cells=8)
+    conv = M.run(levels=8)
+    lv, ev = M.adapt(tol)
+    assert set(np.unique(lv)) <= {2, 4, 8}
+    r = M.run(levels=lv)
+    M.close()
+    assert rel_err(r["m3mr"], conv["m3mr"]) <= tol
+    assert ev == r["evals_pre"] + r["evals_post"]
+    assert ev <= conv["evals_pre"] + conv["evals_post"]

@@ -165,16 +165,17 @@ def _same_values(zones, e, ncol):
 def test_front_mesher_visual_modal_and_nodal(gpu_lib, tmp_path):
     p = _damped("tiny_u", 20)
     M = _front_mesher(p)
+    lv, _ = M.adapt(1e-3)                 # the front's default refinement
     nm = M.nm
     idx = (1, nm, 2)
     q = idx[0] - 1 + (idx[1] - 1) * nm + (idx[2] - 1) * nm * nm
     unit = np.zeros((nm ** 3, 1))
     unit[q, 0] = 1.0
-    em = M.export(coef=unit)
+    em = M.export(levels=lv, coef=unit)
     d1 = tmp_path / "modal"
     d1.mkdir()
     out, f = _front_export(p, d1, "visual=%d,%d,%d" % idx)
-    ref = M.run()
+    ref = M.run(levels=lv)
     assert rel_err(out["m3mr_msh"], ref["m3mr"]) < 1e-13          # moments unchanged
     assert f["nm"] == 3 and list(f["indexes"]) == list(idx)
     assert (f["ncomb"], f["ispsym"], f["nzones"], f["npts"]) == (1, 0, M.ntiles, em["fi"].size)
@@ -183,7 +184,7 @@ def test_front_mesher_visual_modal_and_nodal(gpu_lib, tmp_path):
     d = (4 - 1) * p.nlibs + 1
     ph = p.phi[d - 1, M.modes - 1]
     coef = np.einsum("m,n,p->mnp", ph, ph, ph).reshape(-1, 1, order="F")
-    en = M.export(coef=coef)
+    en = M.export(levels=lv, coef=coef)
     d2 = tmp_path / "nodal"
     d2.mkdir()
     _, f = _front_export(p, d2, "visualn=4,1", "sym=4", fname="bsaexport_msh.brn")
@@ -197,7 +198,7 @@ def test_front_mesher_brm_export_base(gpu_lib, tmp_path):
     p = _damped("tiny_uvw", 20)
     M = _front_mesher(p)
     nm3 = M.nm ** 3
-    e = M.export()
+    e = M.export(levels=M.adapt(1e-3)[0])
     M.close()
     _, f = _front_export(p, tmp_path, "brmbase", nval=nm3)
     assert f["nm"] == M.nm and list(f["indexes"]) == list(M.modes)
@@ -229,7 +230,7 @@ def test_front_export_callback_replaces_files(gpu_lib, tmp_path):
     assert not (d1 / "bsaexport_cls.brm").exists()
     q = _damped("tiny_uvw", 20)
     M = _front_mesher(q)
-    e = M.export()
+    e = M.export(levels=M.adapt(1e-3)[0])
     M.close()
     d2 = tmp_path / "msh"
     d2.mkdir()

@@ -21,7 +21,7 @@ from tests.conftest import load_golden, rel_err
 
 pytestmark = pytest.mark.gpu
 
-TINY = ["tiny_u", "tiny_uvw", "tiny_dav"]
+TINY = ["tiny_u", "tiny_uvw", "tiny_dav", "tiny_2dof"]
 
 
 def _nm(g):

@@ -11,7 +11,7 @@ from oracle import oracle as O
 from tests.cases import CASES
 from tests.conftest import load_golden, rel_err
 
-GOLDEN_CASES = ["tiny_u", "tiny_uvw", "tiny_dav", "config1"]
+GOLDEN_CASES = ["tiny_u", "tiny_uvw", "tiny_dav", "config1", "tiny_2dof"]
 
 
 @pytest.mark.parametrize("name", GOLDEN_CASES)
