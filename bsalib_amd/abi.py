@@ -64,7 +64,7 @@ EXPORTS = ["bsahip_init", "bsahip_finalise", "bsahip_last_error", "bsahip_versio
            "bsahip_mesher_load", "bsahip_batch_create", "bsahip_batch_autotune",
            "bsahip_batch_run", "bsahip_batch_info", "bsahip_batch_destroy",
            "bsahip_mesher_destroy", "bsahip_modal_psd_points", "bsahip_modal_recombination",
-           "bsahip_mesher_adapt"]
+           "bsahip_mesher_adapt", "bsahip_dgemm_strided_batched"]
 
 
 def lib():

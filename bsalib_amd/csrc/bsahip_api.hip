@@ -7,6 +7,7 @@
 // plane epilogue -> modal PSD moments -> copy of the moments).  There is no
 // CPU fallback: without a usable GPU every entry point returns BSAHIP_EDEVICE.
 #include "kernels.hip"
+#include "gemm.hip"
 #include "pod.hip"
 #include "diag.hip"
 #include "../../include/bsahip.h"
@@ -327,6 +328,20 @@ int blas_handle()
    return 0;
 }
 
+// C_b = alpha op(A_b) op(B_b) + beta C_b on the library stream (k_dgemm_sb).
+int dgemm_sb(int transa, int transb, int M, int N, int K, double alpha, const double *A, long lda,
+             long sA, const double *B, long ldb, long sB, double beta, double *C, long ldc,
+             long sC, int batch)
+{
+   if (M <= 0 || N <= 0 || batch <= 0) return 0;
+   if (batch > 65535) return fail(BSAHIP_EUNSUPPORTED, "GEMM batch above 65535");
+   dim3 gr((unsigned)((M + 63) / 64), (unsigned)((N + 63) / 64), (unsigned)batch);
+   k_dgemm_sb<<<gr, 256, 0, g.s>>>(transa, transb, M, N, K, alpha, A, lda, sA, B, ldb, sB, beta, C,
+                                    ldc, sC);
+   HIPC(hipGetLastError());
+   return 0;
+}
+
 // A_K(y, i; f) = (Psi_a V) diag(w) V^T for the truncated POD of S(f) (see
 // pod.hip), written into the table layout of k_project.  sq: sqrt(S_i) at fl.
 int pod_project(Bufs &B, const Prob &D, const double *sq, const double *fl, int nfl,
@@ -349,7 +364,6 @@ int pod_project(Bufs &B, const Prob &D, const double *sq, const double *fl, int 
    CHK(B.get("pod_info", (size_t)cap, &info));
    CHK(B.get("pod_kept", (size_t)cap, &kept));
    std::vector<int> hk((size_t)cap);
-   const double one = 1.0, zero = 0.0;
    for (long b0 = 0; b0 < total; b0 += cap) {
       const int nb = (int)std::min<long>(cap, total - b0);
       k_pod_smat<<<dim3((unsigned)((nn2 + 255) / 256), nb), 256, 0, g.s>>>(D.lnr, sq, fl, nfl, n, b0, S);
@@ -363,17 +377,13 @@ int pod_project(Bufs &B, const Prob &D, const double *sq, const double *fl, int 
       for (long bb = b0; bb < b0 + nb;) {
          const int c = (int)(bb / nfl);
          const long be = std::min<long>(b0 + nb, (long)(c + 1) * nfl);
-         BLASC(rocblas_dgemm_strided_batched(
-            g.blas, rocblas_operation_transpose, rocblas_operation_none, nm, n, n, &one,
-            D.pa_t + (size_t)c * mn, n, 0, S + (size_t)(bb - b0) * nn2, n, (rocblas_stride)nn2, &zero,
-            T + (size_t)(bb - b0) * mn, nm, (rocblas_stride)mn, (int)(be - bb)));
+         CHK(dgemm_sb(1, 0, nm, n, n, 1.0, D.pa_t + (size_t)c * mn, n, 0, S + (size_t)(bb - b0) * nn2,
+                      n, (long)nn2, 0.0, T + (size_t)(bb - b0) * mn, nm, (long)mn, (int)(be - bb)));
          bb = be;
       }
       k_pod_scale<<<dim3((unsigned)((mn + 255) / 256), nb), 256, 0, g.s>>>(T, w, nm, n);
       HIPC(hipGetLastError());
-      BLASC(rocblas_dgemm_strided_batched(g.blas, rocblas_operation_none, rocblas_operation_transpose,
-                                          nm, n, n, &one, T, nm, (rocblas_stride)mn, S, n,
-                                          (rocblas_stride)nn2, &zero, A, nm, (rocblas_stride)mn, nb));
+      CHK(dgemm_sb(0, 1, nm, n, n, 1.0, T, nm, (long)mn, S, n, (long)nn2, 0.0, A, nm, (long)mn, nb));
       k_pod_scatter<<<dim3((unsigned)((mn + 255) / 256), nb), 256, 0, g.s>>>(A, nm, n, nfl, b0, D.nnlp,
                                                                          rows_per_mode, U, ldU);
       HIPC(hipGetLastError());
@@ -904,7 +914,6 @@ namespace {
 int diag_tables(Bufs &B, const Prob &D, int nh, double df, bool with_T, double **X, double **T,
                 double2 **H)
 {
-   CHK(blas_handle());
    const int nft = 2 * nh - 1, nm = D.nm, K = D.K;
    std::vector<double> fl(nh);
    for (int u = 0; u < nh; ++u) fl[u] = (double)u * df;
@@ -924,10 +933,8 @@ int diag_tables(Bufs &B, const Prob &D, int nh, double df, bool with_T, double *
    *T = nullptr;
    if (with_T) {
       CHK(B.get("dg_T", (size_t)nm * nh * nh, T));
-      const double one = 1.0, zero = 0.0;
-      BLASC(rocblas_dgemm_strided_batched(g.blas, rocblas_operation_none, rocblas_operation_transpose,
-                                          nh, nh, K, &one, *X, nh, 0, Y, nh, (rocblas_stride)K * nh,
-                                          &zero, *T, nh, (rocblas_stride)nh * nh, nm));
+      CHK(dgemm_sb(0, 1, nh, nh, K, 1.0, *X, nh, 0, Y, nh, (long)K * nh, 0.0, *T, nh, (long)nh * nh,
+                   nm));
    }
    return 0;
 }
@@ -1232,6 +1239,35 @@ int bsahip_selftest_mfma_layout(double *out512)
    k_mfma_f64_layout<<<1, 64, 0, g.s>>>(d);
    HIPC(hipGetLastError());
    HIPC(hipMemcpyAsync(out512, d, 512 * sizeof(double), hipMemcpyDeviceToHost, g.s));
+   HIPC(hipStreamSynchronize(g.s));
+   return 0;
+}
+
+int bsahip_dgemm_strided_batched(int32_t transa, int32_t transb, int32_t M, int32_t N, int32_t K,
+                                 double alpha, const double *A, int64_t lda, int64_t sA,
+                                 const double *B, int64_t ldb, int64_t sB, double beta, double *C,
+                                 int64_t ldc, int64_t sC, int32_t batch)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   CHK(ensure_init());
+   if (M < 0 || N < 0 || K < 0 || batch < 1 || !A || !B || !C || (transa != 0 && transa != 1) ||
+       (transb != 0 && transb != 1) || lda < (transa ? K : M) || ldb < (transb ? N : K) || ldc < M ||
+       sA < 0 || sB < 0 || sC < 0)
+      return fail(BSAHIP_EINVAL, "bad GEMM arguments");
+   // extents of the host arrays (column-major, last batch included)
+   const size_t na = (size_t)sA * (batch - 1) + (size_t)lda * (transa ? M : K);
+   const size_t nb = (size_t)sB * (batch - 1) + (size_t)ldb * (transb ? K : N);
+   const size_t nc = (size_t)sC * (batch - 1) + (size_t)ldc * N;
+   Bufs &Bf = oneshot_bufs();
+   double *dA, *dB, *dC;
+   CHK(Bf.get("gemm_A", std::max<size_t>(na, 1), &dA));
+   CHK(Bf.get("gemm_B", std::max<size_t>(nb, 1), &dB));
+   CHK(Bf.get("gemm_C", std::max<size_t>(nc, 1), &dC));
+   HIPC(hipMemcpyAsync(dA, A, na * sizeof(double), hipMemcpyHostToDevice, g.s));
+   HIPC(hipMemcpyAsync(dB, B, nb * sizeof(double), hipMemcpyHostToDevice, g.s));
+   HIPC(hipMemcpyAsync(dC, C, nc * sizeof(double), hipMemcpyHostToDevice, g.s));
+   CHK(dgemm_sb(transa, transb, M, N, K, alpha, dA, lda, sA, dB, ldb, sB, beta, dC, ldc, sC, batch));
+   HIPC(hipMemcpyAsync(C, dC, nc * sizeof(double), hipMemcpyDeviceToHost, g.s));
    HIPC(hipStreamSynchronize(g.s));
    return 0;
 }

@@ -248,6 +248,19 @@ int bsahip_selftest_mfma_bf16_layout(float *out768);
 int bsahip_selftest_mfma_f32_layout(float *out512);
 
 /*
+ * The library's FP64 strided-batched GEMM on host arrays (column-major):
+ * C_b = alpha op(A_b) op(B_b) + beta C_b, b = 0 .. batch-1, op = transpose when
+ * trans* = 1, X_b = X + b sX (sX = 0 shares one matrix).  The same kernel
+ * (v_mfma_f64_16x16x4f64, 64 x 64 tiles) forms the ONLY_DIAG model's per-mode
+ * T_m = S W_m S^T and the truncated-POD tables on the device.  No reference
+ * counterpart.
+ */
+int bsahip_dgemm_strided_batched(int32_t transa, int32_t transb, int32_t M, int32_t N, int32_t K,
+                                 double alpha, const double *A, int64_t lda, int64_t sA,
+                                 const double *B, int64_t ldb, int64_t sB, double beta, double *C,
+                                 int64_t ldc, int64_t sC, int32_t batch);
+
+/*
  * Last measured device time (milliseconds, HIP events on the library stream)
  * of the dominant kernel of the previous call, and its launch count.
  */
