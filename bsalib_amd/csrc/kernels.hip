@@ -742,9 +742,10 @@ __global__ __launch_bounds__(256, 4) void k_gsym_f64(GsymArgs a)
 // Operands come from the FP32 copy of the A table; the row operand is scaled
 // by Psi_b(x,k) in FP32 before the split.  Same plane epilogue (FP64) as
 // k_gsym_f64.  Workgroup tile 128x128, 4 waves of 64x64 (4x4 blocks of
-// 16x16), K chunk 32 double-buffered in LDS.  LDS rows hold four 16-byte
-// k-groups stored at slot g ^ ((row >> 2) & 3): the 16 lanes of a fragment read
-// touch 16 distinct 16-byte bank groups.
+// 16x16), K chunk 32 double-buffered in LDS.  LDS rows (64 bytes) hold four
+// 16-byte k-groups stored at slot g ^ ((row >> 1) & 3): the 8 rows 0-7 (8-15)
+// of a fragment read fill the 8 distinct 16-byte slots of a 128-byte line
+// (SQ_LDS_BANK_CONFLICT reads zero for this kernel).
 // ---------------------------------------------------------------------------
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf2v __attribute__((ext_vector_type(2)));
@@ -767,7 +768,7 @@ __device__ __forceinline__ void split_bf16(float v, uint32_t &hi, uint32_t &lo)
    lo = bf16_rne_bits(r);
 }
 
-__device__ __forceinline__ int swz(int row, int g) { return row * 4 + (g ^ ((row >> 2) & 3)); }
+__device__ __forceinline__ int swz(int row, int g) { return row * 4 + (g ^ ((row >> 1) & 3)); }
 
 struct GsymArgs16 {
    const float *UR;       // [rows][ldk]  A table, FP32, K contiguous (row operand source)
@@ -918,13 +919,13 @@ __global__ __launch_bounds__(256, 2) void k_gsym_bf16x3(GsymArgs a, GsymArgs16 b
 // in LDS (two FP32 planes, 64 KB).  Within a chunk, MFMA step s (0..7) takes
 // k = 8 (l/16) + s for lane l in both operands (any assignment of the chunk's
 // 32 k to (step, lane group) gives the same sum), so a lane reads its 8 k of
-// one row as two 16-byte LDS loads.  LDS rows hold eight 16-byte k-groups
-// stored at slot q ^ ((row >> 1) & 7): the 16 rows of a fragment read hit 16
-// distinct (row parity, slot) pairs.
+// one row as two 16-byte LDS loads.  LDS rows (128 bytes) hold eight 16-byte
+// k-groups stored at slot q ^ (row & 7): 8 consecutive rows of a fragment
+// read fill the 8 distinct 16-byte slots of a 128-byte line.
 // ---------------------------------------------------------------------------
 #define KC4 32
 
-__device__ __forceinline__ int swz8(int row, int q) { return row * 8 + (q ^ ((row >> 1) & 7)); }
+__device__ __forceinline__ int swz8(int row, int q) { return row * 8 + (q ^ (row & 7)); }
 
 __device__ __forceinline__ void gsym_f32_item(const GsymArgs &a, const GsymArgs16 &b, int x, long tp)
 {
