@@ -920,8 +920,9 @@ __global__ __launch_bounds__(256, 2) void k_gsym_bf16x3(GsymArgs a, GsymArgs16 b
 // k = 8 (l/16) + s for lane l in both operands (any assignment of the chunk's
 // 32 k to (step, lane group) gives the same sum), so a lane reads its 8 k of
 // one row as two 16-byte LDS loads.  LDS rows (128 bytes) hold eight 16-byte
-// k-groups stored at slot q ^ (row & 7): 8 consecutive rows of a fragment
-// read fill the 8 distinct 16-byte slots of a 128-byte line.
+// k-groups stored at slot q ^ (row & 7).  Measured at NN=1000, NM=30: half the
+// bank-conflict cycles of slot q ^ ((row >> 1) & 7) and 2.4 % faster; slot
+// q ^ ((row ^ (row >> 3)) & 7) has none but ran 1.7 % slower than this one.
 // ---------------------------------------------------------------------------
 #define KC4 32
 
