@@ -169,13 +169,6 @@ __global__ __launch_bounds__(512) void k_bfm_batch(
    }
 }
 
-// ---------------------------------------------------------------------------
-// Tile integration: bilinear BFM on the BRM grid, exact FRF, trapezoid weights.
-// BFM grid nb1 x nb2 (point q = i + j*ldq, values bf[q*nm3 + mnp]); each BFM
-// cell holds s1 x s2 BRM cells.  BRM point (a, b): f1 = f1a + a*h1,
-// f2 = f2a + b*h2.  Adds sum w*B into m3f and sum w*B*Re[H_m(f1)H_n(f2)H_p*(f1+f2)]
-// into m3r.  grid (ceil(nm3/256), nchunk), block 256.
-// ---------------------------------------------------------------------------
 // FRF table H[i*nm + m] = 1/(K_m - w^2 M_m + i w C_mm), w = 2 pi (f0 + i h)
 __global__ void k_frf_axis(double f0, double h, int n, const double *__restrict__ Mm,
                            const double *__restrict__ Km, const double *__restrict__ Cm, int nm,
@@ -202,42 +195,101 @@ __device__ __forceinline__ void msh_entry(int e, int nm, int diag, int &m, int &
    }
 }
 
-__global__ __launch_bounds__(256) void k_msh_integrate(
-   const double *__restrict__ bf, long ldq, int nm, int nc, int diag, int nb1, int nb2, int s1, int s2,
-   const double2 *__restrict__ H1, const double2 *__restrict__ H2,
-   const double2 *__restrict__ H3, double f3a, double h1, double h2, double wscale,
-   const double *__restrict__ Mm, const double *__restrict__ Km, const double *__restrict__ Cm,
-   double *__restrict__ m3f, double *__restrict__ m3r)
+// ---------------------------------------------------------------------------
+// The post-mesh of many tiles in one launch each (bsahip_mesher_run):
+// k_frf_segs fills the FRF tables of every tile, k_msh_copy_pts places the
+// resident level-0 BFM into the fine grids of refined tiles, k_msh_integrate
+// runs the quadrature over a list of (tile, cell range) items.
+// ---------------------------------------------------------------------------
+// FRF segment s: n points f0 + i h, out H[off + i*nm + m]; segment of thread t
+// by binary search over the entry offsets eoff (n*nm per segment).
+struct MshSeg {
+   double f0, h;
+   long off;
+};
+
+__global__ void k_frf_segs(const MshSeg *__restrict__ seg, const long *__restrict__ eoff, int nseg,
+                           const double *__restrict__ Mm, const double *__restrict__ Km,
+                           const double *__restrict__ Cm, int nm, double2 *__restrict__ H)
 {
-   // H1[a*nm + m] at f1 = f1a + a h1, H2[b*nm + n] at f2 = f2a + b h2; H3[(a+b)*nm + p]
-   // at f1 + f2 when h1 == h2 (H3 != NULL), else H_p(f1+f2) is evaluated here
+   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= eoff[nseg]) return;
+   int lo = 0, hi = nseg - 1;                      // eoff[lo] <= t < eoff[lo + 1]
+   while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (eoff[mid] <= t) lo = mid;
+      else hi = mid - 1;
+   }
+   const long e = t - eoff[lo];
+   const long i = e / nm;
+   const int m = (int)(e % nm);
+   const double w = 2.0 * M_PI * (seg[lo].f0 + (double)i * seg[lo].h);
+   const double r = Km[m] - w * w * Mm[m], im = w * Cm[m], d = r * r + im * im;
+   H[seg[lo].off + e] = make_double2(r / d, -im / d);
+}
+
+// fine[dst[q]*nm3 + e] = lv0[src[q]*nm3 + e]; grid (ceil(nm3/256), npairs)
+__global__ void k_msh_copy_pts(const double *__restrict__ lv0, const long *__restrict__ src,
+                               const long *__restrict__ dst, int nm3, double *__restrict__ fine)
+{
+   const int e = blockIdx.x * blockDim.x + threadIdx.x;
+   if (e >= nm3) return;
+   const long q = blockIdx.y;
+   fine[dst[q] * nm3 + e] = lv0[src[q] * nm3 + e];
+}
+
+// One tile of a batched post-mesh.
+struct MshTileRun {
+   const double *bf;           // the tile's BFM grid (lv0 or fine)
+   double *m3r;                // response accumulator of this tile
+   long ldq;
+   long h1, h2, h3;            // FRF table offsets (h3 < 0: f1 + f2 evaluated here)
+   int nb1, nb2, s1, s2;
+   double f3a, dh1, dh2, ws;
+};
+
+// Tile quadrature: bilinear BFM on the BRM grid, exact FRF, trapezoid
+// weights.  A tile's BFM grid is nb1 x nb2 (point q = i + j*ldq, values
+// bf[q*nm3 + mnp]); each BFM cell holds s1 x s2 BRM cells.  BRM point (a, b):
+// f1 = f1a + a*dh1, f2 = f2a + b*dh2.  Adds sum w*B into m3f and
+// sum w*B*Re[H_m(f1)H_n(f2)H_p*(f1+f2)] into the tile's m3r.
+// grid (nitems * bx), block 256: block -> (item blockIdx.x / bx, entries
+// (blockIdx.x % bx) * 256 ..); item q covers cells [c0, c1) of tile it[q]
+__global__ __launch_bounds__(256) void k_msh_integrate(
+   const MshTileRun *__restrict__ tr, const int *__restrict__ it, const int2 *__restrict__ cr,
+   int bx, int nm, int nc, int diag, const double2 *__restrict__ H,
+   const double *__restrict__ Mm, const double *__restrict__ Km, const double *__restrict__ Cm,
+   double *__restrict__ m3f)
+{
    const int nm3 = nc;
-   const int mnp = blockIdx.x * blockDim.x + threadIdx.x;
+   const long q = blockIdx.x / bx;
+   const int mnp = (int)(blockIdx.x % bx) * blockDim.x + threadIdx.x;
    if (mnp >= nm3) return;
+   const MshTileRun T = tr[it[q]];
+   const int c0 = cr[q].x, c1 = cr[q].y;
    int m, n, p;
    msh_entry(mnp, nm, diag, m, n, p);
-   const int nc1 = nb1 - 1, nc2 = nb2 - 1, ncell = nc1 * nc2;
-   const int per = (ncell + gridDim.y - 1) / gridDim.y;
-   const int c0 = blockIdx.y * per, c1 = min(ncell, c0 + per);
-   const int NA = nc1 * s1, NB = nc2 * s2;          // last BRM index per axis
+   const int nc1 = T.nb1 - 1, nc2 = T.nb2 - 1;
+   const int NA = nc1 * T.s1, NB = nc2 * T.s2;
    const double Mp = Mm[p], Kp = Km[p], Cp = Cm[p];
-   const double inv1 = 1.0 / s1, inv2 = 1.0 / s2;
+   const double inv1 = 1.0 / T.s1, inv2 = 1.0 / T.s2;
+   const double2 *H1 = H + T.h1, *H2 = H + T.h2, *H3 = T.h3 >= 0 ? H + T.h3 : nullptr;
    double sf = 0.0, sr = 0.0;
    for (int c = c0; c < c1; ++c) {
       const int ci = c % nc1, cj = c / nc1;
-      const size_t q00 = (size_t)ci + (size_t)cj * ldq;
-      const double b00 = bf[q00 * nm3 + mnp], b10 = bf[(q00 + 1) * nm3 + mnp];
-      const double b01 = bf[(q00 + ldq) * nm3 + mnp], b11 = bf[(q00 + ldq + 1) * nm3 + mnp];
-      const int ea = (ci == nc1 - 1) ? s1 : s1 - 1;   // owned BRM points per axis
-      const int eb = (cj == nc2 - 1) ? s2 : s2 - 1;
+      const size_t q00 = (size_t)ci + (size_t)cj * T.ldq;
+      const double b00 = T.bf[q00 * nm3 + mnp], b10 = T.bf[(q00 + 1) * nm3 + mnp];
+      const double b01 = T.bf[(q00 + T.ldq) * nm3 + mnp], b11 = T.bf[(q00 + T.ldq + 1) * nm3 + mnp];
+      const int ea = (ci == nc1 - 1) ? T.s1 : T.s1 - 1;   // owned BRM points per axis
+      const int eb = (cj == nc2 - 1) ? T.s2 : T.s2 - 1;
       for (int bb = 0; bb <= eb; ++bb) {
-         const int b = cj * s2 + bb;
+         const int b = cj * T.s2 + bb;
          const double wb = (b == 0 || b == NB) ? 0.5 : 1.0;
          const double sv = bb * inv2;
          const double2 h2v = H2[(size_t)b * nm + n];
          const double e0 = (1.0 - sv) * b00 + sv * b01, e1 = (1.0 - sv) * b10 + sv * b11;
          for (int aa = 0; aa <= ea; ++aa) {
-            const int a = ci * s1 + aa;
+            const int a = ci * T.s1 + aa;
             const double wa = (a == 0 || a == NA) ? 0.5 : 1.0;
             const double t = aa * inv1;
             const double bv = (1.0 - t) * e0 + t * e1;
@@ -246,7 +298,7 @@ __global__ __launch_bounds__(256) void k_msh_integrate(
             if (H3) {
                h3v = H3[(size_t)(a + b) * nm + p];
             } else {
-               const double w3 = 2.0 * M_PI * (f3a + a * h1 + b * h2);
+               const double w3 = 2.0 * M_PI * (T.f3a + a * T.dh1 + b * T.dh2);
                const double r = Kp - w3 * w3 * Mp, im = w3 * Cp, d = r * r + im * im;
                h3v = make_double2(r / d, -im / d);
             }
@@ -256,8 +308,8 @@ __global__ __launch_bounds__(256) void k_msh_integrate(
          }
       }
    }
-   atomicAdd(m3f + mnp, sf * wscale);
-   atomicAdd(m3r + mnp, sr * wscale);
+   atomicAdd(m3f + mnp, sf * T.ws);
+   atomicAdd(T.m3r + mnp, sr * T.ws);
 }
 
 // ---------------------------------------------------------------------------
@@ -315,18 +367,6 @@ __global__ __launch_bounds__(256) void k_msh_export(
       }
       val[(size_t)pt * ncol + c] = v;
    }
-}
-
-// Level-0 points of a tile into its level-l grid:
-// fine[(i*l + j*l*nb1)] = lv0[i + j*ldq].  grid (ceil(nm3/256), (n1+1)*(n2+1))
-__global__ void k_lv0_to_fine(const double *__restrict__ lv0, long ldq, int n1, int l, int nb1,
-                              int nm3, double *__restrict__ fine)
-{
-   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-   if (t >= nm3) return;
-   const int q = blockIdx.y, i = q % (n1 + 1), j = q / (n1 + 1);
-   fine[((size_t)i * l + (size_t)j * l * nb1) * nm3 + t] =
-      lv0[((size_t)i + (size_t)j * ldq) * nm3 + t];
 }
 
 // Feature stencil of a tile's level-0 BFM: out[s] = sqrt(sum_mnp B^2) at the
@@ -748,34 +788,102 @@ int msh_frf_tables(bsahip_mesher *M, const MshTile &t, MshBrmGrid &G)
    return 0;
 }
 
-int msh_integrate(bsahip_mesher *M, const MshTile &t, int l, const double *bf, long ldq, int nb1,
-                  int nb2, double *m3f, double *m3r, long &brm_pts)
-{
-   Prob &D = M->D;
-   MshBrmGrid G = msh_brm_grid(t, l);
-   CHK(msh_frf_tables(M, t, G));
-   const double ws = (2.0 * M_PI * G.h1) * (2.0 * M_PI * G.h2);
-   const int nm3 = M->nc;
-   const int ncell = (nb1 - 1) * (nb2 - 1);
-   const int bx = (nm3 + 255) / 256;
-   const int nchunk = std::max(1, std::min(ncell, std::max(1, 4096 / bx)));
-   dim3 gs(bx, nchunk);
-   k_msh_integrate<<<gs, 256, 0, g.s>>>(bf, ldq, D.nm, M->nc, M->diag ? 1 : 0, nb1, nb2, G.s1,
-                                         G.s2, G.H1, G.H2, G.H3,
-                                         t.a1 + t.a2, G.h1, G.h2, ws, D.Mm, D.Km, D.Cm, m3f, m3r);
-   HIPC(hipGetLastError());
-   brm_pts += (long)(G.NA + 1) * (G.NB + 1);
-   return 0;
-}
-
 // Per-tile visitor of the post-mesh: the tile's index, its level and its BFM
 // grid (nb1 x nb2 points, bf[(i + j*ldq)*nm3 + mnp], on the device).
 using MshVisit = std::function<int(size_t, const MshTile &, int, const double *, long, int, int)>;
 
+// One visited tile whose quadrature is still to be launched.
+struct MshRef {
+   size_t k;
+   int l, nb1, nb2;
+   const double *bf;
+   long ldq;
+};
+
+// The quadrature of many visited tiles in three launches: FRF tables of every
+// tile (k_frf_segs), then k_msh_integrate over (tile, cell range)
+// items; m3r_of(k) is tile k's response accumulator.
+int msh_integrate_many(bsahip_mesher *M, const std::vector<MshRef> &refs, double *m3f,
+                       const std::function<double *(size_t)> &m3r_of, long &brm_pts)
+{
+   if (refs.empty()) return 0;
+   Prob &D = M->D;
+   const int nm3 = M->nc, bx = (nm3 + 255) / 256;
+   std::vector<MshSeg> seg;
+   std::vector<long> eoff(1, 0);
+   std::vector<MshTileRun> tr(refs.size());
+   long ncells = 0;
+   auto add_seg = [&](double f0, double h, int n) -> long {
+      const long off = eoff.back();
+      seg.push_back(MshSeg{f0, h, off});
+      eoff.push_back(off + (long)n * D.nm);
+      return off;
+   };
+   for (size_t r = 0; r < refs.size(); ++r) {
+      const MshRef &R = refs[r];
+      const MshTile &t = M->tiles[R.k];
+      const MshBrmGrid G = msh_brm_grid(t, R.l);
+      const bool eq = std::fabs(G.h1 - G.h2) <= 1e-13 * std::max(std::fabs(G.h1), std::fabs(G.h2));
+      MshTileRun &T = tr[r];
+      T.bf = R.bf;
+      T.m3r = m3r_of(R.k);
+      T.ldq = R.ldq;
+      T.h1 = add_seg(t.a1, G.h1, G.NA + 1);
+      T.h2 = add_seg(t.a2, G.h2, G.NB + 1);
+      T.h3 = eq ? add_seg(t.a1 + t.a2, G.h1, G.NA + G.NB + 1) : -1;
+      T.nb1 = R.nb1;
+      T.nb2 = R.nb2;
+      T.s1 = G.s1;
+      T.s2 = G.s2;
+      T.f3a = t.a1 + t.a2;
+      T.dh1 = G.h1;
+      T.dh2 = G.h2;
+      T.ws = (2.0 * M_PI * G.h1) * (2.0 * M_PI * G.h2);
+      ncells += (long)(R.nb1 - 1) * (R.nb2 - 1);
+      brm_pts += (long)(G.NA + 1) * (G.NB + 1);
+   }
+   // items: each tile's cells cut in ranges of at most `per` cells, so that
+   // the launch holds some 8192 (item, entry block) workgroups or more
+   const long per = std::max<long>(1, ncells * bx / 8192);
+   std::vector<int> it;
+   std::vector<int2> cr;
+   for (size_t r = 0; r < refs.size(); ++r) {
+      const int nc = (refs[r].nb1 - 1) * (refs[r].nb2 - 1);
+      for (long c0 = 0; c0 < nc; c0 += per) {
+         it.push_back((int)r);
+         cr.push_back(make_int2((int)c0, (int)std::min<long>(nc, c0 + per)));
+      }
+   }
+   if ((long)it.size() * bx > 0x7fffffffL)
+      return fail(BSAHIP_EUNSUPPORTED, "Mesher post-mesh launch above 2^31 workgroups");
+   MshSeg *dseg;
+   long *doff;
+   double2 *H;
+   MshTileRun *dtr;
+   int *dit;
+   int2 *dcr;
+   CHK(upload(M->B, "m_seg", seg, &dseg));
+   CHK(upload(M->B, "m_segoff", eoff, &doff));
+   CHK(M->B.get("m_H", (size_t)eoff.back(), &H));
+   CHK(upload(M->B, "m_tiles_run", tr, &dtr));
+   CHK(upload(M->B, "m_items", it, &dit));
+   CHK(upload(M->B, "m_ranges", cr, &dcr));
+   k_frf_segs<<<(unsigned)((eoff.back() + 255) / 256), 256, 0, g.s>>>(dseg, doff, (int)seg.size(),
+                                                                       D.Mm, D.Km, D.Cm, D.nm, H);
+   HIPC(hipGetLastError());
+   k_msh_integrate<<<(unsigned)(it.size() * bx), 256, 0, g.s>>>(
+      dtr, dit, dcr, bx, D.nm, M->nc, M->diag ? 1 : 0, H, D.Mm, D.Km, D.Cm, m3f);
+   HIPC(hipGetLastError());
+   return 0;
+}
+
 // Post-mesh traversal shared by the run and the export: level-1 tiles read the
 // resident level-0 BFM in place; refined tiles are evaluated in groups whose
-// fine grids fit the staging buffer.  Returns the refinement evaluations.
-int msh_post(bsahip_mesher *m, const int32_t *levels, const MshVisit &visit, long &evals_post)
+// fine grids fit the staging buffer.  `done`, when given, is called once the
+// tiles visited so far may be consumed (after each group, before its staging
+// buffer is reused, and at the end).  Returns the refinement evaluations.
+int msh_post(bsahip_mesher *m, const int32_t *levels, const MshVisit &visit, long &evals_post,
+             const std::function<int()> &done = nullptr)
 {
    const size_t nm3 = (size_t)m->nc, nt = m->tiles.size();
    const size_t cap = std::max<size_t>(66049, ((size_t)1 << 29) / nm3);   // points (4 GiB)
@@ -784,23 +892,44 @@ int msh_post(bsahip_mesher *m, const int32_t *levels, const MshVisit &visit, lon
    std::vector<size_t> grp, goff;
    size_t used = 0;
    evals_post = 0;
+   std::vector<long> csrc, cdst;
    auto flush = [&]() -> int {
       if (grp.empty()) return 0;
       double *fine;
       CHK(m->B.get("m_fine", used * nm3, &fine));
       CHK(msh_eval(m, e1, e2, slot, fine));
       evals_post += (long)e1.size();
+      // the level-0 points of every refined tile into its fine grid, one launch
+      csrc.clear();
+      cdst.clear();
+      for (size_t k = 0; k < grp.size(); ++k) {
+         const MshTile &t = m->tiles[grp[k]];
+         const int l = levels ? levels[grp[k]] : t.rule;
+         const int nb1 = t.n1 * l + 1;
+         const long base = msh_lv0_at(m, t);
+         for (int j = 0; j <= t.n2; ++j)
+            for (int i = 0; i <= t.n1; ++i) {
+               csrc.push_back(base + i + (long)j * m->nx);
+               cdst.push_back((long)goff[k] + (long)i * l + (long)j * l * nb1);
+            }
+      }
+      if (csrc.size() > 65535u * 65535u) return fail(BSAHIP_EUNSUPPORTED, "too many level-0 points");
+      long *dsrc, *ddst;
+      CHK(upload(m->B, "m_csrc", csrc, &dsrc));
+      CHK(upload(m->B, "m_cdst", cdst, &ddst));
+      for (size_t q0 = 0; q0 < csrc.size(); q0 += 65535) {
+         const unsigned nq = (unsigned)std::min<size_t>(65535, csrc.size() - q0);
+         dim3 gc((unsigned)((nm3 + 255) / 256), nq);
+         k_msh_copy_pts<<<gc, 256, 0, g.s>>>(m->lv0, dsrc + q0, ddst + q0, (int)nm3, fine);
+         HIPC(hipGetLastError());
+      }
       for (size_t k = 0; k < grp.size(); ++k) {
          const MshTile &t = m->tiles[grp[k]];
          const int l = levels ? levels[grp[k]] : t.rule;
          const int nb1 = t.n1 * l + 1, nb2 = t.n2 * l + 1;
-         double *fg = fine + goff[k] * nm3;
-         dim3 gc((unsigned)((nm3 + 255) / 256), (unsigned)((t.n1 + 1) * (t.n2 + 1)));
-         k_lv0_to_fine<<<gc, 256, 0, g.s>>>(m->lv0 + msh_lv0_at(m, t) * nm3, m->nx, t.n1, l, nb1,
-                                            (int)nm3, fg);
-         HIPC(hipGetLastError());
-         CHK(visit(grp[k], t, l, fg, nb1, nb1, nb2));
+         CHK(visit(grp[k], t, l, fine + goff[k] * nm3, nb1, nb1, nb2));
       }
+      if (done) CHK(done());
       HIPC(hipStreamSynchronize(g.s));
       grp.clear();
       goff.clear();
@@ -833,7 +962,9 @@ int msh_post(bsahip_mesher *m, const int32_t *levels, const MshVisit &visit, lon
       goff.push_back(used);
       used += np;
    }
-   return flush();
+   CHK(flush());
+   if (done) CHK(done());
+   return 0;
 }
 
 int msh_check_levels(const bsahip_mesher *m, const int32_t *levels)
@@ -1004,11 +1135,20 @@ int bsahip_mesher_run_tiles(bsahip_mesher *m, const int32_t *levels, double *til
    CHK(fill(acc, nacc, 0.0));
    long evals_post = 0, brm = 0;
    auto slot_r = [&](size_t k) { return tile_m3mr ? acc + 2 * nm3 + k * nm3 : acc + nm3; };
-   CHK(msh_post(m, levels,
-                [&](size_t k, const MshTile &t, int l, const double *bf, long ldq, int nb1, int nb2) {
-                   return msh_integrate(m, t, l, bf, ldq, nb1, nb2, acc, slot_r(k), brm);
-                },
-                evals_post));
+   // tiles are collected as visited and integrated together (msh_integrate_many)
+   std::vector<MshRef> refs;
+   CHK(msh_post(
+      m, levels,
+      [&](size_t k, const MshTile &, int l, const double *bf, long ldq, int nb1, int nb2) {
+         refs.push_back(MshRef{k, l, nb1, nb2, bf, ldq});
+         return 0;
+      },
+      evals_post,
+      [&]() -> int {
+         CHK(msh_integrate_many(m, refs, acc, slot_r, brm));
+         refs.clear();
+         return 0;
+      }));
    if (tile_m3mr) {
       HIPC(hipMemcpyAsync(tile_m3mr, acc + 2 * nm3, nt * nm3 * sizeof(double),
                           hipMemcpyDeviceToHost, g.s));
@@ -1054,18 +1194,24 @@ int bsahip_mesher_adapt(bsahip_mesher *m, double tol, int32_t *levels, int64_t *
    double M = 0.0;
    for (size_t q = 0; q < nm3; ++q) M = std::max(M, std::fabs(m3r[q]));
    const double budget = 0.5 * tol * M;
-   // error estimate of a tile's contribution at level L from the change since
-   // L/2 (bilinear interpolation is second order: err(L) ~ (c_L - c_L/2) / 3),
-   // kept signed per entry: the error of m3mr is their sum over the tiles
-   std::vector<double> delta(nt * nm3), dnorm(nt);
+   // error estimate of a tile's contribution at level L from the change
+   // D_L = c_L - c_L/2, as the tail of a geometric series of ratio rho:
+   // err(L) ~ D_L rho / (1 - rho).  rho starts at 1/4 (bilinear interpolation
+   // is second order, giving D_L / 3) and, once a tile has two changes, is the
+   // observed ratio max|D_L| / max|D_L/2| clipped to [1/64, 1/2] (the change
+   // falls much faster than 1/4 once a tile resolves its peak).  Kept signed
+   // per entry: the error of m3mr is their sum over the tiles.
+   std::vector<double> delta(nt * nm3), dnorm(nt), dlast(nt, 0.0);
    auto set_delta = [&](size_t t) {
-      double e = 0.0;
-      for (size_t q = 0; q < nm3; ++q) {
-         const double d = (cur[t * nm3 + q] - prev[t * nm3 + q]) / 3.0;
-         delta[t * nm3 + q] = d;
-         e = std::max(e, std::fabs(d));
-      }
-      dnorm[t] = e;
+      double dn = 0.0;
+      for (size_t q = 0; q < nm3; ++q)
+         dn = std::max(dn, std::fabs(cur[t * nm3 + q] - prev[t * nm3 + q]));
+      double rho = 0.25;
+      if (dlast[t] > 0.0) rho = std::min(0.5, std::max(1.0 / 64.0, dn / dlast[t]));
+      const double f = rho / (1.0 - rho);
+      for (size_t q = 0; q < nm3; ++q) delta[t * nm3 + q] = f * (cur[t * nm3 + q] - prev[t * nm3 + q]);
+      dnorm[t] = f * dn;
+      dlast[t] = dn;
    };
    for (size_t t = 0; t < nt; ++t) set_delta(t);
    std::swap(prev, cur);
@@ -1081,7 +1227,7 @@ int bsahip_mesher_adapt(bsahip_mesher *m, double tol, int32_t *levels, int64_t *
          for (size_t q = 0; q < nm3; ++q) E[q] += delta[t * nm3 + q];
       if (emax() <= budget) break;
       // refine the tiles of largest estimate, in that order, until the
-      // estimated error with theirs cut by one level (about 1/4) fits
+      // estimated error with theirs cut by one level (by about 1/4) fits
       std::vector<size_t> ord;
       for (size_t t = 0; t < nt; ++t)
          if (lev[t] < 8 && dnorm[t] > 0.0) ord.push_back(t);

@@ -329,8 +329,10 @@ int bsahip_mesher_run(bsahip_mesher *m, const int32_t *levels, double *m3mf, dou
  * max|m3mr| (a plain adaptive Mesher: no counterpart in the reference, whose
  * levels come from the policy table alone): every tile is integrated at
  * levels 1 and 2; the error of a tile's contribution at level L is estimated
- * per entry as (c_L - c_L/2) / 3 (bilinear interpolation is second order) and
- * the error of m3mr as their sum over the tiles; while its largest entry
+ * per entry as (c_L - c_L/2) rho / (1 - rho), rho = 1/4 (bilinear
+ * interpolation is second order) until the tile has two changes, then their
+ * observed ratio (clipped to [1/64, 1/2]), and the error of m3mr as the sum
+ * of these estimates over the tiles; while its largest entry
  * exceeds tol max|m3mr| / 2, the tiles of largest estimate go up one level
  * (two rounds at most: levels 2, 4, 8).  levels (ntiles) receives 2, 4 or 8
  * per tile; *evals the BFM evaluations of those levels (level-0 points
