@@ -312,6 +312,105 @@ __global__ __launch_bounds__(256) void k_msh_integrate(
    atomicAdd(T.m3r + mnp, sr * T.ws);
 }
 
+// The same quadrature for the full model (nm^3 entries), one thread per
+// (m, p) and MSH_NQ consecutive n: Re[H_m(f1) H_n(f2) H_p*(f1+f2)] is
+// Re[H_n(f2) Q] with Q = H_m(f1) conj H_p(f1+f2) shared by the thread's n, and
+// the sum over a BRM row (fixed b) is taken on Q before the H_n(f2) factor:
+//   sum_a w_a w_b B Re[H_n Q] = w_b Re[H_n sum_a w_a B Q].
+// About a third of the arithmetic and a quarter of the FRF loads of
+// k_msh_integrate per (point, entry).  grid (nitems * bx), bx covering
+// nm^2 ceil(nm / MSH_NQ) threads, block 256.
+#define MSH_NQ 4
+
+__global__ __launch_bounds__(256) void k_msh_integrate_full(
+   const MshTileRun *__restrict__ tr, const int *__restrict__ it, const int2 *__restrict__ cr,
+   int bx, int nm, const double2 *__restrict__ H, const double *__restrict__ Mm,
+   const double *__restrict__ Km, const double *__restrict__ Cm, double *__restrict__ m3f)
+{
+   const int nm2 = nm * nm, nm3 = nm2 * nm;
+   const int nthr = nm2 * ((nm + MSH_NQ - 1) / MSH_NQ);
+   const long q = blockIdx.x / bx;
+   const int th = (int)(blockIdx.x % bx) * blockDim.x + threadIdx.x;
+   if (th >= nthr) return;
+   const int m = th % nm, p = (th / nm) % nm, n0 = (th / nm2) * MSH_NQ;
+   const int nn = min(MSH_NQ, nm - n0);
+   const MshTileRun T = tr[it[q]];
+   const int c0 = cr[q].x, c1 = cr[q].y;
+   const int nc1 = T.nb1 - 1, nc2 = T.nb2 - 1;
+   const int NA = nc1 * T.s1, NB = nc2 * T.s2;
+   const double Mp = Mm[p], Kp = Km[p], Cp = Cm[p];
+   const double inv1 = 1.0 / T.s1, inv2 = 1.0 / T.s2;
+   const double2 *H1 = H + T.h1, *H2 = H + T.h2, *H3 = T.h3 >= 0 ? H + T.h3 : nullptr;
+   int e[MSH_NQ];
+#pragma unroll
+   for (int j = 0; j < MSH_NQ; ++j) e[j] = m + min(n0 + j, nm - 1) * nm + p * nm2;
+   double sf[MSH_NQ], sr[MSH_NQ];
+#pragma unroll
+   for (int j = 0; j < MSH_NQ; ++j) sf[j] = sr[j] = 0.0;
+   for (int c = c0; c < c1; ++c) {
+      const int ci = c % nc1, cj = c / nc1;
+      const size_t q00 = (size_t)ci + (size_t)cj * T.ldq;
+      double b00[MSH_NQ], b10[MSH_NQ], b01[MSH_NQ], b11[MSH_NQ];
+#pragma unroll
+      for (int j = 0; j < MSH_NQ; ++j) {
+         b00[j] = T.bf[q00 * nm3 + e[j]];
+         b10[j] = T.bf[(q00 + 1) * nm3 + e[j]];
+         b01[j] = T.bf[(q00 + T.ldq) * nm3 + e[j]];
+         b11[j] = T.bf[(q00 + T.ldq + 1) * nm3 + e[j]];
+      }
+      const int ea = (ci == nc1 - 1) ? T.s1 : T.s1 - 1;   // owned BRM points per axis
+      const int eb = (cj == nc2 - 1) ? T.s2 : T.s2 - 1;
+      for (int bb = 0; bb <= eb; ++bb) {
+         const int b = cj * T.s2 + bb;
+         const double wb = (b == 0 || b == NB) ? 0.5 : 1.0;
+         const double sv = bb * inv2;
+         double e0[MSH_NQ], e1[MSH_NQ], Sf[MSH_NQ], Dr[MSH_NQ], Di[MSH_NQ];
+#pragma unroll
+         for (int j = 0; j < MSH_NQ; ++j) {
+            e0[j] = (1.0 - sv) * b00[j] + sv * b01[j];
+            e1[j] = (1.0 - sv) * b10[j] + sv * b11[j];
+            Sf[j] = Dr[j] = Di[j] = 0.0;
+         }
+         for (int aa = 0; aa <= ea; ++aa) {
+            const int a = ci * T.s1 + aa;
+            const double wa = (a == 0 || a == NA) ? 0.5 : 1.0;
+            const double t = aa * inv1;
+            const double2 h1v = H1[(size_t)a * nm + m];
+            double2 h3v;
+            if (H3) {
+               h3v = H3[(size_t)(a + b) * nm + p];
+            } else {
+               const double w3 = 2.0 * M_PI * (T.f3a + a * T.dh1 + b * T.dh2);
+               const double r = Kp - w3 * w3 * Mp, im = w3 * Cp, d = r * r + im * im;
+               h3v = make_double2(r / d, -im / d);
+            }
+            // Q = h1 conj(h3)
+            const double Qr = h1v.x * h3v.x + h1v.y * h3v.y;
+            const double Qi = h1v.y * h3v.x - h1v.x * h3v.y;
+#pragma unroll
+            for (int j = 0; j < MSH_NQ; ++j) {
+               const double x = wa * ((1.0 - t) * e0[j] + t * e1[j]);
+               Sf[j] += x;
+               Dr[j] += x * Qr;
+               Di[j] += x * Qi;
+            }
+         }
+#pragma unroll
+         for (int j = 0; j < MSH_NQ; ++j) {
+            const double2 h2v = H2[(size_t)b * nm + min(n0 + j, nm - 1)];
+            sf[j] += wb * Sf[j];
+            sr[j] += wb * (h2v.x * Dr[j] - h2v.y * Di[j]);
+         }
+      }
+   }
+#pragma unroll
+   for (int j = 0; j < MSH_NQ; ++j)
+      if (j < nn) {
+         atomicAdd(m3f + e[j], sf[j] * T.ws);
+         atomicAdd(T.m3r + e[j], sr[j] * T.ws);
+      }
+}
+
 // ---------------------------------------------------------------------------
 // Tile export: the integrand of k_msh_integrate without the weights at every
 // BRM point (a, b) of the tile, pt = a + b*(NA+1).  Column c of val
@@ -808,7 +907,10 @@ int msh_integrate_many(bsahip_mesher *M, const std::vector<MshRef> &refs, double
 {
    if (refs.empty()) return 0;
    Prob &D = M->D;
-   const int nm3 = M->nc, bx = (nm3 + 255) / 256;
+   // threads per (tile, cell range) item: one per entry (ONLY_DIAG), or one per
+   // (m, p) and MSH_NQ n (full model, k_msh_integrate_full)
+   const int nthr = M->diag ? M->nc : D.nm * D.nm * ((D.nm + MSH_NQ - 1) / MSH_NQ);
+   const int bx = (nthr + 255) / 256;
    std::vector<MshSeg> seg;
    std::vector<long> eoff(1, 0);
    std::vector<MshTileRun> tr(refs.size());
@@ -871,8 +973,12 @@ int msh_integrate_many(bsahip_mesher *M, const std::vector<MshRef> &refs, double
    k_frf_segs<<<(unsigned)((eoff.back() + 255) / 256), 256, 0, g.s>>>(dseg, doff, (int)seg.size(),
                                                                        D.Mm, D.Km, D.Cm, D.nm, H);
    HIPC(hipGetLastError());
-   k_msh_integrate<<<(unsigned)(it.size() * bx), 256, 0, g.s>>>(
-      dtr, dit, dcr, bx, D.nm, M->nc, M->diag ? 1 : 0, H, D.Mm, D.Km, D.Cm, m3f);
+   if (M->diag)
+      k_msh_integrate<<<(unsigned)(it.size() * bx), 256, 0, g.s>>>(
+         dtr, dit, dcr, bx, D.nm, M->nc, 1, H, D.Mm, D.Km, D.Cm, m3f);
+   else
+      k_msh_integrate_full<<<(unsigned)(it.size() * bx), 256, 0, g.s>>>(dtr, dit, dcr, bx, D.nm, H,
+                                                                     D.Mm, D.Km, D.Cm, m3f);
    HIPC(hipGetLastError());
    return 0;
 }
