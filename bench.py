@@ -207,6 +207,42 @@ def batch_leg(log):
     return out
 
 
+def mesher_batch_leg(log, n: int = 64):
+    """Post-mesh of n small towers (NN 20-60, NM 3-6, rule-based levels) as n
+    bsahip_mesher_run calls and as one bsahip_mesher_run_batch (the FRF tables
+    and quadrature of every Mesher's tiles in one launch each); wall time of
+    each form after a warmup, and the largest relative difference of m3mr."""
+    import time
+    from bsalib_amd import abi, proposer, synth
+    rng = np.random.default_rng(7)
+    ms = []
+    for s in range(n):
+        nm = int(rng.integers(3, 7))
+        p = synth.tower(nn=int(rng.integers(20, 61)), nm=nm, natf=np.sort(rng.uniform(0.15, 2.0, nm)),
+                        xi=rng.uniform(0.005, 0.02, nm), nfreqs=65, df=1 / 16, seed=100 + s,
+                        zone_per_node=False)
+        ms.append(abi.Mesher(p, **proposer.mesher_settings(p)))
+    seq = [M.run() for M in ms]
+    abi.mesher_run_batch(ms)
+    t0 = time.perf_counter()
+    seq = [M.run() for M in ms]
+    t_seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    bat = abi.mesher_run_batch(ms)
+    t_bat = time.perf_counter() - t0
+    diff = max(float(np.abs(b["m3mr"] - a["m3mr"]).max() / np.abs(a["m3mr"]).max())
+               for a, b in zip(seq, bat))
+    out = {"structures": n, "tiles": int(sum(M.ntiles for M in ms)),
+           "evals": int(sum(r["evals_pre"] + r["evals_post"] for r in seq)),
+           "brm_points": int(sum(r["brm_points"] for r in seq)),
+           "separate_ms": 1e3 * t_seq, "batched_ms": 1e3 * t_bat, "speedup": t_seq / t_bat,
+           "max_rel_diff_m3mr": diff}
+    for M in ms:
+        M.close()
+    log(f"mesher batch: {out}")
+    return out
+
+
 def mesher_leg(log):
     """Learned refinement proposer on the deck of configuration 5 (NN=500,
     NM=20): BFM evaluations and third-moment error against the converged Mesher
@@ -306,6 +342,8 @@ def main():
         log(f"errors vs fp64: {errors}")
 
     batch = None if args.no_batch else batch_leg(log)
+    if batch is not None:
+        batch["mesher_small_towers_64"] = mesher_batch_leg(log)
     mesher = None if args.no_mesher else mesher_leg(log)
     cpu = None
     cpu2 = None

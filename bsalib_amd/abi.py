@@ -64,7 +64,7 @@ EXPORTS = ["bsahip_init", "bsahip_finalise", "bsahip_last_error", "bsahip_versio
            "bsahip_mesher_load", "bsahip_batch_create", "bsahip_batch_autotune",
            "bsahip_batch_run", "bsahip_batch_info", "bsahip_batch_destroy",
            "bsahip_mesher_destroy", "bsahip_modal_psd_points", "bsahip_modal_recombination",
-           "bsahip_mesher_adapt", "bsahip_dgemm_strided_batched"]
+           "bsahip_mesher_adapt", "bsahip_dgemm_strided_batched", "bsahip_mesher_run_batch"]
 
 
 def lib():
@@ -95,6 +95,11 @@ def lib():
                                            C.POINTER(C.c_double), C.c_int64, C.POINTER(C.c_int32),
                                            C.POINTER(C.c_double), C.POINTER(C.c_double),
                                            C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+        L.bsahip_mesher_run_batch.argtypes = [C.c_int32, C.POINTER(C.c_void_p),
+                                              C.POINTER(C.POINTER(C.c_int32)),
+                                              C.POINTER(C.POINTER(C.c_double)),
+                                              C.POINTER(C.POINTER(C.c_double)),
+                                              C.POINTER(C.c_int64)]
         L.bsahip_mesher_save.argtypes = [C.c_void_p, C.c_char_p]
         L.bsahip_mesher_load.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_char_p,
                                          C.POINTER(C.c_void_p)]
@@ -471,6 +476,27 @@ class Mesher:
         except Exception:  # noqa: BLE001
             pass
 
+
+
+def mesher_run_batch(meshers, levels=None) -> list:
+    """bsahip_mesher_run_batch: the post-mesh of several Meshers at once, with
+    per-Mesher levels (each None for the rule-based ones).  One dict per Mesher
+    with m3mf, m3mr and the counts of Mesher.run."""
+    n = len(meshers)
+    hs = (C.c_void_p * n)(*[m.h for m in meshers])
+    lv = [None if levels is None or levels[i] is None else meshers[i]._levels(levels[i])
+          for i in range(n)]
+    LP = C.POINTER(C.c_int32)
+    larr = (LP * n)(*[_ptr(v, C.c_int32) if v is not None else LP() for v in lv])
+    outs = [(np.zeros(m.nc), np.zeros(m.nc)) for m in meshers]
+    DP = C.POINTER(C.c_double)
+    f = (DP * n)(*[_ptr(o[0], C.c_double) for o in outs])
+    r = (DP * n)(*[_ptr(o[1], C.c_double) for o in outs])
+    counts = np.zeros(3 * n, dtype=np.int64)
+    _check(lib().bsahip_mesher_run_batch(C.c_int32(n), hs, larr, f, r, _ptr(counts, C.c_int64)))
+    return [{"m3mf": outs[i][0], "m3mr": outs[i][1], "evals_pre": int(counts[3 * i]),
+             "evals_post": int(counts[3 * i + 1]), "brm_points": int(counts[3 * i + 2]),
+             "modes": meshers[i].modes} for i in range(n)]
 
 class Batch:
     """Structures of uneven size resident on the device (bsahip_batch_*):

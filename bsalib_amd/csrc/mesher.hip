@@ -206,11 +206,12 @@ __device__ __forceinline__ void msh_entry(int e, int nm, int diag, int &m, int &
 struct MshSeg {
    double f0, h;
    long off;
+   const double *Mm, *Km, *Cm;   // modal mass, stiffness, damping of the segment's structure
+   int nm;
 };
 
 __global__ void k_frf_segs(const MshSeg *__restrict__ seg, const long *__restrict__ eoff, int nseg,
-                           const double *__restrict__ Mm, const double *__restrict__ Km,
-                           const double *__restrict__ Cm, int nm, double2 *__restrict__ H)
+                           double2 *__restrict__ H)
 {
    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
    if (t >= eoff[nseg]) return;
@@ -220,12 +221,13 @@ __global__ void k_frf_segs(const MshSeg *__restrict__ seg, const long *__restric
       if (eoff[mid] <= t) lo = mid;
       else hi = mid - 1;
    }
+   const MshSeg S = seg[lo];
    const long e = t - eoff[lo];
-   const long i = e / nm;
-   const int m = (int)(e % nm);
-   const double w = 2.0 * M_PI * (seg[lo].f0 + (double)i * seg[lo].h);
-   const double r = Km[m] - w * w * Mm[m], im = w * Cm[m], d = r * r + im * im;
-   H[seg[lo].off + e] = make_double2(r / d, -im / d);
+   const long i = e / S.nm;
+   const int m = (int)(e % S.nm);
+   const double w = 2.0 * M_PI * (S.f0 + (double)i * S.h);
+   const double r = S.Km[m] - w * w * S.Mm[m], im = w * S.Cm[m], d = r * r + im * im;
+   H[S.off + e] = make_double2(r / d, -im / d);
 }
 
 // fine[dst[q]*nm3 + e] = lv0[src[q]*nm3 + e]; grid (ceil(nm3/256), npairs)
@@ -238,13 +240,15 @@ __global__ void k_msh_copy_pts(const double *__restrict__ lv0, const long *__res
    fine[dst[q] * nm3 + e] = lv0[src[q] * nm3 + e];
 }
 
-// One tile of a batched post-mesh.
+// One tile of a batched post-mesh (tiles of several structures may share a
+// launch: the modal data and accumulators are per tile).
 struct MshTileRun {
    const double *bf;           // the tile's BFM grid (lv0 or fine)
-   double *m3r;                // response accumulator of this tile
+   double *m3f, *m3r;          // force accumulator of its structure, response of this tile
+   const double *Mm, *Km, *Cm; // modal mass, stiffness, damping of its structure
    long ldq;
    long h1, h2, h3;            // FRF table offsets (h3 < 0: f1 + f2 evaluated here)
-   int nb1, nb2, s1, s2;
+   int nm, nb1, nb2, s1, s2;
    double f3a, dh1, dh2, ws;
 };
 
@@ -257,21 +261,20 @@ struct MshTileRun {
 // (blockIdx.x % bx) * 256 ..); item q covers cells [c0, c1) of tile it[q]
 __global__ __launch_bounds__(256) void k_msh_integrate(
    const MshTileRun *__restrict__ tr, const int *__restrict__ it, const int2 *__restrict__ cr,
-   int bx, int nm, int nc, int diag, const double2 *__restrict__ H,
-   const double *__restrict__ Mm, const double *__restrict__ Km, const double *__restrict__ Cm,
-   double *__restrict__ m3f)
+   int bx, int diag, const double2 *__restrict__ H)
 {
-   const int nm3 = nc;
    const long q = blockIdx.x / bx;
+   const MshTileRun T = tr[it[q]];
+   const int nm = T.nm;
+   const int nm3 = diag ? nm : nm * nm * nm;
    const int mnp = (int)(blockIdx.x % bx) * blockDim.x + threadIdx.x;
    if (mnp >= nm3) return;
-   const MshTileRun T = tr[it[q]];
    const int c0 = cr[q].x, c1 = cr[q].y;
    int m, n, p;
    msh_entry(mnp, nm, diag, m, n, p);
    const int nc1 = T.nb1 - 1, nc2 = T.nb2 - 1;
    const int NA = nc1 * T.s1, NB = nc2 * T.s2;
-   const double Mp = Mm[p], Kp = Km[p], Cp = Cm[p];
+   const double Mp = T.Mm[p], Kp = T.Km[p], Cp = T.Cm[p];
    const double inv1 = 1.0 / T.s1, inv2 = 1.0 / T.s2;
    const double2 *H1 = H + T.h1, *H2 = H + T.h2, *H3 = T.h3 >= 0 ? H + T.h3 : nullptr;
    double sf = 0.0, sr = 0.0;
@@ -308,7 +311,7 @@ __global__ __launch_bounds__(256) void k_msh_integrate(
          }
       }
    }
-   atomicAdd(m3f + mnp, sf * T.ws);
+   atomicAdd(T.m3f + mnp, sf * T.ws);
    atomicAdd(T.m3r + mnp, sr * T.ws);
 }
 
@@ -319,26 +322,26 @@ __global__ __launch_bounds__(256) void k_msh_integrate(
 //   sum_a w_a w_b B Re[H_n Q] = w_b Re[H_n sum_a w_a B Q].
 // About a third of the arithmetic and a quarter of the FRF loads of
 // k_msh_integrate per (point, entry).  grid (nitems * bx), bx covering
-// nm^2 ceil(nm / MSH_NQ) threads, block 256.
+// nm^2 ceil(nm / MSH_NQ) threads of the item's structure, block 256.
 #define MSH_NQ 4
 
 __global__ __launch_bounds__(256) void k_msh_integrate_full(
    const MshTileRun *__restrict__ tr, const int *__restrict__ it, const int2 *__restrict__ cr,
-   int bx, int nm, const double2 *__restrict__ H, const double *__restrict__ Mm,
-   const double *__restrict__ Km, const double *__restrict__ Cm, double *__restrict__ m3f)
+   int bx, const double2 *__restrict__ H)
 {
+   const long q = blockIdx.x / bx;
+   const MshTileRun T = tr[it[q]];
+   const int nm = T.nm;
    const int nm2 = nm * nm, nm3 = nm2 * nm;
    const int nthr = nm2 * ((nm + MSH_NQ - 1) / MSH_NQ);
-   const long q = blockIdx.x / bx;
    const int th = (int)(blockIdx.x % bx) * blockDim.x + threadIdx.x;
    if (th >= nthr) return;
    const int m = th % nm, p = (th / nm) % nm, n0 = (th / nm2) * MSH_NQ;
    const int nn = min(MSH_NQ, nm - n0);
-   const MshTileRun T = tr[it[q]];
    const int c0 = cr[q].x, c1 = cr[q].y;
    const int nc1 = T.nb1 - 1, nc2 = T.nb2 - 1;
    const int NA = nc1 * T.s1, NB = nc2 * T.s2;
-   const double Mp = Mm[p], Kp = Km[p], Cp = Cm[p];
+   const double Mp = T.Mm[p], Kp = T.Km[p], Cp = T.Cm[p];
    const double inv1 = 1.0 / T.s1, inv2 = 1.0 / T.s2;
    const double2 *H1 = H + T.h1, *H2 = H + T.h2, *H3 = T.h3 >= 0 ? H + T.h3 : nullptr;
    int e[MSH_NQ];
@@ -406,7 +409,7 @@ __global__ __launch_bounds__(256) void k_msh_integrate_full(
 #pragma unroll
    for (int j = 0; j < MSH_NQ; ++j)
       if (j < nn) {
-         atomicAdd(m3f + e[j], sf[j] * T.ws);
+         atomicAdd(T.m3f + e[j], sf[j] * T.ws);
          atomicAdd(T.m3r + e[j], sr[j] * T.ws);
       }
 }
@@ -470,18 +473,23 @@ __global__ __launch_bounds__(256) void k_msh_export(
 
 // Feature stencil of a tile's level-0 BFM: out[s] = sqrt(sum_mnp B^2) at the
 // level-0 point nearest to stencil node s (ns x ns nodes over the tile).
-__global__ void k_bfm_stencil(const double *__restrict__ bf, long ldq, int nm3, int nb1, int nb2,
-                              int ns, double *__restrict__ out)
+// All tiles in one launch: tile k (blockIdx.y) has its level-0 values at
+// lv0 + t0[k] * nm3 (row stride ldq) on an nb1[k] x nb2[k] grid; out[k*ns*ns + s].
+__global__ void k_bfm_stencil(const double *__restrict__ lv0, const long *__restrict__ t0,
+                              const int2 *__restrict__ nb, long ldq, int nm3, int ns,
+                              double *__restrict__ out)
 {
    const int s = blockIdx.x;
+   const long k = blockIdx.y;
    const int si = s % ns, sj = s / ns;
+   const int nb1 = nb[k].x, nb2 = nb[k].y;
    const int i = (int)lround((double)si * (nb1 - 1) / (ns - 1));
    const int j = (int)lround((double)sj * (nb2 - 1) / (ns - 1));
-   const double *b = bf + ((size_t)i + (size_t)j * ldq) * nm3;
+   const double *b = lv0 + ((size_t)t0[k] + (size_t)i + (size_t)j * ldq) * nm3;
    double acc = 0.0;
    for (int t = threadIdx.x; t < nm3; t += 64) acc += b[t] * b[t];
    acc = wave_sum(acc);
-   if (threadIdx.x == 0) out[s] = sqrt(acc);
+   if (threadIdx.x == 0) out[k * ns * ns + s] = sqrt(acc);
 }
 
 // ===========================================================================
@@ -899,97 +907,122 @@ struct MshRef {
    long ldq;
 };
 
-// The quadrature of many visited tiles in three launches: FRF tables of every
-// tile (k_frf_segs), then k_msh_integrate over (tile, cell range)
-// items; m3r_of(k) is tile k's response accumulator.
-int msh_integrate_many(bsahip_mesher *M, const std::vector<MshRef> &refs, double *m3f,
-                       const std::function<double *(size_t)> &m3r_of, long &brm_pts)
+// The visited tiles of one structure awaiting their quadrature: its m3f
+// accumulator and each tile's m3r accumulator (m3r_of(tile index)).
+struct MshJob {
+   bsahip_mesher *M = nullptr;
+   std::vector<MshRef> refs;
+   double *m3f = nullptr;
+   std::function<double *(size_t)> m3r_of;
+   long brm = 0;                  // BRM points integrated
+};
+
+// The quadrature of the visited tiles of one or more structures (all of the
+// full model, or all ONLY_DIAG) in two launches: FRF tables of every tile
+// (k_frf_segs), then k_msh_integrate_full / k_msh_integrate over (tile, cell
+// range) items.  Scratch buffers come from the first structure's buffers.
+int msh_integrate_jobs(const std::vector<MshJob *> &jobs)
 {
-   if (refs.empty()) return 0;
-   Prob &D = M->D;
-   // threads per (tile, cell range) item: one per entry (ONLY_DIAG), or one per
-   // (m, p) and MSH_NQ n (full model, k_msh_integrate_full)
-   const int nthr = M->diag ? M->nc : D.nm * D.nm * ((D.nm + MSH_NQ - 1) / MSH_NQ);
-   const int bx = (nthr + 255) / 256;
+   size_t ntr = 0;
+   for (MshJob *J : jobs) ntr += J->refs.size();
+   if (ntr == 0) return 0;
+   const bool diag = jobs[0]->M->diag;
+   int bx = 1;
    std::vector<MshSeg> seg;
    std::vector<long> eoff(1, 0);
-   std::vector<MshTileRun> tr(refs.size());
+   std::vector<MshTileRun> tr;
+   std::vector<long> ncell_of;
+   tr.reserve(ntr);
    long ncells = 0;
-   auto add_seg = [&](double f0, double h, int n) -> long {
-      const long off = eoff.back();
-      seg.push_back(MshSeg{f0, h, off});
-      eoff.push_back(off + (long)n * D.nm);
-      return off;
-   };
-   for (size_t r = 0; r < refs.size(); ++r) {
-      const MshRef &R = refs[r];
-      const MshTile &t = M->tiles[R.k];
-      const MshBrmGrid G = msh_brm_grid(t, R.l);
-      const bool eq = std::fabs(G.h1 - G.h2) <= 1e-13 * std::max(std::fabs(G.h1), std::fabs(G.h2));
-      MshTileRun &T = tr[r];
-      T.bf = R.bf;
-      T.m3r = m3r_of(R.k);
-      T.ldq = R.ldq;
-      T.h1 = add_seg(t.a1, G.h1, G.NA + 1);
-      T.h2 = add_seg(t.a2, G.h2, G.NB + 1);
-      T.h3 = eq ? add_seg(t.a1 + t.a2, G.h1, G.NA + G.NB + 1) : -1;
-      T.nb1 = R.nb1;
-      T.nb2 = R.nb2;
-      T.s1 = G.s1;
-      T.s2 = G.s2;
-      T.f3a = t.a1 + t.a2;
-      T.dh1 = G.h1;
-      T.dh2 = G.h2;
-      T.ws = (2.0 * M_PI * G.h1) * (2.0 * M_PI * G.h2);
-      ncells += (long)(R.nb1 - 1) * (R.nb2 - 1);
-      brm_pts += (long)(G.NA + 1) * (G.NB + 1);
+   for (MshJob *J : jobs) {
+      bsahip_mesher *M = J->M;
+      Prob &D = M->D;
+      if (M->diag != diag) return fail(BSAHIP_EINVAL, "a Mesher batch mixes the ONLY_DIAG and full models");
+      // threads per item: one per entry (ONLY_DIAG), or one per (m, p) and
+      // MSH_NQ n (full model)
+      const int nthr = diag ? M->nc : D.nm * D.nm * ((D.nm + MSH_NQ - 1) / MSH_NQ);
+      bx = std::max(bx, (nthr + 255) / 256);
+      auto add_seg = [&](double f0, double h, int n) -> long {
+         const long off = eoff.back();
+         seg.push_back(MshSeg{f0, h, off, D.Mm, D.Km, D.Cm, D.nm});
+         eoff.push_back(off + (long)n * D.nm);
+         return off;
+      };
+      for (const MshRef &R : J->refs) {
+         const MshTile &t = M->tiles[R.k];
+         const MshBrmGrid G = msh_brm_grid(t, R.l);
+         const bool eq = std::fabs(G.h1 - G.h2) <= 1e-13 * std::max(std::fabs(G.h1), std::fabs(G.h2));
+         MshTileRun T;
+         T.bf = R.bf;
+         T.m3f = J->m3f;
+         T.m3r = J->m3r_of(R.k);
+         T.Mm = D.Mm;
+         T.Km = D.Km;
+         T.Cm = D.Cm;
+         T.ldq = R.ldq;
+         T.h1 = add_seg(t.a1, G.h1, G.NA + 1);
+         T.h2 = add_seg(t.a2, G.h2, G.NB + 1);
+         T.h3 = eq ? add_seg(t.a1 + t.a2, G.h1, G.NA + G.NB + 1) : -1;
+         T.nm = D.nm;
+         T.nb1 = R.nb1;
+         T.nb2 = R.nb2;
+         T.s1 = G.s1;
+         T.s2 = G.s2;
+         T.f3a = t.a1 + t.a2;
+         T.dh1 = G.h1;
+         T.dh2 = G.h2;
+         T.ws = (2.0 * M_PI * G.h1) * (2.0 * M_PI * G.h2);
+         tr.push_back(T);
+         ncell_of.push_back((long)(R.nb1 - 1) * (R.nb2 - 1));
+         ncells += ncell_of.back();
+         J->brm += (long)(G.NA + 1) * (G.NB + 1);
+      }
    }
    // items: each tile's cells cut in ranges of at most `per` cells, so that
-   // the launch holds some 8192 (item, entry block) workgroups or more
+   // the launch holds some 8192 (item, thread block) workgroups or more
    const long per = std::max<long>(1, ncells * bx / 8192);
    std::vector<int> it;
    std::vector<int2> cr;
-   for (size_t r = 0; r < refs.size(); ++r) {
-      const int nc = (refs[r].nb1 - 1) * (refs[r].nb2 - 1);
-      for (long c0 = 0; c0 < nc; c0 += per) {
+   for (size_t r = 0; r < tr.size(); ++r)
+      for (long c0 = 0; c0 < ncell_of[r]; c0 += per) {
          it.push_back((int)r);
-         cr.push_back(make_int2((int)c0, (int)std::min<long>(nc, c0 + per)));
+         cr.push_back(make_int2((int)c0, (int)std::min<long>(ncell_of[r], c0 + per)));
       }
-   }
    if ((long)it.size() * bx > 0x7fffffffL)
       return fail(BSAHIP_EUNSUPPORTED, "Mesher post-mesh launch above 2^31 workgroups");
+   Bufs &B = jobs[0]->M->B;
    MshSeg *dseg;
    long *doff;
    double2 *H;
    MshTileRun *dtr;
    int *dit;
    int2 *dcr;
-   CHK(upload(M->B, "m_seg", seg, &dseg));
-   CHK(upload(M->B, "m_segoff", eoff, &doff));
-   CHK(M->B.get("m_H", (size_t)eoff.back(), &H));
-   CHK(upload(M->B, "m_tiles_run", tr, &dtr));
-   CHK(upload(M->B, "m_items", it, &dit));
-   CHK(upload(M->B, "m_ranges", cr, &dcr));
-   k_frf_segs<<<(unsigned)((eoff.back() + 255) / 256), 256, 0, g.s>>>(dseg, doff, (int)seg.size(),
-                                                                       D.Mm, D.Km, D.Cm, D.nm, H);
+   CHK(upload(B, "m_seg", seg, &dseg));
+   CHK(upload(B, "m_segoff", eoff, &doff));
+   CHK(B.get("m_H", (size_t)eoff.back(), &H));
+   CHK(upload(B, "m_tiles_run", tr, &dtr));
+   CHK(upload(B, "m_items", it, &dit));
+   CHK(upload(B, "m_ranges", cr, &dcr));
+   k_frf_segs<<<(unsigned)((eoff.back() + 255) / 256), 256, 0, g.s>>>(dseg, doff, (int)seg.size(), H);
    HIPC(hipGetLastError());
-   if (M->diag)
-      k_msh_integrate<<<(unsigned)(it.size() * bx), 256, 0, g.s>>>(
-         dtr, dit, dcr, bx, D.nm, M->nc, 1, H, D.Mm, D.Km, D.Cm, m3f);
+   if (diag)
+      k_msh_integrate<<<(unsigned)(it.size() * bx), 256, 0, g.s>>>(dtr, dit, dcr, bx, 1, H);
    else
-      k_msh_integrate_full<<<(unsigned)(it.size() * bx), 256, 0, g.s>>>(dtr, dit, dcr, bx, D.nm, H,
-                                                                     D.Mm, D.Km, D.Cm, m3f);
+      k_msh_integrate_full<<<(unsigned)(it.size() * bx), 256, 0, g.s>>>(dtr, dit, dcr, bx, H);
    HIPC(hipGetLastError());
+   for (MshJob *J : jobs) J->refs.clear();
    return 0;
 }
 
 // Post-mesh traversal shared by the run and the export: level-1 tiles read the
 // resident level-0 BFM in place; refined tiles are evaluated in groups whose
-// fine grids fit the staging buffer.  `done`, when given, is called once the
-// tiles visited so far may be consumed (after each group, before its staging
-// buffer is reused, and at the end).  Returns the refinement evaluations.
+// fine grids fit the staging buffer.  `done(last)`, when given, is called once
+// the tiles visited so far may be consumed: after each group, before its
+// staging buffer is reused (last = false), and at the end (last = true; those
+// tiles stay valid until this Mesher's next post-mesh).  Returns the
+// refinement evaluations.
 int msh_post(bsahip_mesher *m, const int32_t *levels, const MshVisit &visit, long &evals_post,
-             const std::function<int()> &done = nullptr)
+             const std::function<int(bool)> &done = nullptr)
 {
    const size_t nm3 = (size_t)m->nc, nt = m->tiles.size();
    const size_t cap = std::max<size_t>(66049, ((size_t)1 << 29) / nm3);   // points (4 GiB)
@@ -999,7 +1032,7 @@ int msh_post(bsahip_mesher *m, const int32_t *levels, const MshVisit &visit, lon
    size_t used = 0;
    evals_post = 0;
    std::vector<long> csrc, cdst;
-   auto flush = [&]() -> int {
+   auto flush = [&](bool last) -> int {
       if (grp.empty()) return 0;
       double *fine;
       CHK(m->B.get("m_fine", used * nm3, &fine));
@@ -1035,7 +1068,7 @@ int msh_post(bsahip_mesher *m, const int32_t *levels, const MshVisit &visit, lon
          const int nb1 = t.n1 * l + 1, nb2 = t.n2 * l + 1;
          CHK(visit(grp[k], t, l, fine + goff[k] * nm3, nb1, nb1, nb2));
       }
-      if (done) CHK(done());
+      if (done && !last) CHK(done(false));
       HIPC(hipStreamSynchronize(g.s));
       grp.clear();
       goff.clear();
@@ -1053,7 +1086,7 @@ int msh_post(bsahip_mesher *m, const int32_t *levels, const MshVisit &visit, lon
          continue;
       }
       const size_t np = (size_t)(t.n1 * l + 1) * (t.n2 * l + 1);
-      if (!grp.empty() && used + np > cap) CHK(flush());
+      if (!grp.empty() && used + np > cap) CHK(flush(false));
       int nb1, nb2;
       msh_grid(t, l, f1, f2, nb1, nb2);
       for (int j = 0; j < nb2; ++j)
@@ -1068,8 +1101,8 @@ int msh_post(bsahip_mesher *m, const int32_t *levels, const MshVisit &visit, lon
       goff.push_back(used);
       used += np;
    }
-   CHK(flush());
-   if (done) CHK(done());
+   CHK(flush(true));   // the last group's tiles are consumed with the others by done(true)
+   if (done) CHK(done(true));
    return 0;
 }
 
@@ -1216,10 +1249,21 @@ int bsahip_mesher_features(bsahip_mesher *m, int32_t ns, double *stencil)
    const size_t nt = m->tiles.size(), per = (size_t)ns * ns;
    double *d;
    CHK(m->B.get("m_stencil", nt * per, &d));
+   std::vector<long> t0(nt);
+   std::vector<int2> nb(nt);
    for (size_t i = 0; i < nt; ++i) {
       const MshTile &t = m->tiles[i];
-      k_bfm_stencil<<<(unsigned)per, 64, 0, g.s>>>(m->lv0 + msh_lv0_at(m, t) * nm3, m->nx, nm3,
-                                                   t.n1 + 1, t.n2 + 1, ns, d + i * per);
+      t0[i] = msh_lv0_at(m, t);
+      nb[i] = make_int2(t.n1 + 1, t.n2 + 1);
+   }
+   long *dt0;
+   int2 *dnb;
+   CHK(upload(m->B, "m_st_t0", t0, &dt0));
+   CHK(upload(m->B, "m_st_nb", nb, &dnb));
+   for (size_t k0 = 0; k0 < nt; k0 += 65535) {
+      const unsigned nk = (unsigned)std::min<size_t>(65535, nt - k0);
+      k_bfm_stencil<<<dim3((unsigned)per, nk), 64, 0, g.s>>>(m->lv0, dt0 + k0, dnb + k0, m->nx, nm3,
+                                                            ns, d + k0 * per);
       HIPC(hipGetLastError());
    }
    HIPC(hipMemcpyAsync(stencil, d, nt * per * sizeof(double), hipMemcpyDeviceToHost, g.s));
@@ -1239,22 +1283,20 @@ int bsahip_mesher_run_tiles(bsahip_mesher *m, const int32_t *levels, double *til
    const size_t nacc = 2 * nm3 + (tile_m3mr ? nt * nm3 : 0);
    CHK(m->B.get("m_acc", nacc, &acc));
    CHK(fill(acc, nacc, 0.0));
-   long evals_post = 0, brm = 0;
-   auto slot_r = [&](size_t k) { return tile_m3mr ? acc + 2 * nm3 + k * nm3 : acc + nm3; };
-   // tiles are collected as visited and integrated together (msh_integrate_many)
-   std::vector<MshRef> refs;
+   long evals_post = 0;
+   // tiles are collected as visited and integrated together (msh_integrate_jobs)
+   MshJob job;
+   job.M = m;
+   job.m3f = acc;
+   job.m3r_of = [&](size_t k) { return tile_m3mr ? acc + 2 * nm3 + k * nm3 : acc + nm3; };
    CHK(msh_post(
       m, levels,
       [&](size_t k, const MshTile &, int l, const double *bf, long ldq, int nb1, int nb2) {
-         refs.push_back(MshRef{k, l, nb1, nb2, bf, ldq});
+         job.refs.push_back(MshRef{k, l, nb1, nb2, bf, ldq});
          return 0;
       },
-      evals_post,
-      [&]() -> int {
-         CHK(msh_integrate_many(m, refs, acc, slot_r, brm));
-         refs.clear();
-         return 0;
-      }));
+      evals_post, [&](bool) { return msh_integrate_jobs({&job}); }));
+   const long brm = job.brm;
    if (tile_m3mr) {
       HIPC(hipMemcpyAsync(tile_m3mr, acc + 2 * nm3, nt * nm3 * sizeof(double),
                           hipMemcpyDeviceToHost, g.s));
@@ -1284,6 +1326,61 @@ int bsahip_mesher_run(bsahip_mesher *m, const int32_t *levels, double *m3mf, dou
                       int64_t *counts)
 {
    return bsahip_mesher_run_tiles(m, levels, nullptr, m3mf, m3mr, counts);
+}
+
+int bsahip_mesher_run_batch(int32_t n, bsahip_mesher *const *ms, const int32_t *const *levels,
+                            double *const *m3mf, double *const *m3mr, int64_t *counts)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   if (n < 0 || (n > 0 && !ms)) return fail(BSAHIP_EINVAL, "need n >= 0 and the Meshers");
+   for (int i = 0; i < n; ++i) {
+      if (!ms[i]) return fail(BSAHIP_EINVAL, "NULL Mesher in the batch");
+      for (int j = 0; j < i; ++j)
+         if (ms[j] == ms[i]) return fail(BSAHIP_EINVAL, "a Mesher appears twice in the batch");
+      CHK(msh_check_levels(ms[i], levels ? levels[i] : nullptr));
+   }
+   std::vector<MshJob> jobs(n);
+   std::vector<long> evals(n, 0);
+   std::vector<double *> acc(n);
+   for (int i = 0; i < n; ++i) {
+      bsahip_mesher *m = ms[i];
+      const size_t nm3 = (size_t)m->nc;
+      CHK(m->B.get("m_acc", 2 * nm3, &acc[i]));
+      CHK(fill(acc[i], 2 * nm3, 0.0));
+      MshJob &J = jobs[i];
+      J.M = m;
+      J.m3f = acc[i];
+      double *r = acc[i] + nm3;
+      J.m3r_of = [r](size_t) { return r; };
+      // each Mesher's last group of tiles waits for the shared launch below;
+      // earlier groups (their staging buffer is reused) are integrated at once
+      CHK(msh_post(
+         m, levels ? levels[i] : nullptr,
+         [&J](size_t k, const MshTile &, int l, const double *bf, long ldq, int nb1, int nb2) {
+            J.refs.push_back(MshRef{k, l, nb1, nb2, bf, ldq});
+            return 0;
+         },
+         evals[i], [&J](bool last) { return last ? 0 : msh_integrate_jobs({&J}); }));
+   }
+   std::vector<MshJob *> full, diag;
+   for (MshJob &J : jobs) (J.M->diag ? diag : full).push_back(&J);
+   if (!full.empty()) CHK(msh_integrate_jobs(full));
+   if (!diag.empty()) CHK(msh_integrate_jobs(diag));
+   for (int i = 0; i < n; ++i) {
+      const size_t nm3 = (size_t)ms[i]->nc;
+      if (m3mf && m3mf[i])
+         HIPC(hipMemcpyAsync(m3mf[i], acc[i], nm3 * sizeof(double), hipMemcpyDeviceToHost, g.s));
+      if (m3mr && m3mr[i])
+         HIPC(hipMemcpyAsync(m3mr[i], acc[i] + nm3, nm3 * sizeof(double), hipMemcpyDeviceToHost, g.s));
+   }
+   HIPC(hipStreamSynchronize(g.s));
+   if (counts)
+      for (int i = 0; i < n; ++i) {
+         counts[3 * i] = ms[i]->evals_lv0;
+         counts[3 * i + 1] = evals[i];
+         counts[3 * i + 2] = jobs[i].brm;
+      }
+   return 0;
 }
 
 int bsahip_mesher_adapt(bsahip_mesher *m, double tol, int32_t *levels, int64_t *evals)

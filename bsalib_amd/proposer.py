@@ -334,38 +334,89 @@ def evaluate(prob, proposer: Proposer, log=print, target: float = 1e-3) -> dict:
 THRESHOLDS = (8.0, 6.0, 5.0, 4.0, 3.0, 2.0, 1.0, 0.0, -1.0, -2.0, -4.0)
 
 
+def select_threshold(net: MLP, nval: int, target: float, thresholds=THRESHOLDS, log=print):
+    """The most economical threshold whose errors on the first nval held-out
+    decks all stay within half the target (the last one if none does)."""
+    res = []
+    for s in range(nval):
+        r = sweep(validation_structure(s), net, thresholds, plain=False)
+        log(f"[proposer] validation {r['name']}: converged {r['evals_conv']}; MLP " +
+            ", ".join(f"{th:g}: {m['evals']} / {m['err']:.1e}" for th, m in r["mlp"].items()))
+        res.append(r)
+    for th in thresholds:
+        if max(r["mlp"][th]["err"] for r in res) <= 0.5 * target:
+            return th, res
+    return thresholds[-1], res
+
+
+def learning_curve(sizes, nval: int, epochs: int, target: float, log=print,
+                   on_row=None, save_largest: Path = None) -> list:
+    """One training set of max(sizes) structures; for each size n, a network
+    trained on the tiles of the first n structures, its threshold chosen on
+    nval held-out decks, and its evaluations and error on the configuration-5
+    deck against the cheapest plain setting that meets the target.  on_row(rows)
+    is called after each size; save_largest keeps the largest size's network."""
+    nmax = max(sizes)
+    X, y, err, cost, grp = build_dataset(nmax, 0, log=log)
+    rows = []
+    for n in sorted(sizes):
+        sel = grp < n
+        net, mean, inv, losses = train(X[sel], y[sel], cost[sel], epochs=epochs, log=log)
+        th, _ = select_threshold(net, nval, target, log=log)
+        r = sweep(synth.config5(), net, [th], target=target)
+        m = r["mlp"][th]
+        rows.append(dict(structures=n, tiles=int(sel.sum()), refined=float((y[sel] > 0).mean()),
+                         loss=float(losses[-1]), threshold=th, evals_mlp=m["evals"],
+                         err_mlp=m["err"], cheapest_plain=r["cheapest_plain"],
+                         reduction_vs_cheapest_plain=r["cheapest_plain"]["evals"] / m["evals"]))
+        log(f"[proposer] curve {rows[-1]}")
+        if on_row:
+            on_row(rows)
+        if save_largest and n == nmax:
+            save(net, mean, inv, th, Path(save_largest),
+                 meta=dict(structures=n, epochs=epochs, samples=int(sel.sum()),
+                           refined=rows[-1]["refined"], loss0=float(losses[0]),
+                           loss=float(losses[-1]), validation=nval))
+            log(f"[proposer] saved {save_largest} threshold {th}")
+        net.close()
+    return rows
+
+
 def main():
     import argparse
+    import json
     ap = argparse.ArgumentParser(description="train the Mesher proposer on the GPU")
     ap.add_argument("--structures", type=int, default=200)
     ap.add_argument("--val", type=int, default=4)
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--out", default=str(WEIGHTS))
     ap.add_argument("--target", type=float, default=1e-3)
+    ap.add_argument("--curve", default="",
+                    help="comma-separated training-set sizes: write a learning curve (JSON, to "
+                         "--out, after each size) instead of a network")
+    ap.add_argument("--save-largest", default="",
+                    help="with --curve: also save the largest size's network there")
     args = ap.parse_args()
     abi.init(0)
-    X, y, err, cost, grp = build_dataset(args.structures, 0)
-    print(f"[proposer] {len(y)} tiles, {float((y > 0).mean()):.4f} refined", flush=True)
-    net, mean, inv, losses = train(X, y, cost, epochs=args.epochs)
-    res = []
-    for s in range(args.val):
-        r = sweep(validation_structure(s), net, THRESHOLDS, plain=False)
-        print(f"[proposer] validation {r['name']}: converged {r['evals_conv']}; MLP " +
-              ", ".join(f"{th:g}: {m['evals']} / {m['err']:.1e}" for th, m in r["mlp"].items()),
-              flush=True)
-        res.append(r)
-    # the most economical threshold whose validation errors all stay within
-    # half the target
-    best = THRESHOLDS[-1]
-    for th in THRESHOLDS:
-        if max(r["mlp"][th]["err"] for r in res) <= 0.5 * args.target:
-            best = th
-            break
+    log = lambda m: print(m, flush=True)  # noqa: E731
+    if args.curve:
+        def write(rows):
+            Path(args.out).write_text(json.dumps(dict(target=args.target, validation=args.val,
+                                                      epochs=args.epochs, rows=rows), indent=1))
+        learning_curve([int(v) for v in args.curve.split(",")], args.val, args.epochs,
+                       args.target, log=log, on_row=write,
+                       save_largest=Path(args.save_largest) if args.save_largest else None)
+        log(f"[proposer] wrote {args.out}")
+        return
+    X, y, err, cost, grp = build_dataset(args.structures, 0, log=log)
+    log(f"[proposer] {len(y)} tiles, {float((y > 0).mean()):.4f} refined")
+    net, mean, inv, losses = train(X, y, cost, epochs=args.epochs, log=log)
+    best, _ = select_threshold(net, args.val, args.target, log=log)
     save(net, mean, inv, best, Path(args.out),
          meta=dict(structures=args.structures, epochs=args.epochs, samples=int(len(y)),
                    refined=float((y > 0).mean()), loss0=float(losses[0]),
                    loss=float(losses[-1]), validation=args.val))
-    print(f"[proposer] saved {args.out} threshold {best}", flush=True)
+    log(f"[proposer] saved {args.out} threshold {best}")
 
 
 if __name__ == "__main__":

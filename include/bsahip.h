@@ -339,6 +339,15 @@ int bsahip_mesher_run(bsahip_mesher *m, const int32_t *levels, double *m3mf, dou
  * included, each point once).  Run bsahip_mesher_run with the levels for the
  * moments. */
 int bsahip_mesher_adapt(bsahip_mesher *m, double tol, int32_t *levels, int64_t *evals);
+/* bsahip_mesher_run for n Meshers (distinct, of any structures) at once:
+ * levels[i] (ntiles of Mesher i, or NULL for its rule-based levels; levels
+ * itself NULL: all rule-based), m3mf[i] / m3mr[i] its outputs (either may be
+ * NULL), counts (3 n, or NULL) its counts as bsahip_mesher_run's.  The BFM
+ * evaluations still run per Mesher; the FRF tables and the quadrature of every
+ * Mesher's tiles share one launch each (per model: full or ONLY_DIAG), which
+ * pays for many small structures.  No reference counterpart. */
+int bsahip_mesher_run_batch(int32_t n, bsahip_mesher *const *ms, const int32_t *const *levels,
+                            double *const *m3mf, double *const *m3mr, int64_t *counts);
 /* As bsahip_mesher_run, plus each tile's own m3mr contribution in tile_m3mr
  * (ntiles x NM^3; m3mr is then their sum). */
 int bsahip_mesher_run_tiles(bsahip_mesher *m, const int32_t *levels, double *tile_m3mr,

@@ -80,8 +80,8 @@ def test_level0_features_are_bfm_norms(gpu_lib):
     p = CASES["tiny_u"]()
     M = abi.Mesher(p, bkg_base_rfmnt=5, tile_cells=4, bkg_peakw=0.3)
     geom, _ = M.tiles()
-    st = M.features(ns=2)                        # corners of each tile
-    for t in (0, M.ntiles // 2, M.ntiles - 1):
+    st = M.features(ns=2)                        # corners of each tile (one launch for all)
+    for t in range(M.ntiles):
         f1 = np.array([geom[t, 0], geom[t, 1], geom[t, 0], geom[t, 1]])
         f2 = np.array([geom[t, 2], geom[t, 2], geom[t, 3], geom[t, 3]])
         want = np.linalg.norm(O.msh_points(p, f1, f2), axis=0)
@@ -133,3 +133,29 @@ def test_adaptive_levels_meet_their_target(gpu_lib, name, tol):
     assert rel_err(r["m3mr"], conv["m3mr"]) <= tol
     assert ev == r["evals_pre"] + r["evals_post"]
     assert ev <= conv["evals_pre"] + conv["evals_post"]
+
+
+def test_mesher_batch_equals_separate_runs(gpu_lib):
+    """bsahip_mesher_run_batch over Meshers of different structures, mode counts
+    and models (full and ONLY_DIAG), mixed and rule-based levels: each Mesher's
+    moments equal its own bsahip_mesher_run (to the order of the atomic sums)
+    and its counts are the same."""
+    rng = np.random.default_rng(11)
+    ms, lvs = [], []
+    for name, kw in (("tiny_u", {}), ("tiny_uvw", {}), ("config1", {}), ("tiny_dav", {}),
+                     ("tiny_u", {"only_diag": True})):
+        M = abi.Mesher(CASES[name](), bkg_base_rfmnt=5, tile_cells=2, bkg_peakw=0.25, **kw)
+        ms.append(M)
+        lvs.append(rng.choice([1, 2, 4, 8], size=M.ntiles).astype(np.int32))
+    lvs[2] = None                                 # rule-based levels for one of them
+    sep = [M.run(levels=lv) for M, lv in zip(ms, lvs)]
+    bat = abi.mesher_run_batch(ms, lvs)
+    for a, b in zip(sep, bat):
+        for k in ("m3mf", "m3mr"):
+            assert rel_err(b[k], a[k]) < 1e-13, k
+        assert (a["evals_pre"], a["evals_post"], a["brm_points"]) == \
+            (b["evals_pre"], b["evals_post"], b["brm_points"])
+    with pytest.raises(abi.BsaHipError):
+        abi.mesher_run_batch([ms[0], ms[0]])
+    for M in ms:
+        M.close()
