@@ -20,6 +20,7 @@ QUAD_VECT, QUAD_RIEMANN, QUAD_TRAPZ = 0 << 4, 1 << 4, 2 << 4
 SUM_TRUNCATED, SUM_EXACT = 0, 1 << 8
 NO_PSD, NO_BISP = 1 << 9, 1 << 10
 ONLY_DIAG = 1 << 11
+FRF_PSD, FRF_GRID, FRF_POINTS = 0, 1, 2
 EPI_GENERIC = 1 << 12          # testing: the generic quadrature epilogue only
 DETERMINISTIC = 1 << 13        # fixed-order (bitwise reproducible) moment sums
 # return codes (include/bsahip.h)
@@ -64,7 +65,8 @@ EXPORTS = ["bsahip_init", "bsahip_finalise", "bsahip_last_error", "bsahip_versio
            "bsahip_mesher_load", "bsahip_batch_create", "bsahip_batch_autotune",
            "bsahip_batch_run", "bsahip_batch_info", "bsahip_batch_destroy",
            "bsahip_mesher_destroy", "bsahip_modal_psd_points", "bsahip_modal_recombination",
-           "bsahip_mesher_adapt", "bsahip_dgemm_strided_batched", "bsahip_mesher_run_batch"]
+           "bsahip_mesher_adapt", "bsahip_dgemm_strided_batched", "bsahip_mesher_run_batch",
+           "bsahip_frf_apply", "bsahip_bfm_points_cls"]
 
 
 def lib():
@@ -240,6 +242,44 @@ def bfm_points(prob, fi, fj, with_brm: bool = True, bound: Bound = None):
                                    _ptr(fj, C.c_double), C.c_int32(0), _ptr(bfm, C.c_double),
                                    _ptr(brm, C.c_double)))
     return bfm, brm
+
+
+def bfm_points_cls(prob, fi, fj, with_brm: bool = True, bound: Bound = None):
+    """Modal bispectra at points of the Classic grid of prob (nfreqs, df) with
+    its truncation, S(|f|) = 0 beyond (nfreqs - 1) df (bsahip_bfm_points_cls)."""
+    b = bound or Bound(prob)
+    fi = np.ascontiguousarray(fi, dtype=np.float64)
+    fj = np.ascontiguousarray(fj, dtype=np.float64)
+    n = fi.size
+    bfm = np.zeros((b.nm ** 3, n), order="F")
+    brm = np.zeros((b.nm ** 3, n), order="F") if with_brm else None
+    _check(lib().bsahip_bfm_points_cls(C.byref(b.struct), C.c_int32(prob.nfreqs),
+                                       C.c_double(prob.df), C.c_int32(n), _ptr(fi, C.c_double),
+                                       _ptr(fj, C.c_double), _ptr(bfm, C.c_double),
+                                       _ptr(brm, C.c_double)))
+    return bfm, brm
+
+
+def frf_apply(prob, kind: int, data, f1, f2=None, flags: int = 0, bound: Bound = None):
+    """FRF weighting of modal spectra (bsahip_frf_apply), returned as a new
+    array: kind FRF_PSD data (n1, NM^2); FRF_GRID data (n1, n2, NM^3);
+    FRF_POINTS data (NM^3, n1) at the points (f1[q], f2[q])."""
+    b = bound or Bound(prob)
+    f1 = np.ascontiguousarray(f1, dtype=np.float64).reshape(-1)
+    f2 = None if f2 is None else np.ascontiguousarray(f2, dtype=np.float64).reshape(-1)
+    out = np.array(data, dtype=np.float64, order="F", copy=True)
+    n2 = 0 if f2 is None else f2.size
+    nm = b.nm
+    want = {FRF_PSD: f1.size * nm * nm, FRF_GRID: f1.size * n2 * nm ** 3,
+            FRF_POINTS: f1.size * nm ** 3}.get(kind)
+    if want is not None and out.size != want:
+        raise ValueError(f"data holds {out.size} values, kind {kind} needs {want}")
+    if kind == FRF_POINTS and n2 != f1.size:
+        raise ValueError("FRF_POINTS needs one f2 per f1")
+    _check(lib().bsahip_frf_apply(C.byref(b.struct), C.c_int32(flags), C.c_int32(kind),
+                                  C.c_int64(f1.size), _ptr(f1, C.c_double), C.c_int64(n2),
+                                  _ptr(f2, C.c_double), _ptr(out, C.c_double)))
+    return out
 
 
 def modal_psd_points(prob, f, bound: Bound = None):

@@ -1183,12 +1183,42 @@ int bsahip_classic_bispectra_sel(const bsahip_problem *p, int32_t nfh, double df
    return 0;
 }
 
+}  // extern "C"
+
+namespace {
+int bfm_points_impl(const bsahip_problem *p, int32_t npts, const double *fi, const double *fj,
+                    double *bfm, double *brm, double fmax);
+}  // namespace
+
+extern "C" {
+
 int bsahip_bfm_points(const bsahip_problem *p, int32_t npts, const double *fi, const double *fj,
                       int32_t flags, double *bfm, double *brm)
 {
    std::lock_guard<std::recursive_mutex> lk(g.mu);
    CHK(ensure_init());
    CHK(check_flags(flags, false, false));
+   return bfm_points_impl(p, npts, fi, fj, bfm, brm, -1.0);
+}
+
+int bsahip_bfm_points_cls(const bsahip_problem *p, int32_t nfh, double df, int32_t npts,
+                          const double *fi, const double *fj, double *bfm, double *brm)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   CHK(ensure_init());
+   if (nfh < 2 || !(df > 0.0)) return fail(BSAHIP_EINVAL, "need nfh >= 2 and df > 0");
+   return bfm_points_impl(p, npts, fi, fj, bfm, brm, (nfh - 1) * df);
+}
+
+}  // extern "C"
+
+namespace {
+
+// Point BFM/BRM; fmax > 0: the Classic grid's truncation, S(|f|) = 0 beyond
+// fmax (it reaches only f1 + f2 of grid points)
+int bfm_points_impl(const bsahip_problem *p, int32_t npts, const double *fi, const double *fj,
+                    double *bfm, double *brm, double fmax)
+{
    if (npts < 0 || (npts > 0 && (!fi || !fj || !bfm)))
       return fail(BSAHIP_EINVAL, "npts < 0 or NULL point/output arrays");
    if (npts == 0) return 0;
@@ -1205,6 +1235,15 @@ int bsahip_bfm_points(const bsahip_problem *p, int32_t npts, const double *fi, c
    CHK(upload(B, "fl", uq, &dfl));
    CHK(B.get("U", (size_t)D.K * ldU, &U));
    CHK(build_table(B, D, dfl, nu, nu, U, ldU));
+   if (fmax > 0.0) {
+      // uq is ascending: columns u0.. lie beyond the grid
+      const long u0 = std::upper_bound(uq.begin(), uq.end(), fmax * (1.0 + 1e-12)) - uq.begin();
+      if (u0 < nu) {
+         const long n = (long)D.K * D.nm * (nu - u0);
+         k_zero_cols<<<(unsigned)((n + 255) / 256), 256, 0, g.s>>>(U, ldU, nu, u0, D.nm, n);
+         HIPC(hipGetLastError());
+      }
+   }
    std::vector<double> f1(fi, fi + npts), f2(fj, fj + npts);
    int *di1, *di2, *di3;
    double *df1, *df2, *ob, *orr = nullptr;
@@ -1225,6 +1264,48 @@ int bsahip_bfm_points(const bsahip_problem *p, int32_t npts, const double *fi, c
    HIPC(hipGetLastError());
    HIPC(hipMemcpyAsync(bfm, ob, nm3 * npts * sizeof(double), hipMemcpyDeviceToHost, g.s));
    if (brm) HIPC(hipMemcpyAsync(brm, orr, nm3 * npts * sizeof(double), hipMemcpyDeviceToHost, g.s));
+   HIPC(hipStreamSynchronize(g.s));
+   return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bsahip_frf_apply(const bsahip_problem *p, int32_t flags, int32_t kind, int64_t n1,
+                     const double *f1, int64_t n2, const double *f2, double *data)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   CHK(ensure_init());
+   if (flags & BSAHIP_ONLY_DIAG) return fail(BSAHIP_EUNSUPPORTED, "FRF weighting of the full model only");
+   if (kind < BSAHIP_FRF_PSD || kind > BSAHIP_FRF_POINTS)
+      return fail(BSAHIP_EINVAL, "kind must be BSAHIP_FRF_PSD, _GRID or _POINTS");
+   if (n1 < 0 || n2 < 0 || (kind == BSAHIP_FRF_GRID && n2 == 0 && n1 > 0))
+      return fail(BSAHIP_EINVAL, "negative sizes, or a grid without f2");
+   if (n1 == 0) return 0;
+   if (!f1 || !data || (kind != BSAHIP_FRF_PSD && !f2))
+      return fail(BSAHIP_EINVAL, "NULL frequencies or data");
+   Bufs &B = oneshot_bufs();
+   Prob D;
+   CHK(load_problem(p, 0, B, D));
+   const long nm = D.nm, nm2 = nm * nm, nm3 = nm2 * nm;
+   const long nf2 = kind == BSAHIP_FRF_GRID ? n2 : (kind == BSAHIP_FRF_POINTS ? n1 : 0);
+   const double tot_d = kind == BSAHIP_FRF_PSD ? (double)n1 * nm2
+                       : (kind == BSAHIP_FRF_GRID ? (double)n1 * n2 * nm3 : (double)n1 * nm3);
+   if (tot_d > (double)0xffffff00L)  // one thread per value, grid within 2^32 threads
+      return fail(BSAHIP_EUNSUPPORTED, "more than 2^32 spectral values in one call");
+   const long total = (long)tot_d;
+   std::vector<double> h1(f1, f1 + n1), h2;
+   if (nf2) h2.assign(f2, f2 + nf2);
+   double *d1, *d2 = nullptr, *dd;
+   CHK(upload(B, "frf_f1", h1, &d1));
+   if (nf2) CHK(upload(B, "frf_f2", h2, &d2));
+   CHK(B.get("frf_data", (size_t)total, &dd));
+   HIPC(hipMemcpyAsync(dd, data, (size_t)total * sizeof(double), hipMemcpyHostToDevice, g.s));
+   k_frf_apply<<<(unsigned)((total + 255) / 256), 256, 0, g.s>>>(kind, n1, d1, n2, d2, D.nm, D.Mm,
+                                                                  D.Km, D.Cm, total, dd);
+   HIPC(hipGetLastError());
+   HIPC(hipMemcpyAsync(data, dd, (size_t)total * sizeof(double), hipMemcpyDeviceToHost, g.s));
    HIPC(hipStreamSynchronize(g.s));
    return 0;
 }

@@ -1394,6 +1394,66 @@ __global__ void k_mfma_f64_layout(double *__restrict__ out)
 // are not a symmetric uniform grid (BsaCL computation frequencies).
 // grid (ceil(nm3/256), nchunk).
 // ---------------------------------------------------------------------------
+// FRF weighting of given modal spectra (getRM_full_vect_cls_,
+// functionsImpl.F90:1598-1738, and getRM_full_scalar_msh_, :1031-1115), in
+// place, H_m(f) = 1 / (K_m - w^2 M_m + i w C_mm), w = 2 pi f:
+//   kind 0: psd(n1, NM^2), column n + p NM: times Re[H_p(f1_i) conj H_n(f1_i)]
+//   kind 1: bisp(n1, n2, NM^3), entry m + n NM + p NM^2 at (f1_i, f2_j):
+//           times Re[H_m(f1_i) H_n(f2_j) conj H_p(f1_i + f2_j)]
+//   kind 2: bisp(NM^3, n1) at the points (f1_q, f2_q): the same factor.
+// grid (ceil(total/256)), block 256
+// ---------------------------------------------------------------------------
+// U[k*ldU + y*nu + u] = 0 for u >= u0 (n = K * nm * (nu - u0) values)
+__global__ void k_zero_cols(double *__restrict__ U, long ldU, int nu, long u0, int nm, long n)
+{
+   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= n) return;
+   const long w = nu - u0;
+   const long u = u0 + t % w, y = (t / w) % nm, k = t / (w * nm);
+   U[k * ldU + y * nu + u] = 0.0;
+}
+
+__device__ __forceinline__ double2 frf_at(double f, const double *Mm, const double *Km,
+                                          const double *Cm, int m)
+{
+   const double w = 2.0 * M_PI * f;
+   const double r = Km[m] - w * w * Mm[m], im = w * Cm[m], d = r * r + im * im;
+   return make_double2(r / d, -im / d);
+}
+
+__global__ void k_frf_apply(int kind, long n1, const double *__restrict__ f1, long n2,
+                            const double *__restrict__ f2, int nm, const double *__restrict__ Mm,
+                            const double *__restrict__ Km, const double *__restrict__ Cm,
+                            long total, double *__restrict__ data)
+{
+   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (t >= total) return;
+   if (kind == 0) {
+      const long i = t % n1, c = t / n1;
+      const int n = (int)(c % nm), p = (int)(c / nm);
+      const double2 hn = frf_at(f1[i], Mm, Km, Cm, n), hp = frf_at(f1[i], Mm, Km, Cm, p);
+      data[t] *= hp.x * hn.x + hp.y * hn.y;
+      return;
+   }
+   double a, b;
+   long e;
+   if (kind == 1) {
+      const long i = t % n1, j = (t / n1) % n2;
+      e = t / (n1 * n2);
+      a = f1[i];
+      b = f2[j];
+   } else {
+      const long nm3 = (long)nm * nm * nm;
+      const long q = t / nm3;
+      e = t % nm3;
+      a = f1[q];
+      b = f2[q];
+   }
+   const int m = (int)(e % nm), n = (int)((e / nm) % nm), p = (int)(e / ((long)nm * nm));
+   data[t] *= re3(frf_at(a, Mm, Km, Cm, m), frf_at(b, Mm, Km, Cm, n), frf_at(a + b, Mm, Km, Cm, p));
+}
+
+// ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bfm_points_sum(
    const double *__restrict__ U, long ldU, int rows, const double *__restrict__ pb, int K,
    int nm, int npts, const int *__restrict__ i1, const int *__restrict__ i2,

@@ -17,6 +17,9 @@ module BsaHip_C
    integer(c_int32_t), parameter, public :: BSAHIP_NO_PSD = 512
    integer(c_int32_t), parameter, public :: BSAHIP_NO_BISP = 1024
    integer(c_int32_t), parameter, public :: BSAHIP_ONLY_DIAG = 2048
+   integer(c_int32_t), parameter, public :: BSAHIP_FRF_PSD = 0
+   integer(c_int32_t), parameter, public :: BSAHIP_FRF_GRID = 1
+   integer(c_int32_t), parameter, public :: BSAHIP_FRF_POINTS = 2
 
    type, bind(c), public :: bsahip_problem_t
       integer(c_int32_t) :: nn = 0, nnl = 0, nm = 0, ntc = 0, ndegw = 0, nwz = 0
@@ -41,6 +44,7 @@ module BsaHip_C
    public :: bsahip_mesher_adapt
    public :: bsahip_mesher_ntiles
    public :: bsahip_last_error_f, bsahip_modal_psd_points, bsahip_modal_recombination
+   public :: bsahip_classic_spectra, bsahip_frf_apply, bsahip_bfm_points_cls
 
    interface
       integer(c_int) function bsahip_init(device) bind(c, name="bsahip_init")
@@ -76,6 +80,33 @@ module BsaHip_C
          type(bsahip_problem_t), intent(in) :: p
          integer(c_int32_t), value :: npts, flags
          type(c_ptr), value :: fi, fj, bfm, brm
+      end function
+
+      integer(c_int) function bsahip_bfm_points_cls(p, nfh, df, npts, fi, fj, bfm, brm) &
+            bind(c, name="bsahip_bfm_points_cls")
+         import c_int, c_int32_t, c_double, c_ptr, bsahip_problem_t
+         type(bsahip_problem_t), intent(in) :: p
+         integer(c_int32_t), value :: nfh, npts
+         real(c_double), value :: df
+         type(c_ptr), value :: fi, fj, bfm, brm
+      end function
+
+      integer(c_int) function bsahip_classic_spectra(p, nfh, df, flags, psd_f, psd_r, bfm, brm) &
+            bind(c, name="bsahip_classic_spectra")
+         import c_int, c_int32_t, c_double, c_ptr, bsahip_problem_t
+         type(bsahip_problem_t), intent(in) :: p
+         integer(c_int32_t), value :: nfh, flags
+         real(c_double), value :: df
+         type(c_ptr), value :: psd_f, psd_r, bfm, brm
+      end function
+
+      integer(c_int) function bsahip_frf_apply(p, flags, kind, n1, f1, n2, f2, data) &
+            bind(c, name="bsahip_frf_apply")
+         import c_int, c_int32_t, c_int64_t, c_ptr, bsahip_problem_t
+         type(bsahip_problem_t), intent(in) :: p
+         integer(c_int32_t), value :: flags, kind
+         integer(c_int64_t), value :: n1, n2
+         type(c_ptr), value :: f1, f2, data
       end function
 
       integer(c_int) function bsahip_mesher_create(p, cfg, flags, mesher) &

@@ -247,6 +247,69 @@ module BsaLib
    real(bsa_real_t), allocatable, target :: lp_psi_(:, :, :), lp_Mk_(:), lp_Kk_(:), lp_Ck_(:)
    real(bsa_real_t), allocatable, target :: lp_tscl_(:, :, :), lp_tstd_(:, :), lp_umref_(:)
    integer(bsa_int_t), allocatable, target :: lp_tcs_(:)
+   ! the Mesher's problem (POD settings), Classic grid of the last Classic run
+   type(bsahip_problem_t), save :: lp_msh_
+   integer(bsa_int_t) :: lp_nfh_ = 0
+   real(bsa_real_t) :: lp_df_ = 0
+
+   ! ---- operator slots (src/bsa/data/BsaLibData.F90:166-310) ----
+   ! The reference's spectral operators behind procedure pointers, bound by
+   ! bsa_Run for the full modal model (the ONLY_DIAG model leaves them null);
+   ! the Classic ones once a Classic run has fixed the grid.  Each evaluates
+   ! on the device from the problem of the last bsa_Run: the nodal wind PSD
+   ! arguments (Suvw, Suvw_pad) are taken for the interface and checked for
+   ! shape only, and the scalar BFM is not pure (it calls the library).
+   ! Thread-safe: the library serialises calls.
+   abstract interface
+      subroutine getBFMClsVect(f, Suvw, psd, bisp)
+         import :: bsa_real_t
+         real(bsa_real_t), intent(in) :: f(:)
+         real(bsa_real_t), intent(in) :: Suvw(:, :)
+         real(bsa_real_t), allocatable, intent(inout) :: psd(:, :), bisp(:, :, :)
+      end subroutine
+
+      subroutine getBRMClsVect(f, psd, bisp)
+         import :: bsa_real_t
+         real(bsa_real_t), intent(in) :: f(:)
+         real(bsa_real_t), allocatable, intent(inout) :: psd(:, :), bisp(:, :, :)
+      end subroutine
+
+      subroutine getBFMClsScalar(ii, ij, fi, fj, Suvw, Suvw_pad, psd, bisp)
+         import :: bsa_int_t, bsa_real_t
+         integer(bsa_int_t), intent(in) :: ii, ij
+         real(bsa_real_t), intent(in) :: fi, fj
+         real(bsa_real_t), intent(in) :: Suvw(:, :), Suvw_pad(:)
+         real(bsa_real_t), intent(inout) :: psd(:), bisp(:)
+      end subroutine
+
+      subroutine getBRMClsScalar(ii, ij, fi, fj, psdin, psdout, bispin, bispout)
+         import :: bsa_int_t, bsa_real_t
+         integer(bsa_int_t), intent(in) :: ii, ij
+         real(bsa_real_t), intent(in) :: fi, fj
+         real(bsa_real_t), intent(in) :: psdin(:), bispin(:)
+         real(bsa_real_t), intent(out) :: psdout(:), bispout(:)
+      end subroutine
+
+      subroutine getMshBFM(bfm, fi, fj)
+         import :: bsa_real_t
+         real(bsa_real_t), intent(inout), contiguous :: bfm(:, :)
+         real(bsa_real_t), intent(in), contiguous :: fi(:), fj(:)
+      end subroutine
+
+      subroutine getMshBRM(brm, fi, fj, bfm)
+         import :: bsa_real_t
+         real(bsa_real_t), intent(inout), contiguous :: brm(:, :)
+         real(bsa_real_t), intent(in), contiguous :: fi(:), fj(:)
+         real(bsa_real_t), intent(in), contiguous :: bfm(:, :)
+      end subroutine
+   end interface
+   public :: getBFMClsVect, getBRMClsVect, getBFMClsScalar, getBRMClsScalar, getMshBFM, getMshBRM
+   procedure(getBFMClsVect), pointer, public :: getBFM_vect_cls => null()
+   procedure(getBRMClsVect), pointer, public :: getBRM_vect_cls => null()
+   procedure(getBFMClsScalar), pointer, public :: getBFM_scalar_cls => null()
+   procedure(getBRMClsScalar), pointer, public :: getBRM_scalar_cls => null()
+   procedure(getMshBFM), pointer, public :: getBFM_msh => null()
+   procedure(getMshBRM), pointer, public :: getBRM_msh => null()
 
 contains
 
@@ -306,6 +369,7 @@ contains
       gen_inputs_ = .false.
       run_after_gen_ = .true.
       lp_ready_ = .false.
+      call unbindSlots_()
       nn_ = 0; nlibs_ = 0; ndofs_ = 0; nm_ = 0; nz_lim_ = 0; ipsd_ = 0
       if (device_ready_) rc = bsahip_finalise()
       device_ready_ = .false.
@@ -839,6 +903,12 @@ contains
       lp_%Km = c_loc(lp_Kk_)
       lp_%Cm = c_loc(lp_Ck_)
       lp_ready_ = .true.
+      lp_msh_ = meshProblem_(lp_)
+      call unbindSlots_()
+      if (ionly_diag_ == 0) then
+         getBFM_msh => opBFMMsh_
+         getBRM_msh => opBRMMsh_
+      end if
 
       if (.not. device_ready_) then
          rc = bsahip_init(0_c_int)
@@ -851,6 +921,14 @@ contains
       if (.not. do_cls) return
 
       call classicGrid_(nfh, df)
+      lp_nfh_ = nfh
+      lp_df_ = df
+      if (ionly_diag_ == 0) then
+         getBFM_vect_cls => opBFMVectCls_
+         getBRM_vect_cls => opBRMVectCls_
+         getBFM_scalar_cls => opBFMScalarCls_
+         getBRM_scalar_cls => opBRMScalarCls_
+      end if
 
       ! Classic version as checkMaxAllocation_ (BsaClassicImpl.F90:384-434): outside
       ! test mode a vectorised bispectrum above 1e9 elements switches to the scalar
@@ -965,16 +1043,9 @@ contains
          ! POD BFM (isvd = 1): coherence column 1 for every component (the POD
          ! path passes idir = 1, functionsImpl.F90:628-629, 770); truncation
          ! settings validated as validateAll_ (BsaLibImpl.F90:479-486)
-         pm = p
-         if (isvd_ == 1) then
-            pm%corr_col_mode = 1
-            if (do_trunc_pod_ .and. (pod_trunc_lim_ == 0.0_bsa_real_t .or. &
-                                     pod_trunc_lim_ == 1.0_bsa_real_t)) do_trunc_pod_ = .false.
-            if (npod_set_ .and. (npod_ <= 0 .or. npod_ >= p%nnl)) npod_set_ = .false.
-            if (npod_set_) write(output_unit, '(/ 1x, 2a, i0, a /)') INFOMSG, 'Using  ', npod_, '  POD modes.'
-            if (do_trunc_pod_) pm%pod_trunc_lim = pod_trunc_lim_
-            if (npod_set_) pm%pod_nmodes = npod_
-         end if
+         pm = meshProblem_(p)
+         if (isvd_ == 1 .and. npod_set_) &
+            write(output_unit, '(/ 1x, 2a, i0, a /)') INFOMSG, 'Using  ', npod_, '  POD modes.'
          ! ONLY_DIAG: NM entries (dimM_bisp_ = NM), no POD (BsaMesherImpl.F90:68)
          nc = nmk**3
          mflags = 0
@@ -1015,6 +1086,247 @@ contains
          allocate(a(n), source=0.0_bsa_real_t)
       end subroutine
    end subroutine bsa_Run
+
+   !> The Mesher's problem: with the POD BFM (isvd = 1) coherence column 1 for
+   !> every component (the POD path passes idir = 1, functionsImpl.F90:628-629,
+   !> 770) and the truncation settings validated as validateAll_
+   !> (BsaLibImpl.F90:479-486).
+   function meshProblem_(p) result(pm)
+      type(bsahip_problem_t), intent(in) :: p
+      type(bsahip_problem_t) :: pm
+      pm = p
+      if (isvd_ /= 1) return
+      pm%corr_col_mode = 1
+      if (do_trunc_pod_ .and. (pod_trunc_lim_ == 0.0_bsa_real_t .or. &
+                               pod_trunc_lim_ == 1.0_bsa_real_t)) do_trunc_pod_ = .false.
+      if (npod_set_ .and. (npod_ <= 0 .or. npod_ >= p%nnl)) npod_set_ = .false.
+      if (do_trunc_pod_) pm%pod_trunc_lim = pod_trunc_lim_
+      if (npod_set_) pm%pod_nmodes = npod_
+   end function
+
+   subroutine unbindSlots_()
+      nullify(getBFM_vect_cls, getBRM_vect_cls, getBFM_scalar_cls, getBRM_scalar_cls)
+      nullify(getBFM_msh, getBRM_msh)
+   end subroutine
+
+   subroutine slotCheck_(rc, what)
+      integer(c_int), intent(in) :: rc
+      character(len=*), intent(in) :: what
+      if (rc /= 0) call bsa_Abort(what//' failed: '//bsahip_last_error_f())
+   end subroutine
+
+   !> Half-length and spacing of a symmetric Classic grid f(i) = (i - nfh) df.
+   subroutine slotGrid_(f, nfh, df)
+      real(bsa_real_t), intent(in) :: f(:)
+      integer(bsa_int_t), intent(out) :: nfh
+      real(bsa_real_t), intent(out) :: df
+      integer(bsa_int_t) :: nf
+      nf = size(f)
+      if (nf < 3 .or. mod(nf, 2) == 0) call bsa_Abort('The Classic grid needs an odd number (>= 3) of frequencies.')
+      nfh = (nf + 1) / 2
+      df = f(2) - f(1)
+      if (.not. df > 0 .or. abs(f(nfh)) > 1e-9_bsa_real_t * df .or. &
+          abs(f(1) + f(nf)) > 1e-9_bsa_real_t * df * nf) &
+         call bsa_Abort('The Classic grid must be symmetric about f = 0 with a constant step.')
+   end subroutine
+
+   !> getFM_full_tnm_vect_cls_ (functionsImpl.F90:1312-1587): modal PSDs
+   !> psd(NF, NM^2) and, when bispectra are computed, bisp(NF, NF, NM^3) on the
+   !> grid f, S(f1 + f2) zero beyond the grid; allocated here as there.
+   subroutine opBFMVectCls_(f, Suvw, psd, bisp)
+      real(bsa_real_t), intent(in) :: f(:)
+      real(bsa_real_t), intent(in) :: Suvw(:, :)
+      real(bsa_real_t), allocatable, intent(inout) :: psd(:, :), bisp(:, :, :)
+      integer(bsa_int_t) :: nfh, nf, nm
+      real(bsa_real_t) :: df
+      call slotGrid_(f, nfh, df)
+      nf = size(f)
+      nm = lp_%nm
+      if (size(Suvw, 1) /= nf) call bsa_Abort('getBFM_vect_cls: Suvw needs one row per frequency.')
+      if (allocated(psd)) deallocate(psd)
+      if (allocated(bisp)) deallocate(bisp)
+      allocate(psd(nf, nm * nm))
+      if (icomp_bisp_ /= 0) then
+         allocate(bisp(nf, nf, nm**3))
+         call spectra_(psd, bisp)
+      else
+         call psdOnly_(psd)
+      end if
+   contains
+      subroutine spectra_(ps, bs)
+         real(bsa_real_t), target, intent(inout) :: ps(nf, nm * nm), bs(nf, nf, nm**3)
+         call slotCheck_(bsahip_classic_spectra(lp_, int(nfh, c_int32_t), df, 0_c_int32_t, &
+                                                c_loc(ps), c_null_ptr, c_loc(bs), c_null_ptr), &
+                         'getBFM_vect_cls')
+      end subroutine
+      subroutine psdOnly_(ps)
+         real(bsa_real_t), target, intent(inout) :: ps(nf, nm * nm)
+         call slotCheck_(bsahip_classic_spectra(lp_, int(nfh, c_int32_t), df, 0_c_int32_t, &
+                                                c_loc(ps), c_null_ptr, c_null_ptr, c_null_ptr), &
+                         'getBFM_vect_cls')
+      end subroutine
+   end subroutine
+
+   !> getRM_full_vect_cls_ (functionsImpl.F90:1598-1738): psd(:, n + (p-1) NM)
+   !> times Re[H_p conj H_n] at f, bisp(i, j, m + (n-1) NM + (p-1) NM^2) times
+   !> Re[H_m(f_i) H_n(f_j) conj H_p(f_i + f_j)], in place; no state kept from
+   !> one call to the next.
+   subroutine opBRMVectCls_(f, psd, bisp)
+      real(bsa_real_t), intent(in) :: f(:)
+      real(bsa_real_t), allocatable, intent(inout) :: psd(:, :), bisp(:, :, :)
+      integer(bsa_int_t) :: nf, nm
+      nf = size(f)
+      nm = lp_%nm
+      if (allocated(psd)) then
+         if (size(psd, 1) /= nf .or. size(psd, 2) /= nm * nm) call bsa_Abort('getBRM_vect_cls: psd is not (NF, NM^2).')
+         call frfPsd_(f, psd)
+      end if
+      if (allocated(bisp) .and. icomp_bisp_ /= 0) then
+         if (size(bisp, 1) /= nf .or. size(bisp, 2) /= nf .or. size(bisp, 3) /= nm**3) &
+            call bsa_Abort('getBRM_vect_cls: bisp is not (NF, NF, NM^3).')
+         call frfGrid_(f, bisp)
+      end if
+   contains
+      subroutine frfPsd_(fr, ps)
+         real(bsa_real_t), target, intent(in) :: fr(nf)
+         real(bsa_real_t), target, intent(inout) :: ps(nf, nm * nm)
+         call slotCheck_(bsahip_frf_apply(lp_, 0_c_int32_t, BSAHIP_FRF_PSD, int(nf, c_int64_t), &
+                                          c_loc(fr), 0_c_int64_t, c_null_ptr, c_loc(ps)), &
+                         'getBRM_vect_cls')
+      end subroutine
+      subroutine frfGrid_(fr, bs)
+         real(bsa_real_t), target, intent(in) :: fr(nf)
+         real(bsa_real_t), target, intent(inout) :: bs(nf, nf, nm**3)
+         call slotCheck_(bsahip_frf_apply(lp_, 0_c_int32_t, BSAHIP_FRF_GRID, int(nf, c_int64_t), &
+                                          c_loc(fr), int(nf, c_int64_t), c_loc(fr), c_loc(bs)), &
+                         'getBRM_vect_cls')
+      end subroutine
+   end subroutine
+
+   !> getFM_full_tnm_scalar_cls_ (functionsImpl.F90:2122-2318) at (fi, fj), a
+   !> point of the grid of the last Classic run: bisp(NM^3) with S(fi + fj)
+   !> zero beyond that grid, psd(NM^2) at fi when ij = 1 and zero otherwise.
+   subroutine opBFMScalarCls_(ii, ij, fi, fj, Suvw, Suvw_pad, psd, bisp)
+      integer(bsa_int_t), intent(in) :: ii, ij
+      real(bsa_real_t), intent(in) :: fi, fj
+      real(bsa_real_t), intent(in) :: Suvw(:, :), Suvw_pad(:)
+      real(bsa_real_t), intent(inout) :: psd(:), bisp(:)
+      integer(bsa_int_t) :: nm
+      nm = lp_%nm
+      if (size(psd) < nm * nm .or. size(bisp) < nm**3) &
+         call bsa_Abort('getBFM_scalar_cls: psd needs NM^2 and bisp NM^3 entries.')
+      if (size(Suvw, 2) /= size(Suvw_pad) .or. ii < 1 .or. ij < 1) &
+         call bsa_Abort('getBFM_scalar_cls: bad grid indexes or Suvw / Suvw_pad shapes.')
+      psd = 0
+      bisp = 0
+      if (icomp_bisp_ /= 0) call point_(bisp)
+      if (ij == 1) call psdAt_(psd)
+   contains
+      subroutine point_(bs)
+         real(bsa_real_t), target, intent(inout) :: bs(nm**3)
+         real(bsa_real_t), target :: a(1), b(1)
+         a(1) = fi
+         b(1) = fj
+         call slotCheck_(bsahip_bfm_points_cls(lp_, int(lp_nfh_, c_int32_t), lp_df_, 1_c_int32_t, &
+                                               c_loc(a), c_loc(b), c_loc(bs), c_null_ptr), &
+                         'getBFM_scalar_cls')
+      end subroutine
+      subroutine psdAt_(ps)
+         real(bsa_real_t), target, intent(inout) :: ps(nm * nm)
+         real(bsa_real_t), target :: a(1)
+         a(1) = fi
+         call slotCheck_(bsahip_modal_psd_points(lp_, 1_c_int32_t, c_loc(a), c_loc(ps)), &
+                         'getBFM_scalar_cls')
+      end subroutine
+   end subroutine
+
+   !> getRM_full_scalar_cls_ (functionsImpl.F90:2329-2450): psdout at fi when
+   !> ij = 1 (zero otherwise), bispout at (fi, fj), each psdin / bispin times
+   !> the FRF products of getBRM_vect_cls.
+   subroutine opBRMScalarCls_(ii, ij, fi, fj, psdin, psdout, bispin, bispout)
+      integer(bsa_int_t), intent(in) :: ii, ij
+      real(bsa_real_t), intent(in) :: fi, fj
+      real(bsa_real_t), intent(in) :: psdin(:), bispin(:)
+      real(bsa_real_t), intent(out) :: psdout(:), bispout(:)
+      integer(bsa_int_t) :: nm
+      nm = lp_%nm
+      if (size(psdin) < nm * nm .or. size(psdout) < nm * nm .or. size(bispin) < nm**3 .or. &
+          size(bispout) < nm**3 .or. ii < 1 .or. ij < 1) &
+         call bsa_Abort('getBRM_scalar_cls: psd needs NM^2 and bisp NM^3 entries.')
+      psdout = 0
+      bispout = 0
+      if (ij == 1) then
+         psdout(1:nm * nm) = psdin(1:nm * nm)
+         call frfPsd_(psdout)
+      end if
+      if (icomp_bisp_ /= 0) then
+         bispout(1:nm**3) = bispin(1:nm**3)
+         call frfPt_(bispout)
+      end if
+   contains
+      subroutine frfPsd_(ps)
+         real(bsa_real_t), target, intent(inout) :: ps(nm * nm)
+         real(bsa_real_t), target :: a(1)
+         a(1) = fi
+         call slotCheck_(bsahip_frf_apply(lp_, 0_c_int32_t, BSAHIP_FRF_PSD, 1_c_int64_t, c_loc(a), &
+                                          0_c_int64_t, c_null_ptr, c_loc(ps)), 'getBRM_scalar_cls')
+      end subroutine
+      subroutine frfPt_(bs)
+         real(bsa_real_t), target, intent(inout) :: bs(nm**3)
+         real(bsa_real_t), target :: a(1), b(1)
+         a(1) = fi
+         b(1) = fj
+         call slotCheck_(bsahip_frf_apply(lp_, 0_c_int32_t, BSAHIP_FRF_POINTS, 1_c_int64_t, c_loc(a), &
+                                          1_c_int64_t, c_loc(b), c_loc(bs)), 'getBRM_scalar_cls')
+      end subroutine
+   end subroutine
+
+   !> getBFM_msh (functionsImpl.F90:247-465, the POD form at :582-1020 when
+   !> isvd = 1): bfm(NM^3, npts) at the points (fi(q), fj(q)), S(fi + fj)
+   !> evaluated exactly.
+   subroutine opBFMMsh_(bfm, fi, fj)
+      real(bsa_real_t), intent(inout), contiguous :: bfm(:, :)
+      real(bsa_real_t), intent(in), contiguous :: fi(:), fj(:)
+      integer(bsa_int_t) :: n, nm
+      n = size(fi)
+      nm = lp_msh_%nm
+      if (size(fj) /= n .or. size(bfm, 1) /= nm**3 .or. size(bfm, 2) < n) &
+         call bsa_Abort('getBFM_msh: bfm must be (NM^3, npts) with one fj per fi.')
+      if (n == 0) return
+      call pts_(bfm, fi, fj)
+   contains
+      subroutine pts_(b, a1, a2)
+         real(bsa_real_t), target, intent(inout) :: b(nm**3, n)
+         real(bsa_real_t), target, intent(in) :: a1(n), a2(n)
+         call slotCheck_(bsahip_bfm_points(lp_msh_, int(n, c_int32_t), c_loc(a1), c_loc(a2), &
+                                           0_c_int32_t, c_loc(b), c_null_ptr), 'getBFM_msh')
+      end subroutine
+   end subroutine
+
+   !> getBRM_msh (functionsImpl.F90:1031-1115): brm = bfm times
+   !> Re[H_m(fi) H_n(fj) conj H_p(fi + fj)] at each point.
+   subroutine opBRMMsh_(brm, fi, fj, bfm)
+      real(bsa_real_t), intent(inout), contiguous :: brm(:, :)
+      real(bsa_real_t), intent(in), contiguous :: fi(:), fj(:)
+      real(bsa_real_t), intent(in), contiguous :: bfm(:, :)
+      integer(bsa_int_t) :: n, nm
+      n = size(fi)
+      nm = lp_%nm
+      if (size(fj) /= n .or. size(brm, 1) /= nm**3 .or. size(bfm, 1) /= nm**3 .or. &
+          size(brm, 2) < n .or. size(bfm, 2) < n) &
+         call bsa_Abort('getBRM_msh: brm and bfm must be (NM^3, npts) with one fj per fi.')
+      if (n == 0) return
+      brm(:, 1:n) = bfm(:, 1:n)
+      call frf_(brm, fi, fj)
+   contains
+      subroutine frf_(b, a1, a2)
+         real(bsa_real_t), target, intent(inout) :: b(nm**3, n)
+         real(bsa_real_t), target, intent(in) :: a1(n), a2(n)
+         call slotCheck_(bsahip_frf_apply(lp_, 0_c_int32_t, BSAHIP_FRF_POINTS, int(n, c_int64_t), &
+                                          c_loc(a1), int(n, c_int64_t), c_loc(a2), c_loc(b)), &
+                         'getBRM_msh')
+      end subroutine
+   end subroutine
 
    ! ======================================================================
    !  post-processing

@@ -198,12 +198,41 @@ int bsahip_classic_bispectra_sel(const bsahip_problem *p, int32_t nfh, double df
                                  int32_t nsel, const int32_t *sel, double *bfm, double *brm);
 
 /*
+ * FRF weighting of given modal spectra, in place (the reference's response
+ * operators getRM_full_vect_cls_, functionsImpl.F90:1598-1738, and
+ * getRM_full_scalar_msh_, :1031-1115, full model):
+ * H_m(f) = 1 / (K_m - w^2 M_m + i w C_mm), w = 2 pi f, over the kept modes.
+ *   BSAHIP_FRF_PSD:    data (n1, NM^2), column n + p NM (0-based) times
+ *                      Re[H_p(f1_i) conj H_n(f1_i)]; f2, n2 unused
+ *   BSAHIP_FRF_GRID:   data (n1, n2, NM^3), entry m + n NM + p NM^2 at
+ *                      (f1_i, f2_j) times Re[H_m(f1_i) H_n(f2_j) conj H_p(f1_i+f2_j)]
+ *   BSAHIP_FRF_POINTS: data (NM^3, n1) at the points (f1_q, f2_q): the same
+ * The reference's vectorised operator keeps its column counters from one call
+ * to the next (trap T3: correct on the first call only); this one has no state.
+ */
+#define BSAHIP_FRF_PSD    0
+#define BSAHIP_FRF_GRID   1
+#define BSAHIP_FRF_POINTS 2
+int bsahip_frf_apply(const bsahip_problem *p, int32_t flags, int32_t kind, int64_t n1,
+                     const double *f1, int64_t n2, const double *f2, double *data);
+
+/*
  * Modal bispectra at an arbitrary list of points (fi[q], fj[q]), S(f1+f2)
  * evaluated exactly: getBFM_msh / getBRM_msh (functionsImpl.F90:247-465,
  * 1031-1115).  bfm, brm: (NM^3, npts); brm may be NULL.
  */
 int bsahip_bfm_points(const bsahip_problem *p, int32_t npts, const double *fi,
                       const double *fj, int32_t flags, double *bfm, double *brm);
+
+/*
+ * The same at points of the Classic grid of (nfh, df) with its truncation:
+ * S(|f|) = 0 beyond (nfh-1) df, which reaches f1 + f2 only (the padded S of
+ * getFM_full_tnm_scalar_cls_, functionsImpl.F90:2122-2318, trap T2 read as
+ * zero): the point form of the reference's scalar Classic operators.  Full
+ * model, FP64.  bfm, brm: (NM^3, npts); brm may be NULL.
+ */
+int bsahip_bfm_points_cls(const bsahip_problem *p, int32_t nfh, double df, int32_t npts,
+                          const double *fi, const double *fj, double *bfm, double *brm);
 
 /*
  * Modal force PSD S_Q(n, p; f) at arbitrary frequencies: psd (NM*NM, npts),

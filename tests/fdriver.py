@@ -43,6 +43,8 @@ def run(prob, *opts, check=True, timeout=300, workdir=None):
         keys += ["bkg", "res", "m2_diag", "m3_diag", "m2o2_diag", "sk_diag", "m2_full",
                  "m3_full", "m2o2_full", "sk_full", "peak_g_diag", "peak_pos_diag",
                  "peak_neg_diag", "peak_g_full", "peak_pos_full", "peak_neg_full"]
+    if "slots" in opts:
+        keys += ["slot_psd_f", "slot_psd_r", "slot_bisp_f", "slot_bisp_r"]
     for k in keys:
         n = int(np.frombuffer(raw[o:o + 4], dtype=np.int32)[0])
         o += 4

@@ -6,7 +6,7 @@
 !!                        [pod] [podn=<modes>] [podt=<percent>] [post] [gen] [diag]
 !!                        [visual=<i>,<j>,<k>] [visualn=<node>,<dof>] [sym=<n>] [brmbase]
 !!                        [mtol=<tolerance>]
-!!                        [cb] [premesh] [modal]
+!!                        [cb] [premesh] [modal] [slots]
 !! The input layout is the one tests write for the reference harness (header
 !! of 14 int32, then integer and real64 arrays in Fortran order).  "grid"
 !! leaves test mode off and passes damping ratios, so bsa_Run applies the
@@ -29,6 +29,11 @@
 !! end, "CALLBACK <zones> <points> <sum of values>"; "premesh" calls
 !! bsa_enableOnlyPremesh, "modal" bsa_exportModalData; "mtol=" sets the
 !! Mesher's target error (bsa_setMesherTolerance; 0: rule-based levels).
+!! "slots" (with "both") calls the operator slots bsa_Run bound
+!! (getBFM_vect_cls ... getBRM_msh) on the Classic grid, prints
+!! "SLOT <check> <max relative difference>" for the scalar and Mesher
+!! operators against the vectorised ones at a strided set of grid points,
+!! and appends the vectorised psd_f, psd_r, bisp_f, bisp_r.
 !! Output: nme (int32), the used modes, then for each of m2mf, m2mr, m2o2mr,
 !! m3mf_cls, m3mr_cls, m3mf_msh, m3mr_msh: its size (int32, 0 when bsa_Run
 !! left it unallocated) and its values (real64).
@@ -70,7 +75,8 @@ program bsalib_driver
    real(real64), allocatable :: m3mf(:), m3mr(:)
    real(real64), allocatable :: bkg(:), res(:), m2d(:), m3d(:), m22d(:), skd(:), m2c(:), m3c(:)
    real(real64), allocatable :: m22c(:), skc(:), pgd(:), ppd(:), pnd(:), pgc(:), ppc(:), pnc(:)
-   logical :: post
+   logical :: post, slots
+   real(real64), allocatable :: sl_psdf(:, :), sl_psdr(:, :), sl_bf(:, :, :), sl_br(:, :, :)
    character(len=512) :: fin, fout, arg
    logical :: grid_checks
    integer :: uin, uout, isuban, isvd, npod
@@ -81,6 +87,7 @@ program bsalib_driver
    call get_command_argument(2, fout)
    grid_checks = .false.
    post = .false.
+   slots = .false.
    isuban = 1
    isvd = 0
 
@@ -113,6 +120,7 @@ program bsalib_driver
       if (trim(arg) == 'both') isuban = 3
       if (trim(arg) == 'pod') isvd = 1
       if (trim(arg) == 'post') post = .true.
+      if (trim(arg) == 'slots') slots = .true.
       if (trim(arg) == 'gen') call bsa_generateBSAInputFiles(.false.)
       if (trim(arg) == 'diag') call bsa_setSpectraExtension(1)
       if (arg(1:7) == 'visual=') then
@@ -180,6 +188,7 @@ program bsalib_driver
 
    call bsa_Run(m2mf, m2mr, m2o2mr, m3mf_msh, m3mr_msh, m3mf, m3mr)
    used = bsa_getUsedModeShapes()
+   if (slots) call check_slots()
    if (post) then
       call bsa_computeBRdecomp(m2mf, bkg, res)
       call bsa_modalRecombination(phi(:, used), m2mr, m3mr, m2o2mr, m2d, m3d, m22d, skd, &
@@ -215,6 +224,12 @@ program bsalib_driver
       call put(ppc)
       call put(pnc)
    end if
+   if (slots) then
+      call put_vals(reshape(sl_psdf, [size(sl_psdf)]))
+      call put_vals(reshape(sl_psdr, [size(sl_psdr)]))
+      call put_vals(reshape(sl_bf, [size(sl_bf)]))
+      call put_vals(reshape(sl_br, [size(sl_br)]))
+   end if
    close(uout)
    if (cb_zones > 0) print '(a, 1x, i0, 1x, i0, 1x, es25.17)', 'CALLBACK', cb_zones, cb_pts, cb_sum
    call bsa_Finalise()
@@ -227,5 +242,77 @@ contains
       else
          write(uout) 0_int32
       end if
+   end subroutine
+
+   subroutine put_vals(a)
+      real(real64), intent(in) :: a(:)
+      write(uout) int(size(a), int32), a
+   end subroutine
+
+   subroutine check_slots()
+      real(real64), allocatable :: f(:), suvw(:, :), spad(:), psd(:, :), bisp(:, :, :)
+      real(real64), allocatable :: p1(:), b1(:), pr1(:), br1(:), fi(:), fj(:), bm(:, :), rm(:, :)
+      real(real64) :: e(6), sf, sr, pf, pr
+      integer(int32) :: nf, n3, n2, ii, ij, st, q, npt
+      if (.not. (associated(getBFM_vect_cls) .and. associated(getBRM_vect_cls) .and. &
+                 associated(getBFM_scalar_cls) .and. associated(getBRM_scalar_cls) .and. &
+                 associated(getBFM_msh) .and. associated(getBRM_msh))) error stop 'slots not bound'
+      nf = 2 * nfh - 1
+      n2 = size(used)**2
+      n3 = size(used)**3
+      allocate(f(nf), suvw(nf, 1), spad(1), p1(n2), b1(n3), pr1(n2), br1(n3))
+      suvw = 0
+      spad = 0
+      do i = 1, nf
+         f(i) = (i - nfh) * df
+      end do
+      call getBFM_vect_cls(f, suvw, psd, bisp)
+      sl_psdf = psd
+      sl_bf = bisp
+      call getBRM_vect_cls(f, psd, bisp)
+      sl_psdr = psd
+      sl_br = bisp
+      sf = maxval(abs(sl_bf)); sr = maxval(abs(sl_br))
+      pf = maxval(abs(sl_psdf)); pr = maxval(abs(sl_psdr))
+      e = 0
+      st = max(1, nf / 9)
+      npt = 0
+      allocate(fi(nf * nf), fj(nf * nf))
+      do ii = 1, nf, st
+         do ij = 1, nf, st
+            call getBFM_scalar_cls(ii, ij, f(ii), f(ij), suvw, spad, p1, b1)
+            call getBRM_scalar_cls(ii, ij, f(ii), f(ij), p1, pr1, b1, br1)
+            e(1) = max(e(1), maxval(abs(b1 - sl_bf(ii, ij, :))) / sf)
+            e(2) = max(e(2), maxval(abs(br1 - sl_br(ii, ij, :))) / sr)
+            if (ij == 1) then
+               e(3) = max(e(3), maxval(abs(p1 - sl_psdf(ii, :))) / pf)
+               e(4) = max(e(4), maxval(abs(pr1 - sl_psdr(ii, :))) / pr)
+            else if (any(p1 /= 0) .or. any(pr1 /= 0)) then
+               error stop 'scalar psd not zero off the first column'
+            end if
+            ! Mesher points: inside the grid S(fi + fj) is the same on both paths
+            if (abs(f(ii) + f(ij)) <= f(nf)) then
+               npt = npt + 1
+               fi(npt) = f(ii)
+               fj(npt) = f(ij)
+            end if
+         end do
+      end do
+      allocate(bm(n3, npt), rm(n3, npt))
+      call getBFM_msh(bm, fi(1:npt), fj(1:npt))
+      call getBRM_msh(rm, fi(1:npt), fj(1:npt), bm)
+      do q = 1, npt
+         ii = nint(fi(q) / df) + nfh
+         ij = nint(fj(q) / df) + nfh
+         e(5) = max(e(5), maxval(abs(bm(:, q) - sl_bf(ii, ij, :))) / sf)
+         e(6) = max(e(6), maxval(abs(rm(:, q) - sl_br(ii, ij, :))) / sr)
+      end do
+      print '(a, 1x, a, 1x, es12.5)', 'SLOT', 'bfm_scalar', e(1)
+      print '(a, 1x, a, 1x, es12.5)', 'SLOT', 'brm_scalar', e(2)
+      print '(a, 1x, a, 1x, es12.5)', 'SLOT', 'psdf_scalar', e(3)
+      print '(a, 1x, a, 1x, es12.5)', 'SLOT', 'psdr_scalar', e(4)
+      print '(a, 1x, a, 1x, es12.5)', 'SLOT', 'bfm_msh', e(5)
+      print '(a, 1x, a, 1x, es12.5)', 'SLOT', 'brm_msh', e(6)
+      print '(a, 1x, a, 1x, i0)', 'SLOT', 'msh_points', npt
    end subroutine
 end program bsalib_driver
