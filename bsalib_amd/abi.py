@@ -66,7 +66,7 @@ EXPORTS = ["bsahip_init", "bsahip_finalise", "bsahip_last_error", "bsahip_versio
            "bsahip_batch_run", "bsahip_batch_info", "bsahip_batch_destroy",
            "bsahip_mesher_destroy", "bsahip_modal_psd_points", "bsahip_modal_recombination",
            "bsahip_mesher_adapt", "bsahip_dgemm_strided_batched", "bsahip_mesher_run_batch",
-           "bsahip_frf_apply", "bsahip_bfm_points_cls"]
+           "bsahip_frf_apply", "bsahip_bfm_points_cls", "bsahip_mesher_adapt_run"]
 
 
 def lib():
@@ -108,6 +108,9 @@ def lib():
         L.bsahip_mesher_destroy.argtypes = [C.c_void_p]
         L.bsahip_mesher_adapt.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_int32),
                                           C.POINTER(C.c_int64)]
+        L.bsahip_mesher_adapt_run.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_int32),
+                                              C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                              C.POINTER(C.c_double), C.POINTER(C.c_int64)]
         _lib = L
     return _lib
 
@@ -473,6 +476,20 @@ class Mesher:
         ev = C.c_int64(0)
         _check(lib().bsahip_mesher_adapt(self.h, C.c_double(tol), _ptr(lv, C.c_int32), C.byref(ev)))
         return lv, int(ev.value)
+
+    def adapt_run(self, tol: float = 1e-3) -> dict:
+        """adapt() and the run at its levels in one call (bsahip_mesher_adapt_run):
+        the dict of run() plus "levels" and "evals"."""
+        lv = np.zeros(self.ntiles, dtype=np.int32)
+        ev = C.c_int64(0)
+        m3mf, m3mr = np.zeros(self.nc), np.zeros(self.nc)
+        counts = np.zeros(3, dtype=np.int64)
+        _check(lib().bsahip_mesher_adapt_run(self.h, C.c_double(tol), _ptr(lv, C.c_int32),
+                                             C.byref(ev), _ptr(m3mf, C.c_double),
+                                             _ptr(m3mr, C.c_double), _ptr(counts, C.c_int64)))
+        return {"m3mf": m3mf, "m3mr": m3mr, "evals_pre": int(counts[0]),
+                "evals_post": int(counts[1]), "brm_points": int(counts[2]), "modes": self.modes,
+                "levels": lv, "evals": int(ev.value)}
 
     def _levels(self, levels):
         if levels is None:

@@ -1305,10 +1305,11 @@ int bsahip_mesher_run_tiles(bsahip_mesher *m, const int32_t *levels, double *til
    HIPC(hipStreamSynchronize(g.s));
    if (m3mr) {
       if (tile_m3mr) {
-         for (size_t q = 0; q < nm3; ++q) {
-            double sum = 0.0;
-            for (size_t k = 0; k < nt; ++k) sum += tile_m3mr[k * nm3 + q];
-            m3mr[q] = sum;
+         // tile by tile, so that each pass reads one contiguous slot
+         std::fill(m3mr, m3mr + nm3, 0.0);
+         for (size_t k = 0; k < nt; ++k) {
+            const double *c = tile_m3mr + k * nm3;
+            for (size_t q = 0; q < nm3; ++q) m3mr[q] += c[q];
          }
       } else {
          HIPC(hipMemcpy(m3mr, acc + nm3, nm3 * sizeof(double), hipMemcpyDeviceToHost));
@@ -1383,17 +1384,28 @@ int bsahip_mesher_run_batch(int32_t n, bsahip_mesher *const *ms, const int32_t *
    return 0;
 }
 
-int bsahip_mesher_adapt(bsahip_mesher *m, double tol, int32_t *levels, int64_t *evals)
+}  // extern "C"
+
+namespace {
+
+// A-posteriori levels (bsahip_mesher_adapt).  With outputs (m3mf, m3mr or
+// counts non-NULL) every refinement round runs the tiles it keeps at their
+// current level instead of level 1, so that the last run is the run at the
+// final levels and its moments are returned without another run.
+int msh_adapt(bsahip_mesher *m, double tol, int32_t *levels, int64_t *evals, double *m3mf_out,
+              double *m3mr_out, int64_t *counts_out)
 {
-   std::lock_guard<std::recursive_mutex> lk(g.mu);
    if (!m || !levels || !(tol > 0.0)) return fail(BSAHIP_EINVAL, "need a Mesher, levels and tol > 0");
+   const bool want = m3mf_out || m3mr_out || counts_out;
    const size_t nt = m->tiles.size(), nm3 = (size_t)m->nc;
    std::vector<double> prev(nt * nm3), cur(nt * nm3), m3f(nm3), m3r(nm3);
    std::vector<int32_t> lev(nt, 1), lr(nt);
+   std::vector<char> up(nt);
+   int64_t cnt[3] = {0, 0, 0};
    // level 1 (the resident level-0 samples), then level 2 on every tile
    CHK(bsahip_mesher_run_tiles(m, lev.data(), prev.data(), m3f.data(), m3r.data(), nullptr));
    std::fill(lev.begin(), lev.end(), 2);
-   CHK(bsahip_mesher_run_tiles(m, lev.data(), cur.data(), m3f.data(), m3r.data(), nullptr));
+   CHK(bsahip_mesher_run_tiles(m, lev.data(), cur.data(), m3f.data(), m3r.data(), cnt));
    double M = 0.0;
    for (size_t q = 0; q < nm3; ++q) M = std::max(M, std::fabs(m3r[q]));
    const double budget = 0.5 * tol * M;
@@ -1435,18 +1447,20 @@ int bsahip_mesher_adapt(bsahip_mesher *m, double tol, int32_t *levels, int64_t *
       for (size_t t = 0; t < nt; ++t)
          if (lev[t] < 8 && dnorm[t] > 0.0) ord.push_back(t);
       std::sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return dnorm[a] > dnorm[b]; });
-      std::fill(lr.begin(), lr.end(), 1);
+      for (size_t t = 0; t < nt; ++t) lr[t] = want ? lev[t] : 1;
+      std::fill(up.begin(), up.end(), 0);
       bool any = false;
       for (size_t t : ord) {
          if (emax() <= budget) break;
          lr[t] = 2 * lev[t];
+         up[t] = 1;
          any = true;
          for (size_t q = 0; q < nm3; ++q) E[q] -= 0.75 * delta[t * nm3 + q];
       }
       if (!any) break;
-      CHK(bsahip_mesher_run_tiles(m, lr.data(), cur.data(), m3f.data(), m3r.data(), nullptr));
+      CHK(bsahip_mesher_run_tiles(m, lr.data(), cur.data(), m3f.data(), m3r.data(), cnt));
       for (size_t t = 0; t < nt; ++t)
-         if (lr[t] > 1) {
+         if (up[t]) {
             set_delta(t);
             std::copy(cur.begin() + t * nm3, cur.begin() + (t + 1) * nm3, prev.begin() + t * nm3);
             lev[t] = lr[t];
@@ -1459,7 +1473,29 @@ int bsahip_mesher_adapt(bsahip_mesher *m, double tol, int32_t *levels, int64_t *
       n += (int64_t)(T.n1 * lev[t] + 1) * (T.n2 * lev[t] + 1) - (int64_t)(T.n1 + 1) * (T.n2 + 1);
    }
    if (evals) *evals = n;
+   // with outputs, the last run was at the final levels: level 2 everywhere,
+   // or a refinement round whose kept tiles ran at their own levels
+   if (m3mf_out) std::copy(m3f.begin(), m3f.end(), m3mf_out);
+   if (m3mr_out) std::copy(m3r.begin(), m3r.end(), m3mr_out);
+   if (counts_out) std::copy(cnt, cnt + 3, counts_out);
    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bsahip_mesher_adapt(bsahip_mesher *m, double tol, int32_t *levels, int64_t *evals)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   return msh_adapt(m, tol, levels, evals, nullptr, nullptr, nullptr);
+}
+
+int bsahip_mesher_adapt_run(bsahip_mesher *m, double tol, int32_t *levels, int64_t *evals,
+                            double *m3mf, double *m3mr, int64_t *counts)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   return msh_adapt(m, tol, levels, evals, m3mf, m3mr, counts);
 }
 
 int bsahip_mesher_export(bsahip_mesher *m, const int32_t *levels, int32_t ncol, const double *coef,

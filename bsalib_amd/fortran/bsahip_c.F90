@@ -41,7 +41,7 @@ module BsaHip_C
    public :: bsahip_init, bsahip_finalise, bsahip_classic_run, bsahip_bfm_points
    public :: bsahip_classic_bispectra_sel
    public :: bsahip_mesher_create, bsahip_mesher_run, bsahip_mesher_destroy, bsahip_mesher_export
-   public :: bsahip_mesher_adapt
+   public :: bsahip_mesher_adapt, bsahip_mesher_adapt_run
    public :: bsahip_mesher_ntiles
    public :: bsahip_last_error_f, bsahip_modal_psd_points, bsahip_modal_recombination
    public :: bsahip_classic_spectra, bsahip_frf_apply, bsahip_bfm_points_cls
@@ -128,6 +128,13 @@ module BsaHip_C
             bind(c, name="bsahip_mesher_adapt")
          import c_int, c_double, c_ptr
          type(c_ptr), value :: mesher, levels, evals
+         real(c_double), value :: tol
+      end function
+
+      integer(c_int) function bsahip_mesher_adapt_run(mesher, tol, levels, evals, m3mf, m3mr, &
+                                                      counts) bind(c, name="bsahip_mesher_adapt_run")
+         import c_int, c_double, c_ptr
+         type(c_ptr), value :: mesher, levels, evals, m3mf, m3mr, counts
          real(c_double), value :: tol
       end function
 

@@ -1012,7 +1012,6 @@ contains
          integer(bsa_int_t) :: z, nmk, nc
          integer(c_int32_t) :: mflags, ntl
          integer(c_int) :: rc2
-         type(c_ptr) :: plev
          if (.not. allocated(xsi_)) &
             call bsa_Abort('The Mesher needs the modal damping ratios (bsa_setTotDamping).')
          ! background peak width: 1 / max_z(L/U), maximised over (component, direction)
@@ -1062,17 +1061,17 @@ contains
          end if
          allocate(m3mf(nc), m3mr(nc))
          if (allocated(msh_levels_)) deallocate(msh_levels_)
-         plev = c_null_ptr
          if (mesher_tol_ > 0.0_bsa_real_t) then
+            ! a-posteriori levels; the moments come from the same integration
             rc2 = bsahip_mesher_ntiles(msh, ntl)
             allocate(msh_levels_(ntl))
-            rc2 = bsahip_mesher_adapt(msh, real(mesher_tol_, c_double), c_loc(msh_levels_), &
-                                      c_loc(adapt_evals))
-            if (rc2 /= 0) call bsa_Abort('bsahip_mesher_adapt failed: '//bsahip_last_error_f())
-            plev = c_loc(msh_levels_)
+            rc2 = bsahip_mesher_adapt_run(msh, real(mesher_tol_, c_double), c_loc(msh_levels_), &
+                                          c_loc(adapt_evals), c_loc(m3mf), c_loc(m3mr), c_loc(counts))
+            if (rc2 /= 0) call bsa_Abort('bsahip_mesher_adapt_run failed: '//bsahip_last_error_f())
+         else
+            rc2 = bsahip_mesher_run(msh, c_null_ptr, c_loc(m3mf), c_loc(m3mr), c_loc(counts))
+            if (rc2 /= 0) call bsa_Abort('bsahip_mesher_run failed: '//bsahip_last_error_f())
          end if
-         rc2 = bsahip_mesher_run(msh, plev, c_loc(m3mf), c_loc(m3mr), c_loc(counts))
-         if (rc2 /= 0) call bsa_Abort('bsahip_mesher_run failed: '//bsahip_last_error_f())
          if (visual_ .or. brm_base_) call exportMesher_(msh, nmk)
          rc2 = bsahip_mesher_destroy(msh)
          write(output_unit, '(1x, 2a, i0, a, i0, a, i0, a)') INFOMSG, 'Mesher on the GPU: ', &

@@ -299,9 +299,8 @@ def sweep(prob, net: MLP, thresholds, log=print, target: float = 1e-3, plain: bo
             r = M.run()
             out["plain"]["rule"] = dict(evals=evals(r), err=err(r))
             t1 = time.perf_counter()
-            lv, ev = M.adapt(target)
-            r = M.run(levels=lv)
-            out["plain"]["adaptive"] = dict(evals=ev, err=err(r), tol=target,
+            r = M.adapt_run(target)
+            out["plain"]["adaptive"] = dict(evals=r["evals"], err=err(r), tol=target,
                                             run_s=time.perf_counter() - t1)
             out["plain"]["uniform_8"] = dict(evals=evals(conv), err=0.0)
             ok = {k: v for k, v in out["plain"].items() if v["err"] <= target}

@@ -368,6 +368,12 @@ int bsahip_mesher_run(bsahip_mesher *m, const int32_t *levels, double *m3mf, dou
  * included, each point once).  Run bsahip_mesher_run with the levels for the
  * moments. */
 int bsahip_mesher_adapt(bsahip_mesher *m, double tol, int32_t *levels, int64_t *evals);
+/* bsahip_mesher_adapt, then the moments and counts of bsahip_mesher_run at the
+ * levels it picked, from its own last integration (each refinement round runs
+ * the tiles it keeps at their current level, so no further run is needed).
+ * Any of m3mf, m3mr, counts may be NULL.  No reference counterpart. */
+int bsahip_mesher_adapt_run(bsahip_mesher *m, double tol, int32_t *levels, int64_t *evals,
+                            double *m3mf, double *m3mr, int64_t *counts);
 /* bsahip_mesher_run for n Meshers (distinct, of any structures) at once:
  * levels[i] (ntiles of Mesher i, or NULL for its rule-based levels; levels
  * itself NULL: all rule-based), m3mf[i] / m3mr[i] its outputs (either may be

@@ -135,6 +135,48 @@ def test_adaptive_levels_meet_their_target(gpu_lib, name, tol):
     assert ev <= conv["evals_pre"] + conv["evals_post"]
 
 
+@pytest.mark.parametrize("name,tol,diag", [("config1", 1e-3, False), ("tiny_uvw", 1e-3, False),
+                                           ("tiny_uvw", 1e-4, False), ("tiny_dav", 1e-3, True)])
+def test_adapt_run_equals_adapt_then_run(gpu_lib, name, tol, diag):
+    """bsahip_mesher_adapt_run: the levels of bsahip_mesher_adapt and the moments
+    and counts of bsahip_mesher_run at those levels (to the order of the sums),
+    from the adaptation's own last integration."""
+    p = CASES[name]()
+    M = abi.Mesher(p, bkg_base_rfmnt=20, tile_cells=8, only_diag=diag)
+    lv, ev = M.adapt(tol)
+    want = M.run(levels=lv)
+    got = M.adapt_run(tol)
+    M.close()
+    np.testing.assert_array_equal(got["levels"], lv)
+    assert got["evals"] == ev
+    for k in ("evals_pre", "evals_post", "brm_points"):
+        assert got[k] == want[k], k
+    assert rel_err(got["m3mf"], want["m3mf"]) < 1e-13
+    assert rel_err(got["m3mr"], want["m3mr"]) < 1e-13
+    print(f"{name} tol {tol}: levels {np.bincount(lv, minlength=9)[[2, 4, 8]]}")
+
+
+def test_adapt_run_with_tiles_kept_below_level_8(gpu_lib):
+    """Looser targets stop the refinement with tiles at levels 2 and 4 as well:
+    adapt_run's moments then come from a run in which the kept tiles ran at
+    their own levels.  At least one target must give mixed levels."""
+    p = CASES["config1"]()
+    M = abi.Mesher(p, bkg_base_rfmnt=20, tile_cells=8)
+    mixed = 0
+    for tol in (1e-1, 3e-2, 1e-2, 3e-3):
+        lv, ev = M.adapt(tol)
+        want = M.run(levels=lv)
+        got = M.adapt_run(tol)
+        np.testing.assert_array_equal(got["levels"], lv)
+        assert got["evals"] == ev == want["evals_pre"] + want["evals_post"]
+        assert rel_err(got["m3mf"], want["m3mf"]) < 1e-13
+        assert rel_err(got["m3mr"], want["m3mr"]) < 1e-13
+        mixed += np.unique(lv).size > 1
+        print(f"tol {tol}: tiles at levels 2/4/8 {np.bincount(lv, minlength=9)[[2, 4, 8]]}")
+    M.close()
+    assert mixed > 0
+
+
 def test_mesher_batch_equals_separate_runs(gpu_lib):
     """bsahip_mesher_run_batch over Meshers of different structures, mode counts
     and models (full and ONLY_DIAG), mixed and rule-based levels: each Mesher's
