@@ -320,10 +320,14 @@ __global__ __launch_bounds__(256) void k_msh_integrate(
 // Re[H_n(f2) Q] with Q = H_m(f1) conj H_p(f1+f2) shared by the thread's n, and
 // the sum over a BRM row (fixed b) is taken on Q before the H_n(f2) factor:
 //   sum_a w_a w_b B Re[H_n Q] = w_b Re[H_n sum_a w_a B Q].
-// About a third of the arithmetic and a quarter of the FRF loads of
-// k_msh_integrate per (point, entry).  grid (nitems * bx), bx covering
-// nm^2 ceil(nm / MSH_NQ) threads of the item's structure, block 256.
-#define MSH_NQ 4
+// Along the row B is linear in t = (a - a0) / s1, B = (1 - t) e0 + t e1 with
+// e0, e1 per n, so sum_a w_a B Q = e0 sum_a w_a (1 - t) Q + e1 sum_a w_a t Q:
+// the point loop forms the two n-independent sums, each n uses them once per
+// row.  grid (nitems * bx), bx covering nm^2 ceil(nm / MSH_NQ) threads of the
+// item's structure, block 256.
+#ifndef MSH_NQ
+#define MSH_NQ 8
+#endif
 
 __global__ __launch_bounds__(256) void k_msh_integrate_full(
    const MshTileRun *__restrict__ tr, const int *__restrict__ it, const int2 *__restrict__ cr,
@@ -367,13 +371,10 @@ __global__ __launch_bounds__(256) void k_msh_integrate_full(
          const int b = cj * T.s2 + bb;
          const double wb = (b == 0 || b == NB) ? 0.5 : 1.0;
          const double sv = bb * inv2;
-         double e0[MSH_NQ], e1[MSH_NQ], Sf[MSH_NQ], Dr[MSH_NQ], Di[MSH_NQ];
-#pragma unroll
-         for (int j = 0; j < MSH_NQ; ++j) {
-            e0[j] = (1.0 - sv) * b00[j] + sv * b01[j];
-            e1[j] = (1.0 - sv) * b10[j] + sv * b11[j];
-            Sf[j] = Dr[j] = Di[j] = 0.0;
-         }
+         // along the row the BFM is e0 + t (e1 - e0): the sums over a of
+         // w_a (1 - t) Q and w_a t Q do not depend on n, so they are taken
+         // once per point and combined with each n's e0, e1 once per row
+         double W0 = 0.0, W1 = 0.0, A0r = 0.0, A0i = 0.0, A1r = 0.0, A1i = 0.0;
          for (int aa = 0; aa <= ea; ++aa) {
             const int a = ci * T.s1 + aa;
             const double wa = (a == 0 || a == NA) ? 0.5 : 1.0;
@@ -390,19 +391,22 @@ __global__ __launch_bounds__(256) void k_msh_integrate_full(
             // Q = h1 conj(h3)
             const double Qr = h1v.x * h3v.x + h1v.y * h3v.y;
             const double Qi = h1v.y * h3v.x - h1v.x * h3v.y;
-#pragma unroll
-            for (int j = 0; j < MSH_NQ; ++j) {
-               const double x = wa * ((1.0 - t) * e0[j] + t * e1[j]);
-               Sf[j] += x;
-               Dr[j] += x * Qr;
-               Di[j] += x * Qi;
-            }
+            const double w0 = wa * (1.0 - t), w1 = wa * t;
+            W0 += w0;
+            W1 += w1;
+            A0r += w0 * Qr;
+            A0i += w0 * Qi;
+            A1r += w1 * Qr;
+            A1i += w1 * Qi;
          }
 #pragma unroll
          for (int j = 0; j < MSH_NQ; ++j) {
+            const double e0 = (1.0 - sv) * b00[j] + sv * b01[j];
+            const double e1 = (1.0 - sv) * b10[j] + sv * b11[j];
+            const double Dr = e0 * A0r + e1 * A1r, Di = e0 * A0i + e1 * A1i;
             const double2 h2v = H2[(size_t)b * nm + min(n0 + j, nm - 1)];
-            sf[j] += wb * Sf[j];
-            sr[j] += wb * (h2v.x * Dr[j] - h2v.y * Di[j]);
+            sf[j] += wb * (e0 * W0 + e1 * W1);
+            sr[j] += wb * (h2v.x * Dr - h2v.y * Di);
          }
       }
    }
