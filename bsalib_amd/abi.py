@@ -66,7 +66,8 @@ EXPORTS = ["bsahip_init", "bsahip_finalise", "bsahip_last_error", "bsahip_versio
            "bsahip_batch_run", "bsahip_batch_info", "bsahip_batch_destroy",
            "bsahip_mesher_destroy", "bsahip_modal_psd_points", "bsahip_modal_recombination",
            "bsahip_mesher_adapt", "bsahip_dgemm_strided_batched", "bsahip_mesher_run_batch",
-           "bsahip_frf_apply", "bsahip_bfm_points_cls", "bsahip_mesher_adapt_run"]
+           "bsahip_frf_apply", "bsahip_bfm_points_cls", "bsahip_mesher_adapt_run",
+           "bsahip_diag_psd_points"]
 
 
 def lib():
@@ -233,48 +234,69 @@ def classic_bispectra_sel(prob, sel, exact: bool = False, only_diag: bool = Fals
     return {"bisp_f": bfm, "bisp_r": brm}
 
 
-def bfm_points(prob, fi, fj, with_brm: bool = True, bound: Bound = None):
-    """Modal bispectra at arbitrary points, S(f1+f2) exact (bsahip_bfm_points)."""
+def bfm_points(prob, fi, fj, with_brm: bool = True, bound: Bound = None, only_diag: bool = False):
+    """Modal bispectra at arbitrary points, S(f1+f2) exact (bsahip_bfm_points):
+    (NM^3, npts), or (NM, npts) for the diagonal model."""
     b = bound or Bound(prob)
     fi = np.ascontiguousarray(fi, dtype=np.float64)
     fj = np.ascontiguousarray(fj, dtype=np.float64)
+    if fj.size != fi.size:
+        raise ValueError("one fj per fi")
     n = fi.size
-    bfm = np.zeros((b.nm ** 3, n), order="F")
-    brm = np.zeros((b.nm ** 3, n), order="F") if with_brm else None
+    ne = b.nm if only_diag else b.nm ** 3
+    bfm = np.zeros((ne, n), order="F")
+    brm = np.zeros((ne, n), order="F") if with_brm else None
     _check(lib().bsahip_bfm_points(C.byref(b.struct), C.c_int32(n), _ptr(fi, C.c_double),
-                                   _ptr(fj, C.c_double), C.c_int32(0), _ptr(bfm, C.c_double),
-                                   _ptr(brm, C.c_double)))
+                                   _ptr(fj, C.c_double), C.c_int32(ONLY_DIAG if only_diag else 0),
+                                   _ptr(bfm, C.c_double), _ptr(brm, C.c_double)))
     return bfm, brm
 
 
-def bfm_points_cls(prob, fi, fj, with_brm: bool = True, bound: Bound = None):
+def bfm_points_cls(prob, fi, fj, with_brm: bool = True, bound: Bound = None,
+                   only_diag: bool = False):
     """Modal bispectra at points of the Classic grid of prob (nfreqs, df) with
-    its truncation, S(|f|) = 0 beyond (nfreqs - 1) df (bsahip_bfm_points_cls)."""
+    its truncation, S(|f|) = 0 beyond (nfreqs - 1) df (bsahip_bfm_points_cls):
+    (NM^3, npts), or (NM, npts) for the diagonal model."""
     b = bound or Bound(prob)
     fi = np.ascontiguousarray(fi, dtype=np.float64)
     fj = np.ascontiguousarray(fj, dtype=np.float64)
+    if fj.size != fi.size:
+        raise ValueError("one fj per fi")
     n = fi.size
-    bfm = np.zeros((b.nm ** 3, n), order="F")
-    brm = np.zeros((b.nm ** 3, n), order="F") if with_brm else None
+    ne = b.nm if only_diag else b.nm ** 3
+    bfm = np.zeros((ne, n), order="F")
+    brm = np.zeros((ne, n), order="F") if with_brm else None
     _check(lib().bsahip_bfm_points_cls(C.byref(b.struct), C.c_int32(prob.nfreqs),
-                                       C.c_double(prob.df), C.c_int32(n), _ptr(fi, C.c_double),
-                                       _ptr(fj, C.c_double), _ptr(bfm, C.c_double),
-                                       _ptr(brm, C.c_double)))
+                                       C.c_double(prob.df), C.c_int32(ONLY_DIAG if only_diag else 0),
+                                       C.c_int32(n), _ptr(fi, C.c_double), _ptr(fj, C.c_double),
+                                       _ptr(bfm, C.c_double), _ptr(brm, C.c_double)))
     return bfm, brm
+
+
+def diag_psd_points(prob, f, bound: Bound = None):
+    """Modal force PSD of the diagonal model at frequencies f (bsahip_diag_psd_points):
+    array (NM, npts)."""
+    b = bound or Bound(prob)
+    f = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
+    out = np.zeros((b.nm, f.size), order="F")
+    _check(lib().bsahip_diag_psd_points(C.byref(b.struct), C.c_int32(f.size), _ptr(f, C.c_double),
+                                        _ptr(out, C.c_double)))
+    return out
 
 
 def frf_apply(prob, kind: int, data, f1, f2=None, flags: int = 0, bound: Bound = None):
     """FRF weighting of modal spectra (bsahip_frf_apply), returned as a new
     array: kind FRF_PSD data (n1, NM^2); FRF_GRID data (n1, n2, NM^3);
-    FRF_POINTS data (NM^3, n1) at the points (f1[q], f2[q])."""
+    FRF_POINTS data (NM^3, n1) at the points (f1[q], f2[q]); NM in place of
+    NM^2 and NM^3 with flags=ONLY_DIAG."""
     b = bound or Bound(prob)
     f1 = np.ascontiguousarray(f1, dtype=np.float64).reshape(-1)
     f2 = None if f2 is None else np.ascontiguousarray(f2, dtype=np.float64).reshape(-1)
     out = np.array(data, dtype=np.float64, order="F", copy=True)
     n2 = 0 if f2 is None else f2.size
     nm = b.nm
-    want = {FRF_PSD: f1.size * nm * nm, FRF_GRID: f1.size * n2 * nm ** 3,
-            FRF_POINTS: f1.size * nm ** 3}.get(kind)
+    e2, e3 = (nm, nm) if flags & ONLY_DIAG else (nm * nm, nm ** 3)
+    want = {FRF_PSD: f1.size * e2, FRF_GRID: f1.size * n2 * e3, FRF_POINTS: f1.size * e3}.get(kind)
     if want is not None and out.size != want:
         raise ValueError(f"data holds {out.size} values, kind {kind} needs {want}")
     if kind == FRF_POINTS and n2 != f1.size:

@@ -1187,7 +1187,8 @@ int bsahip_classic_bispectra_sel(const bsahip_problem *p, int32_t nfh, double df
 
 namespace {
 int bfm_points_impl(const bsahip_problem *p, int32_t npts, const double *fi, const double *fj,
-                    double *bfm, double *brm, double fmax);
+                    double *bfm, double *brm, double fmax, bool diag);
+int diag_sq_table(Bufs &B, const Prob &D, const std::vector<double> &uq, double fmax, double **sq);
 }  // namespace
 
 extern "C" {
@@ -1198,26 +1199,83 @@ int bsahip_bfm_points(const bsahip_problem *p, int32_t npts, const double *fi, c
    std::lock_guard<std::recursive_mutex> lk(g.mu);
    CHK(ensure_init());
    CHK(check_flags(flags, false, false));
-   return bfm_points_impl(p, npts, fi, fj, bfm, brm, -1.0);
+   if (flags & ~(BSAHIP_PREC_MASK | BSAHIP_ONLY_DIAG))
+      return fail(BSAHIP_EINVAL, "bsahip_bfm_points takes BSAHIP_ONLY_DIAG and FP64 only");
+   return bfm_points_impl(p, npts, fi, fj, bfm, brm, -1.0, (flags & BSAHIP_ONLY_DIAG) != 0);
 }
 
-int bsahip_bfm_points_cls(const bsahip_problem *p, int32_t nfh, double df, int32_t npts,
-                          const double *fi, const double *fj, double *bfm, double *brm)
+int bsahip_bfm_points_cls(const bsahip_problem *p, int32_t nfh, double df, int32_t flags,
+                          int32_t npts, const double *fi, const double *fj, double *bfm,
+                          double *brm)
 {
    std::lock_guard<std::recursive_mutex> lk(g.mu);
    CHK(ensure_init());
    if (nfh < 2 || !(df > 0.0)) return fail(BSAHIP_EINVAL, "need nfh >= 2 and df > 0");
-   return bfm_points_impl(p, npts, fi, fj, bfm, brm, (nfh - 1) * df);
+   if (flags & ~BSAHIP_ONLY_DIAG)
+      return fail(BSAHIP_EINVAL, "bsahip_bfm_points_cls takes BSAHIP_ONLY_DIAG only");
+   return bfm_points_impl(p, npts, fi, fj, bfm, brm, (nfh - 1) * df,
+                          (flags & BSAHIP_ONLY_DIAG) != 0);
+}
+
+int bsahip_diag_psd_points(const bsahip_problem *p, int32_t npts, const double *f, double *psd)
+{
+   std::lock_guard<std::recursive_mutex> lk(g.mu);
+   CHK(ensure_init());
+   if (npts < 0 || (npts > 0 && (!f || !psd))) return fail(BSAHIP_EINVAL, "npts < 0 or NULL arrays");
+   if (npts == 0) return 0;
+   Bufs &B = oneshot_bufs();
+   Prob D;
+   CHK(load_problem(p, 0, B, D));
+   std::vector<double> af(npts);
+   for (int q = 0; q < npts; ++q) af[q] = std::fabs(f[q]);
+   std::vector<double> uq(af);
+   std::sort(uq.begin(), uq.end());
+   uq.erase(std::unique(uq.begin(), uq.end()), uq.end());
+   std::vector<int> iu(npts);
+   for (int q = 0; q < npts; ++q)
+      iu[q] = (int)(std::lower_bound(uq.begin(), uq.end(), af[q]) - uq.begin());
+   double *sq, *out;
+   int *diu;
+   CHK(diag_sq_table(B, D, uq, -1.0, &sq));
+   CHK(upload(B, "dp_iu", iu, &diu));
+   const long tot = (long)npts * D.nm;
+   CHK(B.get("dp_out", (size_t)tot, &out));
+   k_diag_psd_points<<<(unsigned)((tot + 255) / 256), 256, 0, g.s>>>(
+      sq, (int)uq.size(), D.nnl, D.nnlp, D.pa_k, D.K, D.nm, npts, diu, out);
+   HIPC(hipGetLastError());
+   HIPC(hipMemcpyAsync(psd, out, (size_t)tot * sizeof(double), hipMemcpyDeviceToHost, g.s));
+   HIPC(hipStreamSynchronize(g.s));
+   return 0;
 }
 
 }  // extern "C"
 
 namespace {
 
+// sqrt-PSD table (ntc, nu, nnl) of the nodes at the distinct |f| uq; fmax > 0:
+// rows with |f| beyond fmax zero (the Classic grid's truncation)
+int diag_sq_table(Bufs &B, const Prob &D, const std::vector<double> &uq, double fmax, double **sq)
+{
+   const int nu = (int)uq.size();
+   double *fl;
+   CHK(upload(B, "dg_fl", uq, &fl));
+   CHK(B.get("dg_sq", (size_t)D.ntc * nu * D.nnl, sq));
+   dim3 gp((D.nnl + 63) / 64, nu, D.ntc);
+   k_psd_sqrt<<<gp, 64, 0, g.s>>>(fl, nu, D.prm, D.formula, D.nnl, D.ntc, D.pulsation, *sq);
+   HIPC(hipGetLastError());
+   if (fmax > 0.0) {
+      const long u0 = std::upper_bound(uq.begin(), uq.end(), fmax * (1.0 + 1e-12)) - uq.begin();
+      for (int c = 0; c < D.ntc && u0 < nu; ++c)
+         CHK(fill(*sq + ((size_t)c * nu + u0) * D.nnl, (size_t)(nu - u0) * D.nnl, 0.0));
+   }
+   return 0;
+}
+
 // Point BFM/BRM; fmax > 0: the Classic grid's truncation, S(|f|) = 0 beyond
-// fmax (it reaches only f1 + f2 of grid points)
+// fmax (it reaches only f1 + f2 of grid points); diag: the ONLY_DIAG model,
+// NM entries per point (k_diag_points, the Mesher's diagonal evaluation)
 int bfm_points_impl(const bsahip_problem *p, int32_t npts, const double *fi, const double *fj,
-                    double *bfm, double *brm, double fmax)
+                    double *bfm, double *brm, double fmax, bool diag)
 {
    if (npts < 0 || (npts > 0 && (!fi || !fj || !bfm)))
       return fail(BSAHIP_EINVAL, "npts < 0 or NULL point/output arrays");
@@ -1225,10 +1283,43 @@ int bfm_points_impl(const bsahip_problem *p, int32_t npts, const double *fi, con
    Bufs &B = oneshot_bufs();
    Prob D;
    CHK(load_problem(p, 0, B, D));
-   CHK(set_pod(p, D));
    std::vector<double> uq;
    std::vector<int> i1, i2, i3;
    point_tables(npts, fi, fj, uq, i1, i2, i3);
+   if (diag) {
+      const long tot = (long)npts * D.nm;
+      if (tot > 0xffffff00L) return fail(BSAHIP_EUNSUPPORTED, "npts * NM exceeds 2^32 values");
+      double *sq, *ob, *df1, *df2;
+      int *d1, *d2, *d3;
+      long *ds;
+      std::vector<long> slot(npts);
+      for (int q = 0; q < npts; ++q) slot[q] = q;
+      CHK(diag_sq_table(B, D, uq, fmax, &sq));
+      CHK(upload(B, "i1", i1, &d1));
+      CHK(upload(B, "i2", i2, &d2));
+      CHK(upload(B, "i3", i3, &d3));
+      CHK(upload(B, "dg_slot", slot, &ds));
+      CHK(B.get("bfm_out", (size_t)tot, &ob));
+      k_diag_points<<<(unsigned)((tot + 255) / 256), 256, 0, g.s>>>(
+         sq, (int)uq.size(), D.nnl, D.nnlp, D.pa_k, D.pb_k, D.K, D.nm, npts, d1, d2, d3, ds, ob);
+      HIPC(hipGetLastError());
+      HIPC(hipMemcpyAsync(bfm, ob, (size_t)tot * sizeof(double), hipMemcpyDeviceToHost, g.s));
+      if (brm) {
+         std::vector<double> f1(fi, fi + npts), f2(fj, fj + npts);
+         double *orr;
+         CHK(upload(B, "f1", f1, &df1));
+         CHK(upload(B, "f2", f2, &df2));
+         CHK(B.get("brm_out", (size_t)tot, &orr));
+         HIPC(hipMemcpyAsync(orr, ob, (size_t)tot * sizeof(double), hipMemcpyDeviceToDevice, g.s));
+         k_frf_apply<<<(unsigned)((tot + 255) / 256), 256, 0, g.s>>>(
+            BSAHIP_FRF_POINTS, 1, npts, df1, npts, df2, D.nm, D.Mm, D.Km, D.Cm, tot, orr);
+         HIPC(hipGetLastError());
+         HIPC(hipMemcpyAsync(brm, orr, (size_t)tot * sizeof(double), hipMemcpyDeviceToHost, g.s));
+      }
+      HIPC(hipStreamSynchronize(g.s));
+      return 0;
+   }
+   CHK(set_pod(p, D));
    const int nu = (int)uq.size();
    const long ldU = (long)D.nm * nu;
    double *dfl, *U;
@@ -1277,7 +1368,8 @@ int bsahip_frf_apply(const bsahip_problem *p, int32_t flags, int32_t kind, int64
 {
    std::lock_guard<std::recursive_mutex> lk(g.mu);
    CHK(ensure_init());
-   if (flags & BSAHIP_ONLY_DIAG) return fail(BSAHIP_EUNSUPPORTED, "FRF weighting of the full model only");
+   if (flags & ~BSAHIP_ONLY_DIAG) return fail(BSAHIP_EINVAL, "bsahip_frf_apply takes BSAHIP_ONLY_DIAG only");
+   const bool diag = (flags & BSAHIP_ONLY_DIAG) != 0;
    if (kind < BSAHIP_FRF_PSD || kind > BSAHIP_FRF_POINTS)
       return fail(BSAHIP_EINVAL, "kind must be BSAHIP_FRF_PSD, _GRID or _POINTS");
    if (n1 < 0 || n2 < 0 || (kind == BSAHIP_FRF_GRID && n2 == 0 && n1 > 0))
@@ -1288,7 +1380,7 @@ int bsahip_frf_apply(const bsahip_problem *p, int32_t flags, int32_t kind, int64
    Bufs &B = oneshot_bufs();
    Prob D;
    CHK(load_problem(p, 0, B, D));
-   const long nm = D.nm, nm2 = nm * nm, nm3 = nm2 * nm;
+   const long nm = D.nm, nm2 = diag ? nm : nm * nm, nm3 = diag ? nm : nm * nm * nm;
    const long nf2 = kind == BSAHIP_FRF_GRID ? n2 : (kind == BSAHIP_FRF_POINTS ? n1 : 0);
    const double tot_d = kind == BSAHIP_FRF_PSD ? (double)n1 * nm2
                        : (kind == BSAHIP_FRF_GRID ? (double)n1 * n2 * nm3 : (double)n1 * nm3);
@@ -1302,8 +1394,8 @@ int bsahip_frf_apply(const bsahip_problem *p, int32_t flags, int32_t kind, int64
    if (nf2) CHK(upload(B, "frf_f2", h2, &d2));
    CHK(B.get("frf_data", (size_t)total, &dd));
    HIPC(hipMemcpyAsync(dd, data, (size_t)total * sizeof(double), hipMemcpyHostToDevice, g.s));
-   k_frf_apply<<<(unsigned)((total + 255) / 256), 256, 0, g.s>>>(kind, n1, d1, n2, d2, D.nm, D.Mm,
-                                                                  D.Km, D.Cm, total, dd);
+   k_frf_apply<<<(unsigned)((total + 255) / 256), 256, 0, g.s>>>(kind, diag ? 1 : 0, n1, d1, n2, d2,
+                                                                  D.nm, D.Mm, D.Km, D.Cm, total, dd);
    HIPC(hipGetLastError());
    HIPC(hipMemcpyAsync(data, dd, (size_t)total * sizeof(double), hipMemcpyDeviceToHost, g.s));
    HIPC(hipStreamSynchronize(g.s));

@@ -1421,7 +1421,12 @@ __device__ __forceinline__ double2 frf_at(double f, const double *Mm, const doub
    return make_double2(r / d, -im / d);
 }
 
-__global__ void k_frf_apply(int kind, long n1, const double *__restrict__ f1, long n2,
+// FRF weighting in place (bsahip_frf_apply).  kind 0: (n1, E2) spectra at f1,
+// column c = n + p NM times Re[H_p conj H_n] (diag: column m times |H_m|^2);
+// kind 1: (n1, n2, E3) at (f1_i, f2_j); kind 2: (E3, n1) at (f1_q, f2_q); entry
+// e = m + n NM + p NM^2 times Re[H_m(a) H_n(b) conj H_p(a+b)] (diag: e = m, all
+// three modes m).  E2 = NM^2, E3 = NM^3, or NM both for the diagonal model.
+__global__ void k_frf_apply(int kind, int diag, long n1, const double *__restrict__ f1, long n2,
                             const double *__restrict__ f2, int nm, const double *__restrict__ Mm,
                             const double *__restrict__ Km, const double *__restrict__ Cm,
                             long total, double *__restrict__ data)
@@ -1430,11 +1435,12 @@ __global__ void k_frf_apply(int kind, long n1, const double *__restrict__ f1, lo
    if (t >= total) return;
    if (kind == 0) {
       const long i = t % n1, c = t / n1;
-      const int n = (int)(c % nm), p = (int)(c / nm);
+      const int n = diag ? (int)c : (int)(c % nm), p = diag ? (int)c : (int)(c / nm);
       const double2 hn = frf_at(f1[i], Mm, Km, Cm, n), hp = frf_at(f1[i], Mm, Km, Cm, p);
       data[t] *= hp.x * hn.x + hp.y * hn.y;
       return;
    }
+   const long ne = diag ? (long)nm : (long)nm * nm * nm;
    double a, b;
    long e;
    if (kind == 1) {
@@ -1443,14 +1449,42 @@ __global__ void k_frf_apply(int kind, long n1, const double *__restrict__ f1, lo
       a = f1[i];
       b = f2[j];
    } else {
-      const long nm3 = (long)nm * nm * nm;
-      const long q = t / nm3;
-      e = t % nm3;
+      const long q = t / ne;
+      e = t % ne;
       a = f1[q];
       b = f2[q];
    }
-   const int m = (int)(e % nm), n = (int)((e / nm) % nm), p = (int)(e / ((long)nm * nm));
+   int m, n, p;
+   if (diag) {
+      m = n = p = (int)e;
+   } else {
+      m = (int)(e % nm);
+      n = (int)((e / nm) % nm);
+      p = (int)(e / ((long)nm * nm));
+   }
    data[t] *= re3(frf_at(a, Mm, Km, Cm, m), frf_at(b, Mm, Km, Cm, n), frf_at(a + b, Mm, Km, Cm, p));
+}
+
+// Diagonal-model modal PSD at points: psd[q nm + m] = sum_k pa(m,k)^2 S_k(f_q),
+// S from the sqrt-PSD table sq (ntc, nu, nnl) at row iu[q] (getFM_diag_tnlm_*'s
+// psd, functionsImpl.F90:1747-1970).
+__global__ void k_diag_psd_points(const double *__restrict__ sq, int nu, int nnl, int nnlp,
+                                  const double *__restrict__ pa, int K, int nm, int npts,
+                                  const int *__restrict__ iu, double *__restrict__ psd)
+{
+   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+   if (gid >= (long)npts * nm) return;
+   const int q = (int)(gid / nm), m = (int)(gid % nm);
+   const int u = iu[q];
+   double s = 0.0;
+   for (int k = 0; k < K; ++k) {
+      const int c = k / nnlp, n = k % nnlp;
+      if (n >= nnl) continue;
+      const double r = sq[((size_t)c * nu + u) * nnl + n];
+      const double a = pa[(size_t)m * K + k];
+      s += a * a * r * r;
+   }
+   psd[gid] = s;
 }
 
 // ---------------------------------------------------------------------------

@@ -44,7 +44,7 @@ module BsaHip_C
    public :: bsahip_mesher_adapt, bsahip_mesher_adapt_run
    public :: bsahip_mesher_ntiles
    public :: bsahip_last_error_f, bsahip_modal_psd_points, bsahip_modal_recombination
-   public :: bsahip_classic_spectra, bsahip_frf_apply, bsahip_bfm_points_cls
+   public :: bsahip_classic_spectra, bsahip_frf_apply, bsahip_bfm_points_cls, bsahip_diag_psd_points
 
    interface
       integer(c_int) function bsahip_init(device) bind(c, name="bsahip_init")
@@ -82,13 +82,21 @@ module BsaHip_C
          type(c_ptr), value :: fi, fj, bfm, brm
       end function
 
-      integer(c_int) function bsahip_bfm_points_cls(p, nfh, df, npts, fi, fj, bfm, brm) &
+      integer(c_int) function bsahip_bfm_points_cls(p, nfh, df, flags, npts, fi, fj, bfm, brm) &
             bind(c, name="bsahip_bfm_points_cls")
          import c_int, c_int32_t, c_double, c_ptr, bsahip_problem_t
          type(bsahip_problem_t), intent(in) :: p
-         integer(c_int32_t), value :: nfh, npts
+         integer(c_int32_t), value :: nfh, flags, npts
          real(c_double), value :: df
          type(c_ptr), value :: fi, fj, bfm, brm
+      end function
+
+      integer(c_int) function bsahip_diag_psd_points(p, npts, f, psd) &
+            bind(c, name="bsahip_diag_psd_points")
+         import c_int, c_int32_t, c_ptr, bsahip_problem_t
+         type(bsahip_problem_t), intent(in) :: p
+         integer(c_int32_t), value :: npts
+         type(c_ptr), value :: f, psd
       end function
 
       integer(c_int) function bsahip_classic_spectra(p, nfh, df, flags, psd_f, psd_r, bfm, brm) &

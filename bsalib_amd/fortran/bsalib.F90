@@ -251,15 +251,17 @@ module BsaLib
    type(bsahip_problem_t), save :: lp_msh_
    integer(bsa_int_t) :: lp_nfh_ = 0
    real(bsa_real_t) :: lp_df_ = 0
+   logical :: lp_diag_ = .false.
 
    ! ---- operator slots (src/bsa/data/BsaLibData.F90:166-310) ----
    ! The reference's spectral operators behind procedure pointers, bound by
-   ! bsa_Run for the full modal model (the ONLY_DIAG model leaves them null);
-   ! the Classic ones once a Classic run has fixed the grid.  Each evaluates
-   ! on the device from the problem of the last bsa_Run: the nodal wind PSD
-   ! arguments (Suvw, Suvw_pad) are taken for the interface and checked for
-   ! shape only, and the scalar BFM is not pure (it calls the library).
-   ! Thread-safe: the library serialises calls.
+   ! bsa_Run (BsaLibImpl.F90:313-372): the Mesher ones always, the Classic ones
+   ! once a Classic run has fixed the grid, for the model of that run (full:
+   ! NM^2 PSD and NM^3 bispectrum entries; ONLY_DIAG: NM of each).  Each
+   ! evaluates on the device from the problem of the last bsa_Run: the nodal
+   ! wind PSD arguments (Suvw, Suvw_pad) are taken for the interface and
+   ! checked for shape only, and the scalar BFM is not pure (it calls the
+   ! library).  Thread-safe: the library serialises calls.
    abstract interface
       subroutine getBFMClsVect(f, Suvw, psd, bisp)
          import :: bsa_real_t
@@ -904,11 +906,10 @@ contains
       lp_%Cm = c_loc(lp_Ck_)
       lp_ready_ = .true.
       lp_msh_ = meshProblem_(lp_)
+      lp_diag_ = ionly_diag_ /= 0
       call unbindSlots_()
-      if (ionly_diag_ == 0) then
-         getBFM_msh => opBFMMsh_
-         getBRM_msh => opBRMMsh_
-      end if
+      getBFM_msh => opBFMMsh_
+      getBRM_msh => opBRMMsh_
 
       if (.not. device_ready_) then
          rc = bsahip_init(0_c_int)
@@ -923,12 +924,10 @@ contains
       call classicGrid_(nfh, df)
       lp_nfh_ = nfh
       lp_df_ = df
-      if (ionly_diag_ == 0) then
-         getBFM_vect_cls => opBFMVectCls_
-         getBRM_vect_cls => opBRMVectCls_
-         getBFM_scalar_cls => opBFMScalarCls_
-         getBRM_scalar_cls => opBRMScalarCls_
-      end if
+      getBFM_vect_cls => opBFMVectCls_
+      getBRM_vect_cls => opBRMVectCls_
+      getBFM_scalar_cls => opBFMScalarCls_
+      getBRM_scalar_cls => opBRMScalarCls_
 
       ! Classic version as checkMaxAllocation_ (BsaClassicImpl.F90:384-434): outside
       ! test mode a vectorised bispectrum above 1e9 elements switches to the scalar
@@ -1129,91 +1128,131 @@ contains
          call bsa_Abort('The Classic grid must be symmetric about f = 0 with a constant step.')
    end subroutine
 
-   !> getFM_full_tnm_vect_cls_ (functionsImpl.F90:1312-1587): modal PSDs
-   !> psd(NF, NM^2) and, when bispectra are computed, bisp(NF, NF, NM^3) on the
-   !> grid f, S(f1 + f2) zero beyond the grid; allocated here as there.
+   !> Entries per point of the modal PSD and bispectrum of the bound model.
+   subroutine slotDims_(n2, n3, flags)
+      integer(bsa_int_t), intent(out) :: n2, n3
+      integer(c_int32_t), intent(out) :: flags
+      n2 = lp_%nm**2
+      n3 = lp_%nm**3
+      flags = 0
+      if (lp_diag_) then
+         n2 = lp_%nm
+         n3 = lp_%nm
+         flags = BSAHIP_ONLY_DIAG
+      end if
+   end subroutine
+
+   !> getFM_full_tnm_vect_cls_ (functionsImpl.F90:1312-1587) or, ONLY_DIAG,
+   !> getFM_diag_tnlm_vect_cls_ (:1747-1970): modal PSDs psd(NF, E2) and, when
+   !> bispectra are computed, bisp(NF, NF, E3) on the grid f, S(f1 + f2) zero
+   !> beyond the grid; allocated here as there.
    subroutine opBFMVectCls_(f, Suvw, psd, bisp)
       real(bsa_real_t), intent(in) :: f(:)
       real(bsa_real_t), intent(in) :: Suvw(:, :)
       real(bsa_real_t), allocatable, intent(inout) :: psd(:, :), bisp(:, :, :)
-      integer(bsa_int_t) :: nfh, nf, nm
+      integer(bsa_int_t) :: nfh, nf, n2, n3, i
+      integer(c_int32_t) :: fl
       real(bsa_real_t) :: df
       call slotGrid_(f, nfh, df)
+      call slotDims_(n2, n3, fl)
       nf = size(f)
-      nm = lp_%nm
       if (size(Suvw, 1) /= nf) call bsa_Abort('getBFM_vect_cls: Suvw needs one row per frequency.')
       if (allocated(psd)) deallocate(psd)
       if (allocated(bisp)) deallocate(bisp)
-      allocate(psd(nf, nm * nm))
-      if (icomp_bisp_ /= 0) then
-         allocate(bisp(nf, nf, nm**3))
-         call spectra_(psd, bisp)
+      allocate(psd(nf, n2))
+      if (icomp_bisp_ /= 0) allocate(bisp(nf, nf, n3))
+      if (.not. lp_diag_) then
+         if (icomp_bisp_ /= 0) then
+            call spectra_(psd, bisp)
+         else
+            call psdOnly_(psd)
+         end if
       else
-         call psdOnly_(psd)
+         call diagPsd_(f, psd)
+         if (icomp_bisp_ /= 0) call diagBisp_(bisp, [(i, i = 1, n3)])
       end if
    contains
       subroutine spectra_(ps, bs)
-         real(bsa_real_t), target, intent(inout) :: ps(nf, nm * nm), bs(nf, nf, nm**3)
+         real(bsa_real_t), target, intent(inout) :: ps(nf, n2), bs(nf, nf, n3)
          call slotCheck_(bsahip_classic_spectra(lp_, int(nfh, c_int32_t), df, 0_c_int32_t, &
                                                 c_loc(ps), c_null_ptr, c_loc(bs), c_null_ptr), &
                          'getBFM_vect_cls')
       end subroutine
       subroutine psdOnly_(ps)
-         real(bsa_real_t), target, intent(inout) :: ps(nf, nm * nm)
+         real(bsa_real_t), target, intent(inout) :: ps(nf, n2)
          call slotCheck_(bsahip_classic_spectra(lp_, int(nfh, c_int32_t), df, 0_c_int32_t, &
                                                 c_loc(ps), c_null_ptr, c_null_ptr, c_null_ptr), &
                          'getBFM_vect_cls')
       end subroutine
+      subroutine diagPsd_(fr, ps)
+         real(bsa_real_t), target, intent(in) :: fr(nf)
+         real(bsa_real_t), intent(inout) :: ps(nf, n2)
+         real(bsa_real_t), allocatable, target :: pt(:, :)
+         allocate(pt(n2, nf))
+         call slotCheck_(bsahip_diag_psd_points(lp_, int(nf, c_int32_t), c_loc(fr), c_loc(pt)), &
+                         'getBFM_vect_cls')
+         ps = transpose(pt)
+      end subroutine
+      subroutine diagBisp_(bs, sel)
+         real(bsa_real_t), target, intent(inout) :: bs(nf, nf, n3)
+         integer(c_int32_t), target, intent(in) :: sel(n3)
+         call slotCheck_(bsahip_classic_bispectra_sel(lp_, int(nfh, c_int32_t), df, BSAHIP_ONLY_DIAG, &
+                                                      int(n3, c_int32_t), c_loc(sel), c_loc(bs), &
+                                                      c_null_ptr), 'getBFM_vect_cls')
+      end subroutine
    end subroutine
 
-   !> getRM_full_vect_cls_ (functionsImpl.F90:1598-1738): psd(:, n + (p-1) NM)
-   !> times Re[H_p conj H_n] at f, bisp(i, j, m + (n-1) NM + (p-1) NM^2) times
-   !> Re[H_m(f_i) H_n(f_j) conj H_p(f_i + f_j)], in place; no state kept from
-   !> one call to the next.
+   !> getRM_full_vect_cls_ (functionsImpl.F90:1598-1738) or, ONLY_DIAG,
+   !> getRM_diag_vect_cls_ (:1975-2101): psd times Re[H_p conj H_n] at f (|H_m|^2),
+   !> bisp(i, j, e) times Re[H_m(f_i) H_n(f_j) conj H_p(f_i + f_j)] (m = n = p
+   !> for the diagonal model), in place; no state kept from one call to the next.
    subroutine opBRMVectCls_(f, psd, bisp)
       real(bsa_real_t), intent(in) :: f(:)
       real(bsa_real_t), allocatable, intent(inout) :: psd(:, :), bisp(:, :, :)
-      integer(bsa_int_t) :: nf, nm
+      integer(bsa_int_t) :: nf, n2, n3
+      integer(c_int32_t) :: fl
       nf = size(f)
-      nm = lp_%nm
+      call slotDims_(n2, n3, fl)
       if (allocated(psd)) then
-         if (size(psd, 1) /= nf .or. size(psd, 2) /= nm * nm) call bsa_Abort('getBRM_vect_cls: psd is not (NF, NM^2).')
+         if (size(psd, 1) /= nf .or. size(psd, 2) /= n2) call bsa_Abort('getBRM_vect_cls: psd is not (NF, NM^2) (NM: ONLY_DIAG).')
          call frfPsd_(f, psd)
       end if
       if (allocated(bisp) .and. icomp_bisp_ /= 0) then
-         if (size(bisp, 1) /= nf .or. size(bisp, 2) /= nf .or. size(bisp, 3) /= nm**3) &
-            call bsa_Abort('getBRM_vect_cls: bisp is not (NF, NF, NM^3).')
+         if (size(bisp, 1) /= nf .or. size(bisp, 2) /= nf .or. size(bisp, 3) /= n3) &
+            call bsa_Abort('getBRM_vect_cls: bisp is not (NF, NF, NM^3) (NM: ONLY_DIAG).')
          call frfGrid_(f, bisp)
       end if
    contains
       subroutine frfPsd_(fr, ps)
          real(bsa_real_t), target, intent(in) :: fr(nf)
-         real(bsa_real_t), target, intent(inout) :: ps(nf, nm * nm)
-         call slotCheck_(bsahip_frf_apply(lp_, 0_c_int32_t, BSAHIP_FRF_PSD, int(nf, c_int64_t), &
+         real(bsa_real_t), target, intent(inout) :: ps(nf, n2)
+         call slotCheck_(bsahip_frf_apply(lp_, fl, BSAHIP_FRF_PSD, int(nf, c_int64_t), &
                                           c_loc(fr), 0_c_int64_t, c_null_ptr, c_loc(ps)), &
                          'getBRM_vect_cls')
       end subroutine
       subroutine frfGrid_(fr, bs)
          real(bsa_real_t), target, intent(in) :: fr(nf)
-         real(bsa_real_t), target, intent(inout) :: bs(nf, nf, nm**3)
-         call slotCheck_(bsahip_frf_apply(lp_, 0_c_int32_t, BSAHIP_FRF_GRID, int(nf, c_int64_t), &
+         real(bsa_real_t), target, intent(inout) :: bs(nf, nf, n3)
+         call slotCheck_(bsahip_frf_apply(lp_, fl, BSAHIP_FRF_GRID, int(nf, c_int64_t), &
                                           c_loc(fr), int(nf, c_int64_t), c_loc(fr), c_loc(bs)), &
                          'getBRM_vect_cls')
       end subroutine
    end subroutine
 
-   !> getFM_full_tnm_scalar_cls_ (functionsImpl.F90:2122-2318) at (fi, fj), a
-   !> point of the grid of the last Classic run: bisp(NM^3) with S(fi + fj)
-   !> zero beyond that grid, psd(NM^2) at fi when ij = 1 and zero otherwise.
+   !> getFM_full_tnm_scalar_cls_ (functionsImpl.F90:2122-2318) or, ONLY_DIAG,
+   !> getFM_diag_tnlm_scalar_cls_ (:2460-2563) at (fi, fj), a point of the grid
+   !> of the last Classic run: bisp(E3) with S(fi + fj) zero beyond that grid,
+   !> psd(E2) at fi when ij = 1 and zero otherwise.
    subroutine opBFMScalarCls_(ii, ij, fi, fj, Suvw, Suvw_pad, psd, bisp)
       integer(bsa_int_t), intent(in) :: ii, ij
       real(bsa_real_t), intent(in) :: fi, fj
       real(bsa_real_t), intent(in) :: Suvw(:, :), Suvw_pad(:)
       real(bsa_real_t), intent(inout) :: psd(:), bisp(:)
-      integer(bsa_int_t) :: nm
-      nm = lp_%nm
-      if (size(psd) < nm * nm .or. size(bisp) < nm**3) &
-         call bsa_Abort('getBFM_scalar_cls: psd needs NM^2 and bisp NM^3 entries.')
+      integer(bsa_int_t) :: n2, n3
+      integer(c_int32_t) :: fl
+      call slotDims_(n2, n3, fl)
+      if (size(psd) < n2 .or. size(bisp) < n3) &
+         call bsa_Abort('getBFM_scalar_cls: psd needs NM^2 and bisp NM^3 entries (NM: ONLY_DIAG).')
       if (size(Suvw, 2) /= size(Suvw_pad) .or. ii < 1 .or. ij < 1) &
          call bsa_Abort('getBFM_scalar_cls: bad grid indexes or Suvw / Suvw_pad shapes.')
       psd = 0
@@ -1222,106 +1261,115 @@ contains
       if (ij == 1) call psdAt_(psd)
    contains
       subroutine point_(bs)
-         real(bsa_real_t), target, intent(inout) :: bs(nm**3)
+         real(bsa_real_t), target, intent(inout) :: bs(n3)
          real(bsa_real_t), target :: a(1), b(1)
          a(1) = fi
          b(1) = fj
-         call slotCheck_(bsahip_bfm_points_cls(lp_, int(lp_nfh_, c_int32_t), lp_df_, 1_c_int32_t, &
+         call slotCheck_(bsahip_bfm_points_cls(lp_, int(lp_nfh_, c_int32_t), lp_df_, fl, 1_c_int32_t, &
                                                c_loc(a), c_loc(b), c_loc(bs), c_null_ptr), &
                          'getBFM_scalar_cls')
       end subroutine
       subroutine psdAt_(ps)
-         real(bsa_real_t), target, intent(inout) :: ps(nm * nm)
+         real(bsa_real_t), target, intent(inout) :: ps(n2)
          real(bsa_real_t), target :: a(1)
          a(1) = fi
-         call slotCheck_(bsahip_modal_psd_points(lp_, 1_c_int32_t, c_loc(a), c_loc(ps)), &
-                         'getBFM_scalar_cls')
+         if (lp_diag_) then
+            call slotCheck_(bsahip_diag_psd_points(lp_, 1_c_int32_t, c_loc(a), c_loc(ps)), &
+                            'getBFM_scalar_cls')
+         else
+            call slotCheck_(bsahip_modal_psd_points(lp_, 1_c_int32_t, c_loc(a), c_loc(ps)), &
+                            'getBFM_scalar_cls')
+         end if
       end subroutine
    end subroutine
 
-   !> getRM_full_scalar_cls_ (functionsImpl.F90:2329-2450): psdout at fi when
-   !> ij = 1 (zero otherwise), bispout at (fi, fj), each psdin / bispin times
-   !> the FRF products of getBRM_vect_cls.
+   !> getRM_full_scalar_cls_ (functionsImpl.F90:2329-2450) or, ONLY_DIAG,
+   !> getRM_diag_scalar_cls_ (:2570-2637): psdout at fi when ij = 1 (zero
+   !> otherwise), bispout at (fi, fj), each psdin / bispin times the FRF
+   !> products of getBRM_vect_cls.
    subroutine opBRMScalarCls_(ii, ij, fi, fj, psdin, psdout, bispin, bispout)
       integer(bsa_int_t), intent(in) :: ii, ij
       real(bsa_real_t), intent(in) :: fi, fj
       real(bsa_real_t), intent(in) :: psdin(:), bispin(:)
       real(bsa_real_t), intent(out) :: psdout(:), bispout(:)
-      integer(bsa_int_t) :: nm
-      nm = lp_%nm
-      if (size(psdin) < nm * nm .or. size(psdout) < nm * nm .or. size(bispin) < nm**3 .or. &
-          size(bispout) < nm**3 .or. ii < 1 .or. ij < 1) &
-         call bsa_Abort('getBRM_scalar_cls: psd needs NM^2 and bisp NM^3 entries.')
+      integer(bsa_int_t) :: n2, n3
+      integer(c_int32_t) :: fl
+      call slotDims_(n2, n3, fl)
+      if (size(psdin) < n2 .or. size(psdout) < n2 .or. size(bispin) < n3 .or. &
+          size(bispout) < n3 .or. ii < 1 .or. ij < 1) &
+         call bsa_Abort('getBRM_scalar_cls: psd needs NM^2 and bisp NM^3 entries (NM: ONLY_DIAG).')
       psdout = 0
       bispout = 0
       if (ij == 1) then
-         psdout(1:nm * nm) = psdin(1:nm * nm)
+         psdout(1:n2) = psdin(1:n2)
          call frfPsd_(psdout)
       end if
       if (icomp_bisp_ /= 0) then
-         bispout(1:nm**3) = bispin(1:nm**3)
+         bispout(1:n3) = bispin(1:n3)
          call frfPt_(bispout)
       end if
    contains
       subroutine frfPsd_(ps)
-         real(bsa_real_t), target, intent(inout) :: ps(nm * nm)
+         real(bsa_real_t), target, intent(inout) :: ps(n2)
          real(bsa_real_t), target :: a(1)
          a(1) = fi
-         call slotCheck_(bsahip_frf_apply(lp_, 0_c_int32_t, BSAHIP_FRF_PSD, 1_c_int64_t, c_loc(a), &
+         call slotCheck_(bsahip_frf_apply(lp_, fl, BSAHIP_FRF_PSD, 1_c_int64_t, c_loc(a), &
                                           0_c_int64_t, c_null_ptr, c_loc(ps)), 'getBRM_scalar_cls')
       end subroutine
       subroutine frfPt_(bs)
-         real(bsa_real_t), target, intent(inout) :: bs(nm**3)
+         real(bsa_real_t), target, intent(inout) :: bs(n3)
          real(bsa_real_t), target :: a(1), b(1)
          a(1) = fi
          b(1) = fj
-         call slotCheck_(bsahip_frf_apply(lp_, 0_c_int32_t, BSAHIP_FRF_POINTS, 1_c_int64_t, c_loc(a), &
+         call slotCheck_(bsahip_frf_apply(lp_, fl, BSAHIP_FRF_POINTS, 1_c_int64_t, c_loc(a), &
                                           1_c_int64_t, c_loc(b), c_loc(bs)), 'getBRM_scalar_cls')
       end subroutine
    end subroutine
 
    !> getBFM_msh (functionsImpl.F90:247-465, the POD form at :582-1020 when
-   !> isvd = 1): bfm(NM^3, npts) at the points (fi(q), fj(q)), S(fi + fj)
-   !> evaluated exactly.
+   !> isvd = 1; ONLY_DIAG: getFM_diag_tnm_scalar_msh_, :1123-1216): bfm(E3, npts)
+   !> at the points (fi(q), fj(q)), S(fi + fj) evaluated exactly.
    subroutine opBFMMsh_(bfm, fi, fj)
       real(bsa_real_t), intent(inout), contiguous :: bfm(:, :)
       real(bsa_real_t), intent(in), contiguous :: fi(:), fj(:)
-      integer(bsa_int_t) :: n, nm
+      integer(bsa_int_t) :: n, n2, n3
+      integer(c_int32_t) :: fl
+      call slotDims_(n2, n3, fl)
       n = size(fi)
-      nm = lp_msh_%nm
-      if (size(fj) /= n .or. size(bfm, 1) /= nm**3 .or. size(bfm, 2) < n) &
-         call bsa_Abort('getBFM_msh: bfm must be (NM^3, npts) with one fj per fi.')
+      if (size(fj) /= n .or. size(bfm, 1) /= n3 .or. size(bfm, 2) < n) &
+         call bsa_Abort('getBFM_msh: bfm must be (NM^3, npts) (NM: ONLY_DIAG) with one fj per fi.')
       if (n == 0) return
       call pts_(bfm, fi, fj)
    contains
       subroutine pts_(b, a1, a2)
-         real(bsa_real_t), target, intent(inout) :: b(nm**3, n)
+         real(bsa_real_t), target, intent(inout) :: b(n3, n)
          real(bsa_real_t), target, intent(in) :: a1(n), a2(n)
          call slotCheck_(bsahip_bfm_points(lp_msh_, int(n, c_int32_t), c_loc(a1), c_loc(a2), &
-                                           0_c_int32_t, c_loc(b), c_null_ptr), 'getBFM_msh')
+                                           fl, c_loc(b), c_null_ptr), 'getBFM_msh')
       end subroutine
    end subroutine
 
-   !> getBRM_msh (functionsImpl.F90:1031-1115): brm = bfm times
-   !> Re[H_m(fi) H_n(fj) conj H_p(fi + fj)] at each point.
+   !> getBRM_msh (functionsImpl.F90:1031-1115; ONLY_DIAG: getRM_diag_scalar_msh_,
+   !> :1224-1286): brm = bfm times Re[H_m(fi) H_n(fj) conj H_p(fi + fj)] at each point.
    subroutine opBRMMsh_(brm, fi, fj, bfm)
       real(bsa_real_t), intent(inout), contiguous :: brm(:, :)
       real(bsa_real_t), intent(in), contiguous :: fi(:), fj(:)
       real(bsa_real_t), intent(in), contiguous :: bfm(:, :)
-      integer(bsa_int_t) :: n, nm
+      integer(bsa_int_t) :: n, n2, n3
+      integer(c_int32_t) :: fl
+      call slotDims_(n2, n3, fl)
       n = size(fi)
-      nm = lp_%nm
-      if (size(fj) /= n .or. size(brm, 1) /= nm**3 .or. size(bfm, 1) /= nm**3 .or. &
+      if (size(fj) /= n .or. size(brm, 1) /= n3 .or. size(bfm, 1) /= n3 .or. &
           size(brm, 2) < n .or. size(bfm, 2) < n) &
-         call bsa_Abort('getBRM_msh: brm and bfm must be (NM^3, npts) with one fj per fi.')
+         call bsa_Abort('getBRM_msh: brm and bfm must be (NM^3, npts) (NM: ONLY_DIAG) with one fj per fi.')
       if (n == 0) return
       brm(:, 1:n) = bfm(:, 1:n)
       call frf_(brm, fi, fj)
    contains
       subroutine frf_(b, a1, a2)
-         real(bsa_real_t), target, intent(inout) :: b(nm**3, n)
+         real(bsa_real_t), target, intent(inout) :: b(n3, n)
          real(bsa_real_t), target, intent(in) :: a1(n), a2(n)
-         call slotCheck_(bsahip_frf_apply(lp_, 0_c_int32_t, BSAHIP_FRF_POINTS, int(n, c_int64_t), &
+         call slotCheck_(bsahip_frf_apply(lp_, fl, BSAHIP_FRF_POINTS, int(n, c_int64_t), &
                                           c_loc(a1), int(n, c_int64_t), c_loc(a2), c_loc(b)), &
                          'getBRM_msh')
       end subroutine

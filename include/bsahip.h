@@ -199,14 +199,17 @@ int bsahip_classic_bispectra_sel(const bsahip_problem *p, int32_t nfh, double df
 
 /*
  * FRF weighting of given modal spectra, in place (the reference's response
- * operators getRM_full_vect_cls_, functionsImpl.F90:1598-1738, and
- * getRM_full_scalar_msh_, :1031-1115, full model):
+ * operators getRM_full_vect_cls_, functionsImpl.F90:1598-1738,
+ * getRM_full_scalar_cls_, :2329-2450, and getRM_full_scalar_msh_, :1031-1115):
  * H_m(f) = 1 / (K_m - w^2 M_m + i w C_mm), w = 2 pi f, over the kept modes.
  *   BSAHIP_FRF_PSD:    data (n1, NM^2), column n + p NM (0-based) times
  *                      Re[H_p(f1_i) conj H_n(f1_i)]; f2, n2 unused
  *   BSAHIP_FRF_GRID:   data (n1, n2, NM^3), entry m + n NM + p NM^2 at
  *                      (f1_i, f2_j) times Re[H_m(f1_i) H_n(f2_j) conj H_p(f1_i+f2_j)]
  *   BSAHIP_FRF_POINTS: data (NM^3, n1) at the points (f1_q, f2_q): the same
+ * flags BSAHIP_ONLY_DIAG: the diagonal model (getRM_diag_vect_cls_ :1975-2101,
+ * getRM_diag_scalar_cls_ :2570-2637, getRM_diag_scalar_msh_ :1224-1286): NM in place of
+ * NM^2 and NM^3, entry m times |H_m|^2 (PSD) or Re[H_m(f1) H_m(f2) conj H_m(f1+f2)].
  * The reference's vectorised operator keeps its column counters from one call
  * to the next (trap T3: correct on the first call only); this one has no state.
  */
@@ -219,7 +222,9 @@ int bsahip_frf_apply(const bsahip_problem *p, int32_t flags, int32_t kind, int64
 /*
  * Modal bispectra at an arbitrary list of points (fi[q], fj[q]), S(f1+f2)
  * evaluated exactly: getBFM_msh / getBRM_msh (functionsImpl.F90:247-465,
- * 1031-1115).  bfm, brm: (NM^3, npts); brm may be NULL.
+ * 1031-1115).  bfm, brm: (NM^3, npts); brm may be NULL.  flags: 0, or
+ * BSAHIP_ONLY_DIAG for the diagonal model (getFM_diag_tnm_scalar_msh_ /
+ * getRM_diag_scalar_msh_, :1123-1216, :1224-1286): (NM, npts).
  */
 int bsahip_bfm_points(const bsahip_problem *p, int32_t npts, const double *fi,
                       const double *fj, int32_t flags, double *bfm, double *brm);
@@ -228,11 +233,19 @@ int bsahip_bfm_points(const bsahip_problem *p, int32_t npts, const double *fi,
  * The same at points of the Classic grid of (nfh, df) with its truncation:
  * S(|f|) = 0 beyond (nfh-1) df, which reaches f1 + f2 only (the padded S of
  * getFM_full_tnm_scalar_cls_, functionsImpl.F90:2122-2318, trap T2 read as
- * zero): the point form of the reference's scalar Classic operators.  Full
- * model, FP64.  bfm, brm: (NM^3, npts); brm may be NULL.
+ * zero): the point form of the reference's scalar Classic operators.  FP64.
+ * flags 0: bfm, brm (NM^3, npts); BSAHIP_ONLY_DIAG: the diagonal model
+ * (getFM_diag_tnlm_scalar_cls_, :2460-2563), (NM, npts).  brm may be NULL.
  */
-int bsahip_bfm_points_cls(const bsahip_problem *p, int32_t nfh, double df, int32_t npts,
-                          const double *fi, const double *fj, double *bfm, double *brm);
+int bsahip_bfm_points_cls(const bsahip_problem *p, int32_t nfh, double df, int32_t flags,
+                          int32_t npts, const double *fi, const double *fj, double *bfm,
+                          double *brm);
+/*
+ * Modal force PSD of the diagonal model at arbitrary frequencies, psd (NM,
+ * npts): sum over components and loaded nodes of Psi_l(m)^2 S_n(f) (the psd of
+ * getFM_diag_tnlm_vect_cls_ / _scalar_cls_, functionsImpl.F90:1747-1970).
+ */
+int bsahip_diag_psd_points(const bsahip_problem *p, int32_t npts, const double *f, double *psd);
 
 /*
  * Modal force PSD S_Q(n, p; f) at arbitrary frequencies: psd (NM*NM, npts),

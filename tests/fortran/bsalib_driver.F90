@@ -29,8 +29,9 @@
 !! end, "CALLBACK <zones> <points> <sum of values>"; "premesh" calls
 !! bsa_enableOnlyPremesh, "modal" bsa_exportModalData; "mtol=" sets the
 !! Mesher's target error (bsa_setMesherTolerance; 0: rule-based levels).
-!! "slots" (with "both") calls the operator slots bsa_Run bound
-!! (getBFM_vect_cls ... getBRM_msh) on the Classic grid, prints
+!! "slots" (with "both", and "diag" for the ONLY_DIAG operators) calls the
+!! operator slots bsa_Run bound (getBFM_vect_cls ... getBRM_msh) on the Classic
+!! grid, prints
 !! "SLOT <check> <max relative difference>" for the scalar and Mesher
 !! operators against the vectorised ones at a strided set of grid points,
 !! and appends the vectorised psd_f, psd_r, bisp_f, bisp_r.
@@ -75,7 +76,7 @@ program bsalib_driver
    real(real64), allocatable :: m3mf(:), m3mr(:)
    real(real64), allocatable :: bkg(:), res(:), m2d(:), m3d(:), m22d(:), skd(:), m2c(:), m3c(:)
    real(real64), allocatable :: m22c(:), skc(:), pgd(:), ppd(:), pnd(:), pgc(:), ppc(:), pnc(:)
-   logical :: post, slots
+   logical :: post, slots, diagm
    real(real64), allocatable :: sl_psdf(:, :), sl_psdr(:, :), sl_bf(:, :, :), sl_br(:, :, :)
    character(len=512) :: fin, fout, arg
    logical :: grid_checks
@@ -88,6 +89,7 @@ program bsalib_driver
    grid_checks = .false.
    post = .false.
    slots = .false.
+   diagm = .false.
    isuban = 1
    isvd = 0
 
@@ -122,7 +124,10 @@ program bsalib_driver
       if (trim(arg) == 'post') post = .true.
       if (trim(arg) == 'slots') slots = .true.
       if (trim(arg) == 'gen') call bsa_generateBSAInputFiles(.false.)
-      if (trim(arg) == 'diag') call bsa_setSpectraExtension(1)
+      if (trim(arg) == 'diag') then
+         call bsa_setSpectraExtension(1)
+         diagm = .true.
+      end if
       if (arg(1:7) == 'visual=') then
          read(arg(8:), *) vis
          call bsa_enableVisualMode()
@@ -260,6 +265,10 @@ contains
       nf = 2 * nfh - 1
       n2 = size(used)**2
       n3 = size(used)**3
+      if (diagm) then
+         n2 = size(used)
+         n3 = size(used)
+      end if
       allocate(f(nf), suvw(nf, 1), spad(1), p1(n2), b1(n3), pr1(n2), br1(n3))
       suvw = 0
       spad = 0
