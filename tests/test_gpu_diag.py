@@ -56,6 +56,22 @@ def test_fortran_front_diag(gpu_lib, name):
         assert rel_err(out[k], g[f"{k}_diag"]) < 1e-12, k
 
 
+@pytest.mark.parametrize("name", ["tiny_uvw", "config1"])
+def test_fortran_front_diag_scalar_version(gpu_lib, name):
+    """The scalar Classic version on the diagonal model: third moments are plain
+    Riemann sums of the diagonal bispectra (BsaClassicImpl.F90:331,341), second
+    moments keep the vectorised rule (the golden diagonal ones)."""
+    p = CASES[name]()
+    g = load_golden(name)
+    out, r = fdriver.run(p, "diag", "scalar")
+    assert "SCALAR" in r.stdout
+    o = O.classic_diag(p)
+    d = 2 * np.pi * p.df
+    assert rel_err(out["m3mf"], o["bisp_f"].sum(axis=(0, 1)) * d * d) < 1e-12
+    assert rel_err(out["m3mr"], o["bisp_r"].sum(axis=(0, 1)) * d * d) < 1e-12
+    assert rel_err(out["m2mr"], g["m2mr_diag"]) < 1e-12
+
+
 def test_mesher_diag_matches_oracle(gpu_lib, tmp_path):
     p = CASES["tiny_uvw"]()
     kw = dict(bkg_base_rfmnt=5, tile_cells=2, bkg_peakw=0.25)
