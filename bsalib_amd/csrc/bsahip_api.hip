@@ -1054,6 +1054,9 @@ int bsahip_classic_spectra(const bsahip_problem *p, int32_t nfh, double df, int3
    std::lock_guard<std::recursive_mutex> lk(g.mu);
    CHK(ensure_init());
    CHK(check_flags(flags, true, false));
+   if (flags & BSAHIP_ONLY_DIAG)
+      return fail(BSAHIP_EUNSUPPORTED, "bsahip_classic_spectra is the full model; the diagonal model's "
+                                       "spectra: bsahip_diag_psd_points, bsahip_classic_bispectra_sel");
    if (nfh < 2 || !(df > 0.0)) return fail(BSAHIP_EINVAL, "need nfh >= 2 and df > 0");
    Bufs &B = oneshot_bufs();
    Prob D;

@@ -174,7 +174,8 @@ int bsahip_plan_destroy(bsahip_plan *plan);
  * bfm = getFM_full_tnm_vect_cls_ output, brm = after getRM_full_vect_cls_
  * (functionsImpl.F90:1312-1738).  psd_f/psd_r: (NF, NM^2) modal PSDs.
  * Any output may be NULL.  Meant for parity checks on small grids; the
- * moments path (bsahip_classic_run) never materialises them.
+ * moments path (bsahip_classic_run) never materialises them.  Full model only
+ * (BSAHIP_ONLY_DIAG: BSAHIP_EUNSUPPORTED).
  */
 int bsahip_classic_spectra(const bsahip_problem *p, int32_t nfh, double df, int32_t flags,
                            double *psd_f, double *psd_r, double *bfm, double *brm);

@@ -108,6 +108,13 @@ def test_frf_apply_errors(gpu_lib):
     with pytest.raises(abi.BsaHipError) as e:
         gpu_lib.frf_apply(p, 7, np.zeros(3), f, bound=b)
     assert e.value.code == abi.EINVAL
+    # the full-model spectra entry refuses the diagonal flag instead of writing NM^3 entries
+    nf = p.nf
+    buf = np.zeros(nf * nf * nm ** 3)
+    rc = abi.lib().bsahip_classic_spectra(abi.C.byref(b.struct), abi.C.c_int32(p.nfreqs),
+                                          abi.C.c_double(p.df), abi.C.c_int32(abi.ONLY_DIAG),
+                                          None, None, abi._ptr(buf, abi.C.c_double), None)
+    assert rc == abi.EUNSUPPORTED
     # an empty list is a no-op
     out = gpu_lib.frf_apply(p, abi.FRF_POINTS, np.zeros((nm ** 3, 0)), np.zeros(0), np.zeros(0), bound=b)
     assert out.size == 0
