@@ -1260,6 +1260,7 @@ namespace {
 int diag_sq_table(Bufs &B, const Prob &D, const std::vector<double> &uq, double fmax, double **sq)
 {
    const int nu = (int)uq.size();
+   if (nu > 65535) return fail(BSAHIP_EUNSUPPORTED, "more than 65535 distinct frequencies in one table");
    double *fl;
    CHK(upload(B, "dg_fl", uq, &fl));
    CHK(B.get("dg_sq", (size_t)D.ntc * nu * D.nnl, sq));
