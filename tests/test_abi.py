@@ -103,3 +103,26 @@ def test_fortran_types_list_the_c_fields_in_order():
             decl = line.split("::", 1)[1]
             fields += [re.match(r"\s*(\w+)", part).group(1) for part in decl.split(",")]
         assert [f.lower() for f in fields] == [f[0].lower() for f in T._fields_]
+
+
+def test_wrappers_check_sizes_before_calling_the_library():
+    """frf_apply / bfm_points / bfm_points_cls refuse inconsistent arrays on the
+    host (ValueError), so the C ABI never reads past a caller's buffer."""
+    import numpy as np
+    from bsalib_amd import abi
+    from tests.cases import CASES
+    p = CASES["tiny_uvw"]()
+    b = abi.Bound(p)
+    nm, f = b.nm, p.freqs()
+    with pytest.raises(ValueError):
+        abi.frf_apply(p, abi.FRF_PSD, np.zeros((f.size, nm)), f, bound=b)              # NM^2 columns
+    with pytest.raises(ValueError):
+        abi.frf_apply(p, abi.FRF_PSD, np.zeros((f.size, nm * nm)), f, flags=abi.ONLY_DIAG, bound=b)
+    with pytest.raises(ValueError):
+        abi.frf_apply(p, abi.FRF_GRID, np.zeros((f.size, f.size, nm)), f, f, bound=b)
+    with pytest.raises(ValueError):
+        abi.frf_apply(p, abi.FRF_POINTS, np.zeros((nm ** 3, 3)), f[:3], f[:2], bound=b)
+    with pytest.raises(ValueError):
+        abi.bfm_points(p, f[:3], f[:2], bound=b)
+    with pytest.raises(ValueError):
+        abi.bfm_points_cls(p, f[:3], f[:2], bound=b, only_diag=True)
