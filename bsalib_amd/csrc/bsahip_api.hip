@@ -194,7 +194,9 @@ int validate(const bsahip_problem *p)
 
 // Device-resident description of one structure.
 struct Prob {
-   int nn = 0, nnl = 0, nnlp = 0, nm = 0, ntc = 0, K = 0, pulsation = 1;
+   // K rows: component c, loaded node i at k = c*nnlp + i (nnlp = nnl), the
+   // kv = ntc*nnl valid rows padded with zero rows to a multiple of the K chunk
+   int nn = 0, nnl = 0, nnlp = 0, nm = 0, ntc = 0, K = 0, kv = 0, pulsation = 1;
    double *pa_t = nullptr;   // [tc][m][nnl]
    double *pa_k = nullptr;   // [m][K]
    double *pb_k = nullptr;   // [m][K]
@@ -217,8 +219,9 @@ int load_problem(const bsahip_problem *p, int psd_mode, Bufs &B, Prob &D)
    D.nnl = p->nnl;
    D.nm = p->nm;
    D.ntc = p->ntc;
-   D.nnlp = round_up(p->nnl, KC);
-   D.K = D.ntc * D.nnlp;
+   D.nnlp = p->nnl;
+   D.kv = D.ntc * D.nnl;
+   D.K = round_up(D.kv, KC);
    D.pulsation = p->pulsation;
    const int nm = D.nm, nnl = D.nnl, K = D.K;
    std::vector<double> pa_t((size_t)D.ntc * nm * nnl, 0.0), pa_k((size_t)nm * K, 0.0),
@@ -927,7 +930,7 @@ int diag_tables(Bufs &B, const Prob &D, int nh, double df, bool with_T, double *
    k_psd_sqrt<<<gp, 64, 0, g.s>>>(dfl, nh, D.prm, D.formula, D.nnl, D.ntc, D.pulsation, sq);
    HIPC(hipGetLastError());
    k_diag_tables<<<dim3((unsigned)((nh + 63) / 64), (unsigned)K), 64, 0, g.s>>>(
-      sq, nh, D.nnl, D.nnlp, D.pa_k, D.pb_k, K, nm, *X, Y);
+      sq, nh, D.nnl, D.kv, D.pa_k, D.pb_k, K, nm, *X, Y);
    HIPC(hipGetLastError());
    CHK(launch_frf(D, nft, df, *H));
    *T = nullptr;
@@ -1244,7 +1247,7 @@ int bsahip_diag_psd_points(const bsahip_problem *p, int32_t npts, const double *
    const long tot = (long)npts * D.nm;
    CHK(B.get("dp_out", (size_t)tot, &out));
    k_diag_psd_points<<<(unsigned)((tot + 255) / 256), 256, 0, g.s>>>(
-      sq, (int)uq.size(), D.nnl, D.nnlp, D.pa_k, D.K, D.nm, npts, diu, out);
+      sq, (int)uq.size(), D.nnl, D.kv, D.pa_k, D.K, D.nm, npts, diu, out);
    HIPC(hipGetLastError());
    HIPC(hipMemcpyAsync(psd, out, (size_t)tot * sizeof(double), hipMemcpyDeviceToHost, g.s));
    HIPC(hipStreamSynchronize(g.s));
@@ -1305,7 +1308,7 @@ int bfm_points_impl(const bsahip_problem *p, int32_t npts, const double *fi, con
       CHK(upload(B, "dg_slot", slot, &ds));
       CHK(B.get("bfm_out", (size_t)tot, &ob));
       k_diag_points<<<(unsigned)((tot + 255) / 256), 256, 0, g.s>>>(
-         sq, (int)uq.size(), D.nnl, D.nnlp, D.pa_k, D.pb_k, D.K, D.nm, npts, d1, d2, d3, ds, ob);
+         sq, (int)uq.size(), D.nnl, D.kv, D.pa_k, D.pb_k, D.K, D.nm, npts, d1, d2, d3, ds, ob);
       HIPC(hipGetLastError());
       HIPC(hipMemcpyAsync(bfm, ob, (size_t)tot * sizeof(double), hipMemcpyDeviceToHost, g.s));
       if (brm) {

@@ -12,15 +12,15 @@
 
 // X[k*nh + u] = S_k(u df); Y[(m*K + k)*nh + u] = W_m(k) S_k(u df).
 // grid (ceil(nh/64), K), block 64
-__global__ void k_diag_tables(const double *__restrict__ sq, int nh, int nnl, int nnlp,
+__global__ void k_diag_tables(const double *__restrict__ sq, int nh, int nnl, int kv,
                               const double *__restrict__ pa, const double *__restrict__ pb, int K,
                               int nm, double *__restrict__ X, double *__restrict__ Y)
 {
    const int u = blockIdx.x * 64 + threadIdx.x;
    const int k = blockIdx.y;
    if (u >= nh) return;
-   const int c = k / nnlp, n = k % nnlp;
-   const double s = (n < nnl) ? sq[((size_t)c * nh + u) * nnl + n] : 0.0;
+   const int c = k / nnl, n = k % nnl;                // rows k >= kv are padding
+   const double s = (k < kv) ? sq[((size_t)c * nh + u) * nnl + n] : 0.0;
    const double S = s * s;
    X[(size_t)k * nh + u] = S;
    for (int m = 0; m < nm; ++m) {
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(256) void k_diag_bisp_sel(const double *__restrict_
 // W_m(k) = Psi_q Psi_l^2, S_k = sq^2 from the table sq[(c*nu + u)*nnl + n] at rows
 // i1, i2, i3 (|f1|, |f2|, |f1+f2|).  grid (ceil(npts*nm/256)), block 256
 __global__ __launch_bounds__(256) void k_diag_points(
-   const double *__restrict__ sq, int nu, int nnl, int nnlp, const double *__restrict__ pa,
+   const double *__restrict__ sq, int nu, int nnl, int kv, const double *__restrict__ pa,
    const double *__restrict__ pb, int K, int nm, int npts, const int *__restrict__ i1,
    const int *__restrict__ i2, const int *__restrict__ i3, const long *__restrict__ slot,
    double *__restrict__ out)
@@ -148,9 +148,8 @@ __global__ __launch_bounds__(256) void k_diag_points(
    const int q = (int)(gid / nm), m = (int)(gid % nm);
    const int a1 = i1[q], a2 = i2[q], a3 = i3[q];
    double s = 0.0;
-   for (int k = 0; k < K; ++k) {
-      const int c = k / nnlp, n = k % nnlp;
-      if (n >= nnl) continue;
+   for (int k = 0; k < kv; ++k) {                      // rows k >= kv are padding
+      const int c = k / nnl, n = k % nnl;
       const double *col = sq + (size_t)c * nu * nnl + n;
       const double r1 = col[(size_t)a1 * nnl], r2 = col[(size_t)a2 * nnl];
       const double r3 = a3 >= 0 ? col[(size_t)a3 * nnl] : 0.0;

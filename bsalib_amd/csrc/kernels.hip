@@ -1478,7 +1478,7 @@ __global__ void k_frf_apply(int kind, int diag, long n1, const double *__restric
 // Diagonal-model modal PSD at points: psd[q nm + m] = sum_k pa(m,k)^2 S_k(f_q),
 // S from the sqrt-PSD table sq (ntc, nu, nnl) at row iu[q] (getFM_diag_tnlm_*'s
 // psd, functionsImpl.F90:1747-1970).
-__global__ void k_diag_psd_points(const double *__restrict__ sq, int nu, int nnl, int nnlp,
+__global__ void k_diag_psd_points(const double *__restrict__ sq, int nu, int nnl, int kv,
                                   const double *__restrict__ pa, int K, int nm, int npts,
                                   const int *__restrict__ iu, double *__restrict__ psd)
 {
@@ -1487,9 +1487,8 @@ __global__ void k_diag_psd_points(const double *__restrict__ sq, int nu, int nnl
    const int q = (int)(gid / nm), m = (int)(gid % nm);
    const int u = iu[q];
    double s = 0.0;
-   for (int k = 0; k < K; ++k) {
-      const int c = k / nnlp, n = k % nnlp;
-      if (n >= nnl) continue;
+   for (int k = 0; k < kv; ++k) {                      // rows k >= kv are padding
+      const int c = k / nnl, n = k % nnl;
       const double r = sq[((size_t)c * nu + u) * nnl + n];
       const double a = pa[(size_t)m * K + k];
       s += a * a * r * r;

@@ -802,7 +802,7 @@ int msh_eval_chunk(bsahip_mesher *M, int npts, const double *f1p, const double *
       CHK(upload(M->B, "m_slot", slot, &ds));
       const long tot = (long)npts * D.nm;
       k_diag_points<<<(unsigned)((tot + 255) / 256), 256, 0, g.s>>>(
-         sq, nu, D.nnl, D.nnlp, D.pa_k, D.pb_k, D.K, D.nm, npts, d1, d2, d3, ds, out);
+         sq, nu, D.nnl, D.kv, D.pa_k, D.pb_k, D.K, D.nm, npts, d1, d2, d3, ds, out);
       HIPC(hipGetLastError());
       return 0;
    }
