@@ -310,7 +310,7 @@ __device__ __forceinline__ void epi_accum(const GsymArgs &a, int x, int y, int z
    // and contribute with weight 0 (divergent exits would make the compiler
    // copy the nine accumulators at every join)
    const int vr = vq0 + (lane & 15);
-   const bool vok = vr < a.nu;
+   const bool vok = vr >= 0 && vr < a.nu;          // a block may start in the previous mode
    const int v = vok ? vr : 0;
    const double2 hxv = Hx[v], hyv = Hy[v];
    const double2 cyv = make_double2(hyv.x, -hyv.y);
@@ -318,11 +318,11 @@ __device__ __forceinline__ void epi_accum(const GsymArgs &a, int x, int y, int z
    const bool ev = (v == n), ov = (v > n);
 #pragma unroll
    for (int t = 0; t < 4; ++t) {
-      // rows past the plane (padding up to a multiple of 16) add nothing: stop
-      // once the lowest row of this register is one of them (uniform)
+      // rows past the mode's last row (the next mode's rows, or padding) add
+      // nothing: stop once the lowest row of this register is one of them (uniform)
       if (ur0 + (interleaved ? 4 * t : t) >= a.nu) break;
       const int ur = ur0 + (interleaved ? ((lane >> 4) + 4 * t) : (4 * (lane >> 4) + t));
-      const bool ok = vok && ur < a.nu;
+      const bool ok = vok && ur >= 0 && ur < a.nu;
       const int u = ok ? ur : 0;
       const double g = ok ? 2.0 * gval[t] : 0.0;   // B = 2 * sum of the three G terms
       const double ms = (u == 0 && v == 0) ? 1.0 : 2.0;
@@ -592,8 +592,11 @@ __device__ __forceinline__ void epi_wave(const GsymArgs &a, int x, bool diag, in
       }
       epi_flush(a, x, y0, z0, diag, lane, s, sbase, f++);
    } else if constexpr (BR * BC > 4) {
-      // the wave's rows (cols) span modes y0..y1 (z0..z1), each 16-block in one
-      // of them (nhp is a multiple of 16): one reduction per (y, z) pair
+      // the wave's rows (cols) span modes y0..y1 (z0..z1): one reduction per
+      // (y, z) pair over the blocks that meet both.  The rows of consecutive
+      // modes are packed, so a 16-block can hold the end of one mode and the
+      // start of the next: it is visited for each, the rows (cols) of the
+      // other mode masked by the generic epilogue (u < 0 or u >= nu there)
 #pragma unroll 1
       for (int y = y0; y <= y1; ++y)
 #pragma unroll 1
@@ -603,7 +606,9 @@ __device__ __forceinline__ void epi_wave(const GsymArgs &a, int x, bool diag, in
             for (int b = 0; b < BR * BC; ++b) {
                const int i = b / BC, j = b % BC;
                const int r0 = rw0 + 16 * i, q0 = cw0 + 16 * j;
-               if (r0 / a.nhp != y || q0 / a.nhp != z) continue;
+               if (r0 / a.nhp > y || (r0 + 15) / a.nhp < y || q0 / a.nhp > z ||
+                   (q0 + 15) / a.nhp < z)
+                  continue;
                double g[4];
                getv(b, g);
                epi_accum_any(a, x, y, z, diag, r0 - y * a.nhp, q0 - z * a.nhp, lane, g, interleaved,
@@ -612,7 +617,9 @@ __device__ __forceinline__ void epi_wave(const GsymArgs &a, int x, bool diag, in
             epi_flush(a, x, y, z, diag, lane, s, sbase, f++);
          }
    } else {
-      // few blocks per wave (and fewer registers to spare): one reduction per block
+      // few blocks per wave (and fewer registers to spare): one reduction per
+      // block.  Each block lies in one mode: the kernels that take this path
+      // (k_gsym_f64) get rows per mode padded to a multiple of 16
 #pragma unroll 1
       for (int b = 0; b < BR * BC; ++b) {
          const int i = b / BC, j = b % BC;
