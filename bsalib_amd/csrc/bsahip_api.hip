@@ -531,15 +531,10 @@ void det_slot_lists(const bsahip_plan *P, std::vector<long> &off, std::vector<lo
             int f = 0;
             if (y0 == y1 && z0 == z1) {
                if (y0 < nm && z0 < nm) emit(lid, w, f++, x, y0, z0, diag);
-            } else if (BR * BC > 4) {
+            } else {
                for (int y = y0; y <= y1; ++y)
                   for (int z = z0; z <= z1; ++z)
                      if (y < nm && z < nm) emit(lid, w, f++, x, y, z, diag);
-            } else {
-               for (int b = 0; b < BR * BC; ++b) {
-                  const int y = (rw0 + 16 * (b / BC)) / nhp, z = (cw0 + 16 * (b % BC)) / nhp;
-                  if (y < nm && z < nm) emit(lid, w, f++, x, y, z, diag);
-               }
             }
          }
       }
@@ -588,11 +583,9 @@ int plan_setup(bsahip_plan *P, const bsahip_problem *p, int32_t nfh, double df, 
    P->nh = nfh;
    P->nf = 2 * nfh - 1;
    P->nu = (flags & BSAHIP_SUM_EXACT) ? P->nf : P->nh;          // rows of A per mode
-   // rows per mode: packed for the 128x128 kernels, whose epilogue masks a
-   // 16-row block that spans two modes (4.8 % fewer tile pairs at 513 rows);
-   // a multiple of 16 for k_gsym_f64, whose one-block flushes would pay the
-   // split twice (it ran 2.6 % slower packed)
-   P->nhp = (P->prec == BSAHIP_PREC_FP64) ? round_up(P->nu, 16) : P->nu;
+   // rows of consecutive modes packed (no padding to 16 per mode): a 16-row
+   // block may span two modes, and the epilogue masks the other mode's rows
+   P->nhp = P->nu;
    P->rows = P->D.nm * P->nhp;
    const int tile = (P->prec == BSAHIP_PREC_FP64) ? TILE : T2;
    P->ntile = (P->rows + tile - 1) / tile;

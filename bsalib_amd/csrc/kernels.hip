@@ -568,7 +568,7 @@ __device__ __forceinline__ void epi_flush(const GsymArgs &a, int x, int y, int z
 }
 
 // Epilogue of a wave's BR x BC blocks of 16x16 (rows rw0.., cols cw0..): one
-// accumulation / reduction when all blocks share (y,z), else per block.
+// accumulation / reduction when all blocks share (y,z), else one per (y,z) pair.
 template <int BR, int BC, class GetVal>
 __device__ __forceinline__ void epi_wave(const GsymArgs &a, int x, bool diag, int rw0, int cw0,
                                          int lane, bool interleaved, long sbase, GetVal getv)
@@ -591,7 +591,7 @@ __device__ __forceinline__ void epi_wave(const GsymArgs &a, int x, bool diag, in
                        lane, g, interleaved, s);
       }
       epi_flush(a, x, y0, z0, diag, lane, s, sbase, f++);
-   } else if constexpr (BR * BC > 4) {
+   } else {
       // the wave's rows (cols) span modes y0..y1 (z0..z1): one reduction per
       // (y, z) pair over the blocks that meet both.  The rows of consecutive
       // modes are packed, so a 16-block can hold the end of one mode and the
@@ -616,21 +616,6 @@ __device__ __forceinline__ void epi_wave(const GsymArgs &a, int x, bool diag, in
             }
             epi_flush(a, x, y, z, diag, lane, s, sbase, f++);
          }
-   } else {
-      // few blocks per wave (and fewer registers to spare): one reduction per
-      // block.  Each block lies in one mode: the kernels that take this path
-      // (k_gsym_f64) get rows per mode padded to a multiple of 16
-#pragma unroll 1
-      for (int b = 0; b < BR * BC; ++b) {
-         const int i = b / BC, j = b % BC;
-         const int r0 = rw0 + 16 * i, q0 = cw0 + 16 * j;
-         const int y = r0 / a.nhp, z = q0 / a.nhp;
-         if (y >= nm || z >= nm) continue;             // uniform per block
-         double g[4];
-         getv(b, g);
-         epi_accum_any(a, x, y, z, diag, r0 - y * a.nhp, q0 - z * a.nhp, lane, g, interleaved, s);
-         epi_flush(a, x, y, z, diag, lane, s, sbase, f++);
-      }
    }
 }
 
