@@ -374,10 +374,11 @@ __device__ __forceinline__ void epi_accum(const GsymArgs &a, int x, int y, int z
 // With Wa = Wb = We = 1, ms = mo = 2 and h = 2 g = 4 G the contributions of
 // epi_accum reduce to three (CLS 1) or five (CLS 2, 3) distinct sums, kept per
 // block and spread over the nine at its end; no lane is out of range.
-template <int CLS, bool DIAG>
+template <int CLS, bool DIAG, bool WIN>
 __device__ __forceinline__ void epi_accum_inner(const GsymArgs &a, int x, int y, int z, int ur0,
                                                 int vq0, int lane, const double (&gval)[4],
-                                                bool interleaved, double (&s)[9])
+                                                bool interleaved, double (&s)[9],
+                                                const double2 *win, int wsb, int wdb)
 {
    const double2 *Hx = a.H + (size_t)x * a.nft;
    const double2 *Hy = a.H + (size_t)y * a.nft;
@@ -399,30 +400,43 @@ __device__ __forceinline__ void epi_accum_inner(const GsymArgs &a, int x, int y,
       const int sp = u + v;
       const int dm = u - v;
       const int ad = dm < 0 ? -dm : dm;
-      const double2 hxu = Hx[u], hzu = Hz[u], hzp = Hz[sp];
-      double2 hym = Hy[ad], hzm = Hz[ad];
-      if (dm < 0) {
-         hym.y = -hym.y;
-         hzm.y = -hzm.y;
+      const double2 hxu = Hx[u], hzu = Hz[u];
+      double2 hzp, hym, hzm;
+      if constexpr (WIN) {                           // the wave's LDS window
+         hzp = win[sp - wsb];
+         hym = win[256 + dm - wdb];
+         hzm = win[384 + dm - wdb];
+      } else {
+         hzp = Hz[sp];
+         hym = Hy[ad];
+         hzm = Hz[ad];
+         if (dm < 0) {
+            hym.y = -hym.y;
+            hzm.y = -hzm.y;
+         }
       }
+      auto hys = [&]() {                                              // H_y(u+v)
+         if constexpr (WIN) return win[128 + sp - wsb];
+         else return Hy[sp];
+      };
       const double F1s = re3(hxu, hyv, hzp);
       const double F1o = re3(hxu, cyv, hzm);
       const double F2s = re3(hym, hxv, hzu);
       lh += h;
       lA += h * (F1s + F1o);
       if constexpr (CLS == 1) {
-         const double F2o = re3(Hy[sp], cxv, hzu);
+         const double F2o = re3(hys(), cxv, hzu);
          lB += h * (F2s + F2o);
       } else if constexpr (CLS == 2) {
          lB += h * F2s;
          lC += h * F1o;
          if constexpr (!DIAG) {
-            const double F2o = re3(Hy[sp], cxv, hzu);
+            const double F2o = re3(hys(), cxv, hzu);
             lD += h * (F2s + F2o);
          }
       } else {
          const double w = sp < n ? 1.0 : (sp == n ? ce : 0.0);
-         const double F2o = re3(Hy[sp], cxv, hzu);
+         const double F2o = re3(hys(), cxv, hzu);
          const double hw = h * w;
          lW += hw;
          lB += h * F2s + hw * F2o;
@@ -482,27 +496,29 @@ __device__ __forceinline__ int epi_class(const GsymArgs &a, int u0, int v0)
    return 3;
 }
 
+template <bool WIN>
 __device__ __forceinline__ void epi_accum_any(const GsymArgs &a, int x, int y, int z, bool diag,
                                               int ur0, int vq0, int lane, const double (&g)[4],
-                                              bool interleaved, double (&s)[9])
+                                              bool interleaved, double (&s)[9],
+                                              const double2 *win, int wsb, int wdb)
 {
    const int cls = epi_class(a, ur0, vq0);           // uniform per block
    if (diag) {
       if (cls == 1)
-         epi_accum_inner<1, true>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
+         epi_accum_inner<1, true, WIN>(a, x, y, z, ur0, vq0, lane, g, interleaved, s, win, wsb, wdb);
       else if (cls == 2)
-         epi_accum_inner<2, true>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
+         epi_accum_inner<2, true, WIN>(a, x, y, z, ur0, vq0, lane, g, interleaved, s, win, wsb, wdb);
       else if (cls == 3)
-         epi_accum_inner<3, true>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
+         epi_accum_inner<3, true, WIN>(a, x, y, z, ur0, vq0, lane, g, interleaved, s, win, wsb, wdb);
       else
          epi_accum<true>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
    } else {
       if (cls == 1)
-         epi_accum_inner<1, false>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
+         epi_accum_inner<1, false, WIN>(a, x, y, z, ur0, vq0, lane, g, interleaved, s, win, wsb, wdb);
       else if (cls == 2)
-         epi_accum_inner<2, false>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
+         epi_accum_inner<2, false, WIN>(a, x, y, z, ur0, vq0, lane, g, interleaved, s, win, wsb, wdb);
       else if (cls == 3)
-         epi_accum_inner<3, false>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
+         epi_accum_inner<3, false, WIN>(a, x, y, z, ur0, vq0, lane, g, interleaved, s, win, wsb, wdb);
       else
          epi_accum<false>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
    }
@@ -569,10 +585,21 @@ __device__ __forceinline__ void epi_flush(const GsymArgs &a, int x, int y, int z
 
 // Epilogue of a wave's BR x BC blocks of 16x16 (rows rw0.., cols cw0..): one
 // accumulation / reduction when all blocks share (y,z), else one per (y,z) pair.
-template <int BR, int BC, class GetVal>
+// WIN: win holds 512 double2 of LDS owned by this wave (free after the K loop).
+// When all blocks share (y,z), the FRF values the interior classes gather per
+// point are staged there first, each indexed by its offset from the wave's
+// origin (ub, vb) within the modes:
+//   [0, 128)   H_z(u+v)      at u+v - (ub+vb)
+//   [128, 256) H_y(u+v)      at u+v - (ub+vb)
+//   [256, 384) H_y(u-v)      at u-v - (ub-vb-16BC+1), conjugated for u < v
+//   [384, 512) H_z(u-v)      likewise
+// (u+v and u-v span 16(BR+BC)-1 <= 127 values over the wave's blocks).
+template <int BR, int BC, bool WIN, class GetVal>
 __device__ __forceinline__ void epi_wave(const GsymArgs &a, int x, bool diag, int rw0, int cw0,
-                                         int lane, bool interleaved, long sbase, GetVal getv)
+                                         int lane, bool interleaved, long sbase, GetVal getv,
+                                         double2 *win)
 {
+   static_assert(16 * (BR + BC) - 1 <= 128, "FRF window of 128 entries");
    const int nm = a.nm;
    int f = 0;                                         // flushes of this wave so far
    const int y0 = rw0 / a.nhp, y1 = (rw0 + 16 * BR - 1) / a.nhp;
@@ -582,13 +609,37 @@ __device__ __forceinline__ void epi_wave(const GsymArgs &a, int x, bool diag, in
    for (int t = 0; t < 9; ++t) s[t] = 0.0;
    if (y0 == y1 && z0 == z1) {
       if (y0 >= nm || z0 >= nm) return;
+      const int ub = rw0 - y0 * a.nhp, vb = cw0 - z0 * a.nhp;
+      const int wsb = ub + vb, wdb = ub - vb - (16 * BC - 1);
+      if constexpr (WIN) {
+         const double2 *Hy = a.H + (size_t)y0 * a.nft;
+         const double2 *Hz = a.H + (size_t)z0 * a.nft;
+         for (int e = lane; e < 128; e += 64) {
+            const int sp = min(wsb + e, a.nft - 1);   // entries past the wave's range unused
+            const int dm = wdb + e;
+            const int ad = min(dm < 0 ? -dm : dm, a.nft - 1);
+            double2 hym = Hy[ad], hzm = Hz[ad];
+            if (dm < 0) {
+               hym.y = -hym.y;
+               hzm.y = -hzm.y;
+            }
+            win[e] = Hz[sp];
+            win[128 + e] = Hy[sp];
+            win[256 + e] = hym;
+            win[384 + e] = hzm;
+         }
+         // the wave's own LDS writes before its reads (other waves never touch it)
+         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+         __builtin_amdgcn_wave_barrier();
+         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
 #pragma unroll 1
       for (int b = 0; b < BR * BC; ++b) {
          const int i = b / BC, j = b % BC;
          double g[4];
          getv(b, g);
-         epi_accum_any(a, x, y0, z0, diag, rw0 + 16 * i - y0 * a.nhp, cw0 + 16 * j - z0 * a.nhp,
-                       lane, g, interleaved, s);
+         epi_accum_any<WIN>(a, x, y0, z0, diag, ub + 16 * i, vb + 16 * j, lane, g, interleaved, s,
+                            win, wsb, wdb);
       }
       epi_flush(a, x, y0, z0, diag, lane, s, sbase, f++);
    } else {
@@ -611,8 +662,8 @@ __device__ __forceinline__ void epi_wave(const GsymArgs &a, int x, bool diag, in
                   continue;
                double g[4];
                getv(b, g);
-               epi_accum_any(a, x, y, z, diag, r0 - y * a.nhp, q0 - z * a.nhp, lane, g, interleaved,
-                             s);
+               epi_accum_any<false>(a, x, y, z, diag, r0 - y * a.nhp, q0 - z * a.nhp, lane, g,
+                                    interleaved, s, nullptr, 0, 0);
             }
             epi_flush(a, x, y, z, diag, lane, s, sbase, f++);
          }
@@ -707,7 +758,7 @@ __device__ __forceinline__ void gsym_f64_item(const GsymArgs &a, int x, long tp)
    // D[(l/16) + 4t][l%16] of its 16x16 block (rows interleaved by 4).
    const long sbase = ((tp * a.nm + x) * 4 + w) * (long)a.mflush * 12;
    __builtin_amdgcn_s_setprio(0);
-   epi_wave<2, 2>(a, x, diag, R * TILE + wr * 32, Q * TILE + wc * 32, lane, true, sbase,
+   epi_wave<2, 2, false>(a, x, diag, R * TILE + wr * 32, Q * TILE + wc * 32, lane, true, sbase,
                   [&](int b, double (&g)[4]) {
                      d4 v;
                      switch (b) {       // uniform branch: constant register indices
@@ -718,7 +769,8 @@ __device__ __forceinline__ void gsym_f64_item(const GsymArgs &a, int x, long tp)
                      }
 #pragma unroll
                      for (int t = 0; t < 4; ++t) g[t] = v[t];
-                  });
+                  },
+                  nullptr);
 }
 
 // Four workgroups per CU (at most 128 VGPRs: a few spill in the epilogue):
@@ -852,6 +904,7 @@ __device__ __forceinline__ void gsym_bf16x3_item(const GsymArgs &a, const GsymAr
    // 1-1.4 % shorter kernel; the reverse order gains nothing)
    __builtin_amdgcn_s_setprio(1);
    gload(0);
+   __syncthreads();      // a previous item's epilogue (persistent batch) may still read LDS
    lstore(0);
    __syncthreads();
    for (int c = 0; c < nchunk; ++c) {
@@ -891,7 +944,9 @@ __device__ __forceinline__ void gsym_bf16x3_item(const GsymArgs &a, const GsymAr
    __builtin_amdgcn_s_setprio(0);
    // epilogue: f32-output 16x16 MFMAs hold D[4(l/16)+t][l%16]
    const long sbase = ((tp * a.nm + x) * 4 + w) * (long)a.mflush * 12;
-   epi_wave<4, 4>(a, x, diag, R * T2 + wr * 64, Q * T2 + wc * 64, lane, false, sbase,
+   // each wave's FRF window: 8 KB of the A planes, free after the last chunk's barrier
+   double2 *win = reinterpret_cast<double2 *>(w < 2 ? &Ah[0][0] : &Al[0][0]) + (w & 1) * 512;
+   epi_wave<4, 4, true>(a, x, diag, R * T2 + wr * 64, Q * T2 + wc * 64, lane, false, sbase,
                   [&](int b, double (&g)[4]) {
                      f4v v;
                      switch (b) {       // uniform branch: constant register indices
@@ -903,7 +958,8 @@ __device__ __forceinline__ void gsym_bf16x3_item(const GsymArgs &a, const GsymAr
                      }
 #pragma unroll
                      for (int t = 0; t < 4; ++t) g[t] = (double)v[t];
-                  });
+                  },
+                  win);
 }
 
 __global__ __launch_bounds__(256, 2) void k_gsym_bf16x3(GsymArgs a, GsymArgs16 b)
@@ -974,6 +1030,7 @@ __device__ __forceinline__ void gsym_f32_item(const GsymArgs &a, const GsymArgs1
    const int nchunk = a.K / KC4;
    __builtin_amdgcn_s_setprio(1);   // K loop at raised wave priority, as k_gsym_bf16x3
    gload(0);
+   __syncthreads();      // a previous item's epilogue (persistent batch) may still read LDS
    lstore(0);
    __syncthreads();
    const int g = lane >> 4;            // lane group: k = 8g .. 8g+7 of the chunk
@@ -1006,7 +1063,9 @@ __device__ __forceinline__ void gsym_f32_item(const GsymArgs &a, const GsymArgs1
    // epilogue: f32-output 16x16 MFMAs hold D[4(l/16)+t][l%16]
    const long sbase = ((tp * a.nm + x) * 4 + w) * (long)a.mflush * 12;
    __builtin_amdgcn_s_setprio(0);
-   epi_wave<4, 4>(a, x, diag, R * T2 + wr * 64, Q * T2 + wc * 64, lane, false, sbase,
+   // each wave's FRF window: 8 KB of the A planes, free after the last chunk's barrier
+   double2 *win = reinterpret_cast<double2 *>(&As[0][0]) + w * 512;
+   epi_wave<4, 4, true>(a, x, diag, R * T2 + wr * 64, Q * T2 + wc * 64, lane, false, sbase,
                   [&](int bb, double (&gv)[4]) {
                      f4v v;
                      switch (bb) {      // uniform branch: constant register indices
@@ -1018,7 +1077,8 @@ __device__ __forceinline__ void gsym_f32_item(const GsymArgs &a, const GsymArgs1
                      }
 #pragma unroll
                      for (int t = 0; t < 4; ++t) gv[t] = (double)v[t];
-                  });
+                  },
+                  win);
 }
 
 __global__ __launch_bounds__(256, 2) void k_gsym_f32(GsymArgs a, GsymArgs16 b)
