@@ -34,7 +34,7 @@ BF16_PEAK_TFLOPS = 2500.0
 # rocprofv3 --pmc FETCH_SIZE x 2 + WRITE_SIZE x 1 (KiB), the scales read off a
 # 1 GiB device copy (bench.py --copy-calibration); commands and counts in
 # profiles/gsym_bf16x3_traffic_deck_nn1000_nm30.md
-GSYM_BF16X3_TRAFFIC_B = int((2 * 4147946.75 + 410062.5) * 1024)
+GSYM_BF16X3_TRAFFIC_B = int((2 * 4184124.25 + 410062.5) * 1024)
 
 
 def log(msg):
