@@ -378,13 +378,21 @@ template <int CLS, bool DIAG, bool WIN>
 __device__ __forceinline__ void epi_accum_inner(const GsymArgs &a, int x, int y, int z, int ur0,
                                                 int vq0, int lane, const double (&gval)[4],
                                                 bool interleaved, double (&s)[9],
-                                                const double2 *win, int wsb, int wdb)
+                                                const double2 *win, int wub, int wvb, int wdb)
 {
    const double2 *Hx = a.H + (size_t)x * a.nft;
    const double2 *Hy = a.H + (size_t)y * a.nft;
    const double2 *Hz = a.H + (size_t)z * a.nft;
    const int v = vq0 + (lane & 15);
-   const double2 hxv = Hx[v], hyv = Hy[v];
+   const int wsb = wub + wvb;
+   double2 hxv, hyv;
+   if constexpr (WIN) {
+      hxv = win[640 + v - wvb];
+      hyv = win[704 + v - wvb];
+   } else {
+      hxv = Hx[v];
+      hyv = Hy[v];
+   }
    const double2 cyv = make_double2(hyv.x, -hyv.y);
    const double2 cxv = make_double2(hxv.x, -hxv.y);
    // lh = sum h, lA = sum h (F1s + F1o), lB = sum h F2s (CLS 2), h (F2s + F2o)
@@ -400,13 +408,16 @@ __device__ __forceinline__ void epi_accum_inner(const GsymArgs &a, int x, int y,
       const int sp = u + v;
       const int dm = u - v;
       const int ad = dm < 0 ? -dm : dm;
-      const double2 hxu = Hx[u], hzu = Hz[u];
-      double2 hzp, hym, hzm;
+      double2 hxu, hzu, hzp, hym, hzm;
       if constexpr (WIN) {                           // the wave's LDS window
+         hxu = win[512 + u - wub];
+         hzu = win[576 + u - wub];
          hzp = win[sp - wsb];
          hym = win[256 + dm - wdb];
          hzm = win[384 + dm - wdb];
       } else {
+         hxu = Hx[u];
+         hzu = Hz[u];
          hzp = Hz[sp];
          hym = Hy[ad];
          hzm = Hz[ad];
@@ -500,25 +511,25 @@ template <bool WIN>
 __device__ __forceinline__ void epi_accum_any(const GsymArgs &a, int x, int y, int z, bool diag,
                                               int ur0, int vq0, int lane, const double (&g)[4],
                                               bool interleaved, double (&s)[9],
-                                              const double2 *win, int wsb, int wdb)
+                                              const double2 *win, int wub, int wvb, int wdb)
 {
    const int cls = epi_class(a, ur0, vq0);           // uniform per block
    if (diag) {
       if (cls == 1)
-         epi_accum_inner<1, true, WIN>(a, x, y, z, ur0, vq0, lane, g, interleaved, s, win, wsb, wdb);
+         epi_accum_inner<1, true, WIN>(a, x, y, z, ur0, vq0, lane, g, interleaved, s, win, wub, wvb, wdb);
       else if (cls == 2)
-         epi_accum_inner<2, true, WIN>(a, x, y, z, ur0, vq0, lane, g, interleaved, s, win, wsb, wdb);
+         epi_accum_inner<2, true, WIN>(a, x, y, z, ur0, vq0, lane, g, interleaved, s, win, wub, wvb, wdb);
       else if (cls == 3)
-         epi_accum_inner<3, true, WIN>(a, x, y, z, ur0, vq0, lane, g, interleaved, s, win, wsb, wdb);
+         epi_accum_inner<3, true, WIN>(a, x, y, z, ur0, vq0, lane, g, interleaved, s, win, wub, wvb, wdb);
       else
          epi_accum<true>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
    } else {
       if (cls == 1)
-         epi_accum_inner<1, false, WIN>(a, x, y, z, ur0, vq0, lane, g, interleaved, s, win, wsb, wdb);
+         epi_accum_inner<1, false, WIN>(a, x, y, z, ur0, vq0, lane, g, interleaved, s, win, wub, wvb, wdb);
       else if (cls == 2)
-         epi_accum_inner<2, false, WIN>(a, x, y, z, ur0, vq0, lane, g, interleaved, s, win, wsb, wdb);
+         epi_accum_inner<2, false, WIN>(a, x, y, z, ur0, vq0, lane, g, interleaved, s, win, wub, wvb, wdb);
       else if (cls == 3)
-         epi_accum_inner<3, false, WIN>(a, x, y, z, ur0, vq0, lane, g, interleaved, s, win, wsb, wdb);
+         epi_accum_inner<3, false, WIN>(a, x, y, z, ur0, vq0, lane, g, interleaved, s, win, wub, wvb, wdb);
       else
          epi_accum<false>(a, x, y, z, ur0, vq0, lane, g, interleaved, s);
    }
@@ -585,21 +596,24 @@ __device__ __forceinline__ void epi_flush(const GsymArgs &a, int x, int y, int z
 
 // Epilogue of a wave's BR x BC blocks of 16x16 (rows rw0.., cols cw0..): one
 // accumulation / reduction when all blocks share (y,z), else one per (y,z) pair.
-// WIN: win holds 512 double2 of LDS owned by this wave (free after the K loop).
-// When all blocks share (y,z), the FRF values the interior classes gather per
+// WIN: win holds 768 double2 of LDS owned by this wave (free after the K loop).
+// When all blocks share (y,z), the FRF values the interior classes read per
 // point are staged there first, each indexed by its offset from the wave's
 // origin (ub, vb) within the modes:
 //   [0, 128)   H_z(u+v)      at u+v - (ub+vb)
 //   [128, 256) H_y(u+v)      at u+v - (ub+vb)
 //   [256, 384) H_y(u-v)      at u-v - (ub-vb-16BC+1), conjugated for u < v
 //   [384, 512) H_z(u-v)      likewise
+//   [512, 576) H_x(u), [576, 640) H_z(u)   at u - ub
+//   [640, 704) H_x(v), [704, 768) H_y(v)   at v - vb
 // (u+v and u-v span 16(BR+BC)-1 <= 127 values over the wave's blocks).
 template <int BR, int BC, bool WIN, class GetVal>
 __device__ __forceinline__ void epi_wave(const GsymArgs &a, int x, bool diag, int rw0, int cw0,
                                          int lane, bool interleaved, long sbase, GetVal getv,
                                          double2 *win)
 {
-   static_assert(16 * (BR + BC) - 1 <= 128, "FRF window of 128 entries");
+   static_assert(16 * (BR + BC) - 1 <= 128 && 16 * BR <= 64 && 16 * BC <= 64,
+                 "FRF windows of 128 and 64 entries");
    const int nm = a.nm;
    int f = 0;                                         // flushes of this wave so far
    const int y0 = rw0 / a.nhp, y1 = (rw0 + 16 * BR - 1) / a.nhp;
@@ -628,6 +642,16 @@ __device__ __forceinline__ void epi_wave(const GsymArgs &a, int x, bool diag, in
             win[256 + e] = hym;
             win[384 + e] = hzm;
          }
+         if (lane < 16 * BR) {
+            const double2 *Hx = a.H + (size_t)x * a.nft;
+            win[512 + lane] = Hx[ub + lane];
+            win[576 + lane] = Hz[ub + lane];
+         }
+         if (lane < 16 * BC) {
+            const double2 *Hx = a.H + (size_t)x * a.nft;
+            win[640 + lane] = Hx[vb + lane];
+            win[704 + lane] = Hy[vb + lane];
+         }
          // the wave's own LDS writes before its reads (other waves never touch it)
          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
          __builtin_amdgcn_wave_barrier();
@@ -639,7 +663,7 @@ __device__ __forceinline__ void epi_wave(const GsymArgs &a, int x, bool diag, in
          double g[4];
          getv(b, g);
          epi_accum_any<WIN>(a, x, y0, z0, diag, ub + 16 * i, vb + 16 * j, lane, g, interleaved, s,
-                            win, wsb, wdb);
+                            win, ub, vb, wdb);
       }
       epi_flush(a, x, y0, z0, diag, lane, s, sbase, f++);
    } else {
@@ -663,7 +687,7 @@ __device__ __forceinline__ void epi_wave(const GsymArgs &a, int x, bool diag, in
                double g[4];
                getv(b, g);
                epi_accum_any<false>(a, x, y, z, diag, r0 - y * a.nhp, q0 - z * a.nhp, lane, g,
-                                    interleaved, s, nullptr, 0, 0);
+                                    interleaved, s, nullptr, 0, 0, 0);
             }
             epi_flush(a, x, y, z, diag, lane, s, sbase, f++);
          }
@@ -944,8 +968,10 @@ __device__ __forceinline__ void gsym_bf16x3_item(const GsymArgs &a, const GsymAr
    __builtin_amdgcn_s_setprio(0);
    // epilogue: f32-output 16x16 MFMAs hold D[4(l/16)+t][l%16]
    const long sbase = ((tp * a.nm + x) * 4 + w) * (long)a.mflush * 12;
-   // each wave's FRF window: 8 KB of the A planes, free after the last chunk's barrier
-   double2 *win = reinterpret_cast<double2 *>(w < 2 ? &Ah[0][0] : &Al[0][0]) + (w & 1) * 512;
+   // each wave's FRF window: 12 KB of one operand plane (16 KB), free after the
+   // last chunk's barrier
+   uint4 *plane = w == 0 ? &Ah[0][0] : (w == 1 ? &Al[0][0] : (w == 2 ? &Bh[0][0] : &Bl[0][0]));
+   double2 *win = reinterpret_cast<double2 *>(plane);
    epi_wave<4, 4, true>(a, x, diag, R * T2 + wr * 64, Q * T2 + wc * 64, lane, false, sbase,
                   [&](int b, double (&g)[4]) {
                      f4v v;
@@ -1063,8 +1089,9 @@ __device__ __forceinline__ void gsym_f32_item(const GsymArgs &a, const GsymArgs1
    // epilogue: f32-output 16x16 MFMAs hold D[4(l/16)+t][l%16]
    const long sbase = ((tp * a.nm + x) * 4 + w) * (long)a.mflush * 12;
    __builtin_amdgcn_s_setprio(0);
-   // each wave's FRF window: 8 KB of the A planes, free after the last chunk's barrier
-   double2 *win = reinterpret_cast<double2 *>(&As[0][0]) + w * 512;
+   // each wave's FRF window: 12 KB of the operand planes (2 x 32 KB), free after
+   // the last chunk's barrier
+   double2 *win = reinterpret_cast<double2 *>(w < 2 ? &As[0][0] : &Bs[0][0]) + (w & 1) * 768;
    epi_wave<4, 4, true>(a, x, diag, R * T2 + wr * 64, Q * T2 + wc * 64, lane, false, sbase,
                   [&](int bb, double (&gv)[4]) {
                      f4v v;
